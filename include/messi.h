@@ -1,0 +1,118 @@
+/*
+ * libmessi -- MI355X (gfx950) likelihood hot path of MESSI behind a C ABI.
+ *
+ * One context per host thread; one HIP stream per context.  Every call is
+ * synchronous from the caller's view, reads and writes caller-owned host
+ * arrays only during the call, and returns 0 on success or a non-zero code
+ * with a message available from messi_last_error().
+ *
+ * Layouts are Julia-native (column-major):
+ *   alignment  data[s + nseq*c + nseq*ncol*a]  (nseq, ncol, 4), 0/1 state indicators
+ *   matrices   P[k][node][a + dim*d] = P(a -> d)  (dim, dim, nnodes, K)
+ *   L x L      M[i + L*j]
+ *   3 x L x L  T[s + 3*(i + L*j)], s = 0 (S), 1 (F), 2 (L) -- the KH99 nonterminals
+ * Trees are pre-order with the root at index 0 (the reference's nodeindex - 1);
+ * children[2*v], children[2*v+1] are -1 for leaves; seqindex[v] is the
+ * alignment row (0-based) of leaf v and -1 for internal nodes.
+ *
+ * prec: MESSI_FP64 is the plain path (matches the reference CPU semantics);
+ * MESSI_FP32 keeps pruning and the inside/outside bulk in FP32 with power-of-two
+ * scaling per pattern and baseline-relative base-2 log tables.
+ */
+#ifndef MESSI_H
+#define MESSI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MESSI_FP64 0
+#define MESSI_FP32 1
+
+typedef struct messi_ctx messi_ctx;
+typedef struct messi_dataset messi_dataset;
+
+/* Context and error handling.  Replaces the per-call CuContext/CuModuleFile setup
+ * of src/FelsensteinCUDA.jl:240-253 and src/InsideCUDA.jl:96-104. */
+int messi_ctx_create(int device, messi_ctx **out);
+void messi_ctx_destroy(messi_ctx *ctx);
+const char *messi_last_error(const messi_ctx *ctx);
+/* Record per-phase device times (HIP events) for messi_last_timings. */
+int messi_set_profiling(messi_ctx *ctx, int on);
+/* Checks the MFMA operand layouts the kernels rely on against a CPU product. */
+int messi_selftest(messi_ctx *ctx);
+
+/* Dataset handle: one-hot alignment, rooted binary tree, base-pair distance
+ * bound and optional pair mask (mask[x + ncol*y] != 0 allows the pair, NULL
+ * allows all).  Builds the per-node pattern dedup (src/Dataset.jl:180-196) and
+ * the per-node pair index lists (getindexdict, src/FelsensteinCUDA.jl:77-142)
+ * once, replacing the reference's global caches (src/FelsensteinCUDA.jl:206-238). */
+int messi_dataset_create(messi_ctx *ctx, int nseq, int ncol, const double *data, int nnodes,
+                         const int *children, const int *seqindex, int maxbasepairdistance,
+                         const uint8_t *mask, messi_dataset **out);
+void messi_dataset_destroy(messi_dataset *ds);
+/* Work counts: stats[0] = sum over internal nodes of distinct pair patterns
+ * (sum U_v), stats[1] = root pair entries, stats[2] = in-range column pairs,
+ * stats[3] = sum over internal nodes of distinct column patterns, stats[4] =
+ * tree levels, stats[5] = device bytes held by the dataset. */
+int messi_dataset_stats(const messi_dataset *ds, int64_t *stats);
+
+/* Site log-likelihoods over K site-rate components:
+ * u[c] = lse_k(logw[k] + log sum_a pi4[k][a] * root_a(c)).
+ * Replaces computeunpairedlikelihoods (src/FARCE.jl:1198-1220), whose per-column
+ * work is felsensteinstack (src/FARCE.jl:2182-2254). */
+int messi_site_loglik(messi_dataset *ds, int K, const double *P4, const double *pi4,
+                      const double *logw, int prec, double *u_out);
+
+/* Pair log-likelihood matrix, mixed over K components with log weights.
+ * out (ncol x ncol) is symmetric; the diagonal, pairs beyond the distance bound
+ * and masked pairs are -inf.  percomp (K x ncol x ncol) receives the
+ * per-component matrices when not NULL.
+ * Replaces felsensteincuda (src/FelsensteinCUDA.jl:214-516). */
+int messi_pair_loglik(messi_dataset *ds, int K, const double *P16, const double *pi16,
+                      const double *logw, int prec, double *out, double *percomp);
+
+/* KH99 inside.  Z receives log Z = S[0, L-1]; when zonly == 0, in3 (3 x L x L)
+ * receives the inside table (-inf below the diagonal).
+ * Replaces computeinsideKH99 (src/InsideCUDA.jl:23-53) and its CPU semantics
+ * computeinsideparallel (src/InsideOutsideAlgorithm.jl:683-749). */
+int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, double B, int zonly,
+                 int prec, double *Z, double *in3);
+
+/* KH99 outside from an inside table.  Replaces computeoutsideKH99
+ * (src/InsideCUDA.jl:55-61) and computeoutside (src/InsideOutsideAlgorithm.jl:447-483). */
+int messi_outside(messi_ctx *ctx, int L, const double *in3, const double *u, const double *paired,
+                  int prec, double *out3);
+
+/* Row-normalised base-pair posteriors.  Replaces computebasepairprobs
+ * (src/InsideOutsideAlgorithm.jl:486-533). */
+int messi_bpp(messi_ctx *ctx, int L, const double *in3, const double *out3, const double *u,
+              const double *paired, int prec, double *unpaired, double *pairprob);
+
+/* One full-likelihood evaluation on the device with no intermediate host
+ * transfers: site log-likelihoods, pair matrix, inside log Z.  Replaces
+ * computestructureintegratedlikelihood without its prior term
+ * (src/FARCE.jl:2322-2345). */
+int messi_eval(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
+               int K16, const double *P16, const double *pi16, const double *logw16, double B,
+               int prec, double *logZ);
+
+/* Evaluation plus outside and base-pair posteriors (the processmax path,
+ * src/MESSI.jl:1097-1103).  unpaired (L) and pairprob (L x L) may be NULL. */
+int messi_posterior(messi_dataset *ds, int K4, const double *P4, const double *pi4,
+                    const double *logw4, int K16, const double *P16, const double *pi16,
+                    const double *logw16, int prec, double *logZ, double *unpaired,
+                    double *pairprob);
+
+/* Device time in milliseconds of the phases of the last messi_eval /
+ * messi_posterior call on this dataset: t[0] site, t[1] pair, t[2] inside,
+ * t[3] outside, t[4] bpp (HIP events on the context's stream). */
+int messi_last_timings(const messi_dataset *ds, double *t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

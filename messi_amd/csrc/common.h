@@ -1,0 +1,92 @@
+// Shared host/device definitions of libmessi (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/messi.h"
+
+#define MESSI_HIP(expr)                                                                      \
+  do {                                                                                       \
+    hipError_t _e = (expr);                                                                  \
+    if (_e != hipSuccess)                                                                    \
+      throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(_e));           \
+  } while (0)
+
+#define MESSI_REQUIRE(cond, msg)                                                             \
+  do {                                                                                       \
+    if (!(cond)) throw std::invalid_argument(msg);                                           \
+  } while (0)
+
+namespace messi {
+
+// KH99 rule log-probabilities (src/InsideOutsideAlgorithm.jl:40-46), natural log.
+// Index: 0 S->LS, 1 S->s, 2 S->dFd, 3 F->dFd, 4 F->LS, 5 L->s, 6 L->dFd.
+inline void kh99_logprobs(double lp[7]) {
+  lp[0] = std::log(0.868534);
+  lp[1] = std::log(0.117609877998);
+  lp[2] = std::log(0.013856122002);
+  lp[3] = std::log(0.787640);
+  lp[4] = std::log(0.21236);
+  lp[5] = std::log(0.894603);
+  lp[6] = std::log(0.105397);
+}
+
+// Device buffer that only grows.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void ensure(size_t n) {
+    if (n <= bytes) return;
+    release();
+    MESSI_HIP(hipMalloc(&p, n < 256 ? 256 : n));
+    bytes = n < 256 ? 256 : n;
+  }
+  template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+template <typename T>
+inline void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+  b.ensure(v.size() * sizeof(T));
+  if (!v.empty())
+    MESSI_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// Scratch of the inside/outside/bpp phase, per precision.
+struct InsideWork {
+  int L = 0;
+  DevBuf Lr, Lc, Sr, Sc, F;       // inside tables (row- and column-major copies)
+  DevBuf oSr, oSc, oFr, oFc, oL;  // outside tables
+  DevBuf ptv;                     // scaled pair term per cell
+  DevBuf lam, C;                  // per-position baseline and its prefix sums (double)
+  DevBuf uhat;                    // u - lambda (double)
+  DevBuf paired;                  // double L x L (row-major == column-major: symmetric)
+  DevBuf u;                       // double L
+  DevBuf lp;                      // bpp log table
+  DevBuf red;                     // reductions
+  DevBuf out3;                    // double 3 x L x L staging
+};
+
+}  // namespace messi
+
+struct messi_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool profiling = false;
+  messi::InsideWork w64, w32;
+};

@@ -1,0 +1,358 @@
+// Felsenstein pruning kernels for gfx950: 16-state column-pair pruning on MFMA
+// and 4-state site pruning.
+//
+// Both run level by level over the tree (all internal nodes of equal height in
+// one launch) on per-node deduplicated entries, the reference's pattern dedup
+// (getindexdict, src/FelsensteinCUDA.jl:77-142, and the FastCache of
+// src/FARCE.jl:2182-2254): an entry of node v is one distinct pair (or single)
+// of subtree column patterns at v, and carries the 16 (or 4) partial
+// likelihoods plus an integer power-of-two exponent.  An internal entry is
+//   part_v = (P_l * part_l[idx_l]) .* (P_r * part_r[idx_r]),
+// rescaled so its maximum lies in [0.5, 1) (exact power-of-two scaling; the
+// reference divides by the maximum when it falls below 1e-20,
+// src/FARCE.jl:2225-2238, and always on the GPU, src/cuda/felsenstein.cu:193-198).
+// At the root the log-likelihood log(sum_a pi_a part_a) + exponent*ln 2 is mixed
+// into a running log-sum-exp over components (src/FelsensteinCUDA.jl:400-419).
+//
+// MFMA mapping (V_MFMA_F64_16X16X4_F64 / V_MFMA_F32_16X16X4_F32, one wavefront
+// per block of 16 entries): the message P_c * part_c is a 16x16 (states) times
+// 16x16 (states x entries) product done as four K=4 steps.  Step s contracts
+// the states {s, 4+s, 8+s, 12+s}: lane l feeds A = P[l%16][4*(l/16)+s] and
+// B = part[4*(l/16)+s][entry l%16], which is exactly register s of the
+// accumulator layout (lane l holds rows 4*(l/16)+r of column l%16).  A node's
+// output therefore feeds its parent's MFMA with no data movement, and every
+// lane loads/stores 4 consecutive states of one entry.
+#include "common.h"
+#include "prune.h"
+
+namespace messi {
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+__device__ __forceinline__ double lse_d(double a, double b) {
+  if (a == -HUGE_VAL) return b;
+  if (b == -HUGE_VAL) return a;
+  return a < b ? b + log1p(exp(a - b)) : a + log1p(exp(b - a));
+}
+
+__device__ __forceinline__ void load4(const double *p, double v[4]) {
+  const double2 *q = reinterpret_cast<const double2 *>(p);
+  double2 a = q[0], b = q[1];
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+__device__ __forceinline__ void load4(const float *p, float v[4]) {
+  float4 a = *reinterpret_cast<const float4 *>(p);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+}
+__device__ __forceinline__ void store4(double *p, const double v[4]) {
+  double2 *q = reinterpret_cast<double2 *>(p);
+  q[0] = make_double2(v[0], v[1]);
+  q[1] = make_double2(v[2], v[3]);
+}
+__device__ __forceinline__ void store4(float *p, const float v[4]) {
+  *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+__device__ __forceinline__ int frexp_exp(double m) {
+  int e;
+  (void)frexp(m, &e);
+  return e;
+}
+__device__ __forceinline__ int frexp_exp(float m) {
+  int e;
+  (void)frexpf(m, &e);
+  return e;
+}
+
+// One wavefront per item of 16*NB consecutive entries of one node.
+template <typename T, int NB, bool ROOT>
+__global__ __launch_bounds__(256) void pair_level_kernel(
+    const int2 *__restrict__ items, int nitems, const NodeDesc *__restrict__ nodes,
+    const int *__restrict__ idx, const uint16_t *__restrict__ leafc, T *__restrict__ part,
+    int *__restrict__ expo, const T *__restrict__ P, const double *__restrict__ pi16, double logw,
+    int first, double *__restrict__ rootacc, double *__restrict__ rootcomp) {
+  typedef typename Mfma<T>::v4 v4;
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+  if (wave >= nitems) return;  // uniform per wavefront
+  const int2 it = items[wave];
+  const NodeDesc nd = nodes[it.x];
+  const int col = lane & 15, g = lane >> 4;
+  T al[4], ar[4];
+  const T *Pl = P + (size_t)nd.node_l * 256;
+  const T *Pr = P + (size_t)nd.node_r * 256;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    al[s] = Pl[col + 16 * (4 * g + s)];
+    ar[s] = Pr[col + 16 * (4 * g + s)];
+  }
+  double pis[4] = {0, 0, 0, 0};
+  if (ROOT) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pis[r] = pi16[4 * g + r];
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int64_t e = (int64_t)it.y + b * 16 + col;
+    const bool valid = e < nd.U;
+    const int64_t ec = valid ? e : nd.U - 1;
+    T bl[4], br[4];
+    int el = 0, er = 0;
+    const int cl = idx[nd.idx_l + ec];
+    const int cr = idx[nd.idx_r + ec];
+    if (nd.leaf_l) {
+      const unsigned cc = leafc[nd.cpart_l + cl];
+      const T xg = (T)((cc >> g) & 1u);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) bl[s] = xg * (T)((cc >> (8 + s)) & 1u);
+    } else {
+      load4(part + (size_t)(nd.cpart_l + cl) * 16 + 4 * g, bl);
+      el = expo[nd.cpart_l + cl];
+    }
+    if (nd.leaf_r) {
+      const unsigned cc = leafc[nd.cpart_r + cr];
+      const T xg = (T)((cc >> g) & 1u);
+#pragma unroll
+      for (int s = 0; s < 4; ++s) br[s] = xg * (T)((cc >> (8 + s)) & 1u);
+    } else {
+      load4(part + (size_t)(nd.cpart_r + cr) * 16 + 4 * g, br);
+      er = expo[nd.cpart_r + cr];
+    }
+    v4 ml = {0, 0, 0, 0}, mr = {0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      ml = Mfma<T>::op(al[s], bl[s], ml);
+      mr = Mfma<T>::op(ar[s], br[s], mr);
+    }
+    T v[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = ml[r] * mr[r];
+    T m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
+    m = fmax(m, __shfl_xor(m, 16));
+    m = fmax(m, __shfl_xor(m, 32));
+    int E = el + er;
+    if (m > (T)0) {
+      const int ex = frexp_exp(m);
+      const T sc = ldexp((T)1, -ex);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v[r] *= sc;
+      E += ex;
+    }
+    if (!ROOT) {
+      if (valid) {
+        store4(part + (size_t)(nd.part + e) * 16 + 4 * g, v);
+        if (g == 0) expo[nd.part + e] = E;
+      }
+    } else {
+      double s = pis[0] * (double)v[0] + pis[1] * (double)v[1] + pis[2] * (double)v[2] +
+                 pis[3] * (double)v[3];
+      s += __shfl_xor(s, 16);
+      s += __shfl_xor(s, 32);
+      if (valid && g == 0) {
+        const double ll = log(s) + (double)E * 0.69314718055994530942;
+        rootcomp[e] = ll;
+        const double t = logw + ll;
+        rootacc[e] = first ? t : lse_d(rootacc[e], t);
+      }
+    }
+  }
+}
+
+// One thread per entry of a node; 256 entries per item.
+template <typename T, bool ROOT>
+__global__ __launch_bounds__(256) void site_level_kernel(
+    const int2 *__restrict__ items, int nitems, const NodeDesc *__restrict__ nodes,
+    const int *__restrict__ idx, const uint8_t *__restrict__ leafc, T *__restrict__ part,
+    int *__restrict__ expo, const T *__restrict__ P4, const double *__restrict__ pi4, double logw,
+    int first, double *__restrict__ rootacc) {
+  if ((int)blockIdx.x >= nitems) return;
+  const int2 it = items[blockIdx.x];
+  const NodeDesc nd = nodes[it.x];
+  const int64_t e = (int64_t)it.y + threadIdx.x;
+  if (e >= nd.U) return;
+  T vl[4], vr[4];
+  int el = 0, er = 0;
+  const int cl = idx[nd.idx_l + e], cr = idx[nd.idx_r + e];
+  if (nd.leaf_l) {
+    const unsigned c = leafc[nd.cpart_l + cl];
+    for (int a = 0; a < 4; ++a) vl[a] = (T)((c >> a) & 1u);
+  } else {
+    load4(part + (size_t)(nd.cpart_l + cl) * 4, vl);
+    el = expo[nd.cpart_l + cl];
+  }
+  if (nd.leaf_r) {
+    const unsigned c = leafc[nd.cpart_r + cr];
+    for (int a = 0; a < 4; ++a) vr[a] = (T)((c >> a) & 1u);
+  } else {
+    load4(part + (size_t)(nd.cpart_r + cr) * 4, vr);
+    er = expo[nd.cpart_r + cr];
+  }
+  const T *Pl = P4 + (size_t)nd.node_l * 16;
+  const T *Pr = P4 + (size_t)nd.node_r * 16;
+  T v[4];
+  T m = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    T sl = 0, sr = 0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      sl += Pl[a + 4 * d] * vl[d];
+      sr += Pr[a + 4 * d] * vr[d];
+    }
+    v[a] = sl * sr;
+    m = fmax(m, v[a]);
+  }
+  int E = el + er;
+  if (m > (T)0) {
+    const int ex = frexp_exp(m);
+    const T sc = ldexp((T)1, -ex);
+    for (int a = 0; a < 4; ++a) v[a] *= sc;
+    E += ex;
+  }
+  if (!ROOT) {
+    store4(part + (size_t)(nd.part + e) * 4, v);
+    expo[nd.part + e] = E;
+  } else {
+    double s = 0.0;
+    for (int a = 0; a < 4; ++a) s += pi4[a] * (double)v[a];
+    const double t = logw + log(s) + (double)E * 0.69314718055994530942;
+    rootacc[e] = first ? t : lse_d(rootacc[e], t);
+  }
+}
+
+// Site expansion: u[c] = root entry of column c's root pattern.
+__global__ void site_expand_kernel(int L, const int *__restrict__ pid0, const double *__restrict__ acc,
+                                   double *__restrict__ u) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < L) u[c] = acc[pid0[c]];
+}
+
+// Pair expansion into a symmetric row-major L x L matrix.
+__global__ void pair_expand_kernel(int L, int maxbp, int n0, const int *__restrict__ pid0,
+                                   const int *__restrict__ T0, const uint8_t *__restrict__ mask,
+                                   const double *__restrict__ acc, double *__restrict__ out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int r = blockIdx.y;
+  if (c >= L) return;
+  int x = min(r, c), y = max(r, c);
+  double v = -HUGE_VAL;
+  if (x != y && y - x <= maxbp && (!mask || mask[(size_t)x + (size_t)L * y])) {
+    int k = T0[(size_t)pid0[x] * n0 + pid0[y]];
+    if (k >= 0) v = acc[k];
+  }
+  out[(size_t)r * L + c] = v;
+}
+
+template <typename T, int NB>
+void launch_pair_level(hipStream_t s, const int2 *items, int nitems, bool root, const NodeDesc *nodes,
+                       const int *idx, const uint16_t *leafc, T *part, int *expo, const T *P,
+                       const double *pi16, double logw, int first, double *rootacc,
+                       double *rootcomp) {
+  if (nitems <= 0) return;
+  int blocks = (nitems + 3) / 4;
+  if (root)
+    hipLaunchKernelGGL((pair_level_kernel<T, NB, true>), dim3(blocks), dim3(256), 0, s, items, nitems,
+                       nodes, idx, leafc, part, expo, P, pi16, logw, first, rootacc, rootcomp);
+  else
+    hipLaunchKernelGGL((pair_level_kernel<T, NB, false>), dim3(blocks), dim3(256), 0, s, items, nitems,
+                       nodes, idx, leafc, part, expo, P, pi16, logw, first, rootacc, rootcomp);
+}
+
+template <typename T>
+void launch_site_level(hipStream_t s, const int2 *items, int nitems, bool root, const NodeDesc *nodes,
+                       const int *idx, const uint8_t *leafc, T *part, int *expo, const T *P4,
+                       const double *pi4, double logw, int first, double *rootacc) {
+  if (nitems <= 0) return;
+  if (root)
+    hipLaunchKernelGGL((site_level_kernel<T, true>), dim3(nitems), dim3(256), 0, s, items, nitems, nodes,
+                       idx, leafc, part, expo, P4, pi4, logw, first, rootacc);
+  else
+    hipLaunchKernelGGL((site_level_kernel<T, false>), dim3(nitems), dim3(256), 0, s, items, nitems, nodes,
+                       idx, leafc, part, expo, P4, pi4, logw, first, rootacc);
+}
+
+void launch_site_expand(hipStream_t s, int L, const int *pid0, const double *acc, double *u) {
+  hipLaunchKernelGGL(site_expand_kernel, dim3((L + 255) / 256), dim3(256), 0, s, L, pid0, acc, u);
+}
+
+void launch_pair_expand(hipStream_t s, int L, int maxbp, int n0, const int *pid0, const int *T0,
+                        const uint8_t *mask, const double *acc, double *out) {
+  hipLaunchKernelGGL(pair_expand_kernel, dim3((L + 255) / 256, L), dim3(256), 0, s, L, maxbp, n0, pid0,
+                     T0, mask, acc, out);
+}
+
+template void launch_pair_level<double, PAIR_NB>(hipStream_t, const int2 *, int, bool, const NodeDesc *,
+                                                 const int *, const uint16_t *, double *, int *,
+                                                 const double *, const double *, double, int, double *,
+                                                 double *);
+template void launch_pair_level<float, PAIR_NB>(hipStream_t, const int2 *, int, bool, const NodeDesc *,
+                                                const int *, const uint16_t *, float *, int *,
+                                                const float *, const double *, double, int, double *,
+                                                double *);
+template void launch_site_level<double>(hipStream_t, const int2 *, int, bool, const NodeDesc *,
+                                        const int *, const uint8_t *, double *, int *, const double *,
+                                        const double *, double, int, double *);
+template void launch_site_level<float>(hipStream_t, const int2 *, int, bool, const NodeDesc *,
+                                       const int *, const uint8_t *, float *, int *, const float *,
+                                       const double *, double, int, double *);
+
+// ---------------------------------------------------------------------------
+// Self-test of the MFMA operand layouts the pair kernel relies on.
+// D = A (16x4) * B (4x16); A[i][k] at lane i + 16k, B[k][j] at lane j + 16k,
+// D[i][j] in register i%4 of lane j + 16*(i/4).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void mfma_probe_kernel(const T *A, const T *Bm, T *D) {
+  typedef typename Mfma<T>::v4 v4;
+  int l = threadIdx.x;
+  T a = A[(l % 16) * 4 + l / 16];
+  T b = Bm[(l / 16) * 16 + l % 16];
+  v4 acc = {0, 0, 0, 0};
+  acc = Mfma<T>::op(a, b, acc);
+  for (int r = 0; r < 4; ++r) D[(4 * (l / 16) + r) * 16 + l % 16] = acc[r];
+}
+
+template <typename T>
+bool mfma_probe(hipStream_t s) {
+  std::vector<T> A(64), Bm(64), D(256), R(256, 0);
+  for (int i = 0; i < 64; ++i) {
+    A[i] = (T)(1 + (i * 7) % 13);
+    Bm[i] = (T)(1 + (i * 5) % 11);
+  }
+  for (int i = 0; i < 16; ++i)
+    for (int j = 0; j < 16; ++j)
+      for (int k = 0; k < 4; ++k) R[i * 16 + j] += A[i * 4 + k] * Bm[k * 16 + j];
+  DevBuf dA, dB, dD;
+  upload(dA, A, s);
+  upload(dB, Bm, s);
+  dD.ensure(256 * sizeof(T));
+  hipLaunchKernelGGL(mfma_probe_kernel<T>, dim3(1), dim3(64), 0, s, dA.as<T>(), dB.as<T>(), dD.as<T>());
+  MESSI_HIP(hipGetLastError());
+  MESSI_HIP(hipMemcpyAsync(D.data(), dD.p, 256 * sizeof(T), hipMemcpyDeviceToHost, s));
+  MESSI_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < 256; ++i)
+    if (D[i] != R[i]) return false;
+  return true;
+}
+template bool mfma_probe<double>(hipStream_t);
+template bool mfma_probe<float>(hipStream_t);
+
+}  // namespace messi

@@ -1,0 +1,133 @@
+"""The reference's hot-path entry points, with their names and argument meaning,
+routed to libmessi (HIP, gfx950).  There is no CPU fallback in this package.
+
+===============================  ==========================================================
+Function here                    Reference
+===============================  ==========================================================
+``felsensteincuda``              src/FelsensteinCUDA.jl:214-516
+``computeunpairedlikelihoods``   src/FARCE.jl:1198-1220
+``coevolutionall``               src/FARCE.jl:1736-1858 (integratesiterates=true, GPU branch)
+``computeinsideKH99``            src/InsideCUDA.jl:23-53
+``computeoutsideKH99``           src/InsideCUDA.jl:55-61
+``computebasepairprobs``         src/InsideOutsideAlgorithm.jl:486-533
+``computestructureintegratedlikelihood``  src/FARCE.jl:2322-2345
+===============================  ==========================================================
+
+``prec`` selects the plain FP64 path (default) or the scaled FP32 path.
+"""
+from __future__ import annotations
+
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import lib
+from .model import (ModelParameters, MuseSpecificParameters, computeprior, pair_components,
+                    site_components)
+
+
+class KH99:
+    """Marker for the KH99 grammar (src/InsideOutsideAlgorithm.jl:30-53); its rule
+    probabilities are compiled into the library."""
+
+    RULES = (("S", "LS", 0.868534), ("S", "s", 0.117609877998), ("S", "dFd", 0.013856122002),
+             ("F", "dFd", 0.787640), ("F", "LS", 0.21236), ("L", "s", 0.894603), ("L", "dFd", 0.105397))
+
+
+def device_dataset(dataset, maxbasepairdistance: int = 1000000, mask: Optional[np.ndarray] = None
+                   ) -> "lib.DeviceDataset":
+    """Device-resident copy of a Dataset for one distance bound (cached on the dataset)."""
+    cache = dataset.__dict__.setdefault("_device", {})
+    key = (int(min(maxbasepairdistance, 2**31 - 1)), None if mask is None else id(mask))
+    if key not in cache:
+        cache[key] = lib.DeviceDataset(dataset, maxbasepairdistance, mask)
+    return cache[key]
+
+
+def _stack(components):
+    P = np.stack([c[0] for c in components])
+    pi = np.stack([c[1] for c in components])
+    lw = np.array([c[2] for c in components])
+    return P, pi, lw
+
+
+def computeunpairedlikelihoods(dataset, params: ModelParameters, prec=lib.FP64,
+                               maxbasepairdistance: int = 1000000) -> np.ndarray:
+    """Site log-likelihoods mixed over site-rate categories."""
+    dd = device_dataset(dataset, maxbasepairdistance)
+    return dd.site_loglik(*_stack(site_components(params)), prec=prec)
+
+
+def felsensteincuda(dataset, params: ModelParameters, museparams: Optional[List[MuseSpecificParameters]] = None,
+                    maxbasepairdistance: int = 1000000, keepmuseconditionals: bool = False,
+                    prec=lib.FP64, mask: Optional[np.ndarray] = None
+                    ) -> Tuple[np.ndarray, List[np.ndarray]]:
+    """All-pairs 16-state pruning with the site-category x lambda-category mixture.
+
+    Returns (L x L matrix, per-component matrices if ``keepmuseconditionals``).
+    The lambda categories come from ``params`` (``museparams`` is accepted for
+    signature parity; with one lambda category it is ``getmusespecificparamsarray``).
+    """
+    dd = device_dataset(dataset, maxbasepairdistance, mask)
+    comps = pair_components(params)
+    P, pi, lw = _stack(comps)
+    if keepmuseconditionals:
+        mat, per = dd.pair_loglik(P, pi, lw, prec=prec, percomp=True)
+        return np.asarray(mat), per
+    return np.asarray(dd.pair_loglik(P, pi, lw, prec=prec)), []
+
+
+def coevolutionall(dataset, params: ModelParameters, parallel: bool = True, keepmatrices: bool = False,
+                   integratesiterates: bool = True, maxbasepairdistance: int = 1000000,
+                   usecuda: bool = True, prec=lib.FP64):
+    if not (integratesiterates and usecuda):
+        raise NotImplementedError("only the integrated-site-rate device path is provided")
+    return felsensteincuda(dataset, params, None, maxbasepairdistance, keepmatrices, prec=prec)
+
+
+def computeinsideKH99(unpairedlogprobs, pairedlogprobs, B: float = 1.0, zonly: bool = True,
+                      usecudain: bool = True, prec=lib.FP64):
+    """log Z, or the (3, L, L) inside table [S, F, L] when ``zonly`` is false."""
+    if not usecudain:
+        raise NotImplementedError("the CPU inside algorithm is not part of this library")
+    r = lib.inside(unpairedlogprobs, pairedlogprobs, B, zonly, prec)
+    return r if zonly else r[1]
+
+
+def computeoutsideKH99(inside, unpairedlogprobs, pairedlogprobs, usecuda: bool = True, prec=lib.FP64):
+    if not usecuda:
+        raise NotImplementedError("the CPU outside algorithm is not part of this library")
+    return lib.outside(inside, unpairedlogprobs, pairedlogprobs, prec)
+
+
+def computebasepairprobs(inside, outside, unpairedlogprobs, pairedlogprobs, grammar: KH99 = None,
+                         prec=lib.FP64):
+    """(unpaired probabilities (L), pair probabilities (L x L)), row-normalised."""
+    return lib.bpp(inside, outside, unpairedlogprobs, pairedlogprobs, prec)
+
+
+def computestructureintegratedlikelihood(dataset, params: ModelParameters, maxbasepairdistance: int = 1000000,
+                                         samplebranchlengths: bool = False, fixLambdaGT: bool = False,
+                                         integratesiterates: bool = True, usecuda: bool = True,
+                                         prec=lib.FP64) -> float:
+    """log prior + log Z of the structure-integrated model (one device evaluation)."""
+    p = params
+    if fixLambdaGT:
+        p = params.copy()
+        p.lambdaGT = 1.0
+        p.lambdaratesGT = np.array([0.0])
+    dd = device_dataset(dataset, maxbasepairdistance)
+    z = dd.eval(_stack(site_components(p)), _stack(pair_components(p)), 1.0, prec)
+    return computeprior(p, samplebranchlengths, fixLambdaGT) + z
+
+
+def loglikelihood(dataset, params: ModelParameters, maxbasepairdistance: int = 1000000, prec=lib.FP64) -> float:
+    """log Z only (the eval without the prior)."""
+    dd = device_dataset(dataset, maxbasepairdistance)
+    return dd.eval(_stack(site_components(params)), _stack(pair_components(params)), 1.0, prec)
+
+
+def posterior(dataset, params: ModelParameters, maxbasepairdistance: int = 1000000, prec=lib.FP64):
+    """(log Z, unpaired probabilities, pair probabilities) in one device pass."""
+    dd = device_dataset(dataset, maxbasepairdistance)
+    return dd.posterior(_stack(site_components(params)), _stack(pair_components(params)), prec)

@@ -1,0 +1,305 @@
+"""ctypes binding of ``libmessi.so`` (the C ABI declared in include/messi.h).
+
+The shared library is built in-tree by ``messi_amd.build``.  There is no CPU
+fallback: loading fails loudly when the library is missing, and every call
+raises ``MessiError`` with the library's message when it returns non-zero.
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes
+import os
+import threading
+import weakref
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmessi.so")
+
+FP64 = 0
+FP32 = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_u8 = ctypes.POINTER(ctypes.c_uint8)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every entry point declared in include/messi.h.
+SIGNATURES = {
+    "messi_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+    "messi_ctx_destroy": (None, [_vp]),
+    "messi_last_error": (ctypes.c_char_p, [_vp]),
+    "messi_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "messi_selftest": (ctypes.c_int, [_vp]),
+    "messi_dataset_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, _ip,
+                                            ctypes.c_int, _u8, ctypes.POINTER(_vp)]),
+    "messi_dataset_destroy": (None, [_vp]),
+    "messi_dataset_stats": (ctypes.c_int, [_vp, _i64]),
+    "messi_site_loglik": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp]),
+    "messi_pair_loglik": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]),
+    "messi_inside": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                    _dp, _dp]),
+    "messi_outside": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp]),
+    "messi_bpp": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]),
+    "messi_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
+                                  ctypes.c_double, ctypes.c_int, _dp]),
+    "messi_posterior": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
+                                       ctypes.c_int, _dp, _dp, _dp]),
+    "messi_last_timings": (ctypes.c_int, [_vp, _dp]),
+}
+
+_lib = None
+_lock = threading.Lock()
+_live = weakref.WeakSet()      # device objects to release before the HIP runtime unloads
+_shutdown = False
+
+
+@atexit.register
+def _close_all():
+    global _shutdown
+    for o in list(_live):
+        if isinstance(o, DeviceDataset):
+            o.close()
+    for o in list(_live):
+        o.close()
+    _shutdown = True
+
+
+class MessiError(RuntimeError):
+    pass
+
+
+def load() -> ctypes.CDLL:
+    """Load libmessi.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIBPATH):
+                raise MessiError(f"{LIBPATH} is missing: build it with `python -m messi_amd.build`")
+            lib = ctypes.CDLL(LIBPATH)
+            for name, (res, args) in SIGNATURES.items():
+                f = getattr(lib, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def ptr(a: Optional[np.ndarray], t=_dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(t)
+
+
+def prec_code(prec) -> int:
+    if prec in (FP64, "fp64", "float64", np.float64):
+        return FP64
+    if prec in (FP32, "fp32", "float32", np.float32):
+        return FP32
+    raise ValueError(f"unknown precision {prec!r}")
+
+
+class Context:
+    """One HIP device context + stream (one per host thread)."""
+
+    def __init__(self, device: int = 0):
+        lib = load()
+        h = _vp()
+        rc = lib.messi_ctx_create(int(device), ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = lib.messi_last_error(h) if h else b"messi_ctx_create failed"
+            if h:
+                lib.messi_ctx_destroy(h)
+            self._h = None
+            raise MessiError((msg or b"").decode())
+        self.lib = lib
+        _live.add(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def check(self, rc: int) -> None:
+        if rc != 0:
+            raise MessiError(self.lib.messi_last_error(self._h).decode())
+
+    def set_profiling(self, on: bool) -> None:
+        self.check(self.lib.messi_set_profiling(self._h, int(bool(on))))
+
+    def selftest(self) -> None:
+        self.check(self.lib.messi_selftest(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            self.lib.messi_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _shutdown:
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_ctx: Optional[Context] = None
+
+
+def context() -> Context:
+    """Process-wide default context on device 0."""
+    global _ctx
+    if _ctx is None:
+        _ctx = Context(0)
+    return _ctx
+
+
+def colmajor_P(P: np.ndarray) -> np.ndarray:
+    """(..., nnodes, dim, dim) [a, d] -> contiguous column-major-per-matrix doubles."""
+    P = np.asarray(P, dtype=np.float64)
+    return np.ascontiguousarray(np.swapaxes(P, -1, -2))
+
+
+class DeviceDataset:
+    """Device-resident dataset (alignment, tree, dedup lists) for one distance bound."""
+
+    def __init__(self, ds, maxbasepairdistance: int = 1000000, mask: Optional[np.ndarray] = None,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or context()
+        self.L = ds.numcols
+        self.nnodes = ds.numnodes
+        data = np.asfortranarray(ds.data, dtype=np.float64)
+        ch = np.ascontiguousarray(ds.tree.children.reshape(-1), dtype=np.int32)
+        sq = np.ascontiguousarray(ds.tree.seqindex, dtype=np.int32)
+        m = None if mask is None else np.asfortranarray(mask > 0, dtype=np.uint8)
+        self.maxbp = int(min(maxbasepairdistance, 2**31 - 1))
+        h = _vp()
+        self.ctx.check(self.ctx.lib.messi_dataset_create(
+            self.ctx.handle, ds.numseqs, ds.numcols, ptr(data), ds.numnodes, ptr(ch, _ip), ptr(sq, _ip),
+            self.maxbp, ptr(m, _u8), ctypes.byref(h)))
+        self._h = h
+        _live.add(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stats(self) -> dict:
+        a = np.zeros(6, dtype=np.int64)
+        self.ctx.check(self.ctx.lib.messi_dataset_stats(self._h, ptr(a, _i64)))
+        return {"sum_U": int(a[0]), "root_entries": int(a[1]), "pairs_in_range": int(a[2]),
+                "sum_site_patterns": int(a[3]), "levels": int(a[4]), "device_bytes": int(a[5])}
+
+    def timings(self) -> np.ndarray:
+        t = np.zeros(5)
+        self.ctx.check(self.ctx.lib.messi_last_timings(self._h, ptr(t)))
+        return t
+
+    def site_loglik(self, P4, pi4, logw, prec=FP64) -> np.ndarray:
+        P = colmajor_P(P4)
+        K = P.shape[0]
+        pi = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K, 4)
+        lw = np.ascontiguousarray(logw, dtype=np.float64).reshape(K)
+        out = np.empty(self.L)
+        self.ctx.check(self.ctx.lib.messi_site_loglik(self._h, K, ptr(P), ptr(pi), ptr(lw), prec_code(prec),
+                                                      ptr(out)))
+        return out
+
+    def pair_loglik(self, P16, pi16, logw, prec=FP64, percomp: bool = False):
+        P = colmajor_P(P16)
+        K = P.shape[0]
+        pi = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K, 16)
+        lw = np.ascontiguousarray(logw, dtype=np.float64).reshape(K)
+        out = np.empty((self.L, self.L), order="F")
+        pc = np.empty((K, self.L, self.L)) if percomp else None
+        self.ctx.check(self.ctx.lib.messi_pair_loglik(self._h, K, ptr(P), ptr(pi), ptr(lw), prec_code(prec),
+                                                      ptr(out), ptr(pc)))
+        if percomp:
+            return out, [np.asfortranarray(pc[k].T) for k in range(K)]
+        return out
+
+    def eval(self, site, pair, B: float = 1.0, prec=FP64) -> float:
+        P4, pi4, lw4 = site
+        P16, pi16, lw16 = pair
+        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
+        K4, K16 = P4c.shape[0], P16c.shape[0]
+        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
+        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
+        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
+        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        z = ctypes.c_double()
+        self.ctx.check(self.ctx.lib.messi_eval(self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c),
+                                               ptr(pi16), ptr(lw16), float(B), prec_code(prec),
+                                               ctypes.byref(z)))
+        return z.value
+
+    def posterior(self, site, pair, prec=FP64, want_pairs: bool = True):
+        P4, pi4, lw4 = site
+        P16, pi16, lw16 = pair
+        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
+        K4, K16 = P4c.shape[0], P16c.shape[0]
+        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
+        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
+        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
+        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        z = ctypes.c_double()
+        unp = np.empty(self.L)
+        pp = np.empty((self.L, self.L), order="F") if want_pairs else None
+        self.ctx.check(self.ctx.lib.messi_posterior(self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c),
+                                                    ptr(pi16), ptr(lw16), prec_code(prec), ctypes.byref(z),
+                                                    ptr(unp), ptr(pp)))
+        return z.value, unp, pp
+
+    def close(self) -> None:
+        if self._h:
+            self.ctx.lib.messi_dataset_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _shutdown:
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def inside(u, paired, B: float = 1.0, zonly: bool = True, prec=FP64, ctx: Optional[Context] = None):
+    ctx = ctx or context()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    L = len(u)
+    pm = np.asfortranarray(paired, dtype=np.float64)
+    z = ctypes.c_double()
+    t = None if zonly else np.empty((3, L, L), order="F")
+    ctx.check(ctx.lib.messi_inside(ctx.handle, L, ptr(u), ptr(pm), float(B), int(bool(zonly)), prec_code(prec),
+                                   ctypes.byref(z), ptr(t)))
+    return z.value if zonly else (z.value, t)
+
+
+def outside(inside_t, u, paired, prec=FP64, ctx: Optional[Context] = None):
+    ctx = ctx or context()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    L = len(u)
+    it = np.asfortranarray(inside_t, dtype=np.float64)
+    pm = np.asfortranarray(paired, dtype=np.float64)
+    out = np.empty((3, L, L), order="F")
+    ctx.check(ctx.lib.messi_outside(ctx.handle, L, ptr(it), ptr(u), ptr(pm), prec_code(prec), ptr(out)))
+    return out
+
+
+def bpp(inside_t, outside_t, u, paired, prec=FP64, ctx: Optional[Context] = None):
+    ctx = ctx or context()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    L = len(u)
+    it = np.asfortranarray(inside_t, dtype=np.float64)
+    ot = np.asfortranarray(outside_t, dtype=np.float64)
+    pm = np.asfortranarray(paired, dtype=np.float64)
+    unp = np.empty(L)
+    pp = np.empty((L, L), order="F")
+    ctx.check(ctx.lib.messi_bpp(ctx.handle, L, ptr(it), ptr(ot), ptr(u), ptr(pm), prec_code(prec), ptr(unp),
+                                ptr(pp)))
+    return unp, pp

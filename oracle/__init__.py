@@ -1,0 +1,136 @@
+"""FP64 CPU oracle (test infrastructure only).
+
+Python bindings for ``messi_oracle.c``, the serial CPU restatement of the
+reference's likelihood algorithms (see the header of that file for the
+reference file:line of each function).  Only the tests, the smoke check and
+the CPU-baseline leg of the benchmark may import this package; the product
+library never does.
+
+``build()`` compiles the C file with gcc into ``oracle/_build/``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SRC = os.path.join(_HERE, "messi_oracle.c")
+_LIB = os.path.join(_HERE, "_build", "libmessi_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_u8 = ctypes.POINTER(ctypes.c_uint8)
+
+
+def build() -> str:
+    os.makedirs(os.path.dirname(_LIB), exist_ok=True)
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fno-fast-math", "-o", _LIB, _SRC, "-lm"]
+    subprocess.run(cmd, check=True)
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        L.oracle_site_loglik.argtypes = [ctypes.c_int, _ip, _ip, ctypes.c_int, ctypes.c_int, _u8, _dp, _dp, _dp]
+        L.oracle_pair_loglik.argtypes = [ctypes.c_int, _ip, _ip, ctypes.c_int, ctypes.c_int, _u8, _dp, _dp,
+                                         ctypes.c_int, _u8, _dp]
+        L.oracle_mix_accumulate.argtypes = [ctypes.c_size_t, _dp, _dp, ctypes.c_double, ctypes.c_int]
+        L.oracle_inside.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, _dp]
+        L.oracle_outside.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.oracle_bpp.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def _colmajor_P(P: np.ndarray) -> np.ndarray:
+    """(nnodes, dim, dim) row-indexed [a, d] -> column-major per matrix."""
+    return np.ascontiguousarray(np.transpose(P, (0, 2, 1)))
+
+
+def _tree(ds):
+    ch = np.ascontiguousarray(ds.tree.children.reshape(-1), dtype=np.int32)
+    sq = np.ascontiguousarray(ds.tree.seqindex, dtype=np.int32)
+    return ch, sq
+
+
+def site_loglik(ds, components) -> np.ndarray:
+    """Site log-likelihoods with the site-rate mixture, src/FARCE.jl:1198-1220.
+    ``components``: list of (P4 (nnodes,4,4), pi4, logw)."""
+    L = ds.numcols
+    ch, sq = _tree(ds)
+    codes = np.ascontiguousarray(ds.codes)
+    acc = np.zeros(L)
+    tmp = np.zeros(L)
+    for i, (P, pi, logw) in enumerate(components):
+        Pc = _colmajor_P(np.asarray(P, dtype=np.float64))
+        pi = np.ascontiguousarray(pi, dtype=np.float64)
+        lib().oracle_site_loglik(ds.numnodes, _p(ch, _ip), _p(sq, _ip), ds.numseqs, L, _p(codes, _u8),
+                                 _p(Pc), _p(pi), _p(tmp))
+        lib().oracle_mix_accumulate(L, _p(acc), _p(tmp), float(logw), int(i == 0))
+    return acc
+
+
+def pair_loglik(ds, components, maxbp: int = 1000000, mask: Optional[np.ndarray] = None) -> np.ndarray:
+    """Pair log-likelihood matrix (L x L, -inf outside the computed band), mixed
+    over components with log weights, src/FARCE.jl:1736-1792 semantics."""
+    L = ds.numcols
+    ch, sq = _tree(ds)
+    codes = np.ascontiguousarray(ds.codes)
+    acc = np.full(L * L, -np.inf)
+    tmp = np.full(L * L, -np.inf)
+    m = None if mask is None else np.asfortranarray(mask, dtype=np.uint8).reshape(-1, order="F")
+    for i, (P, pi, logw) in enumerate(components):
+        Pc = _colmajor_P(np.asarray(P, dtype=np.float64))
+        pi = np.ascontiguousarray(pi, dtype=np.float64)
+        tmp[:] = -np.inf
+        lib().oracle_pair_loglik(ds.numnodes, _p(ch, _ip), _p(sq, _ip), ds.numseqs, L, _p(codes, _u8),
+                                 _p(Pc), _p(pi), int(min(maxbp, 2**31 - 1)),
+                                 None if m is None else _p(m, _u8), _p(tmp))
+        lib().oracle_mix_accumulate(L * L, _p(acc), _p(tmp), float(logw), int(i == 0))
+    out = acc.reshape(L, L, order="F")
+    return out
+
+
+def inside(u: np.ndarray, paired: np.ndarray, B: float = 1.0) -> np.ndarray:
+    """(3, L, L) inside table, index [s, i, j] with S=0, F=1, L=2."""
+    L = len(u)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    pm = np.asfortranarray(paired, dtype=np.float64).reshape(-1, order="F")
+    out = np.empty(3 * L * L)
+    lib().oracle_inside(L, _p(u), _p(pm), float(B), _p(out))
+    return out.reshape((3, L, L), order="F")
+
+
+def outside(inside_t: np.ndarray, u: np.ndarray, paired: np.ndarray) -> np.ndarray:
+    L = len(u)
+    it = np.asfortranarray(inside_t).reshape(-1, order="F")
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    pm = np.asfortranarray(paired, dtype=np.float64).reshape(-1, order="F")
+    out = np.empty(3 * L * L)
+    lib().oracle_outside(L, _p(it), _p(u), _p(pm), _p(out))
+    return out.reshape((3, L, L), order="F")
+
+
+def bpp(inside_t, outside_t, u, paired):
+    L = len(u)
+    it = np.asfortranarray(inside_t).reshape(-1, order="F")
+    ot = np.asfortranarray(outside_t).reshape(-1, order="F")
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    pm = np.asfortranarray(paired, dtype=np.float64).reshape(-1, order="F")
+    res = np.empty(L)
+    pr = np.empty(L * L)
+    lib().oracle_bpp(L, _p(it), _p(ot), _p(u), _p(pm), _p(res), _p(pr))
+    return res, pr.reshape(L, L, order="F")

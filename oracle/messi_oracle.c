@@ -1,0 +1,254 @@
+/*
+ * FP64 CPU oracle for the MESSI likelihood hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates the reference's serial CPU
+ * algorithms so that the HIP path can be checked against them.  Only the test
+ * suite, the smoke check and the CPU-baseline leg of the benchmark load it;
+ * the product library never links or calls it.
+ *
+ * Restated functions (reference file:line):
+ *   oracle_site_loglik   -- felsensteinstack, 4 states (src/FARCE.jl:2182-2254) as driven by
+ *                           computeunpairedlikelihoodscat (src/FARCE.jl:742-773)
+ *   oracle_pair_loglik   -- felsensteinstack, 16 states (src/FARCE.jl:2182-2254) as driven by
+ *                           felsensteinall (src/FARCE.jl:1992-2046); leaf state vector of a
+ *                           column pair is data[x] (x) data[y] with index h1*4+h2
+ *                           (src/FARCE.jl:370-377)
+ *   oracle_inside        -- computeinsideparallel/computeblock (src/InsideOutsideAlgorithm.jl:648-749)
+ *   oracle_outside       -- computeoutside (src/InsideOutsideAlgorithm.jl:447-483)
+ *   oracle_bpp           -- computebasepairprobs (src/InsideOutsideAlgorithm.jl:486-533)
+ *   lse                  -- CommonUtils.logsumexp (src/CommonUtils.jl:145-155)
+ *
+ * Conventions: trees are flat pre-order arrays with the root at index 0,
+ * children[2*v], children[2*v+1] (-1 for leaves), seqindex[v] the alignment
+ * row of leaf v.  Alignment cells are 4-bit state masks codes[s*ncol+c].
+ * Transition matrices are column-major per node (Julia layout):
+ * P[node*dim*dim + a + dim*d] = P(a -> d).  L x L matrices are column-major,
+ * 3 x L x L tables use Julia's (3,L,L) layout: (s,i,j) at s + 3*(i + L*j).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* KH99 rule probabilities, src/InsideOutsideAlgorithm.jl:40-46. */
+#define P_S_LS 0.868534
+#define P_S_s 0.117609877998
+#define P_S_dFd 0.013856122002
+#define P_F_dFd 0.787640
+#define P_F_LS 0.21236
+#define P_L_s 0.894603
+#define P_L_dFd 0.105397
+
+static double lse(double a, double b) {
+  if (b == -INFINITY) return a;
+  if (a == -INFINITY) return b;
+  if (a < b) return b + log1p(exp(a - b));
+  return a + log1p(exp(b - a));
+}
+
+/* Post-order pruning of one column (dim 4) or column pair (dim 16).
+ * part: nnodes*dim scratch, logm: nnodes scratch.  Rescaling follows
+ * src/FARCE.jl:2225-2238: divide by the maximum only when it is below 1e-20. */
+static double prune_one(int nnodes, const int *children, const int *seqindex, int ncol,
+                        const uint8_t *codes, int x, int y, int dim, const double *P,
+                        const double *pi, double *part, double *logm) {
+  for (int v = nnodes - 1; v >= 0; --v) {   /* reverse pre-order: children first */
+    double *pv = part + (size_t)v * dim;
+    int l = children[2 * v], r = children[2 * v + 1];
+    if (l < 0) {
+      int s = seqindex[v];
+      if (dim == 4) {
+        uint8_t c = codes[(size_t)s * ncol + x];
+        for (int a = 0; a < 4; ++a) pv[a] = (double)((c >> a) & 1);
+      } else {
+        uint8_t cx = codes[(size_t)s * ncol + x], cy = codes[(size_t)s * ncol + y];
+        for (int h1 = 0; h1 < 4; ++h1)
+          for (int h2 = 0; h2 < 4; ++h2)
+            pv[h1 * 4 + h2] = (double)((cx >> h1) & 1) * (double)((cy >> h2) & 1);
+      }
+      logm[v] = 0.0;
+      continue;
+    }
+    const double *Pl = P + (size_t)l * dim * dim, *Pr = P + (size_t)r * dim * dim;
+    const double *vl = part + (size_t)l * dim, *vr = part + (size_t)r * dim;
+    double m = 0.0;
+    for (int a = 0; a < dim; ++a) {
+      double sl = 0.0, sr = 0.0;
+      for (int d = 0; d < dim; ++d) {
+        sl += Pl[a + dim * d] * vl[d];
+        sr += Pr[a + dim * d] * vr[d];
+      }
+      pv[a] = sl * sr;
+      if (pv[a] > m) m = pv[a];
+    }
+    if (m < 1e-20) {
+      for (int a = 0; a < dim; ++a) pv[a] /= m;
+      logm[v] = log(m) + logm[l] + logm[r];
+    } else {
+      logm[v] = logm[l] + logm[r];
+    }
+  }
+  double tot = 0.0;
+  for (int a = 0; a < dim; ++a) tot += pi[a] * part[a];
+  return log(tot) + logm[0];
+}
+
+void oracle_site_loglik(int nnodes, const int *children, const int *seqindex, int nseq, int ncol,
+                        const uint8_t *codes, const double *P4, const double *pi4, double *out) {
+  (void)nseq;
+  double *part = (double *)malloc(sizeof(double) * 4 * nnodes);
+  double *logm = (double *)malloc(sizeof(double) * nnodes);
+  for (int c = 0; c < ncol; ++c)
+    out[c] = prune_one(nnodes, children, seqindex, ncol, codes, c, c, 4, P4, pi4, part, logm);
+  free(part);
+  free(logm);
+}
+
+/* One mixture component of the pair log-likelihood matrix.  out is L x L
+ * column-major; entries for x<y<=x+maxbp (and mask[x+L*y] != 0 when a mask is
+ * given) are written symmetrically, everything else is left untouched. */
+void oracle_pair_loglik(int nnodes, const int *children, const int *seqindex, int nseq, int ncol,
+                        const uint8_t *codes, const double *P16, const double *pi16, int maxbp,
+                        const uint8_t *mask, double *out) {
+  (void)nseq;
+  double *part = (double *)malloc(sizeof(double) * 16 * nnodes);
+  double *logm = (double *)malloc(sizeof(double) * nnodes);
+  for (int x = 0; x < ncol; ++x) {
+    int yend = x + maxbp < ncol - 1 ? x + maxbp : ncol - 1;
+    for (int y = x + 1; y <= yend; ++y) {
+      if (mask && !mask[x + (size_t)ncol * y]) continue;
+      double v = prune_one(nnodes, children, seqindex, ncol, codes, x, y, 16, P16, pi16, part, logm);
+      out[x + (size_t)ncol * y] = v;
+      out[y + (size_t)ncol * x] = v;
+    }
+  }
+  free(part);
+  free(logm);
+}
+
+/* Accumulate a component into a running mixture: acc = lse(acc, logw + comp),
+ * elementwise over n values (src/FARCE.jl:1780-1792, src/FARCE.jl:1207-1218). */
+void oracle_mix_accumulate(size_t n, double *acc, const double *comp, double logw, int first) {
+  for (size_t i = 0; i < n; ++i) {
+    if (first) acc[i] = logw + comp[i];
+    else acc[i] = lse(acc[i], logw + comp[i]);
+  }
+}
+
+#define IDX3(s, i, j) ((size_t)(s) + 3 * ((size_t)(i) + (size_t)L * (size_t)(j)))
+
+/* KH99 inside, src/InsideOutsideAlgorithm.jl:648-749.  S=0, F=1, L=2. */
+void oracle_inside(int L, const double *u, const double *paired, double B, double *in) {
+  const double lp[7] = {log(P_S_LS), log(P_S_s), log(P_S_dFd), log(P_F_dFd),
+                        log(P_F_LS), log(P_L_s), log(P_L_dFd)};
+  size_t n = (size_t)3 * L * L;
+  for (size_t k = 0; k < n; ++k) in[k] = -INFINITY;
+  for (int i = 0; i < L; ++i) {
+    in[IDX3(0, i, i)] = (u[i] + lp[1]) * B;
+    in[IDX3(2, i, i)] = (u[i] + lp[5]) * B;
+  }
+  /* type-3 rules: S->LS (lp[0]) and F->LS (lp[4]); type-2 rules: S->dFd (lp[2]),
+   * F->dFd (lp[3]), L->dFd (lp[6]), in the reference's order. */
+  const int t3left[2] = {0, 1};
+  const double t3lp[2] = {lp[0], lp[4]};
+  const int t2left[3] = {0, 1, 2};
+  const double t2lp[3] = {lp[2], lp[3], lp[6]};
+  for (int d = 1; d < L; ++d) {
+    for (int j = 0; j + d < L; ++j) {
+      int y = j + d;
+      for (int r = 0; r < 2; ++r) {
+        double tmp = -INFINITY;
+        for (int h = j; h <= y - 1; ++h) tmp = lse(tmp, in[IDX3(2, j, h)] + in[IDX3(0, h + 1, y)]);
+        size_t c = IDX3(t3left[r], j, y);
+        in[c] = lse(in[c], t3lp[r] * B + tmp);
+      }
+      for (int r = 0; r < 3; ++r) {
+        size_t c = IDX3(t2left[r], j, y);
+        double f = (j + 1 < L && y - 1 >= 0) ? in[IDX3(1, j + 1, y - 1)] : -INFINITY;
+        in[c] = lse(in[c], (t2lp[r] + paired[j + (size_t)L * y]) * B + f);
+      }
+    }
+  }
+}
+
+/* KH99 outside, src/InsideOutsideAlgorithm.jl:447-483. */
+void oracle_outside(int L, const double *in, const double *u, const double *paired, double *out) {
+  (void)u;
+  const double lp[7] = {log(P_S_LS), log(P_S_s), log(P_S_dFd), log(P_F_dFd),
+                        log(P_F_LS), log(P_L_s), log(P_L_dFd)};
+  size_t n = (size_t)3 * L * L;
+  for (size_t k = 0; k < n; ++k) out[k] = -INFINITY;
+  out[IDX3(0, 0, L - 1)] = 0.0;
+  /* type-3 rules (left, right1, right2, logp): S->LS, F->LS. */
+  const int t3left[2] = {0, 1};
+  const double t3lp[2] = {lp[0], lp[4]};
+  const int t2left[3] = {0, 1, 2};
+  const double t2lp[3] = {lp[2], lp[3], lp[6]};
+  for (int c = 2; c <= L; ++c) {
+    int b = L - c;
+    for (int j = 0; j <= L - b - 1; ++j) {
+      for (int r = 0; r < 2; ++r) {
+        int A = t3left[r];
+        double tmp = -INFINITY;
+        for (int k = j + b + 1; k <= L - 1; ++k)
+          tmp = lse(tmp, out[IDX3(A, j, k)] + in[IDX3(0, j + b + 1, k)]);
+        size_t cl = IDX3(2, j, j + b);
+        out[cl] = lse(out[cl], tmp + t3lp[r]);
+        tmp = -INFINITY;
+        for (int k = 0; k <= j - 1; ++k)
+          tmp = lse(tmp, out[IDX3(A, k, j + b)] + in[IDX3(2, k, j - 1)]);
+        size_t cs = IDX3(0, j, j + b);
+        out[cs] = lse(out[cs], tmp + t3lp[r]);
+      }
+      if (j >= 1 && j + b + 1 < L) {
+        for (int r = 0; r < 3; ++r) {
+          size_t cf = IDX3(1, j, j + b);
+          out[cf] = lse(out[cf], t2lp[r] + paired[(j - 1) + (size_t)L * (j + b + 1)] +
+                                     out[IDX3(t2left[r], j - 1, j + b + 1)]);
+        }
+      }
+    }
+  }
+}
+
+/* Base-pair posteriors, src/InsideOutsideAlgorithm.jl:486-533.  res (L) and
+ * pair (L x L column-major) receive the row-normalised probabilities. */
+void oracle_bpp(int L, const double *in, const double *out, const double *u, const double *paired,
+                double *res, double *pair) {
+  const double lp[7] = {log(P_S_LS), log(P_S_s), log(P_S_dFd), log(P_F_dFd),
+                        log(P_F_LS), log(P_L_s), log(P_L_dFd)};
+  double Z = in[IDX3(0, 0, L - 1)];
+  for (int j = 0; j < L; ++j) {
+    double tmp = -INFINITY;
+    tmp = lse(tmp, out[IDX3(0, j, j)] + lp[1]);
+    tmp = lse(tmp, out[IDX3(2, j, j)] + lp[5]);
+    res[j] = tmp + u[j] - Z;
+  }
+  for (size_t k = 0; k < (size_t)L * L; ++k) pair[k] = -INFINITY;
+  const int t2left[3] = {0, 1, 2};
+  const double t2lp[3] = {lp[2], lp[3], lp[6]};
+  for (int j = 0; j < L; ++j) {
+    for (int k = j + 1; k < L; ++k) {
+      double tmp = -INFINITY;
+      for (int r = 0; r < 3; ++r) {
+        double inres = (j + 1 <= k - 1) ? in[IDX3(1, j + 1, k - 1)] : -INFINITY;
+        double outres = out[IDX3(t2left[r], j, k)];
+        tmp = lse(tmp, outres + t2lp[r] + inres);
+      }
+      double v = tmp + paired[j + (size_t)L * k] - Z;
+      pair[j + (size_t)L * k] = v;
+      pair[k + (size_t)L * j] = v;
+    }
+  }
+  double m = -INFINITY;
+  for (int j = 0; j < L; ++j) if (res[j] > m) m = res[j];
+  for (size_t k = 0; k < (size_t)L * L; ++k) if (pair[k] > m) m = pair[k];
+  for (int j = 0; j < L; ++j) res[j] = exp(res[j] - m);
+  for (size_t k = 0; k < (size_t)L * L; ++k) pair[k] = exp(pair[k] - m);
+  for (int x = 0; x < L; ++x) {
+    double norm = res[x];
+    for (int y = 0; y < L; ++y) norm += pair[x + (size_t)L * y];
+    res[x] /= norm;
+    for (int y = 0; y < L; ++y) pair[x + (size_t)L * y] /= norm;
+  }
+}

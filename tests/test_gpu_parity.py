@@ -1,0 +1,201 @@
+"""HIP path versus the FP64 oracle, through the C ABI.
+
+FP64 bar (from the task): log-likelihoods within 1e-9 relative, pairing
+probabilities within 1e-8 absolute.  The FP32 path reports its error against
+the oracle and is held to the bounds written in each test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from messi_amd import hotpath, lib, model, synth
+from messi_amd.dataset import Dataset
+
+pytestmark = pytest.mark.gpu
+
+REL64 = 1e-9
+BPP64 = 1e-8
+
+
+def _say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    _say("context create + MFMA layout self-test")
+    c = lib.context()
+    c.selftest()
+    return c
+
+
+@pytest.fixture(scope="module")
+def c1():
+    a = synth.config1()
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths)
+    u = oracle.site_loglik(ds, model.site_components(p))
+    M = oracle.pair_loglik(ds, model.pair_components(p))
+    ins = oracle.inside(u, M)
+    out = oracle.outside(ins, u, M)
+    res, pair = oracle.bpp(ins, out, u, M)
+    return dict(ds=ds, p=p, u=u, M=M, ins=ins, out=out, res=res, pair=pair)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    fin = np.isfinite(b)
+    assert np.array_equal(np.isfinite(a), fin)
+    assert np.array_equal(a[~fin], b[~fin])
+    if not fin.any():
+        return 0.0
+    return float(np.max(np.abs(a[fin] - b[fin]) / np.maximum(1.0, np.abs(b[fin]))))
+
+
+def test_selftest(ctx):
+    ctx.selftest()
+
+
+def test_site_loglik_fp64(ctx, c1):
+    _say("site loglik fp64 config1")
+    u = hotpath.computeunpairedlikelihoods(c1["ds"], c1["p"])
+    assert _rel(u, c1["u"]) < 1e-12
+
+
+def test_pair_loglik_fp64(ctx, c1):
+    _say("pair loglik fp64 config1")
+    M, _ = hotpath.felsensteincuda(c1["ds"], c1["p"])
+    assert _rel(M, c1["M"]) < 1e-12
+
+
+def test_inside_fp64(ctx, c1):
+    _say("inside fp64 config1")
+    z = hotpath.computeinsideKH99(c1["u"], c1["M"])
+    zr = c1["ins"][0, 0, -1]
+    assert abs(z - zr) <= REL64 * abs(zr)
+    t = hotpath.computeinsideKH99(c1["u"], c1["M"], 1.0, False)
+    assert _rel(t, c1["ins"]) < 1e-11
+
+
+def test_outside_bpp_fp64(ctx, c1):
+    _say("outside + bpp fp64 config1")
+    out = hotpath.computeoutsideKH99(c1["ins"], c1["u"], c1["M"])
+    assert _rel(out, c1["out"]) < 1e-10
+    res, pair = hotpath.computebasepairprobs(c1["ins"], c1["out"], c1["u"], c1["M"])
+    assert np.max(np.abs(res - c1["res"])) < BPP64
+    assert np.max(np.abs(pair - c1["pair"])) < BPP64
+
+
+def test_eval_and_posterior_fp64(ctx, c1):
+    _say("eval + posterior fp64 config1")
+    z = hotpath.loglikelihood(c1["ds"], c1["p"])
+    zr = c1["ins"][0, 0, -1]
+    assert abs(z - zr) <= REL64 * abs(zr)
+    z2, res, pair = hotpath.posterior(c1["ds"], c1["p"])
+    assert abs(z2 - zr) <= REL64 * abs(zr)
+    assert np.max(np.abs(res - c1["res"])) < BPP64
+    assert np.max(np.abs(pair - c1["pair"])) < BPP64
+
+
+def test_fp32_path_error_report(ctx, c1):
+    _say("fp32 path config1")
+    u = hotpath.computeunpairedlikelihoods(c1["ds"], c1["p"], prec="fp32")
+    M, _ = hotpath.felsensteincuda(c1["ds"], c1["p"], prec="fp32")
+    zr = c1["ins"][0, 0, -1]
+    z, res, pair = hotpath.posterior(c1["ds"], c1["p"], prec="fp32")
+    eu, eM = _rel(u, c1["u"]), _rel(M, c1["M"])
+    ez = abs(z - zr) / abs(zr)
+    eb = max(np.max(np.abs(res - c1["res"])), np.max(np.abs(pair - c1["pair"])))
+    _say(f"fp32 errors: site {eu:.2e} pair {eM:.2e} logZ(rel) {ez:.2e} bpp(abs) {eb:.2e}")
+    assert eu < 1e-6 and eM < 1e-6
+    assert ez < 1e-6
+    assert eb < 1e-5
+
+
+def test_mixture_components_match_oracle(ctx):
+    _say("K>1 mixture fp64")
+    a = synth.generate(8, 40, 11)
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths, siteCats=3)
+    sc, pc = model.site_components(p), model.pair_components(p)
+    assert len(sc) == 3 and len(pc) == 9
+    u = hotpath.computeunpairedlikelihoods(ds, p)
+    assert _rel(u, oracle.site_loglik(ds, sc)) < 1e-12
+    M, per = hotpath.felsensteincuda(ds, p, keepmuseconditionals=True)
+    assert _rel(M, oracle.pair_loglik(ds, pc)) < 1e-12
+    for k in (0, 4, 8):
+        assert _rel(per[k], oracle.pair_loglik(ds, [(pc[k][0], pc[k][1], 0.0)])) < 1e-12
+
+
+@pytest.mark.parametrize("maxbp", [1, 7, 1000000])
+def test_distance_bound_and_mask(ctx, maxbp):
+    _say(f"distance bound {maxbp}")
+    a = synth.generate(6, 30, 12)
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths)
+    M, _ = hotpath.felsensteincuda(ds, p, maxbasepairdistance=maxbp)
+    assert _rel(M, oracle.pair_loglik(ds, model.pair_components(p), maxbp=maxbp)) < 1e-12
+    rng = np.random.default_rng(1)
+    mask = (rng.random((30, 30)) < 0.5).astype(np.uint8)
+    M2, _ = hotpath.felsensteincuda(ds, p, maxbasepairdistance=maxbp, mask=mask)
+    assert _rel(M2, oracle.pair_loglik(ds, model.pair_components(p), maxbp=maxbp, mask=mask)) < 1e-12
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 9])
+def test_small_and_degenerate_lengths(ctx, L):
+    _say(f"inside/outside L={L}")
+    rng = np.random.default_rng(L)
+    u = -rng.uniform(2, 6, size=L)
+    pm = np.full((L, L), -np.inf)
+    for i in range(L):
+        for j in range(i + 1, L):
+            pm[i, j] = pm[j, i] = u[i] + u[j] + rng.normal(0, 3)
+    ins = oracle.inside(u, pm)
+    z = hotpath.computeinsideKH99(u, pm)
+    assert abs(z - ins[0, 0, -1]) <= 1e-12 * abs(ins[0, 0, -1])
+    if L >= 2:
+        out = oracle.outside(ins, u, pm)
+        assert _rel(hotpath.computeoutsideKH99(ins, u, pm), out) < 1e-11
+        res, pair = oracle.bpp(ins, out, u, pm)
+        r2, p2 = hotpath.computebasepairprobs(ins, out, u, pm)
+        assert np.max(np.abs(r2 - res)) < BPP64 and np.max(np.abs(p2 - pair)) < BPP64
+
+
+def test_all_unpaired_closed_form(ctx):
+    _say("all-unpaired closed form L=700")
+    L = 700
+    rng = np.random.default_rng(3)
+    u = -rng.uniform(1, 40, size=L)
+    pm = np.full((L, L), -np.inf)
+    closed = np.log(0.117609877998) + (L - 1) * (np.log(0.868534) + np.log(0.894603)) + u.sum()
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-6)):
+        z = hotpath.computeinsideKH99(u, pm, prec=prec)
+        assert abs(z - closed) <= tol * abs(closed), prec
+
+
+def test_config2_scale_properties(ctx):
+    """Full 1,500 nt x 50 sequence workload: dedup consistency against a banded
+    run, symmetry, posterior row sums and FP32 versus FP64 agreement."""
+    _say("config2 properties")
+    a = synth.config2()
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths)
+    u = hotpath.computeunpairedlikelihoods(ds, p)
+    assert _rel(u, oracle.site_loglik(ds, model.site_components(p))) < 1e-12
+    M, _ = hotpath.felsensteincuda(ds, p)
+    assert np.array_equal(M, M.T)
+    Mb, _ = hotpath.felsensteincuda(ds, p, maxbasepairdistance=12)
+    ref = oracle.pair_loglik(ds, model.pair_components(p), maxbp=12)
+    assert _rel(Mb, ref) < 1e-12
+    band = np.abs(np.subtract.outer(np.arange(1500), np.arange(1500))) <= 12
+    assert _rel(np.where(band, M, -np.inf), ref) < 1e-12
+    z64, r64, p64 = hotpath.posterior(ds, p)
+    assert np.max(np.abs(r64 + p64.sum(axis=1) - 1.0)) < 1e-10
+    z32, r32, p32 = hotpath.posterior(ds, p, prec="fp32")
+    err = max(np.max(np.abs(r32 - r64)), np.max(np.abs(p32 - p64)))
+    _say(f"config2 logZ64 {z64:.6f} logZ32 {z32:.6f} rel {abs(z32 - z64) / abs(z64):.2e} bpp err {err:.2e}")
+    assert abs(z32 - z64) <= 1e-6 * abs(z64)
+    assert err < 1e-4
