@@ -70,6 +70,7 @@ inline void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
 struct InsideWork {
   int L = 0;
   DevBuf Lr, Lc, Sr, Sc, F;       // inside tables (row- and column-major copies)
+  DevBuf acc;                     // bulk split-point accumulator of the tiled inside
   DevBuf oSr, oSc, oFr, oFc, oL;  // outside tables
   DevBuf ptv;                     // scaled pair term per cell
   DevBuf lam, C;                  // per-position baseline and its prefix sums (double)

@@ -5,87 +5,58 @@
 //   outside -- computeoutside (src/InsideOutsideAlgorithm.jl:447-483)
 //   bpp     -- computebasepairprobs (src/InsideOutsideAlgorithm.jl:486-533)
 //
-// Representation.  Tables hold log values in "units" relative to a per-position
-// baseline: v_hat[i,j] = v[i,j] * toU - (C[j+1] - C[i]), C the prefix sum of
-// lambda_k.  Because every derivation of a span emits each of its positions
-// exactly once, the baseline is additive over bifurcations and cancels out of
-// every recursion; only the pair term becomes ptv[j,y] = paired[j,y]*B*toU -
-// lambda_j - lambda_y.  The FP64 path uses natural logs with lambda = 0 (the
-// reference's own numbers); the FP32 path uses base-2 logs with lambda_k =
-// (u_k + log p(S->LS) + log p(L->s)) * B / ln 2, which removes the per-column
-// trend of about -20 nats so FP32 keeps its precision, and exp2/log2 map to
-// single gfx950 instructions.
+// Two value representations (sr.h): the plain FP64 path keeps natural-log
+// values with exact log-sum-exp (LogSR); the scaled FP32 path keeps m * 2^e per
+// cell (XF) relative to a per-position baseline lambda_k = (u_k + log p(S->LS) +
+// log p(L->s)) * B, so a bifurcation term is one float multiply plus exponent
+// arithmetic and the bulk split-point sums run on MFMA.  Every derivation of a
+// span emits each of its positions once, so the baseline is additive over
+// bifurcations and cancels out of every recursion; only the pair term becomes
+// ptv[j,y] = paired[j,y]*B - lambda_j - lambda_y.
 //
-// Log-sum-exp over a bifurcation is computed as max + log(sum exp(t - max)),
-// two passes over the split point, with the split range shared by a group of
-// G lanes of one wavefront.
+// Inside schedule: the upper triangle is cut into T x T tiles.  Launch t
+// finalises every tile (I, I+t) with an in-LDS wavefront over its cells, from
+// (a) split points h inside tile I, (c) split points inside tile J and (b) the
+// bulk of split points in the tiles strictly between, accumulated earlier: the
+// tile product (I,K) x (K,J) runs in launch max(K-I, J-K) + 1, as soon as both
+// operand tiles are final, either inside the workgroup that finalises (I,J) or
+// in a workgroup of its own.  A launch therefore carries the cell wavefronts
+// of one tile-diagonal plus the split-point products that just became ready.
 #include "common.h"
+#include "prune.h"
+#include "sr.h"
 
 namespace messi {
 
-template <typename T> struct Lg;
-template <> struct Lg<double> {
-  static __device__ __forceinline__ double ex(double x) { return exp(x); }
-  static __device__ __forceinline__ double lg(double x) { return log(x); }
-  static __device__ __forceinline__ double ninf() { return -HUGE_VAL; }
-  static __device__ __forceinline__ double lse(double a, double b) {
-    if (a == -HUGE_VAL) return b;
-    if (b == -HUGE_VAL) return a;
-    return a < b ? b + log1p(exp(a - b)) : a + log1p(exp(b - a));
-  }
-};
-template <> struct Lg<float> {
-  static __device__ __forceinline__ float ex(float x) { return exp2f(x); }
-  static __device__ __forceinline__ float lg(float x) { return log2f(x); }
-  static __device__ __forceinline__ float ninf() { return -HUGE_VALF; }
-  static __device__ __forceinline__ float lse(float a, float b) {
-    if (a == -HUGE_VALF) return b;
-    if (b == -HUGE_VALF) return a;
-    float m = fmaxf(a, b), d = fabsf(a - b);
-    return m + log2f(1.0f + exp2f(-d));
-  }
+constexpr int TILE = 32;
+constexpr int NTHR = 256;
+
+template <class SR> struct Rules {
+  typename SR::C p[7];
 };
 
-template <typename T> struct Rules {
-  T p[7];
-};
-
-template <typename T>
-__device__ __forceinline__ T group_max(T v, int G) {
-  for (int o = G >> 1; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
-template <typename T>
-__device__ __forceinline__ T group_sum(T v, int G) {
-  for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-static inline int pow2ceil(int x) {
-  int g = 1;
-  while (g < x) g <<= 1;
-  return g;
-}
-static inline int lanes_for(int terms) {
-  int g = pow2ceil(terms / 8 > 1 ? terms / 8 : 1);
-  return g > 64 ? 64 : g;
+template <class SR> static Rules<SR> make_rules(double B) {
+  double lp[7];
+  kh99_logprobs(lp);
+  Rules<SR> r;
+  for (int i = 0; i < 7; ++i) r.p[i] = SR::rule(lp[i] * B);
+  return r;
 }
 
 // ---------------------------------------------------------------------------
-// Preparation: baseline lambda (units), its prefix sums C, u_hat and ptv.
+// Preparation
 // ---------------------------------------------------------------------------
-__global__ void baseline_kernel(int L, const double *u, double B, double toU, double lamconst,
-                                int scaled, double *lam, double *C, double *uhat) {
-  // One workgroup of 1024 threads: chunked serial sums + shared scan.
+__global__ void baseline_kernel(int L, const double *u, double B, double lamconst, int scaled,
+                                double *lam, double *C, double *uhat) {
   __shared__ double part[1024];
   int t = threadIdx.x, nt = blockDim.x;
   int chunk = (L + nt - 1) / nt;
   int b = t * chunk, e = min(L, b + chunk);
   double s = 0.0;
   for (int k = b; k < e; ++k) {
-    double lk = scaled ? (u[k] + lamconst) * B * toU : 0.0;
+    double lk = scaled ? (u[k] + lamconst) * B : 0.0;
     lam[k] = lk;
-    uhat[k] = u[k] * toU - lk;
+    uhat[k] = u[k] - lk;
     s += lk;
   }
   part[t] = s;
@@ -107,144 +78,418 @@ __global__ void baseline_kernel(int L, const double *u, double B, double toU, do
   }
 }
 
-template <typename T>
-__global__ void ptv_kernel(int L, const double *paired, const double *lam, double BtoU, T *ptv) {
+// ptv (row-major) from the Julia column-major paired matrix (upper triangle).
+template <class SR>
+__global__ void ptv_kernel(int L, const double *paired, const double *lam, double B,
+                           typename SR::V *ptv) {
   int y = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
   if (y >= L) return;
-  // Julia column-major element (j, y); only the upper triangle (j < y) is used.
   double v = paired[(size_t)y * L + j];
-  T out = (j == y || v == -HUGE_VAL) ? Lg<T>::ninf() : (T)(v * BtoU - lam[j] - lam[y]);
-  ptv[(size_t)j * L + y] = out;
+  double x = (j >= y || v == -HUGE_VAL) ? -HUGE_VAL : v * B - lam[j] - lam[y];
+  ptv[(size_t)j * L + y] = SR::from_log(x);
 }
 
-template <typename T>
-__global__ void inside_init_kernel(int L, const double *u, const double *lam, double BtoU,
-                                   Rules<T> r0, T *Lr, T *Lc, T *Sr, T *Sc, T *F) {
-  // r0.p[1] = log p(S->s) * B * toU, r0.p[5] = log p(L->s) * B * toU.
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= L) return;
-  size_t c = (size_t)i * L + i;
-  double ub = u[i] * BtoU - lam[i];
-  T s = (T)(ub + (double)r0.p[1]);
-  T l = (T)(ub + (double)r0.p[5]);
-  Sr[c] = s;
-  Sc[c] = s;
-  Lr[c] = l;
-  Lc[c] = l;
-  F[c] = Lg<T>::ninf();
-}
-
-// One anti-diagonal d of the inside recursion; G lanes per cell.
-template <typename T>
-__global__ __launch_bounds__(256) void inside_diag_kernel(int L, int d, int G, T *Lr, T *Lc, T *Sr,
-                                                          T *Sc, T *F, const T *ptv, Rules<T> r) {
-  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int cell = (int)(tid / G), sub = (int)(tid % G);
-  bool valid = cell < L - d;
-  int j = valid ? cell : 0;
-  int y = j + d;
-  const T NINF = Lg<T>::ninf();
-  const T *Lrow = Lr + (size_t)j * L;          // L[j, h]
-  const T *Scol = Sc + (size_t)y * L + 1;      // S[h+1, y]
-  T m = NINF;
-  if (valid)
-    for (int h = j + sub; h < y; h += G) m = fmax(m, Lrow[h] + Scol[h]);
-  m = group_max(m, G);
-  T s = 0;
-  if (valid && m != NINF)
-    for (int h = j + sub; h < y; h += G) s += Lg<T>::ex(Lrow[h] + Scol[h] - m);
-  s = group_sum(s, G);
-  if (valid && sub == 0) {
-    T bif = s > (T)0 ? m + Lg<T>::lg(s) : NINF;
-    T pt = ptv[(size_t)j * L + y];
-    T f = d >= 2 ? F[(size_t)(j + 1) * L + (y - 1)] : NINF;
-    T pb = pt + f;
-    T b0 = bif == NINF ? NINF : r.p[0] + bif;
-    T b4 = bif == NINF ? NINF : r.p[4] + bif;
-    T q2 = pb == NINF ? NINF : r.p[2] + pb;
-    T q3 = pb == NINF ? NINF : r.p[3] + pb;
-    T q6 = pb == NINF ? NINF : r.p[6] + pb;
-    T vS = Lg<T>::lse(b0, q2);
-    T vF = Lg<T>::lse(b4, q3);
-    size_t rc = (size_t)j * L + y, cc = (size_t)y * L + j;
-    Sr[rc] = vS;
-    Sc[cc] = vS;
-    F[rc] = vF;
-    Lr[rc] = q6;
-    Lc[cc] = q6;
+// Zero the bulk accumulator; set the diagonal of S and L (F has no s rule).
+template <class SR>
+__global__ void inside_init_kernel(int L, const double *u, const double *lam, double B, double lp1,
+                                   double lp5, typename SR::V *Lr, typename SR::V *Lc,
+                                   typename SR::V *Sr, typename SR::V *Sc, typename SR::V *F,
+                                   typename SR::V *acc) {
+  int y = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (y >= L) return;
+  size_t c = (size_t)j * L + y;
+  acc[c] = SR::zero();
+  if (y == j) {
+    double ub = u[j] * B - lam[j];
+    typename SR::V s = SR::from_log(ub + lp1 * B), l = SR::from_log(ub + lp5 * B);
+    Sr[c] = s;
+    Sc[c] = s;
+    Lr[c] = l;
+    Lc[c] = l;
+    F[c] = SR::zero();
   }
 }
 
-// Outside, one anti-diagonal b (descending); G lanes per cell.
-template <typename T>
-__global__ __launch_bounds__(256) void outside_diag_kernel(int L, int b, int G, const T *iSr,
-                                                           const T *iLc, T *oSr, T *oSc, T *oFr,
-                                                           T *oFc, T *oL, const T *ptv, Rules<T> r) {
+// ---------------------------------------------------------------------------
+// Group reductions of the two-pass split-point sums
+// ---------------------------------------------------------------------------
+template <class SR>
+__device__ __forceinline__ typename SR::K grp_kmax(typename SR::K k, int G) {
+  for (int o = G >> 1; o > 0; o >>= 1) k = SR::kmax(k, SR::shfl_k(k, o));
+  return k;
+}
+template <class SR>
+__device__ __forceinline__ typename SR::S grp_sum(typename SR::S s, int G) {
+  for (int o = G >> 1; o > 0; o >>= 1) s += SR::shfl_s(s, o);
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Shared memory of one step workgroup.
+// ---------------------------------------------------------------------------
+template <class SR, int T> struct StepLDS {
+  typedef typename SR::V V;
+  V Ld[T][T + 1];   // L of the diagonal tile (I, I)
+  V Sd[T][T + 1];   // S of the diagonal tile (J, J)
+  V cA[T][T + 1];   // bulk accumulator of the target tile
+  V cP[T][T + 1];   // pair term of the target tile
+  union {
+    struct {
+      V cS[T][T + 1], cF[T][T + 1], cL[T][T + 1];
+    } w;              // the tile being finalised
+    struct {
+      V A[T][T + 1];  // L[IT+a, KT+c]
+      V B[T][T + 1];  // S[KT+c+1, JT+b]
+      float Ap[T][T + 1];
+      float Bp[T][T + 1];
+    } p;              // a split-point product
+  } u;
+  int rA[T], cB[T];
+  V Sb[T], Fbot[T], Fleft[T];
+};
+
+// acc (+)= sum over split points h in tile K of L[IT+a, h] * S[h+1, JT+b]
+template <class SR, int T>
+__device__ void tile_product(int L, int I, int K, int J, const typename SR::V *__restrict__ Lr,
+                             const typename SR::V *__restrict__ Sc, StepLDS<SR, T> &sm) {
+  typedef typename SR::V V;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < T * T; idx += NTHR) {
+    int a = idx / T, c = idx % T;
+    int j = I * T + a, h = K * T + c;
+    sm.u.p.A[a][c] = (j < L && h < L) ? Lr[(size_t)j * L + h] : SR::zero();
+  }
+  for (int idx = tid; idx < T * T; idx += NTHR) {
+    int b = idx / T, c = idx % T;
+    int y = J * T + b, r = K * T + c + 1;
+    sm.u.p.B[c][b] = (y < L && r < L) ? Sc[(size_t)y * L + r] : SR::zero();
+  }
+  __syncthreads();
+  if constexpr (SR::scaled) {
+    // Exponent-aligned float tiles, then one MFMA 16x16x4 f32 chain per 16x16 block.
+    if (tid < T) {
+      int m = XF::EZ;
+      for (int c = 0; c < T; ++c) {
+        V v = sm.u.p.A[tid][c];
+        if (v.m != 0.0f) m = max(m, v.e);
+      }
+      sm.rA[tid] = m;
+    } else if (tid >= 64 && tid < 64 + T) {
+      int b = tid - 64, m = XF::EZ;
+      for (int c = 0; c < T; ++c) {
+        V v = sm.u.p.B[c][b];
+        if (v.m != 0.0f) m = max(m, v.e);
+      }
+      sm.cB[b] = m;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < T * T; idx += NTHR) {
+      int a = idx / T, c = idx % T;
+      V v = sm.u.p.A[a][c];
+      sm.u.p.Ap[a][c] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - sm.rA[a], -200));
+      V w = sm.u.p.B[a][c];  // row a of B = split point, column c = target column
+      sm.u.p.Bp[a][c] = w.m == 0.0f ? 0.0f : ldexpf(w.m, max(w.e - sm.cB[c], -200));
+    }
+    __syncthreads();
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const int wave = tid >> 6, lane = tid & 63;
+    constexpr int NB = T / 16;
+    for (int blk = wave; blk < NB * NB; blk += NTHR / 64) {
+      const int bi = blk / NB, bj = blk % NB;
+      v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < T / 4; ++s) {
+        float av = sm.u.p.Ap[bi * 16 + (lane & 15)][4 * s + (lane >> 4)];
+        float bv = sm.u.p.Bp[4 * s + (lane >> 4)][bj * 16 + (lane & 15)];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int a = bi * 16 + 4 * (lane >> 4) + r, b = bj * 16 + (lane & 15);
+        const int ra = sm.rA[a], cb = sm.cB[b];
+        V contrib = SR::zero();
+        if (ra != XF::EZ && cb != XF::EZ) {
+          float cv = acc[r];
+          if (cv >= 0x1p-100f) {
+            contrib = XF::norm(cv, ra + cb);
+          } else {
+            // Alignment lost the significant terms: exact exponent-aligned sum.
+            int km = XF::kmin();
+            for (int c = 0; c < T; ++c) km = max(km, XF::key(sm.u.p.A[a][c], sm.u.p.B[c][b]));
+            float s = 0.0f;
+            for (int c = 0; c < T; ++c) s += XF::contrib(sm.u.p.A[a][c], sm.u.p.B[c][b], km);
+            contrib = XF::finish(km, s);
+          }
+        }
+        sm.cA[a][b] = SR::add(sm.cA[a][b], contrib);
+      }
+    }
+  } else {
+    for (int idx = tid; idx < T * T; idx += NTHR) {
+      int a = idx / T, b = idx % T;
+      typename SR::K km = SR::kmin();
+      for (int c = 0; c < T; ++c) km = SR::kmax(km, SR::key(sm.u.p.A[a][c], sm.u.p.B[c][b]));
+      typename SR::S s = 0;
+      if (km != SR::kmin())
+        for (int c = 0; c < T; ++c) s += SR::contrib(sm.u.p.A[a][c], sm.u.p.B[c][b], km);
+      sm.cA[a][b] = SR::add(sm.cA[a][b], SR::finish(km, s));
+    }
+  }
+  __syncthreads();
+}
+
+// One launch of the inside schedule (see the file header).
+template <class SR, int T>
+__global__ __launch_bounds__(NTHR) void inside_step_kernel(
+    int L, int NT, int t, typename SR::V *__restrict__ Lr, typename SR::V *__restrict__ Lc,
+    typename SR::V *__restrict__ Sr, typename SR::V *__restrict__ Sc, typename SR::V *__restrict__ F,
+    typename SR::V *__restrict__ acc, const typename SR::V *__restrict__ ptv, Rules<SR> r) {
+  typedef typename SR::V V;
+  __shared__ StepLDS<SR, T> sm;
+  const int tid = threadIdx.x;
+  // ---- decode the work item
+  int item = blockIdx.x, I, J;
+  bool fin;
+  if (item < NT - t) {
+    fin = true;
+    I = item;
+    J = item + t;
+  } else {
+    fin = false;
+    int idx = item - (NT - t);
+    int dist = t + 1;
+    while (idx >= NT - dist) {
+      idx -= NT - dist;
+      ++dist;
+    }
+    I = idx;
+    J = idx + dist;
+  }
+  const int IT = I * T, JT = J * T;
+  // ---- bulk accumulator of the target tile
+  if (t >= 2 || !fin) {
+    for (int idx = tid; idx < T * T; idx += NTHR) {
+      int a = idx / T, b = idx % T, j = IT + a, y = JT + b;
+      sm.cA[a][b] = (j < L && y < L) ? acc[(size_t)j * L + y] : SR::zero();
+    }
+  } else {
+    for (int idx = tid; idx < T * T; idx += NTHR) sm.cA[idx / T][idx % T] = SR::zero();
+  }
+  __syncthreads();
+  // ---- split-point products that became ready with tile-diagonal t-1
+  if (t >= 2) {
+    const int K1 = I + t - 1, K2 = J - (t - 1);
+    tile_product<SR, T>(L, I, K1, J, Lr, Sc, sm);
+    if (K2 != K1) tile_product<SR, T>(L, I, K2, J, Lr, Sc, sm);
+  }
+  if (!fin) {
+    for (int idx = tid; idx < T * T; idx += NTHR) {
+      int a = idx / T, b = idx % T, j = IT + a, y = JT + b;
+      if (j < L && y < L) acc[(size_t)j * L + y] = sm.cA[a][b];
+    }
+    return;
+  }
+  // ---- finalise tile (I, J): stage operands
+  for (int idx = tid; idx < T * T; idx += NTHR) {
+    int a = idx / T, b = idx % T;
+    int j = IT + a, y = JT + b;
+    int jd = IT + b;  // column index inside the diagonal tile (I, I)
+    sm.Ld[a][b] = (j < L && jd < L && jd >= j) ? Lr[(size_t)j * L + jd] : SR::zero();
+    sm.cP[a][b] = (j < L && y < L) ? ptv[(size_t)j * L + y] : SR::zero();
+  }
+  for (int idx = tid; idx < T * T; idx += NTHR) {
+    int b = idx / T, c = idx % T;  // S[JT+c, JT+b], c fastest along the column
+    int y = JT + b, rr = JT + c;
+    sm.Sd[c][b] = (y < L && rr <= y) ? Sc[(size_t)y * L + rr] : SR::zero();
+  }
+  if (tid < T) {
+    int b = tid, y = JT + b, rb = (I + 1) * T;
+    sm.Sb[b] = (t >= 1 && y < L && rb <= y) ? Sc[(size_t)y * L + rb] : SR::zero();
+    int yc = JT + b - 1;
+    sm.Fbot[b] = (t >= 1 && rb < L && yc < L && rb <= yc) ? F[(size_t)rb * L + yc] : SR::zero();
+    int ja = IT + b + 1, yl = JT - 1;
+    sm.Fleft[b] = (t >= 1 && ja < L && yl >= 0 && ja <= yl) ? F[(size_t)ja * L + yl] : SR::zero();
+  }
+  if (t == 0) {
+    for (int idx = tid; idx < T * T; idx += NTHR) {
+      int a = idx / T, b = idx % T;
+      int j = IT + a;
+      bool d = (a == b) && j < L;
+      sm.u.w.cS[a][b] = d ? Sr[(size_t)j * L + j] : SR::zero();
+      sm.u.w.cL[a][b] = d ? Lr[(size_t)j * L + j] : SR::zero();
+      sm.u.w.cF[a][b] = SR::zero();
+    }
+  }
+  __syncthreads();
+  constexpr int G = NTHR / T;
+  const int ci = tid / G, sub = tid % G;
+  if (t == 0) {
+    // Diagonal tile: span order inside the tile.
+    for (int d = 1; d < T; ++d) {
+      const int a = ci, b = ci + d;
+      const bool valid = (ci < T - d) && (IT + b < L);
+      typename SR::K km = SR::kmin();
+      if (valid)
+        for (int c = a + sub; c < b; c += G)
+          km = SR::kmax(km, SR::key(sm.u.w.cL[a][c], sm.u.w.cS[c + 1][b]));
+      km = grp_kmax<SR>(km, G);
+      typename SR::S s = 0;
+      if (valid && km != SR::kmin())
+        for (int c = a + sub; c < b; c += G) s += SR::contrib(sm.u.w.cL[a][c], sm.u.w.cS[c + 1][b], km);
+      s = grp_sum<SR>(s, G);
+      if (valid && sub == 0) {
+        V bif = SR::finish(km, s);
+        V f = d >= 2 ? sm.u.w.cF[a + 1][b - 1] : SR::zero();
+        V pf = SR::mul(sm.cP[a][b], f);
+        sm.u.w.cS[a][b] = SR::add(SR::mulc(bif, r.p[0]), SR::mulc(pf, r.p[2]));
+        sm.u.w.cF[a][b] = SR::add(SR::mulc(bif, r.p[4]), SR::mulc(pf, r.p[3]));
+        sm.u.w.cL[a][b] = SR::mulc(pf, r.p[6]);
+      }
+      __syncthreads();
+    }
+  } else {
+    // Off-diagonal tile: anti-diagonals from the bottom-left corner.
+    for (int st = 0; st <= 2 * T - 2; ++st) {
+      const int a0 = st < T ? T - 1 - st : 0;
+      const int ncell = st < T ? st + 1 : 2 * T - 1 - st;
+      const int a = a0 + ci, b = st - (T - 1) + a;
+      const int j = IT + a, y = JT + b;
+      const bool valid = ci < ncell && j < L && y < L;
+      const int na = T - a, n = st + 1;
+      typename SR::K km = SR::kmin();
+      if (valid)
+        for (int q = sub; q < n; q += G) {
+          V lv, sv;
+          if (q < na) {
+            int c = a + q;
+            lv = sm.Ld[a][c];
+            sv = (c + 1 < T) ? sm.u.w.cS[c + 1][b] : sm.Sb[b];
+          } else {
+            int c = q - na;
+            lv = sm.u.w.cL[a][c];
+            sv = sm.Sd[c + 1][b];
+          }
+          km = SR::kmax(km, SR::key(lv, sv));
+        }
+      km = grp_kmax<SR>(km, G);
+      typename SR::S s = 0;
+      if (valid && km != SR::kmin())
+        for (int q = sub; q < n; q += G) {
+          V lv, sv;
+          if (q < na) {
+            int c = a + q;
+            lv = sm.Ld[a][c];
+            sv = (c + 1 < T) ? sm.u.w.cS[c + 1][b] : sm.Sb[b];
+          } else {
+            int c = q - na;
+            lv = sm.u.w.cL[a][c];
+            sv = sm.Sd[c + 1][b];
+          }
+          s += SR::contrib(lv, sv, km);
+        }
+      s = grp_sum<SR>(s, G);
+      if (valid && sub == 0) {
+        V bif = SR::add(sm.cA[a][b], SR::finish(km, s));
+        V f;
+        if (a + 1 < T && b >= 1) f = sm.u.w.cF[a + 1][b - 1];
+        else if (a + 1 == T) f = sm.Fbot[b];
+        else f = sm.Fleft[a];
+        if (j + 1 > y - 1) f = SR::zero();
+        V pf = SR::mul(sm.cP[a][b], f);
+        sm.u.w.cS[a][b] = SR::add(SR::mulc(bif, r.p[0]), SR::mulc(pf, r.p[2]));
+        sm.u.w.cF[a][b] = SR::add(SR::mulc(bif, r.p[4]), SR::mulc(pf, r.p[3]));
+        sm.u.w.cL[a][b] = SR::mulc(pf, r.p[6]);
+      }
+      __syncthreads();
+    }
+  }
+  // ---- write the tile back (row-major copies with b fastest, column-major with a fastest)
+  for (int idx = tid; idx < T * T; idx += NTHR) {
+    int a = idx / T, b = idx % T, j = IT + a, y = JT + b;
+    if (j < L && y < L && j < y) {
+      size_t rc = (size_t)j * L + y;
+      Sr[rc] = sm.u.w.cS[a][b];
+      F[rc] = sm.u.w.cF[a][b];
+      Lr[rc] = sm.u.w.cL[a][b];
+    }
+  }
+  for (int idx = tid; idx < T * T; idx += NTHR) {
+    int b = idx / T, a = idx % T, j = IT + a, y = JT + b;
+    if (j < L && y < L && j < y) {
+      size_t cc = (size_t)y * L + j;
+      Sc[cc] = sm.u.w.cS[a][b];
+      Lc[cc] = sm.u.w.cL[a][b];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Outside, one anti-diagonal b (descending), G lanes per cell.
+// ---------------------------------------------------------------------------
+template <class SR>
+__global__ __launch_bounds__(NTHR) void outside_diag_kernel(
+    int L, int b, int G, const typename SR::V *__restrict__ iSr, const typename SR::V *__restrict__ iLc,
+    typename SR::V *__restrict__ oSr, typename SR::V *__restrict__ oSc, typename SR::V *__restrict__ oFr,
+    typename SR::V *__restrict__ oFc, typename SR::V *__restrict__ oL, const typename SR::V *__restrict__ ptv,
+    Rules<SR> r) {
+  typedef typename SR::V V;
+  typedef typename SR::K K;
+  typedef typename SR::S S;
   long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   int cell = (int)(tid / G), sub = (int)(tid % G);
   bool valid = cell < L - b;
   int j = valid ? cell : 0;
   int y = j + b;
-  const T NINF = Lg<T>::ninf();
-  // right extension: k in (y, L-1]: out[A, j, k] + in[S, y+1, k]
-  const T *oSrow = oSr + (size_t)j * L, *oFrow = oFr + (size_t)j * L;
-  const T *iSrow = iSr + (size_t)(y + 1 < L ? y + 1 : y) * L;
-  // left extension: k in [0, j-1]: out[A, k, y] + in[L, k, j-1]
-  const T *oScol = oSc + (size_t)y * L, *oFcol = oFc + (size_t)y * L;
-  const T *iLcol = iLc + (size_t)(j >= 1 ? j - 1 : 0) * L;
-  T mrS = NINF, mrF = NINF, mlS = NINF, mlF = NINF;
+  const V *oSrow = oSr + (size_t)j * L, *oFrow = oFr + (size_t)j * L;
+  const V *iSrow = iSr + (size_t)(y + 1 < L ? y + 1 : y) * L;
+  const V *oScol = oSc + (size_t)y * L, *oFcol = oFc + (size_t)y * L;
+  const V *iLcol = iLc + (size_t)(j >= 1 ? j - 1 : 0) * L;
+  K krS = SR::kmin(), krF = SR::kmin(), klS = SR::kmin(), klF = SR::kmin();
   if (valid) {
     for (int k = y + 1 + sub; k < L; k += G) {
-      T is = iSrow[k];
-      mrS = fmax(mrS, oSrow[k] + is);
-      mrF = fmax(mrF, oFrow[k] + is);
+      V is = iSrow[k];
+      krS = SR::kmax(krS, SR::key(oSrow[k], is));
+      krF = SR::kmax(krF, SR::key(oFrow[k], is));
     }
     for (int k = sub; k < j; k += G) {
-      T il = iLcol[k];
-      mlS = fmax(mlS, oScol[k] + il);
-      mlF = fmax(mlF, oFcol[k] + il);
+      V il = iLcol[k];
+      klS = SR::kmax(klS, SR::key(oScol[k], il));
+      klF = SR::kmax(klF, SR::key(oFcol[k], il));
     }
   }
-  mrS = group_max(mrS, G);
-  mrF = group_max(mrF, G);
-  mlS = group_max(mlS, G);
-  mlF = group_max(mlF, G);
-  T srS = 0, srF = 0, slS = 0, slF = 0;
+  krS = grp_kmax<SR>(krS, G);
+  krF = grp_kmax<SR>(krF, G);
+  klS = grp_kmax<SR>(klS, G);
+  klF = grp_kmax<SR>(klF, G);
+  S srS = 0, srF = 0, slS = 0, slF = 0;
   if (valid) {
     for (int k = y + 1 + sub; k < L; k += G) {
-      T is = iSrow[k];
-      if (mrS != NINF) srS += Lg<T>::ex(oSrow[k] + is - mrS);
-      if (mrF != NINF) srF += Lg<T>::ex(oFrow[k] + is - mrF);
+      V is = iSrow[k];
+      if (krS != SR::kmin()) srS += SR::contrib(oSrow[k], is, krS);
+      if (krF != SR::kmin()) srF += SR::contrib(oFrow[k], is, krF);
     }
     for (int k = sub; k < j; k += G) {
-      T il = iLcol[k];
-      if (mlS != NINF) slS += Lg<T>::ex(oScol[k] + il - mlS);
-      if (mlF != NINF) slF += Lg<T>::ex(oFcol[k] + il - mlF);
+      V il = iLcol[k];
+      if (klS != SR::kmin()) slS += SR::contrib(oScol[k], il, klS);
+      if (klF != SR::kmin()) slF += SR::contrib(oFcol[k], il, klF);
     }
   }
-  srS = group_sum(srS, G);
-  srF = group_sum(srF, G);
-  slS = group_sum(slS, G);
-  slF = group_sum(slF, G);
+  srS = grp_sum<SR>(srS, G);
+  srF = grp_sum<SR>(srF, G);
+  slS = grp_sum<SR>(slS, G);
+  slF = grp_sum<SR>(slF, G);
   if (valid && sub == 0) {
-    T rS = srS > (T)0 ? mrS + Lg<T>::lg(srS) : NINF;
-    T rF = srF > (T)0 ? mrF + Lg<T>::lg(srF) : NINF;
-    T lS = slS > (T)0 ? mlS + Lg<T>::lg(slS) : NINF;
-    T lF = slF > (T)0 ? mlF + Lg<T>::lg(slF) : NINF;
-    // rule S->LS then F->LS: L gets the right extensions, S the left ones.
-    T vL = Lg<T>::lse(rS == NINF ? NINF : rS + r.p[0], rF == NINF ? NINF : rF + r.p[4]);
-    T vS = Lg<T>::lse(lS == NINF ? NINF : lS + r.p[0], lF == NINF ? NINF : lF + r.p[4]);
-    T vF = NINF;
+    // S->LS then F->LS: L gets the right extensions, S the left ones.
+    V vL = SR::add(SR::mulc(SR::finish(krS, srS), r.p[0]), SR::mulc(SR::finish(krF, srF), r.p[4]));
+    V vS = SR::add(SR::mulc(SR::finish(klS, slS), r.p[0]), SR::mulc(SR::finish(klF, slF), r.p[4]));
+    V vF = SR::zero();
     if (j >= 1 && y + 1 < L) {
       size_t pc = (size_t)(j - 1) * L + (y + 1);
-      T pt = ptv[pc];
-      if (pt != NINF) {
-        T a = oSr[pc], f = oFr[pc], l = oL[pc];
-        vF = Lg<T>::lse(vF, a == NINF ? NINF : r.p[2] + pt + a);
-        vF = Lg<T>::lse(vF, f == NINF ? NINF : r.p[3] + pt + f);
-        vF = Lg<T>::lse(vF, l == NINF ? NINF : r.p[6] + pt + l);
-      }
+      V pt = ptv[pc];
+      V t = SR::add(SR::add(SR::mulc(oSr[pc], r.p[2]), SR::mulc(oFr[pc], r.p[3])), SR::mulc(oL[pc], r.p[6]));
+      vF = SR::mul(pt, t);
     }
     size_t rc = (size_t)j * L + y, cc = (size_t)y * L + j;
     oSr[rc] = vS;
@@ -255,21 +500,24 @@ __global__ __launch_bounds__(256) void outside_diag_kernel(int L, int b, int G, 
   }
 }
 
-template <typename T>
-__global__ void outside_init_kernel(int L, T *oSr, T *oSc, T *oFr, T *oFc, T *oL) {
-  size_t rc = (size_t)0 * L + (L - 1), cc = (size_t)(L - 1) * L + 0;
-  oSr[rc] = (T)0;
-  oSc[cc] = (T)0;
-  oFr[rc] = Lg<T>::ninf();
-  oFc[cc] = Lg<T>::ninf();
-  oL[rc] = Lg<T>::ninf();
+template <class SR>
+__global__ void outside_init_kernel(int L, typename SR::V *oSr, typename SR::V *oSc, typename SR::V *oFr,
+                                    typename SR::V *oFc, typename SR::V *oL) {
+  size_t rc = (size_t)(L - 1), cc = (size_t)(L - 1) * L;
+  oSr[rc] = SR::from_log(0.0);
+  oSc[cc] = SR::from_log(0.0);
+  oFr[rc] = SR::zero();
+  oFc[cc] = SR::zero();
+  oL[rc] = SR::zero();
 }
 
-// Export a table triple to the Julia (3, L, L) layout in natural-log absolute
-// values; cells below the diagonal are -inf.
-template <typename T>
-__global__ void export3_kernel(int L, const T *Sc, const T *Fr, const T *Lrow, const T *Lcol,
-                               const double *C, double fromU, int outside, double CL, double *o3) {
+// ---------------------------------------------------------------------------
+// Export / import of the Julia (3, L, L) layout (natural-log absolute values).
+// ---------------------------------------------------------------------------
+template <class SR>
+__global__ void export3_kernel(int L, const typename SR::V *Scol, const typename SR::V *Frow,
+                               const typename SR::V *Lrow, const typename SR::V *Lcol, const double *C,
+                               int outside, double *o3) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
   if (i >= L) return;
@@ -278,29 +526,28 @@ __global__ void export3_kernel(int L, const T *Sc, const T *Fr, const T *Lrow, c
     o3[base] = o3[base + 1] = o3[base + 2] = -HUGE_VAL;
     return;
   }
-  // inside baseline of span [i, j]: C[j+1] - C[i]; outside: C[L] - C[j+1] + C[i].
-  double bl = outside ? (CL - C[j + 1] + C[i]) : (C[j + 1] - C[i]);
-  T s = Sc[(size_t)j * L + i];
-  T f = Fr[(size_t)i * L + j];
-  T l = Lcol ? Lcol[(size_t)j * L + i] : Lrow[(size_t)i * L + j];
-  o3[base] = s == Lg<T>::ninf() ? -HUGE_VAL : ((double)s + bl) * fromU;
-  o3[base + 1] = f == Lg<T>::ninf() ? -HUGE_VAL : ((double)f + bl) * fromU;
-  o3[base + 2] = l == Lg<T>::ninf() ? -HUGE_VAL : ((double)l + bl) * fromU;
+  double bl = outside ? (C[L] - C[j + 1] + C[i]) : (C[j + 1] - C[i]);
+  double s = SR::to_log(Scol[(size_t)j * L + i]);
+  double f = SR::to_log(Frow[(size_t)i * L + j]);
+  double l = SR::to_log(Lcol ? Lcol[(size_t)j * L + i] : Lrow[(size_t)i * L + j]);
+  o3[base] = s == -HUGE_VAL ? s : s + bl;
+  o3[base + 1] = f == -HUGE_VAL ? f : f + bl;
+  o3[base + 2] = l == -HUGE_VAL ? l : l + bl;
 }
 
-// Import a Julia (3, L, L) table into row/column-major scaled tables.
-template <typename T>
-__global__ void import3_kernel(int L, const double *i3, const double *C, double toU, int outside,
-                               double CL, T *Sr, T *Sc, T *Fr, T *Fc, T *Lr, T *Lc) {
+template <class SR>
+__global__ void import3_kernel(int L, const double *i3, const double *C, int outside, typename SR::V *Sr,
+                               typename SR::V *Sc, typename SR::V *Fr, typename SR::V *Fc, typename SR::V *Lr,
+                               typename SR::V *Lc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
   if (i >= L || i > j) return;
   size_t base = 3 * ((size_t)i + (size_t)L * j);
-  double bl = outside ? (CL - C[j + 1] + C[i]) : (C[j + 1] - C[i]);
+  double bl = outside ? (C[L] - C[j + 1] + C[i]) : (C[j + 1] - C[i]);
   double s = i3[base], f = i3[base + 1], l = i3[base + 2];
-  T ts = s == -HUGE_VAL ? Lg<T>::ninf() : (T)(s * toU - bl);
-  T tf = f == -HUGE_VAL ? Lg<T>::ninf() : (T)(f * toU - bl);
-  T tl = l == -HUGE_VAL ? Lg<T>::ninf() : (T)(l * toU - bl);
+  typename SR::V ts = SR::from_log(s == -HUGE_VAL ? s : s - bl);
+  typename SR::V tf = SR::from_log(f == -HUGE_VAL ? f : f - bl);
+  typename SR::V tl = SR::from_log(l == -HUGE_VAL ? l : l - bl);
   size_t rc = (size_t)i * L + j, cc = (size_t)j * L + i;
   Sr[rc] = ts;
   Sc[cc] = ts;
@@ -310,48 +557,49 @@ __global__ void import3_kernel(int L, const double *i3, const double *C, double 
   if (Lc) Lc[cc] = tl;
 }
 
-// Base-pair posteriors: log table (symmetric) and unpaired logs, scaled units.
-template <typename T>
-__global__ void bpp_logs_kernel(int L, const T *oSr, const T *oFr, const T *oL, const T *iF,
-                                const T *ptv, const double *uhat, Rules<T> r, const T *Zhat, T *lp,
-                                T *res) {
+// log Z (natural, absolute) = S[0, L-1] plus the baseline of the whole span.
+template <class SR>
+__global__ void zlog_kernel(int L, const typename SR::V *Sc, const double *C, double *z) {
+  double v = SR::to_log(Sc[(size_t)(L - 1) * L]);
+  z[0] = v == -HUGE_VAL ? v : v + C[L];
+  z[1] = v;  // baseline-relative
+}
+
+// ---------------------------------------------------------------------------
+// Base-pair posteriors (natural logs in double, then per-row normalisation).
+// ---------------------------------------------------------------------------
+template <class SR>
+__global__ void bpp_logs_kernel(int L, const typename SR::V *oSr, const typename SR::V *oFr,
+                                const typename SR::V *oL, const typename SR::V *iF, const typename SR::V *ptv,
+                                const double *uhat, Rules<SR> r, const double *zz, double *lp, double *res) {
+  typedef typename SR::V V;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
   if (k >= L) return;
-  const T NINF = Lg<T>::ninf();
-  T z = Zhat[0];
+  const double zhat = zz[1];
   if (k == j) {
-    lp[(size_t)j * L + j] = NINF;
-    T a = oSr[(size_t)j * L + j], l = oL[(size_t)j * L + j];
-    T t = Lg<T>::lse(a == NINF ? NINF : a + r.p[1], l == NINF ? NINF : l + r.p[5]);
-    res[j] = t == NINF ? NINF : (T)((double)t + uhat[j] - (double)z);
+    lp[(size_t)j * L + j] = -HUGE_VAL;
+    V t = SR::add(SR::mulc(oSr[(size_t)j * L + j], r.p[1]), SR::mulc(oL[(size_t)j * L + j], r.p[5]));
+    double tl = SR::to_log(t);
+    res[j] = tl == -HUGE_VAL ? tl : tl + uhat[j] - zhat;
     return;
   }
   if (k < j) return;
   size_t c = (size_t)j * L + k;
-  T inres = (j + 1 <= k - 1) ? iF[(size_t)(j + 1) * L + (k - 1)] : NINF;
-  T pt = ptv[c];
-  T v = NINF;
-  if (inres != NINF && pt != NINF) {
-    T a = oSr[c], f = oFr[c], l = oL[c];
-    T t = NINF;
-    t = Lg<T>::lse(t, a == NINF ? NINF : a + r.p[2] + inres);
-    t = Lg<T>::lse(t, f == NINF ? NINF : f + r.p[3] + inres);
-    t = Lg<T>::lse(t, l == NINF ? NINF : l + r.p[6] + inres);
-    v = t == NINF ? NINF : t + pt - z;
-  }
+  V inres = (j + 1 <= k - 1) ? iF[(size_t)(j + 1) * L + (k - 1)] : SR::zero();
+  V t = SR::add(SR::add(SR::mulc(oSr[c], r.p[2]), SR::mulc(oFr[c], r.p[3])), SR::mulc(oL[c], r.p[6]));
+  double v = SR::to_log(SR::mul(SR::mul(t, inres), ptv[c]));
+  v = v == -HUGE_VAL ? v : v - zhat;
   lp[c] = v;
   lp[(size_t)k * L + j] = v;
 }
 
-// Per-row maximum and normaliser (one workgroup per row).
-template <typename T>
-__global__ __launch_bounds__(256) void bpp_rownorm_kernel(int L, const T *lp, const T *res, T *rmax,
-                                                          T *rnorm) {
-  __shared__ T sh[256];
+__global__ __launch_bounds__(256) void bpp_rownorm_kernel(int L, const double *lp, const double *res,
+                                                          double *rmax, double *rnorm) {
+  __shared__ double sh[256];
   int x = blockIdx.x, t = threadIdx.x;
-  const T *row = lp + (size_t)x * L;
-  T m = res[x];
+  const double *row = lp + (size_t)x * L;
+  double m = res[x];
   for (int y = t; y < L; y += 256) m = fmax(m, row[y]);
   sh[t] = m;
   __syncthreads();
@@ -361,8 +609,8 @@ __global__ __launch_bounds__(256) void bpp_rownorm_kernel(int L, const T *lp, co
   }
   m = sh[0];
   __syncthreads();
-  T s = 0;
-  for (int y = t; y < L; y += 256) s += Lg<T>::ex(row[y] - m);
+  double s = 0;
+  for (int y = t; y < L; y += 256) s += exp(row[y] - m);
   sh[t] = s;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -371,186 +619,179 @@ __global__ __launch_bounds__(256) void bpp_rownorm_kernel(int L, const T *lp, co
   }
   if (t == 0) {
     rmax[x] = m;
-    rnorm[x] = sh[0] + Lg<T>::ex(res[x] - m);
+    rnorm[x] = sh[0] + exp(res[x] - m);
   }
 }
 
-template <typename T>
-__global__ void bpp_write_kernel(int L, const T *lp, const T *res, const T *rmax, const T *rnorm,
-                                 double *unpaired, double *pair) {
+__global__ void bpp_write_kernel(int L, const double *lp, const double *res, const double *rmax,
+                                 const double *rnorm, double *unpaired, double *pair) {
   int x = blockIdx.x * blockDim.x + threadIdx.x;
   int y = blockIdx.y;
   if (x >= L) return;
-  T m = rmax[x], n = rnorm[x];
-  pair[(size_t)x + (size_t)L * y] = (double)(Lg<T>::ex(lp[(size_t)y * L + x] - m) / n);
-  if (y == 0 && unpaired) unpaired[x] = (double)(Lg<T>::ex(res[x] - m) / n);
-}
-
-template <typename T>
-__global__ void zhat_kernel(int L, const T *Sc, T *z) {
-  z[0] = Sc[(size_t)(L - 1) * L + 0];
+  double m = rmax[x], n = rnorm[x];
+  pair[(size_t)x + (size_t)L * y] = exp(lp[(size_t)y * L + x] - m) / n;
+  if (y == 0 && unpaired) unpaired[x] = exp(res[x] - m) / n;
 }
 
 // ---------------------------------------------------------------------------
-// Host side
+// Host side (T = double -> LogSR, T = float -> XF)
 // ---------------------------------------------------------------------------
-template <typename T> struct Units;
-template <> struct Units<double> {
-  static constexpr double toU = 1.0;
-  static constexpr int scaled = 0;
-};
-template <> struct Units<float> {
-  static constexpr double toU = 1.4426950408889634;  // 1 / ln 2
-  static constexpr int scaled = 1;
-};
+template <typename T> struct SRof;
+template <> struct SRof<double> { typedef LogSR type; };
+template <> struct SRof<float> { typedef XF type; };
 
-template <typename T>
-static Rules<T> make_rules(double B) {
-  double lp[7];
-  kh99_logprobs(lp);
-  Rules<T> r;
-  for (int i = 0; i < 7; ++i) r.p[i] = (T)(lp[i] * B * Units<T>::toU);
-  return r;
+static inline int pow2ceil(int x) {
+  int g = 1;
+  while (g < x) g <<= 1;
+  return g;
+}
+static inline int lanes_for(int terms) {
+  int g = pow2ceil(terms / 8 > 1 ? terms / 8 : 1);
+  return g > 64 ? 64 : g;
 }
 
 template <typename T>
 void inside_alloc(InsideWork &w, int L, bool outside) {
-  size_t n = (size_t)L * L * sizeof(T);
+  typedef typename SRof<T>::type SR;
+  size_t n = (size_t)L * L * sizeof(typename SR::V);
   w.L = L;
   w.Lr.ensure(n);
   w.Lc.ensure(n);
   w.Sr.ensure(n);
   w.Sc.ensure(n);
   w.F.ensure(n);
+  w.acc.ensure(n);
   w.ptv.ensure(n);
   w.lam.ensure(sizeof(double) * L);
   w.C.ensure(sizeof(double) * (L + 1));
   w.uhat.ensure(sizeof(double) * L);
   w.u.ensure(sizeof(double) * L);
   w.paired.ensure(sizeof(double) * L * L);
-  w.red.ensure(sizeof(double) * 4 * (L + 16));
+  w.red.ensure(sizeof(double) * (3 * L + 16));
   if (outside) {
     w.oSr.ensure(n);
     w.oSc.ensure(n);
     w.oFr.ensure(n);
     w.oFc.ensure(n);
     w.oL.ensure(n);
-    w.lp.ensure(n);
+    w.lp.ensure(sizeof(double) * L * L);
   }
 }
 
-// Baseline, ptv and the diagonal of the inside tables from w.u and w.paired (device).
 template <typename T>
 void inside_prepare(hipStream_t s, InsideWork &w, int L, double B) {
+  typedef typename SRof<T>::type SR;
   double lp[7];
   kh99_logprobs(lp);
-  double toU = Units<T>::toU;
-  hipLaunchKernelGGL(baseline_kernel, dim3(1), dim3(1024), 0, s, L, w.u.as<double>(), B, toU,
-                     lp[0] + lp[5], Units<T>::scaled, w.lam.as<double>(), w.C.as<double>(),
-                     w.uhat.as<double>());
-  hipLaunchKernelGGL(ptv_kernel<T>, dim3((L + 255) / 256, L), dim3(256), 0, s, L,
-                     w.paired.as<double>(), w.lam.as<double>(), B * toU, w.ptv.as<T>());
+  hipLaunchKernelGGL(baseline_kernel, dim3(1), dim3(1024), 0, s, L, w.u.as<double>(), B, lp[0] + lp[5],
+                     (int)SR::scaled, w.lam.as<double>(), w.C.as<double>(), w.uhat.as<double>());
+  hipLaunchKernelGGL(ptv_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, w.paired.as<double>(),
+                     w.lam.as<double>(), B, w.ptv.as<typename SR::V>());
 }
 
 template <typename T>
 void inside_run(hipStream_t s, InsideWork &w, int L, double B) {
-  Rules<T> r = make_rules<T>(B);
-  hipLaunchKernelGGL(inside_init_kernel<T>, dim3((L + 255) / 256), dim3(256), 0, s, L,
-                     w.u.as<double>(), w.lam.as<double>(), B * Units<T>::toU, r, w.Lr.as<T>(),
-                     w.Lc.as<T>(), w.Sr.as<T>(), w.Sc.as<T>(), w.F.as<T>());
-  for (int d = 1; d < L; ++d) {
-    int G = lanes_for(d);
-    long threads = (long)(L - d) * G;
-    int blocks = (int)((threads + 255) / 256);
-    hipLaunchKernelGGL(inside_diag_kernel<T>, dim3(blocks), dim3(256), 0, s, L, d, G, w.Lr.as<T>(),
-                       w.Lc.as<T>(), w.Sr.as<T>(), w.Sc.as<T>(), w.F.as<T>(), w.ptv.as<T>(), r);
+  typedef typename SRof<T>::type SR;
+  typedef typename SR::V V;
+  double lp[7];
+  kh99_logprobs(lp);
+  Rules<SR> r = make_rules<SR>(B);
+  hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, w.u.as<double>(),
+                     w.lam.as<double>(), B, lp[1], lp[5], w.Lr.as<V>(), w.Lc.as<V>(), w.Sr.as<V>(),
+                     w.Sc.as<V>(), w.F.as<V>(), w.acc.as<V>());
+  const int NT = (L + TILE - 1) / TILE;
+  for (int t = 0; t < NT; ++t) {
+    int items = NT - t;
+    for (int dist = t + 1; dist <= std::min(2 * t - 2, NT - 1); ++dist) items += NT - dist;
+    hipLaunchKernelGGL((inside_step_kernel<SR, TILE>), dim3(items), dim3(NTHR), 0, s, L, NT, t, w.Lr.as<V>(),
+                       w.Lc.as<V>(), w.Sr.as<V>(), w.Sc.as<V>(), w.F.as<V>(), w.acc.as<V>(),
+                       (const V *)w.ptv.as<V>(), r);
   }
+  hipLaunchKernelGGL(zlog_kernel<SR>, dim3(1), dim3(1), 0, s, L, (const V *)w.Sc.as<V>(),
+                     (const double *)w.C.as<double>(), w.red.as<double>());
   MESSI_HIP(hipGetLastError());
 }
 
-// log Z in natural units (host value).
 template <typename T>
 double inside_logZ(hipStream_t s, InsideWork &w, int L) {
-  T z;
-  double CL;
-  MESSI_HIP(hipMemcpyAsync(&z, w.Sc.as<T>() + (size_t)(L - 1) * L, sizeof(T), hipMemcpyDeviceToHost, s));
-  MESSI_HIP(hipMemcpyAsync(&CL, w.C.as<double>() + L, sizeof(double), hipMemcpyDeviceToHost, s));
+  double z[2];
+  MESSI_HIP(hipMemcpyAsync(z, w.red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   MESSI_HIP(hipStreamSynchronize(s));
-  if (z == -INFINITY) return -INFINITY;
-  return ((double)z + CL) / Units<T>::toU;
+  return z[0];
 }
 
 template <typename T>
 void inside_export(hipStream_t s, InsideWork &w, int L, double *d_in3) {
-  hipLaunchKernelGGL(export3_kernel<T>, dim3((L + 255) / 256, L), dim3(256), 0, s, L,
-                     (const T *)w.Sc.as<T>(), (const T *)w.F.as<T>(), (const T *)nullptr,
-                     (const T *)w.Lc.as<T>(), (const double *)w.C.as<double>(),
-                     1.0 / Units<T>::toU, 0, 0.0, d_in3);
+  typedef typename SRof<T>::type SR;
+  typedef typename SR::V V;
+  hipLaunchKernelGGL(export3_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, (const V *)w.Sc.as<V>(),
+                     (const V *)w.F.as<V>(), (const V *)nullptr, (const V *)w.Lc.as<V>(),
+                     (const double *)w.C.as<double>(), 0, d_in3);
 }
 
 template <typename T>
 void inside_import(hipStream_t s, InsideWork &w, int L, const double *d_in3) {
-  hipLaunchKernelGGL(import3_kernel<T>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, d_in3,
-                     (const double *)w.C.as<double>(), Units<T>::toU, 0, 0.0, w.Sr.as<T>(),
-                     w.Sc.as<T>(), w.F.as<T>(), (T *)nullptr, w.Lr.as<T>(), w.Lc.as<T>());
+  typedef typename SRof<T>::type SR;
+  typedef typename SR::V V;
+  hipLaunchKernelGGL(import3_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, d_in3,
+                     (const double *)w.C.as<double>(), 0, w.Sr.as<V>(), w.Sc.as<V>(), w.F.as<V>(), (V *)nullptr,
+                     w.Lr.as<V>(), w.Lc.as<V>());
+  hipLaunchKernelGGL(zlog_kernel<SR>, dim3(1), dim3(1), 0, s, L, (const V *)w.Sc.as<V>(),
+                     (const double *)w.C.as<double>(), w.red.as<double>());
 }
 
 template <typename T>
 void outside_run(hipStream_t s, InsideWork &w, int L) {
-  Rules<T> r = make_rules<T>(1.0);
-  hipLaunchKernelGGL(outside_init_kernel<T>, dim3(1), dim3(1), 0, s, L, w.oSr.as<T>(), w.oSc.as<T>(),
-                     w.oFr.as<T>(), w.oFc.as<T>(), w.oL.as<T>());
+  typedef typename SRof<T>::type SR;
+  typedef typename SR::V V;
+  Rules<SR> r = make_rules<SR>(1.0);
+  hipLaunchKernelGGL(outside_init_kernel<SR>, dim3(1), dim3(1), 0, s, L, w.oSr.as<V>(), w.oSc.as<V>(),
+                     w.oFr.as<V>(), w.oFc.as<V>(), w.oL.as<V>());
   for (int b = L - 2; b >= 0; --b) {
     int G = lanes_for(L - b);
     long threads = (long)(L - b) * G;
     int blocks = (int)((threads + 255) / 256);
-    hipLaunchKernelGGL(outside_diag_kernel<T>, dim3(blocks), dim3(256), 0, s, L, b, G,
-                       (const T *)w.Sr.as<T>(), (const T *)w.Lc.as<T>(), w.oSr.as<T>(),
-                       w.oSc.as<T>(), w.oFr.as<T>(), w.oFc.as<T>(), w.oL.as<T>(),
-                       (const T *)w.ptv.as<T>(), r);
+    hipLaunchKernelGGL(outside_diag_kernel<SR>, dim3(blocks), dim3(256), 0, s, L, b, G, (const V *)w.Sr.as<V>(),
+                       (const V *)w.Lc.as<V>(), w.oSr.as<V>(), w.oSc.as<V>(), w.oFr.as<V>(), w.oFc.as<V>(),
+                       w.oL.as<V>(), (const V *)w.ptv.as<V>(), r);
   }
   MESSI_HIP(hipGetLastError());
 }
 
 template <typename T>
 void outside_export(hipStream_t s, InsideWork &w, int L, double *d_out3) {
-  double CL;
-  MESSI_HIP(hipMemcpyAsync(&CL, w.C.as<double>() + L, sizeof(double), hipMemcpyDeviceToHost, s));
-  MESSI_HIP(hipStreamSynchronize(s));
-  hipLaunchKernelGGL(export3_kernel<T>, dim3((L + 255) / 256, L), dim3(256), 0, s, L,
-                     (const T *)w.oSc.as<T>(), (const T *)w.oFr.as<T>(), (const T *)w.oL.as<T>(),
-                     (const T *)nullptr, (const double *)w.C.as<double>(), 1.0 / Units<T>::toU, 1,
-                     CL, d_out3);
+  typedef typename SRof<T>::type SR;
+  typedef typename SR::V V;
+  hipLaunchKernelGGL(export3_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L,
+                     (const V *)w.oSc.as<V>(), (const V *)w.oFr.as<V>(), (const V *)w.oL.as<V>(), (const V *)nullptr,
+                     (const double *)w.C.as<double>(), 1, d_out3);
 }
 
 template <typename T>
 void outside_import(hipStream_t s, InsideWork &w, int L, const double *d_out3) {
-  double CL;
-  MESSI_HIP(hipMemcpyAsync(&CL, w.C.as<double>() + L, sizeof(double), hipMemcpyDeviceToHost, s));
-  MESSI_HIP(hipStreamSynchronize(s));
-  hipLaunchKernelGGL(import3_kernel<T>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, d_out3,
-                     (const double *)w.C.as<double>(), Units<T>::toU, 1, CL, w.oSr.as<T>(),
-                     w.oSc.as<T>(), w.oFr.as<T>(), w.oFc.as<T>(), w.oL.as<T>(), (T *)nullptr);
+  typedef typename SRof<T>::type SR;
+  typedef typename SR::V V;
+  hipLaunchKernelGGL(import3_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, d_out3,
+                     (const double *)w.C.as<double>(), 1, w.oSr.as<V>(), w.oSc.as<V>(), w.oFr.as<V>(),
+                     w.oFc.as<V>(), w.oL.as<V>(), (V *)nullptr);
 }
 
-// Posteriors into device arrays (unpaired may be null).
 template <typename T>
 void bpp_run(hipStream_t s, InsideWork &w, int L, double *d_unpaired, double *d_pair) {
-  Rules<T> r = make_rules<T>(1.0);
-  T *z = reinterpret_cast<T *>(w.red.as<double>());
-  T *res = z + 16;
-  T *rmax = res + L;
-  T *rnorm = rmax + L;
-  hipLaunchKernelGGL(zhat_kernel<T>, dim3(1), dim3(1), 0, s, L, (const T *)w.Sc.as<T>(), z);
-  hipLaunchKernelGGL(bpp_logs_kernel<T>, dim3((L + 255) / 256, L), dim3(256), 0, s, L,
-                     (const T *)w.oSr.as<T>(), (const T *)w.oFr.as<T>(), (const T *)w.oL.as<T>(),
-                     (const T *)w.F.as<T>(), (const T *)w.ptv.as<T>(),
-                     (const double *)w.uhat.as<double>(), r, (const T *)z, w.lp.as<T>(), res);
-  hipLaunchKernelGGL(bpp_rownorm_kernel<T>, dim3(L), dim3(256), 0, s, L, (const T *)w.lp.as<T>(),
-                     (const T *)res, rmax, rnorm);
-  hipLaunchKernelGGL(bpp_write_kernel<T>, dim3((L + 255) / 256, L), dim3(256), 0, s, L,
-                     (const T *)w.lp.as<T>(), (const T *)res, (const T *)rmax, (const T *)rnorm,
-                     d_unpaired, d_pair);
+  typedef typename SRof<T>::type SR;
+  typedef typename SR::V V;
+  Rules<SR> r = make_rules<SR>(1.0);
+  double *zz = w.red.as<double>();  // [0] log Z, [1] baseline-relative log Z (from zlog_kernel)
+  double *res = zz + 16, *rmax = res + L, *rnorm = rmax + L;
+  hipLaunchKernelGGL(bpp_logs_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, (const V *)w.oSr.as<V>(),
+                     (const V *)w.oFr.as<V>(), (const V *)w.oL.as<V>(), (const V *)w.F.as<V>(),
+                     (const V *)w.ptv.as<V>(), (const double *)w.uhat.as<double>(), r, (const double *)zz,
+                     w.lp.as<double>(), res);
+  hipLaunchKernelGGL(bpp_rownorm_kernel, dim3(L), dim3(256), 0, s, L, (const double *)w.lp.as<double>(),
+                     (const double *)res, rmax, rnorm);
+  hipLaunchKernelGGL(bpp_write_kernel, dim3((L + 255) / 256, L), dim3(256), 0, s, L,
+                     (const double *)w.lp.as<double>(), (const double *)res, (const double *)rmax,
+                     (const double *)rnorm, d_unpaired, d_pair);
   MESSI_HIP(hipGetLastError());
 }
 

@@ -16,12 +16,15 @@
 //
 // MFMA mapping (V_MFMA_F64_16X16X4_F64 / V_MFMA_F32_16X16X4_F32, one wavefront
 // per block of 16 entries): the message P_c * part_c is a 16x16 (states) times
-// 16x16 (states x entries) product done as four K=4 steps.  Step s contracts
-// the states {s, 4+s, 8+s, 12+s}: lane l feeds A = P[l%16][4*(l/16)+s] and
-// B = part[4*(l/16)+s][entry l%16], which is exactly register s of the
-// accumulator layout (lane l holds rows 4*(l/16)+r of column l%16).  A node's
-// output therefore feeds its parent's MFMA with no data movement, and every
-// lane loads/stores 4 consecutive states of one entry.
+// 16x16 (states x entries) product done as four K=4 steps.  Both instructions
+// take A[i][k] from lane i + 16k and B[k][j] from lane j + 16k, and return
+// D[i][j] in lane j + 16g, register r, where the row is i = state(g, r):
+// g + 4r for f64 and 4g + r for f32 (checked by messi_selftest).  Step s of a
+// node therefore uses register s of the child's accumulator as B, contracting
+// the states {state(g, s)}, with A = P[i][state(g, s)]: a node's output feeds
+// its parent's MFMA with no data movement.  Partials are stored in lane order
+// (position 4g + r holds state(g, r)) so every lane loads and stores 4
+// consecutive values of one entry.
 #include "common.h"
 #include "prune.h"
 
@@ -33,12 +36,14 @@ template <> struct Mfma<double> {
   static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __host__ __device__ __forceinline__ int state(int g, int r) { return g + 4 * r; }
 };
 template <> struct Mfma<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __host__ __device__ __forceinline__ int state(int g, int r) { return 4 * g + r; }
 };
 
 __device__ __forceinline__ double lse_d(double a, double b) {
@@ -101,13 +106,13 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
   const T *Pr = P + (size_t)nd.node_r * 256;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    al[s] = Pl[col + 16 * (4 * g + s)];
-    ar[s] = Pr[col + 16 * (4 * g + s)];
+    al[s] = Pl[col + 16 * Mfma<T>::state(g, s)];
+    ar[s] = Pr[col + 16 * Mfma<T>::state(g, s)];
   }
   double pis[4] = {0, 0, 0, 0};
   if (ROOT) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) pis[r] = pi16[4 * g + r];
+    for (int r = 0; r < 4; ++r) pis[r] = pi16[Mfma<T>::state(g, r)];
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -120,18 +125,22 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
     const int cr = idx[nd.idx_r + ec];
     if (nd.leaf_l) {
       const unsigned cc = leafc[nd.cpart_l + cl];
-      const T xg = (T)((cc >> g) & 1u);
 #pragma unroll
-      for (int s = 0; s < 4; ++s) bl[s] = xg * (T)((cc >> (8 + s)) & 1u);
+      for (int s = 0; s < 4; ++s) {
+        const int a = Mfma<T>::state(g, s);
+        bl[s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
+      }
     } else {
       load4(part + (size_t)(nd.cpart_l + cl) * 16 + 4 * g, bl);
       el = expo[nd.cpart_l + cl];
     }
     if (nd.leaf_r) {
       const unsigned cc = leafc[nd.cpart_r + cr];
-      const T xg = (T)((cc >> g) & 1u);
 #pragma unroll
-      for (int s = 0; s < 4; ++s) br[s] = xg * (T)((cc >> (8 + s)) & 1u);
+      for (int s = 0; s < 4; ++s) {
+        const int a = Mfma<T>::state(g, s);
+        br[s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
+      }
     } else {
       load4(part + (size_t)(nd.cpart_r + cr) * 16 + 4 * g, br);
       er = expo[nd.cpart_r + cr];
@@ -317,7 +326,7 @@ template void launch_site_level<float>(hipStream_t, const int2 *, int, bool, con
 // ---------------------------------------------------------------------------
 // Self-test of the MFMA operand layouts the pair kernel relies on.
 // D = A (16x4) * B (4x16); A[i][k] at lane i + 16k, B[k][j] at lane j + 16k,
-// D[i][j] in register i%4 of lane j + 16*(i/4).
+// D[state(g, r)][j] in register r of lane j + 16g.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void mfma_probe_kernel(const T *A, const T *Bm, T *D) {
@@ -327,7 +336,7 @@ __global__ void mfma_probe_kernel(const T *A, const T *Bm, T *D) {
   T b = Bm[(l / 16) * 16 + l % 16];
   v4 acc = {0, 0, 0, 0};
   acc = Mfma<T>::op(a, b, acc);
-  for (int r = 0; r < 4; ++r) D[(4 * (l / 16) + r) * 16 + l % 16] = acc[r];
+  for (int r = 0; r < 4; ++r) D[Mfma<T>::state(l / 16, r) * 16 + l % 16] = acc[r];
 }
 
 template <typename T>
