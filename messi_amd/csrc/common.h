@@ -69,6 +69,7 @@ inline void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
 // Scratch of the inside/outside/bpp phase, per precision.
 struct InsideWork {
   int L = 0;
+  int tile = 16;                  // tile edge of the inside schedule (16 or 32)
   DevBuf Lr, Lc, Sr, Sc, F;       // inside tables (row- and column-major copies)
   DevBuf acc;                     // bulk split-point accumulator of the tiled inside
   DevBuf oSr, oSc, oFr, oFc, oL;  // outside tables

@@ -28,18 +28,21 @@
 
 namespace messi {
 
-constexpr int TILE = 32;
-constexpr int NTHR = 256;
+// Tile edge T of the inside schedule; a workgroup has T*T threads (one per cell).
 
 template <class SR> struct Rules {
-  typename SR::C p[7];
+  typename SR::C p[7];    // rule weights in the representation's own form (log or linear)
+  typename SR::C lin[7];  // linear rule probabilities p^B
 };
 
 template <class SR> static Rules<SR> make_rules(double B) {
   double lp[7];
   kh99_logprobs(lp);
   Rules<SR> r;
-  for (int i = 0; i < 7; ++i) r.p[i] = SR::rule(lp[i] * B);
+  for (int i = 0; i < 7; ++i) {
+    r.p[i] = SR::rule(lp[i] * B);
+    r.lin[i] = (typename SR::C)std::exp(lp[i] * B);
+  }
   return r;
 }
 
@@ -150,18 +153,29 @@ template <class SR, int T> struct StepLDS {
   V Sb[T], Fbot[T], Fleft[T];
 };
 
+// Exact exponent-aligned split-point sum of one cell of a staged product (the
+// rare case where aligning a whole row/column lost the significant terms).
+template <int T>
+__device__ __forceinline__ XFv exact_cell_xf(const XFv (*A)[T + 1], const XFv (*B)[T + 1], int a, int b) {
+  int km = XF::kmin();
+  for (int c = 0; c < T; ++c) km = max(km, XF::key(A[a][c], B[c][b]));
+  float s = 0.0f;
+  for (int c = 0; c < T; ++c) s += XF::contrib(A[a][c], B[c][b], km);
+  return XF::finish(km, s);
+}
+
 // acc (+)= sum over split points h in tile K of L[IT+a, h] * S[h+1, JT+b]
 template <class SR, int T>
-__device__ void tile_product(int L, int I, int K, int J, const typename SR::V *__restrict__ Lr,
+__device__ __forceinline__ void tile_product(int L, int I, int K, int J, const typename SR::V *__restrict__ Lr,
                              const typename SR::V *__restrict__ Sc, StepLDS<SR, T> &sm) {
   typedef typename SR::V V;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < T * T; idx += NTHR) {
+  for (int idx = tid; idx < T * T; idx += T * T) {
     int a = idx / T, c = idx % T;
     int j = I * T + a, h = K * T + c;
     sm.u.p.A[a][c] = (j < L && h < L) ? Lr[(size_t)j * L + h] : SR::zero();
   }
-  for (int idx = tid; idx < T * T; idx += NTHR) {
+  for (int idx = tid; idx < T * T; idx += T * T) {
     int b = idx / T, c = idx % T;
     int y = J * T + b, r = K * T + c + 1;
     sm.u.p.B[c][b] = (y < L && r < L) ? Sc[(size_t)y * L + r] : SR::zero();
@@ -185,7 +199,7 @@ __device__ void tile_product(int L, int I, int K, int J, const typename SR::V *_
       sm.cB[b] = m;
     }
     __syncthreads();
-    for (int idx = tid; idx < T * T; idx += NTHR) {
+    for (int idx = tid; idx < T * T; idx += T * T) {
       int a = idx / T, c = idx % T;
       V v = sm.u.p.A[a][c];
       sm.u.p.Ap[a][c] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - sm.rA[a], -200));
@@ -196,7 +210,7 @@ __device__ void tile_product(int L, int I, int K, int J, const typename SR::V *_
     typedef float v4f __attribute__((ext_vector_type(4)));
     const int wave = tid >> 6, lane = tid & 63;
     constexpr int NB = T / 16;
-    for (int blk = wave; blk < NB * NB; blk += NTHR / 64) {
+    for (int blk = wave; blk < NB * NB; blk += T * T / 64) {
       const int bi = blk / NB, bj = blk % NB;
       v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -215,19 +229,14 @@ __device__ void tile_product(int L, int I, int K, int J, const typename SR::V *_
           if (cv >= 0x1p-100f) {
             contrib = XF::norm(cv, ra + cb);
           } else {
-            // Alignment lost the significant terms: exact exponent-aligned sum.
-            int km = XF::kmin();
-            for (int c = 0; c < T; ++c) km = max(km, XF::key(sm.u.p.A[a][c], sm.u.p.B[c][b]));
-            float s = 0.0f;
-            for (int c = 0; c < T; ++c) s += XF::contrib(sm.u.p.A[a][c], sm.u.p.B[c][b], km);
-            contrib = XF::finish(km, s);
+            contrib = exact_cell_xf<T>(sm.u.p.A, sm.u.p.B, a, b);
           }
         }
         sm.cA[a][b] = SR::add(sm.cA[a][b], contrib);
       }
     }
   } else {
-    for (int idx = tid; idx < T * T; idx += NTHR) {
+    for (int idx = tid; idx < T * T; idx += T * T) {
       int a = idx / T, b = idx % T;
       typename SR::K km = SR::kmin();
       for (int c = 0; c < T; ++c) km = SR::kmax(km, SR::key(sm.u.p.A[a][c], sm.u.p.B[c][b]));
@@ -242,7 +251,7 @@ __device__ void tile_product(int L, int I, int K, int J, const typename SR::V *_
 
 // One launch of the inside schedule (see the file header).
 template <class SR, int T>
-__global__ __launch_bounds__(NTHR) void inside_step_kernel(
+__global__ __launch_bounds__(T * T) void inside_step_kernel(
     int L, int NT, int t, typename SR::V *__restrict__ Lr, typename SR::V *__restrict__ Lc,
     typename SR::V *__restrict__ Sr, typename SR::V *__restrict__ Sc, typename SR::V *__restrict__ F,
     typename SR::V *__restrict__ acc, const typename SR::V *__restrict__ ptv, Rules<SR> r) {
@@ -270,12 +279,12 @@ __global__ __launch_bounds__(NTHR) void inside_step_kernel(
   const int IT = I * T, JT = J * T;
   // ---- bulk accumulator of the target tile
   if (t >= 2 || !fin) {
-    for (int idx = tid; idx < T * T; idx += NTHR) {
+    for (int idx = tid; idx < T * T; idx += T * T) {
       int a = idx / T, b = idx % T, j = IT + a, y = JT + b;
       sm.cA[a][b] = (j < L && y < L) ? acc[(size_t)j * L + y] : SR::zero();
     }
   } else {
-    for (int idx = tid; idx < T * T; idx += NTHR) sm.cA[idx / T][idx % T] = SR::zero();
+    for (int idx = tid; idx < T * T; idx += T * T) sm.cA[idx / T][idx % T] = SR::zero();
   }
   __syncthreads();
   // ---- split-point products that became ready with tile-diagonal t-1
@@ -285,21 +294,21 @@ __global__ __launch_bounds__(NTHR) void inside_step_kernel(
     if (K2 != K1) tile_product<SR, T>(L, I, K2, J, Lr, Sc, sm);
   }
   if (!fin) {
-    for (int idx = tid; idx < T * T; idx += NTHR) {
+    for (int idx = tid; idx < T * T; idx += T * T) {
       int a = idx / T, b = idx % T, j = IT + a, y = JT + b;
       if (j < L && y < L) acc[(size_t)j * L + y] = sm.cA[a][b];
     }
     return;
   }
   // ---- finalise tile (I, J): stage operands
-  for (int idx = tid; idx < T * T; idx += NTHR) {
+  for (int idx = tid; idx < T * T; idx += T * T) {
     int a = idx / T, b = idx % T;
     int j = IT + a, y = JT + b;
     int jd = IT + b;  // column index inside the diagonal tile (I, I)
     sm.Ld[a][b] = (j < L && jd < L && jd >= j) ? Lr[(size_t)j * L + jd] : SR::zero();
     sm.cP[a][b] = (j < L && y < L) ? ptv[(size_t)j * L + y] : SR::zero();
   }
-  for (int idx = tid; idx < T * T; idx += NTHR) {
+  for (int idx = tid; idx < T * T; idx += T * T) {
     int b = idx / T, c = idx % T;  // S[JT+c, JT+b], c fastest along the column
     int y = JT + b, rr = JT + c;
     sm.Sd[c][b] = (y < L && rr <= y) ? Sc[(size_t)y * L + rr] : SR::zero();
@@ -313,7 +322,7 @@ __global__ __launch_bounds__(NTHR) void inside_step_kernel(
     sm.Fleft[b] = (t >= 1 && ja < L && yl >= 0 && ja <= yl) ? F[(size_t)ja * L + yl] : SR::zero();
   }
   if (t == 0) {
-    for (int idx = tid; idx < T * T; idx += NTHR) {
+    for (int idx = tid; idx < T * T; idx += T * T) {
       int a = idx / T, b = idx % T;
       int j = IT + a;
       bool d = (a == b) && j < L;
@@ -323,90 +332,84 @@ __global__ __launch_bounds__(NTHR) void inside_step_kernel(
     }
   }
   __syncthreads();
-  constexpr int G = NTHR / T;
-  const int ci = tid / G, sub = tid % G;
+  // One thread per cell.  A cell's split-point sum is accumulated online: at
+  // step s it adds the (at most two) terms whose newest operand was finalised at
+  // step s-1, and it finalises itself at its own step.
+  const int a = tid / T, b = tid % T;
+  const int j = IT + a, y = JT + b;
+  const bool mine = j < L && y < L && (t > 0 || a < b);
+  const typename SR::FinRules rk = SR::fin_rules(r.p[6], r.lin[0], r.lin[2], r.lin[3], r.lin[4], r.lin[6]);
+  typename SR::Acc ac = SR::acc(sm.cA[a][b]);
   if (t == 0) {
-    // Diagonal tile: span order inside the tile.
-    for (int d = 1; d < T; ++d) {
-      const int a = ci, b = ci + d;
-      const bool valid = (ci < T - d) && (IT + b < L);
-      typename SR::K km = SR::kmin();
-      if (valid)
-        for (int c = a + sub; c < b; c += G)
-          km = SR::kmax(km, SR::key(sm.u.w.cL[a][c], sm.u.w.cS[c + 1][b]));
-      km = grp_kmax<SR>(km, G);
-      typename SR::S s = 0;
-      if (valid && km != SR::kmin())
-        for (int c = a + sub; c < b; c += G) s += SR::contrib(sm.u.w.cL[a][c], sm.u.w.cS[c + 1][b], km);
-      s = grp_sum<SR>(s, G);
-      if (valid && sub == 0) {
-        V bif = SR::finish(km, s);
-        V f = d >= 2 ? sm.u.w.cF[a + 1][b - 1] : SR::zero();
-        V pf = SR::mul(sm.cP[a][b], f);
-        sm.u.w.cS[a][b] = SR::add(SR::mulc(bif, r.p[0]), SR::mulc(pf, r.p[2]));
-        sm.u.w.cF[a][b] = SR::add(SR::mulc(bif, r.p[4]), SR::mulc(pf, r.p[3]));
-        sm.u.w.cL[a][b] = SR::mulc(pf, r.p[6]);
+    const int own = b - a;
+    for (int st = 1; st < T; ++st) {
+      const bool act = mine && st <= own, fin = mine && st == own;
+      V l1 = SR::zero(), s1 = SR::zero(), l2 = SR::zero(), s2 = SR::zero(), f = SR::zero(), pt = SR::zero();
+      if (act) {
+        const bool ok = own - st <= st - 1;
+        const int c1 = a + st - 1, c2 = b - st;
+        if (ok && c1 <= b - 1) {
+          l1 = sm.u.w.cL[a][c1];
+          s1 = sm.u.w.cS[c1 + 1][b];
+        }
+        if (ok && c2 >= a && c2 != c1) {
+          l2 = sm.u.w.cL[a][c2];
+          s2 = sm.u.w.cS[c2 + 1][b];
+        }
+      }
+      if (fin) {
+        pt = sm.cP[a][b];
+        if (own >= 2) f = sm.u.w.cF[a + 1][b - 1];
+      }
+      if (act) SR::acc_add2(ac, SR::mul_raw(l1, s1), SR::mul_raw(l2, s2));
+      if (fin) {
+        V vs, vf, vl;
+        SR::finalize(ac, SR::mul_raw(pt, f), rk, vs, vf, vl);
+        sm.u.w.cS[a][b] = vs;
+        sm.u.w.cF[a][b] = vf;
+        sm.u.w.cL[a][b] = vl;
       }
       __syncthreads();
     }
   } else {
-    // Off-diagonal tile: anti-diagonals from the bottom-left corner.
+    const int own = b + T - 1 - a;
+    // split point h = IT+T-1: S[(I+1)T, y] lies in the tile below.
+    if (mine) SR::acc_add2(ac, SR::mul_raw(sm.Ld[a][T - 1], sm.Sb[b]), SR::zero());
+    // F[j+1, y-1] for the pair term: inside this tile, or on its bottom / left border.
+    const bool fin_in = a + 1 < T && b >= 1;
     for (int st = 0; st <= 2 * T - 2; ++st) {
-      const int a0 = st < T ? T - 1 - st : 0;
-      const int ncell = st < T ? st + 1 : 2 * T - 1 - st;
-      const int a = a0 + ci, b = st - (T - 1) + a;
-      const int j = IT + a, y = JT + b;
-      const bool valid = ci < ncell && j < L && y < L;
-      const int na = T - a, n = st + 1;
-      typename SR::K km = SR::kmin();
-      if (valid)
-        for (int q = sub; q < n; q += G) {
-          V lv, sv;
-          if (q < na) {
-            int c = a + q;
-            lv = sm.Ld[a][c];
-            sv = (c + 1 < T) ? sm.u.w.cS[c + 1][b] : sm.Sb[b];
-          } else {
-            int c = q - na;
-            lv = sm.u.w.cL[a][c];
-            sv = sm.Sd[c + 1][b];
-          }
-          km = SR::kmax(km, SR::key(lv, sv));
+      const bool act = mine && st >= 1 && st <= own, fin = mine && st == own;
+      V l1 = SR::zero(), s1 = SR::zero(), l2 = SR::zero(), s2 = SR::zero(), f = SR::zero(), pt = SR::zero();
+      if (act) {
+        const int ap = b + T - st;  // column producer (ap, b), finalised at step st-1
+        if (ap > a && ap < T) {
+          l1 = sm.Ld[a][ap - 1];
+          s1 = sm.u.w.cS[ap][b];
         }
-      km = grp_kmax<SR>(km, G);
-      typename SR::S s = 0;
-      if (valid && km != SR::kmin())
-        for (int q = sub; q < n; q += G) {
-          V lv, sv;
-          if (q < na) {
-            int c = a + q;
-            lv = sm.Ld[a][c];
-            sv = (c + 1 < T) ? sm.u.w.cS[c + 1][b] : sm.Sb[b];
-          } else {
-            int c = q - na;
-            lv = sm.u.w.cL[a][c];
-            sv = sm.Sd[c + 1][b];
-          }
-          s += SR::contrib(lv, sv, km);
+        const int bp = st - T + a;  // row producer (a, bp), finalised at step st-1
+        if (bp >= 0 && bp < b) {
+          l2 = sm.u.w.cL[a][bp];
+          s2 = sm.Sd[bp + 1][b];
         }
-      s = grp_sum<SR>(s, G);
-      if (valid && sub == 0) {
-        V bif = SR::add(sm.cA[a][b], SR::finish(km, s));
-        V f;
-        if (a + 1 < T && b >= 1) f = sm.u.w.cF[a + 1][b - 1];
-        else if (a + 1 == T) f = sm.Fbot[b];
-        else f = sm.Fleft[a];
+      }
+      if (fin) {
+        pt = sm.cP[a][b];
+        f = fin_in ? sm.u.w.cF[a + 1][b - 1] : (a + 1 == T ? sm.Fbot[b] : sm.Fleft[a]);
         if (j + 1 > y - 1) f = SR::zero();
-        V pf = SR::mul(sm.cP[a][b], f);
-        sm.u.w.cS[a][b] = SR::add(SR::mulc(bif, r.p[0]), SR::mulc(pf, r.p[2]));
-        sm.u.w.cF[a][b] = SR::add(SR::mulc(bif, r.p[4]), SR::mulc(pf, r.p[3]));
-        sm.u.w.cL[a][b] = SR::mulc(pf, r.p[6]);
+      }
+      if (act) SR::acc_add2(ac, SR::mul_raw(l1, s1), SR::mul_raw(l2, s2));
+      if (fin) {
+        V vs, vf, vl;
+        SR::finalize(ac, SR::mul_raw(pt, f), rk, vs, vf, vl);
+        sm.u.w.cS[a][b] = vs;
+        sm.u.w.cF[a][b] = vf;
+        sm.u.w.cL[a][b] = vl;
       }
       __syncthreads();
     }
   }
   // ---- write the tile back (row-major copies with b fastest, column-major with a fastest)
-  for (int idx = tid; idx < T * T; idx += NTHR) {
+  for (int idx = tid; idx < T * T; idx += T * T) {
     int a = idx / T, b = idx % T, j = IT + a, y = JT + b;
     if (j < L && y < L && j < y) {
       size_t rc = (size_t)j * L + y;
@@ -415,7 +418,7 @@ __global__ __launch_bounds__(NTHR) void inside_step_kernel(
       Lr[rc] = sm.u.w.cL[a][b];
     }
   }
-  for (int idx = tid; idx < T * T; idx += NTHR) {
+  for (int idx = tid; idx < T * T; idx += T * T) {
     int b = idx / T, a = idx % T, j = IT + a, y = JT + b;
     if (j < L && y < L && j < y) {
       size_t cc = (size_t)y * L + j;
@@ -429,7 +432,7 @@ __global__ __launch_bounds__(NTHR) void inside_step_kernel(
 // Outside, one anti-diagonal b (descending), G lanes per cell.
 // ---------------------------------------------------------------------------
 template <class SR>
-__global__ __launch_bounds__(NTHR) void outside_diag_kernel(
+__global__ __launch_bounds__(256) void outside_diag_kernel(
     int L, int b, int G, const typename SR::V *__restrict__ iSr, const typename SR::V *__restrict__ iLc,
     typename SR::V *__restrict__ oSr, typename SR::V *__restrict__ oSc, typename SR::V *__restrict__ oFr,
     typename SR::V *__restrict__ oFc, typename SR::V *__restrict__ oL, const typename SR::V *__restrict__ ptv,
@@ -699,13 +702,19 @@ void inside_run(hipStream_t s, InsideWork &w, int L, double B) {
   hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, w.u.as<double>(),
                      w.lam.as<double>(), B, lp[1], lp[5], w.Lr.as<V>(), w.Lc.as<V>(), w.Sr.as<V>(),
                      w.Sc.as<V>(), w.F.as<V>(), w.acc.as<V>());
-  const int NT = (L + TILE - 1) / TILE;
+  const int TE = w.tile == 32 ? 32 : 16;
+  const int NT = (L + TE - 1) / TE;
   for (int t = 0; t < NT; ++t) {
     int items = NT - t;
     for (int dist = t + 1; dist <= std::min(2 * t - 2, NT - 1); ++dist) items += NT - dist;
-    hipLaunchKernelGGL((inside_step_kernel<SR, TILE>), dim3(items), dim3(NTHR), 0, s, L, NT, t, w.Lr.as<V>(),
-                       w.Lc.as<V>(), w.Sr.as<V>(), w.Sc.as<V>(), w.F.as<V>(), w.acc.as<V>(),
-                       (const V *)w.ptv.as<V>(), r);
+    if (TE == 32)
+      hipLaunchKernelGGL((inside_step_kernel<SR, 32>), dim3(items), dim3(32 * 32), 0, s, L, NT, t, w.Lr.as<V>(),
+                         w.Lc.as<V>(), w.Sr.as<V>(), w.Sc.as<V>(), w.F.as<V>(), w.acc.as<V>(),
+                         (const V *)w.ptv.as<V>(), r);
+    else
+      hipLaunchKernelGGL((inside_step_kernel<SR, 16>), dim3(items), dim3(16 * 16), 0, s, L, NT, t, w.Lr.as<V>(),
+                         w.Lc.as<V>(), w.Sr.as<V>(), w.Sc.as<V>(), w.F.as<V>(), w.acc.as<V>(),
+                         (const V *)w.ptv.as<V>(), r);
   }
   hipLaunchKernelGGL(zlog_kernel<SR>, dim3(1), dim3(1), 0, s, L, (const V *)w.Sc.as<V>(),
                      (const double *)w.C.as<double>(), w.red.as<double>());

@@ -41,12 +41,57 @@ struct LogSR {
     return (a == -HUGE_VAL || b == -HUGE_VAL) ? -HUGE_VAL : a + b;
   }
   static __device__ __forceinline__ V mulc(V a, C c) { return a == -HUGE_VAL ? a : a + c; }
+  static __device__ __forceinline__ V mul_raw(V a, V b) { return a + b; }
   // natural-log value of a cell (for export); baseline added by the caller
   static __device__ __forceinline__ double to_log(V a) { return a; }
   static __device__ __forceinline__ V from_log(double x) { return x; }
   static __host__ __device__ inline C rule(double logp) { return logp; }
   static __device__ __forceinline__ K shfl_k(K k, int o) { return __shfl_xor(k, o); }
   static __device__ __forceinline__ S shfl_s(S s, int o) { return __shfl_xor(s, o); }
+  // Online accumulator of a split-point sum: value = m + log(s), one exp per term.
+  struct Acc {
+    double m, s;
+  };
+  static __device__ __forceinline__ Acc acc(V v) { return Acc{v, v == -HUGE_VAL ? 0.0 : 1.0}; }
+  static __device__ __forceinline__ void acc_add(Acc &a, V t) {
+    if (t == -HUGE_VAL) return;
+    if (t > a.m) {
+      a.s = (a.s == 0.0 ? 0.0 : a.s * exp(a.m - t)) + 1.0;
+      a.m = t;
+    } else {
+      a.s += exp(t - a.m);
+    }
+  }
+  // Add two terms at once; the three exponentials are independent.
+  static __device__ __forceinline__ void acc_add2(Acc &a, V t1, V t2) {
+    const double m = fmax(a.m, fmax(t1, t2));
+    if (m == -HUGE_VAL) return;
+    a.s = a.s * exp(a.m - m) + exp(t1 - m) + exp(t2 - m);
+    a.m = m;
+  }
+  // Rule weights used when a cell is finalised: log p6 and linear p0, p2, p3, p4.
+  struct FinRules {
+    double l6, p0, p2, p3, p4;
+  };
+  static __device__ __forceinline__ FinRules fin_rules(double lp6, double p0, double p2, double p3, double p4,
+                                                       double) {
+    return FinRules{lp6, p0, p2, p3, p4};
+  }
+  // S = log(p0 e^bif + p2 e^pf), F = log(p4 e^bif + p3 e^pf), L = log p6 + pf, with
+  // bif = a.m + log(a.s).
+  static __device__ __forceinline__ void finalize(const Acc &a, V pf, const FinRules &r, V &S, V &F, V &Lv) {
+    const bool hb = a.s > 0.0, hp = pf != -HUGE_VAL;
+    Lv = hp ? pf + r.l6 : -HUGE_VAL;
+    if (!hb && !hp) {
+      S = F = -HUGE_VAL;
+      return;
+    }
+    double m = hb ? (hp ? fmax(a.m, pf) : a.m) : pf;
+    double ex = hb ? a.s * exp(a.m - m) : 0.0;
+    double ey = hp ? exp(pf - m) : 0.0;
+    S = m + log(r.p0 * ex + r.p2 * ey);
+    F = m + log(r.p4 * ex + r.p3 * ey);
+  }
 };
 
 struct XFv {
@@ -111,6 +156,36 @@ struct XF {
   static __host__ __device__ inline C rule(double logp) { return (C)exp(logp); }
   static __device__ __forceinline__ K shfl_k(K k, int o) { return __shfl_xor(k, o); }
   static __device__ __forceinline__ S shfl_s(S s, int o) { return __shfl_xor(s, o); }
+  // Unnormalised accumulator: terms are raw products (mantissa in [0.25, 1)),
+  // aligned to the larger exponent and added; normalised once when finalised.
+  typedef V Acc;
+  static __device__ __forceinline__ Acc acc(V v) { return v; }
+  static __device__ __forceinline__ V mul_raw(V a, V b) { return V{a.m * b.m, a.e + b.e}; }
+  static __device__ __forceinline__ void acc_add(Acc &a, V t) {
+    const int ne = max(a.e, t.e);
+    a.m = ldexpf(a.m, max(a.e - ne, -200)) + ldexpf(t.m, max(t.e - ne, -200));
+    a.e = ne;
+  }
+  struct FinRules {
+    float p0, p2, p3, p4, p6;
+  };
+  static __device__ __forceinline__ FinRules fin_rules(float, float p0, float p2, float p3, float p4, float p6) {
+    return FinRules{p0, p2, p3, p4, p6};
+  }
+  static __device__ __forceinline__ void acc_add2(Acc &a, V t1, V t2) {
+    const int ne = max(a.e, max(t1.e, t2.e));
+    a.m = ldexpf(a.m, max(a.e - ne, -200)) + ldexpf(t1.m, max(t1.e - ne, -200)) +
+          ldexpf(t2.m, max(t2.e - ne, -200));
+    a.e = ne;
+  }
+  static __device__ __forceinline__ void finalize(const Acc &bif, V pf, const FinRules &r, V &S, V &F, V &Lv) {
+    // pf is a raw product; bif an unnormalised sum.
+    const int ne = max(bif.e, pf.e);
+    const float xb = ldexpf(bif.m, max(bif.e - ne, -200)), xp = ldexpf(pf.m, max(pf.e - ne, -200));
+    S = norm(r.p0 * xb + r.p2 * xp, ne);
+    F = norm(r.p4 * xb + r.p3 * xp, ne);
+    Lv = norm(r.p6 * pf.m, pf.e);
+  }
 };
 
 }  // namespace messi
