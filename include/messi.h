@@ -33,6 +33,7 @@ extern "C" {
 
 typedef struct messi_ctx messi_ctx;
 typedef struct messi_dataset messi_dataset;
+typedef struct messi_batch messi_batch;
 
 /* Context and error handling.  Replaces the per-call CuContext/CuModuleFile setup
  * of src/FelsensteinCUDA.jl:240-253 and src/InsideCUDA.jl:96-104. */
@@ -41,6 +42,8 @@ void messi_ctx_destroy(messi_ctx *ctx);
 const char *messi_last_error(const messi_ctx *ctx);
 /* Record per-phase device times (HIP events) for messi_last_timings. */
 int messi_set_profiling(messi_ctx *ctx, int on);
+/* Tile edge (16 or 32) of the inside schedule used by later calls on this context. */
+int messi_set_tile(messi_ctx *ctx, int tile);
 /* Checks the MFMA operand layouts the kernels rely on against a CPU product. */
 int messi_selftest(messi_ctx *ctx);
 
@@ -105,6 +108,21 @@ int messi_posterior(messi_dataset *ds, int K4, const double *P4, const double *p
                     const double *logw4, int K16, const double *P16, const double *pi16,
                     const double *logw16, int prec, double *logZ, double *unpaired,
                     double *pairprob);
+
+/* Batch of members, each a dataset (the same dataset may appear several times,
+ * e.g. one member per MCMC chain or proposal).  messi_batch_eval evaluates all
+ * members together: each tree level and each tile-diagonal of the inside
+ * schedule is one launch for every member.  Inputs are member-major: member m's
+ * transition matrices are K x nnodes_m x dim x dim (column-major per matrix) and
+ * follow member m-1's; pi is n x K x dim, logw is n x K; logZ receives n values.
+ * No counterpart in the reference (it evaluates one parameter vector per call,
+ * src/FARCE.jl:2322-2345); the per-member result equals messi_eval's. */
+int messi_batch_create(messi_ctx *ctx, int n, messi_dataset *const *members, messi_batch **out);
+void messi_batch_destroy(messi_batch *b);
+int messi_batch_eval(messi_batch *b, int K4, const double *P4, const double *pi4, const double *logw4,
+                     int K16, const double *P16, const double *pi16, const double *logw16, double B,
+                     int prec, double *logZ);
+int messi_batch_timings(const messi_batch *b, double *t);
 
 /* Device time in milliseconds of the phases of the last messi_eval /
  * messi_posterior call on this dataset: t[0] site, t[1] pair, t[2] inside,

@@ -1,7 +1,9 @@
 // C-ABI of libmessi: context, dataset (pattern dedup and pair index lists built
-// natively at creation), and the likelihood entry points.
+// natively at creation), batches of (dataset, parameter set) members, and the
+// likelihood entry points.  Single-dataset calls run as a batch of one.
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <memory>
 
 #include "common.h"
@@ -68,24 +70,42 @@ struct messi_dataset {
   int nseq = 0, L = 0, nn = 0, maxbp = 0, H = 0;
   std::vector<int> ch, seq, height, npat;
   std::vector<std::vector<int>> pid, rep;
-  std::vector<int> site_lv, pair_lv;  // item offsets per level (H+1 entries)
+  // Node descriptors with dataset-local offsets (member 0, local node ids) and the
+  // level work items (index h-1 holds height h).
+  std::vector<NodeDesc> site_nodes, pair_nodes;
+  std::vector<std::vector<int2>> site_items, pair_items;
   int64_t site_entries = 0, pair_entries = 0, U_root = 0, sumU = 0, sumN = 0, inrange = 0;
-  DevBuf d_site_nodes, d_site_idx, d_site_leaf, d_site_items;
-  DevBuf d_pair_nodes, d_pair_idx, d_pair_leaf, d_pair_items;
+  int64_t site_idx_n = 0, pair_idx_n = 0, site_leaf_n = 0, pair_leaf_n = 0;
+  DevBuf d_site_idx, d_site_leaf, d_pair_idx, d_pair_leaf;
   DevBuf d_pid0, d_T0, d_mask;
   bool has_mask = false;
-  DevBuf site_part, site_expo, site_acc;
-  DevBuf pair_part, pair_expo, pair_acc, pair_comp;
-  DevBuf dP, dpi, dtmp;
+  messi_batch *self = nullptr;  // batch of one, created on first use
+  ~messi_dataset();
+};
+
+struct messi_batch {
+  messi_ctx *ctx = nullptr;
+  int n = 0, Hmax = 0;
+  std::vector<messi_dataset *> mem;
+  bool owns_arrays = false;
+  DevBuf c_site_idx, c_site_leaf, c_pair_idx, c_pair_leaf;  // concatenation when datasets differ
+  const int *site_idx = nullptr, *pair_idx = nullptr;
+  const uint8_t *site_leaf = nullptr;
+  const uint16_t *pair_leaf = nullptr;
+  DevBuf site_nodes, pair_nodes, site_items, pair_items;
+  std::vector<int> site_lv, pair_lv;
+  std::vector<int64_t> node_base, site_root, pair_root;
+  int64_t nn_total = 0, site_entries = 0, pair_entries = 0, site_root_total = 0, pair_root_total = 0;
+  int Lmax = 0;
+  DevBuf site_part, site_expo, site_acc, pair_part, pair_expo, pair_acc, pair_comp;
+  DevBuf dP, dpi, dlogw, dexp, douts;
+  std::vector<std::unique_ptr<InsideWork>> w64, w32;
+  InsideBatch ib;
   Events ev;
   double t[5] = {0, 0, 0, 0, 0};
-  size_t dev_bytes() const {
-    return d_site_nodes.bytes + d_site_idx.bytes + d_site_leaf.bytes + d_site_items.bytes +
-           d_pair_nodes.bytes + d_pair_idx.bytes + d_pair_leaf.bytes + d_pair_items.bytes +
-           d_pid0.bytes + d_T0.bytes + d_mask.bytes + site_part.bytes + site_expo.bytes +
-           site_acc.bytes + pair_part.bytes + pair_expo.bytes + pair_acc.bytes + pair_comp.bytes;
-  }
 };
+
+messi_dataset::~messi_dataset() { delete self; }
 
 namespace {
 
@@ -215,21 +235,18 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
       d.leaf_r = isleaf(r);
       d.node_l = l;
       d.node_r = r;
+      d.root = v == 0;
       nodes[v] = d;
     }
-    std::vector<int2> items;
-    ds->site_lv.assign(H + 1, 0);
-    for (int h = 1; h <= H; ++h) {
-      ds->site_lv[h - 1] = (int)items.size();
-      for (int v = 0; v < n; ++v)
-        if (!isleaf(v) && ds->height[v] == h)
-          for (int e = 0; e < ds->npat[v]; e += 256) items.push_back(make_int2(v, e));
-    }
-    ds->site_lv[H] = (int)items.size();
-    upload(ds->d_site_nodes, nodes, s);
+    ds->site_items.assign(H, {});
+    for (int v = 0; v < n; ++v)
+      if (!isleaf(v))
+        for (int e = 0; e < ds->npat[v]; e += 256) ds->site_items[ds->height[v] - 1].push_back(make_int2(v, e));
+    ds->site_nodes = nodes;
+    ds->site_idx_n = (int64_t)idx.size();
+    ds->site_leaf_n = (int64_t)leaf.size();
     upload(ds->d_site_idx, idx, s);
     upload(ds->d_site_leaf, leaf, s);
-    upload(ds->d_site_items, items, s);
   }
 
   // ---- pair (16-state) structures: getindexdict semantics ----
@@ -278,7 +295,6 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
         }
       }
       if (v != 0) {
-        // Only leaves keep their representative pairs (for their state codes).
         std::vector<int>().swap(ex[v]);
         std::vector<int>().swap(ey[v]);
       }
@@ -323,24 +339,21 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
       d.leaf_r = isleaf(r);
       d.node_l = l;
       d.node_r = r;
+      d.root = v == 0;
       nodes[v] = d;
       std::vector<int>().swap(il[v]);
       std::vector<int>().swap(ir[v]);
     }
     const int step = 16 * PAIR_NB;
-    std::vector<int2> items;
-    ds->pair_lv.assign(H + 1, 0);
-    for (int h = 1; h <= H; ++h) {
-      ds->pair_lv[h - 1] = (int)items.size();
-      for (int v = 0; v < n; ++v)
-        if (!isleaf(v) && ds->height[v] == h)
-          for (int64_t e = 0; e < U[v]; e += step) items.push_back(make_int2(v, (int)e));
-    }
-    ds->pair_lv[H] = (int)items.size();
-    upload(ds->d_pair_nodes, nodes, s);
+    ds->pair_items.assign(H, {});
+    for (int v = 0; v < n; ++v)
+      if (!isleaf(v))
+        for (int64_t e = 0; e < U[v]; e += step) ds->pair_items[ds->height[v] - 1].push_back(make_int2(v, (int)e));
+    ds->pair_nodes = nodes;
+    ds->pair_idx_n = (int64_t)idx.size();
+    ds->pair_leaf_n = (int64_t)leaf.size();
     upload(ds->d_pair_idx, idx, s);
     upload(ds->d_pair_leaf, leaf, s);
-    upload(ds->d_pair_items, items, s);
     upload(ds->d_T0, T0, s);
   }
   upload(ds->d_pid0, ds->pid[0], s);
@@ -352,119 +365,317 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
   MESSI_HIP(hipStreamSynchronize(s));
 }
 
+// Combine the members' descriptors into one launch structure.
+void build_batch(messi_batch *b) {
+  hipStream_t s = b->ctx->stream;
+  const int n = b->n;
+  std::map<messi_dataset *, int> distinct;
+  std::vector<messi_dataset *> dlist;
+  for (auto *d : b->mem)
+    if (!distinct.count(d)) {
+      distinct[d] = (int)dlist.size();
+      dlist.push_back(d);
+    }
+  std::vector<int64_t> sidx_base(dlist.size(), 0), pidx_base(dlist.size(), 0), sleaf_base(dlist.size(), 0),
+      pleaf_base(dlist.size(), 0);
+  if (dlist.size() == 1) {
+    b->site_idx = dlist[0]->d_site_idx.as<int>();
+    b->pair_idx = dlist[0]->d_pair_idx.as<int>();
+    b->site_leaf = dlist[0]->d_site_leaf.as<uint8_t>();
+    b->pair_leaf = dlist[0]->d_pair_leaf.as<uint16_t>();
+  } else {
+    int64_t a = 0, c = 0, e = 0, f = 0;
+    for (size_t i = 0; i < dlist.size(); ++i) {
+      sidx_base[i] = a;
+      pidx_base[i] = c;
+      sleaf_base[i] = e;
+      pleaf_base[i] = f;
+      a += dlist[i]->site_idx_n;
+      c += dlist[i]->pair_idx_n;
+      e += dlist[i]->site_leaf_n;
+      f += dlist[i]->pair_leaf_n;
+    }
+    b->c_site_idx.ensure(std::max<int64_t>(1, a) * sizeof(int));
+    b->c_pair_idx.ensure(std::max<int64_t>(1, c) * sizeof(int));
+    b->c_site_leaf.ensure(std::max<int64_t>(1, e));
+    b->c_pair_leaf.ensure(std::max<int64_t>(1, f) * sizeof(uint16_t));
+    for (size_t i = 0; i < dlist.size(); ++i) {
+      messi_dataset *d = dlist[i];
+      if (d->site_idx_n)
+        MESSI_HIP(hipMemcpyAsync(b->c_site_idx.as<int>() + sidx_base[i], d->d_site_idx.p,
+                                 d->site_idx_n * sizeof(int), hipMemcpyDeviceToDevice, s));
+      if (d->pair_idx_n)
+        MESSI_HIP(hipMemcpyAsync(b->c_pair_idx.as<int>() + pidx_base[i], d->d_pair_idx.p,
+                                 d->pair_idx_n * sizeof(int), hipMemcpyDeviceToDevice, s));
+      if (d->site_leaf_n)
+        MESSI_HIP(hipMemcpyAsync(b->c_site_leaf.as<uint8_t>() + sleaf_base[i], d->d_site_leaf.p, d->site_leaf_n,
+                                 hipMemcpyDeviceToDevice, s));
+      if (d->pair_leaf_n)
+        MESSI_HIP(hipMemcpyAsync(b->c_pair_leaf.as<uint16_t>() + pleaf_base[i], d->d_pair_leaf.p,
+                                 d->pair_leaf_n * sizeof(uint16_t), hipMemcpyDeviceToDevice, s));
+    }
+    b->site_idx = b->c_site_idx.as<int>();
+    b->pair_idx = b->c_pair_idx.as<int>();
+    b->site_leaf = b->c_site_leaf.as<uint8_t>();
+    b->pair_leaf = b->c_pair_leaf.as<uint16_t>();
+    b->owns_arrays = true;
+  }
+  b->node_base.assign(n, 0);
+  b->site_root.assign(n, 0);
+  b->pair_root.assign(n, 0);
+  std::vector<int64_t> spart(n), ppart(n);
+  int64_t nb = 0, sp = 0, pp = 0, sr = 0, pr = 0;
+  b->Hmax = 0;
+  b->Lmax = 0;
+  for (int m = 0; m < n; ++m) {
+    messi_dataset *d = b->mem[m];
+    b->node_base[m] = nb;
+    spart[m] = sp;
+    ppart[m] = pp;
+    b->site_root[m] = sr;
+    b->pair_root[m] = pr;
+    nb += d->nn;
+    sp += d->site_entries;
+    pp += d->pair_entries;
+    sr += d->npat[0];
+    pr += d->U_root;
+    b->Hmax = std::max(b->Hmax, d->H);
+    b->Lmax = std::max(b->Lmax, d->L);
+  }
+  b->nn_total = nb;
+  b->site_entries = sp;
+  b->pair_entries = pp;
+  b->site_root_total = sr;
+  b->pair_root_total = pr;
+  std::vector<NodeDesc> sn(nb), pn(nb);
+  std::vector<int2> si, pi;
+  b->site_lv.assign(b->Hmax + 1, 0);
+  b->pair_lv.assign(b->Hmax + 1, 0);
+  for (int m = 0; m < n; ++m) {
+    messi_dataset *d = b->mem[m];
+    const int di = distinct[d];
+    for (int v = 0; v < d->nn; ++v) {
+      if (d->ch[2 * v] < 0) continue;
+      for (int kind = 0; kind < 2; ++kind) {
+        NodeDesc nd = kind ? d->pair_nodes[v] : d->site_nodes[v];
+        const int64_t ib = kind ? pidx_base[di] : sidx_base[di];
+        const int64_t lb = kind ? pleaf_base[di] : sleaf_base[di];
+        const int64_t pb = kind ? ppart[m] : spart[m];
+        nd.idx_l += ib;
+        nd.idx_r += ib;
+        nd.part += pb;
+        nd.cpart_l += nd.leaf_l ? lb : pb;
+        nd.cpart_r += nd.leaf_r ? lb : pb;
+        nd.node_l += (int)b->node_base[m];
+        nd.node_r += (int)b->node_base[m];
+        nd.member = m;
+        nd.out = kind ? b->pair_root[m] : b->site_root[m];
+        (kind ? pn : sn)[b->node_base[m] + v] = nd;
+      }
+    }
+  }
+  for (int h = 1; h <= b->Hmax; ++h) {
+    b->site_lv[h - 1] = (int)si.size();
+    b->pair_lv[h - 1] = (int)pi.size();
+    for (int m = 0; m < n; ++m) {
+      messi_dataset *d = b->mem[m];
+      if (h > d->H) continue;
+      for (int2 it : d->site_items[h - 1]) si.push_back(make_int2(it.x + (int)b->node_base[m], it.y));
+      for (int2 it : d->pair_items[h - 1]) pi.push_back(make_int2(it.x + (int)b->node_base[m], it.y));
+    }
+  }
+  b->site_lv[b->Hmax] = (int)si.size();
+  b->pair_lv[b->Hmax] = (int)pi.size();
+  upload(b->site_nodes, sn, s);
+  upload(b->pair_nodes, pn, s);
+  upload(b->site_items, si, s);
+  upload(b->pair_items, pi, s);
+  b->w64.clear();
+  b->w32.clear();
+  for (int m = 0; m < n; ++m) {
+    b->w64.emplace_back(new InsideWork());
+    b->w32.emplace_back(new InsideWork());
+  }
+  MESSI_HIP(hipStreamSynchronize(s));
+}
+
+messi_batch *self_batch(messi_dataset *ds) {
+  if (!ds->self) {
+    auto *b = new messi_batch();
+    b->ctx = ds->ctx;
+    b->n = 1;
+    b->mem.push_back(ds);
+    try {
+      build_batch(b);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    ds->self = b;
+  }
+  return ds->self;
+}
+
+// Member-major input (member m: K x nn_m x dim^2) -> component-major device arena.
 template <typename T>
-std::vector<T> convert_P(const double *P, size_t n) {
-  std::vector<T> out(n);
-  for (size_t i = 0; i < n; ++i) out[i] = (T)P[i];
-  return out;
+void upload_components(messi_batch *b, int K, int dim2, const double *P, const double *pi, int pidim,
+                       const double *logw) {
+  hipStream_t s = b->ctx->stream;
+  const int n = b->n;
+  std::vector<T> Ph((size_t)K * b->nn_total * dim2);
+  std::vector<double> pih((size_t)K * n * pidim), lwh((size_t)K * n);
+  size_t src = 0;
+  for (int m = 0; m < n; ++m) {
+    const int nn = b->mem[m]->nn;
+    for (int k = 0; k < K; ++k) {
+      T *dst = Ph.data() + ((size_t)k * b->nn_total + b->node_base[m]) * dim2;
+      for (size_t i = 0; i < (size_t)nn * dim2; ++i) dst[i] = (T)P[src + i];
+      src += (size_t)nn * dim2;
+      for (int a = 0; a < pidim; ++a) pih[((size_t)k * n + m) * pidim + a] = pi[((size_t)m * K + k) * pidim + a];
+      lwh[(size_t)k * n + m] = logw[(size_t)m * K + k];
+    }
+  }
+  upload(b->dP, Ph, s);
+  upload(b->dpi, pih, s);
+  upload(b->dlogw, lwh, s);
+}
+
+InsideWork &member_work(messi_batch *b, int m, int prec) {
+  return prec == MESSI_FP32 ? *b->w32[m] : *b->w64[m];
+}
+
+void upload_expand(messi_batch *b, int prec, bool pairs) {
+  std::vector<ExpandDesc> e(b->n);
+  for (int m = 0; m < b->n; ++m) {
+    messi_dataset *d = b->mem[m];
+    InsideWork &w = member_work(b, m, prec);
+    e[m].L = d->L;
+    e[m].maxbp = d->maxbp;
+    e[m].n0 = d->npat[0];
+    e[m].pid0 = d->d_pid0.as<int>();
+    e[m].T0 = d->d_T0.as<int>();
+    e[m].mask = d->has_mask ? d->d_mask.as<uint8_t>() : nullptr;
+    e[m].rootoff = pairs ? b->pair_root[m] : b->site_root[m];
+    e[m].u = w.u.as<double>();
+    e[m].paired = w.paired.as<double>();
+  }
+  upload(b->dexp, e, b->ctx->stream);
 }
 
 template <typename T>
-void run_site(messi_dataset *ds, int K, const double *P4, const double *pi4, const double *logw,
-              double *d_u) {
-  hipStream_t s = ds->ctx->stream;
-  const int nn = ds->nn;
+void run_site(messi_batch *b, int K, const double *P4, const double *pi4, const double *logw, int prec) {
+  hipStream_t s = b->ctx->stream;
   MESSI_REQUIRE(K >= 1, "need at least one site component");
-  std::vector<T> Ph = convert_P<T>(P4, (size_t)K * nn * 16);
-  std::vector<double> pih(pi4, pi4 + 4 * K);
-  upload(ds->dP, Ph, s);
-  upload(ds->dpi, pih, s);
-  ds->site_part.ensure(std::max<int64_t>(1, ds->site_entries) * 4 * sizeof(T));
-  ds->site_expo.ensure(std::max<int64_t>(1, ds->site_entries) * sizeof(int));
-  ds->site_acc.ensure(std::max(1, ds->npat[0]) * sizeof(double));
-  for (int k = 0; k < K; ++k) {
-    for (int h = 1; h <= ds->H; ++h) {
-      int b = ds->site_lv[h - 1], e = ds->site_lv[h];
-      launch_site_level<T>(s, ds->d_site_items.as<int2>() + b, e - b, h == ds->H,
-                           ds->d_site_nodes.as<NodeDesc>(), ds->d_site_idx.as<int>(),
-                           ds->d_site_leaf.as<uint8_t>(), ds->site_part.as<T>(),
-                           ds->site_expo.as<int>(), ds->dP.as<T>() + (size_t)k * nn * 16,
-                           ds->dpi.as<double>() + 4 * k, logw[k], k == 0, ds->site_acc.as<double>());
+  upload_components<T>(b, K, 16, P4, pi4, 4, logw);
+  b->site_part.ensure(std::max<int64_t>(1, b->site_entries) * 4 * sizeof(T));
+  b->site_expo.ensure(std::max<int64_t>(1, b->site_entries) * sizeof(int));
+  b->site_acc.ensure(std::max<int64_t>(1, b->site_root_total) * sizeof(double));
+  for (int m = 0; m < b->n; ++m) member_work(b, m, prec).u.ensure(sizeof(double) * b->mem[m]->L);
+  for (int k = 0; k < K; ++k)
+    for (int h = 1; h <= b->Hmax; ++h) {
+      int lo = b->site_lv[h - 1], hi = b->site_lv[h];
+      launch_site_level<T>(s, b->site_items.as<int2>() + lo, hi - lo, b->site_nodes.as<NodeDesc>(), b->site_idx,
+                           b->site_leaf, b->site_part.as<T>(), b->site_expo.as<int>(),
+                           b->dP.as<T>() + (size_t)k * b->nn_total * 16, b->dpi.as<double>() + (size_t)k * b->n * 4,
+                           b->dlogw.as<double>() + (size_t)k * b->n, k == 0, b->site_acc.as<double>());
     }
-  }
-  launch_site_expand(s, ds->L, ds->d_pid0.as<int>(), ds->site_acc.as<double>(), d_u);
+  upload_expand(b, prec, false);
+  launch_site_expand(s, b->n, b->Lmax, b->dexp.as<ExpandDesc>(), b->site_acc.as<double>());
   MESSI_HIP(hipGetLastError());
 }
 
 template <typename T>
-void run_pair(messi_dataset *ds, int K, const double *P16, const double *pi16, const double *logw,
-              double *d_out, double *h_percomp) {
-  hipStream_t s = ds->ctx->stream;
-  const int nn = ds->nn, L = ds->L;
+void run_pair(messi_batch *b, int K, const double *P16, const double *pi16, const double *logw, int prec,
+              double *h_percomp) {
+  hipStream_t s = b->ctx->stream;
   MESSI_REQUIRE(K >= 1, "need at least one pair component");
-  std::vector<T> Ph = convert_P<T>(P16, (size_t)K * nn * 256);
-  std::vector<double> pih(pi16, pi16 + 16 * K);
-  upload(ds->dP, Ph, s);
-  upload(ds->dpi, pih, s);
-  ds->pair_part.ensure(std::max<int64_t>(1, ds->pair_entries) * 16 * sizeof(T));
-  ds->pair_expo.ensure(std::max<int64_t>(1, ds->pair_entries) * sizeof(int));
-  ds->pair_acc.ensure(std::max<int64_t>(1, ds->U_root) * sizeof(double));
-  ds->pair_comp.ensure(std::max<int64_t>(1, ds->U_root) * sizeof(double));
+  upload_components<T>(b, K, 256, P16, pi16, 16, logw);
+  b->pair_part.ensure(std::max<int64_t>(1, b->pair_entries) * 16 * sizeof(T));
+  b->pair_expo.ensure(std::max<int64_t>(1, b->pair_entries) * sizeof(int));
+  b->pair_acc.ensure(std::max<int64_t>(1, b->pair_root_total) * sizeof(double));
+  b->pair_comp.ensure(std::max<int64_t>(1, b->pair_root_total) * sizeof(double));
+  for (int m = 0; m < b->n; ++m)
+    member_work(b, m, prec).paired.ensure(sizeof(double) * b->mem[m]->L * b->mem[m]->L);
+  upload_expand(b, prec, true);
   for (int k = 0; k < K; ++k) {
-    for (int h = 1; h <= ds->H; ++h) {
-      int b = ds->pair_lv[h - 1], e = ds->pair_lv[h];
-      launch_pair_level<T, PAIR_NB>(s, ds->d_pair_items.as<int2>() + b, e - b, h == ds->H,
-                                    ds->d_pair_nodes.as<NodeDesc>(), ds->d_pair_idx.as<int>(),
-                                    ds->d_pair_leaf.as<uint16_t>(), ds->pair_part.as<T>(),
-                                    ds->pair_expo.as<int>(), ds->dP.as<T>() + (size_t)k * nn * 256,
-                                    ds->dpi.as<double>() + 16 * k, logw[k], k == 0,
-                                    ds->pair_acc.as<double>(), ds->pair_comp.as<double>());
+    for (int h = 1; h <= b->Hmax; ++h) {
+      int lo = b->pair_lv[h - 1], hi = b->pair_lv[h];
+      launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
+                                    b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), b->pair_expo.as<int>(),
+                                    b->dP.as<T>() + (size_t)k * b->nn_total * 256,
+                                    b->dpi.as<double>() + (size_t)k * b->n * 16,
+                                    b->dlogw.as<double>() + (size_t)k * b->n, k == 0, b->pair_acc.as<double>(),
+                                    b->pair_comp.as<double>());
     }
     if (h_percomp) {
-      ds->dtmp.ensure((size_t)L * L * sizeof(double));
-      launch_pair_expand(s, L, ds->maxbp, ds->npat[0], ds->d_pid0.as<int>(), ds->d_T0.as<int>(),
-                         ds->has_mask ? ds->d_mask.as<uint8_t>() : nullptr,
-                         ds->pair_comp.as<double>(), ds->dtmp.as<double>());
-      MESSI_HIP(hipMemcpyAsync(h_percomp + (size_t)k * L * L, ds->dtmp.p, (size_t)L * L * sizeof(double),
-                               hipMemcpyDeviceToHost, s));
+      MESSI_REQUIRE(b->n == 1, "per-component output needs a single member");
+      const int L = b->mem[0]->L;
+      InsideWork &w = member_work(b, 0, prec);
+      w.out3.ensure(sizeof(double) * L * L);
+      double *o = w.out3.as<double>();
+      std::vector<double *> outs(1, o);
+      upload(b->douts, outs, s);
+      launch_pair_expand(s, 1, L, b->dexp.as<ExpandDesc>(), b->pair_comp.as<double>(), b->douts.as<double *>());
+      MESSI_HIP(hipMemcpyAsync(h_percomp + (size_t)k * L * L, o, sizeof(double) * L * L, hipMemcpyDeviceToHost, s));
+      MESSI_HIP(hipStreamSynchronize(s));
     }
   }
-  launch_pair_expand(s, L, ds->maxbp, ds->npat[0], ds->d_pid0.as<int>(), ds->d_T0.as<int>(),
-                     ds->has_mask ? ds->d_mask.as<uint8_t>() : nullptr, ds->pair_acc.as<double>(),
-                     d_out);
+  launch_pair_expand(s, b->n, b->Lmax, b->dexp.as<ExpandDesc>(), b->pair_acc.as<double>(), nullptr);
   MESSI_HIP(hipGetLastError());
 }
 
 InsideWork &work(messi_ctx *ctx, int prec) {
-  MESSI_REQUIRE(prec == MESSI_FP64 || prec == MESSI_FP32, "prec must be MESSI_FP64 or MESSI_FP32");
+  MESSI_REQUIRE(prec == MESSI_FP32 || prec == MESSI_FP64, "prec must be MESSI_FP64 or MESSI_FP32");
   return prec == MESSI_FP32 ? ctx->w32 : ctx->w64;
 }
 
-void record(messi_dataset *ds, int i) {
-  if (ds->ctx->profiling) MESSI_HIP(hipEventRecord(ds->ev.e[i], ds->ctx->stream));
+void check_prec(int prec) {
+  MESSI_REQUIRE(prec == MESSI_FP32 || prec == MESSI_FP64, "prec must be MESSI_FP64 or MESSI_FP32");
 }
 
-void collect(messi_dataset *ds, int upto) {
-  if (!ds->ctx->profiling) return;
-  MESSI_HIP(hipEventSynchronize(ds->ev.e[upto]));
+void record(messi_batch *b, int i) {
+  if (b->ctx->profiling) MESSI_HIP(hipEventRecord(b->ev.e[i], b->ctx->stream));
+}
+
+void collect(messi_batch *b, int upto) {
+  if (!b->ctx->profiling) return;
+  MESSI_HIP(hipEventSynchronize(b->ev.e[upto]));
   for (int i = 0; i < upto; ++i) {
     float ms = 0;
-    MESSI_HIP(hipEventElapsedTime(&ms, ds->ev.e[i], ds->ev.e[i + 1]));
-    ds->t[i] = ms;
+    MESSI_HIP(hipEventElapsedTime(&ms, b->ev.e[i], b->ev.e[i + 1]));
+    b->t[i] = ms;
   }
-  for (int i = upto; i < 5; ++i) ds->t[i] = 0;
+  for (int i = upto; i < 5; ++i) b->t[i] = 0;
 }
 
+// Site + pair + inside for every member; logZ[m] on return.
 template <typename T>
-double eval_impl(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
-                 int K16, const double *P16, const double *pi16, const double *logw16, double B,
-                 bool keep_outside) {
-  messi_ctx *ctx = ds->ctx;
-  InsideWork &w = work(ctx, sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64);
-  const int L = ds->L;
-  inside_alloc<T>(w, L, keep_outside);
-  if (ctx->profiling) ds->ev.make();
-  record(ds, 0);
-  run_site<T>(ds, K4, P4, pi4, logw4, w.u.as<double>());
-  record(ds, 1);
-  run_pair<T>(ds, K16, P16, pi16, logw16, w.paired.as<double>(), nullptr);
-  record(ds, 2);
-  inside_prepare<T>(ctx->stream, w, L, B);
-  inside_run<T>(ctx->stream, w, L, B);
-  record(ds, 3);
-  double z = inside_logZ<T>(ctx->stream, w, L);
-  if (!keep_outside) collect(ds, 3);
-  return z;
+void eval_impl(messi_batch *b, int K4, const double *P4, const double *pi4, const double *logw4, int K16,
+               const double *P16, const double *pi16, const double *logw16, double B, bool keep_outside,
+               double *logZ) {
+  messi_ctx *ctx = b->ctx;
+  hipStream_t s = ctx->stream;
+  const int prec = sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64;
+  std::vector<InsideWork *> ws(b->n);
+  for (int m = 0; m < b->n; ++m) {
+    ws[m] = &member_work(b, m, prec);
+    ws[m]->tile = ctx->tile;
+    inside_alloc<T>(*ws[m], b->mem[m]->L, keep_outside);
+  }
+  if (ctx->profiling) b->ev.make();
+  record(b, 0);
+  run_site<T>(b, K4, P4, pi4, logw4, prec);
+  record(b, 1);
+  run_pair<T>(b, K16, P16, pi16, logw16, prec, nullptr);
+  record(b, 2);
+  inside_batch_run<T>(s, ws.data(), b->n, b->ib, B);
+  record(b, 3);
+  std::vector<double> z(2 * b->n);
+  for (int m = 0; m < b->n; ++m)
+    MESSI_HIP(hipMemcpyAsync(&z[2 * m], ws[m]->red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  MESSI_HIP(hipStreamSynchronize(s));
+  for (int m = 0; m < b->n; ++m) logZ[m] = z[2 * m];
+  if (!keep_outside) collect(b, 3);
 }
 
 }  // namespace
@@ -483,13 +694,8 @@ int messi_ctx_create(int device, messi_ctx **out) {
     ctx->device = device;
     MESSI_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   });
-  if (rc) {
-    // Leave the context to report its error.
-    *out = ctx;
-    return rc;
-  }
   *out = ctx;
-  return 0;
+  return rc;
 }
 
 void messi_ctx_destroy(messi_ctx *ctx) {
@@ -510,6 +716,16 @@ int messi_set_profiling(messi_ctx *ctx, int on) {
   if (!ctx) return 2;
   ctx->profiling = on != 0;
   return 0;
+}
+
+int messi_set_tile(messi_ctx *ctx, int tile) {
+  if (!ctx) return 2;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(tile == 16 || tile == 32, "tile must be 16 or 32");
+    ctx->tile = tile;
+    ctx->w64.tile = tile;
+    ctx->w32.tile = tile;
+  });
 }
 
 int messi_selftest(messi_ctx *ctx) {
@@ -555,45 +771,48 @@ int messi_dataset_stats(const messi_dataset *ds, int64_t *stats) {
   stats[2] = ds->inrange;
   stats[3] = ds->sumN;
   stats[4] = ds->H;
-  stats[5] = (int64_t)ds->dev_bytes();
+  stats[5] = (int64_t)(ds->d_site_idx.bytes + ds->d_site_leaf.bytes + ds->d_pair_idx.bytes +
+                       ds->d_pair_leaf.bytes + ds->d_pid0.bytes + ds->d_T0.bytes + ds->d_mask.bytes);
   return 0;
 }
 
-int messi_site_loglik(messi_dataset *ds, int K, const double *P4, const double *pi4,
-                      const double *logw, int prec, double *u_out) {
+int messi_site_loglik(messi_dataset *ds, int K, const double *P4, const double *pi4, const double *logw, int prec,
+                      double *u_out) {
   if (!ds) return 2;
   messi_ctx *ctx = ds->ctx;
   return guard(ctx, [&] {
     MESSI_REQUIRE(P4 && pi4 && logw && u_out, "null argument");
+    check_prec(prec);
     MESSI_HIP(hipSetDevice(ctx->device));
-    InsideWork &w = work(ctx, prec);
-    w.u.ensure(sizeof(double) * ds->L);
-    if (prec == MESSI_FP32) run_site<float>(ds, K, P4, pi4, logw, w.u.as<double>());
-    else run_site<double>(ds, K, P4, pi4, logw, w.u.as<double>());
+    messi_batch *b = self_batch(ds);
+    if (prec == MESSI_FP32) run_site<float>(b, K, P4, pi4, logw, prec);
+    else run_site<double>(b, K, P4, pi4, logw, prec);
+    InsideWork &w = member_work(b, 0, prec);
     MESSI_HIP(hipMemcpyAsync(u_out, w.u.p, sizeof(double) * ds->L, hipMemcpyDeviceToHost, ctx->stream));
     MESSI_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
 
-int messi_pair_loglik(messi_dataset *ds, int K, const double *P16, const double *pi16,
-                      const double *logw, int prec, double *out, double *percomp) {
+int messi_pair_loglik(messi_dataset *ds, int K, const double *P16, const double *pi16, const double *logw, int prec,
+                      double *out, double *percomp) {
   if (!ds) return 2;
   messi_ctx *ctx = ds->ctx;
   return guard(ctx, [&] {
     MESSI_REQUIRE(P16 && pi16 && logw && out, "null argument");
+    check_prec(prec);
     MESSI_HIP(hipSetDevice(ctx->device));
-    InsideWork &w = work(ctx, prec);
-    const size_t LL = (size_t)ds->L * ds->L;
-    w.paired.ensure(sizeof(double) * LL);
-    if (prec == MESSI_FP32) run_pair<float>(ds, K, P16, pi16, logw, w.paired.as<double>(), percomp);
-    else run_pair<double>(ds, K, P16, pi16, logw, w.paired.as<double>(), percomp);
-    MESSI_HIP(hipMemcpyAsync(out, w.paired.p, sizeof(double) * LL, hipMemcpyDeviceToHost, ctx->stream));
+    messi_batch *b = self_batch(ds);
+    if (prec == MESSI_FP32) run_pair<float>(b, K, P16, pi16, logw, prec, percomp);
+    else run_pair<double>(b, K, P16, pi16, logw, prec, percomp);
+    InsideWork &w = member_work(b, 0, prec);
+    MESSI_HIP(hipMemcpyAsync(out, w.paired.p, sizeof(double) * ds->L * ds->L, hipMemcpyDeviceToHost,
+                             ctx->stream));
     MESSI_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
 
-int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, double B, int zonly,
-                 int prec, double *Z, double *in3) {
+int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, double B, int zonly, int prec,
+                 double *Z, double *in3) {
   if (!ctx) return 2;
   return guard(ctx, [&] {
     MESSI_REQUIRE(L >= 1 && u && paired && Z, "bad argument");
@@ -602,13 +821,13 @@ int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, d
     MESSI_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     InsideWork &w = work(ctx, prec);
+    w.tile = ctx->tile;
     const size_t LL = (size_t)L * L;
     auto go = [&](auto tag) {
       using T = decltype(tag);
       inside_alloc<T>(w, L, false);
       MESSI_HIP(hipMemcpyAsync(w.u.p, u, sizeof(double) * L, hipMemcpyHostToDevice, s));
       MESSI_HIP(hipMemcpyAsync(w.paired.p, paired, sizeof(double) * LL, hipMemcpyHostToDevice, s));
-      inside_prepare<T>(s, w, L, B);
       inside_run<T>(s, w, L, B);
       *Z = inside_logZ<T>(s, w, L);
       if (!zonly) {
@@ -623,8 +842,8 @@ int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, d
   });
 }
 
-int messi_outside(messi_ctx *ctx, int L, const double *in3, const double *u, const double *paired,
-                  int prec, double *out3) {
+int messi_outside(messi_ctx *ctx, int L, const double *in3, const double *u, const double *paired, int prec,
+                  double *out3) {
   if (!ctx) return 2;
   return guard(ctx, [&] {
     MESSI_REQUIRE(L >= 1 && in3 && u && paired && out3, "bad argument");
@@ -651,8 +870,8 @@ int messi_outside(messi_ctx *ctx, int L, const double *in3, const double *u, con
   });
 }
 
-int messi_bpp(messi_ctx *ctx, int L, const double *in3, const double *out3, const double *u,
-              const double *paired, int prec, double *unpaired, double *pairprob) {
+int messi_bpp(messi_ctx *ctx, int L, const double *in3, const double *out3, const double *u, const double *paired,
+              int prec, double *unpaired, double *pairprob) {
   if (!ctx) return 2;
   return guard(ctx, [&] {
     MESSI_REQUIRE(L >= 1 && in3 && out3 && u && paired && pairprob, "bad argument");
@@ -675,8 +894,7 @@ int messi_bpp(messi_ctx *ctx, int L, const double *in3, const double *out3, cons
       MESSI_HIP(hipMemcpyAsync(w.out3.p, out3, sizeof(double) * 3 * LL, hipMemcpyHostToDevice, s));
       outside_import<T>(s, w, L, w.out3.as<double>());
       bpp_run<T>(s, w, L, du.as<double>(), dp.as<double>());
-      if (unpaired)
-        MESSI_HIP(hipMemcpyAsync(unpaired, du.p, sizeof(double) * L, hipMemcpyDeviceToHost, s));
+      if (unpaired) MESSI_HIP(hipMemcpyAsync(unpaired, du.p, sizeof(double) * L, hipMemcpyDeviceToHost, s));
       MESSI_HIP(hipMemcpyAsync(pairprob, dp.p, sizeof(double) * LL, hipMemcpyDeviceToHost, s));
       MESSI_HIP(hipStreamSynchronize(s));
     };
@@ -685,52 +903,49 @@ int messi_bpp(messi_ctx *ctx, int L, const double *in3, const double *out3, cons
   });
 }
 
-int messi_eval(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
-               int K16, const double *P16, const double *pi16, const double *logw16, double B, int prec,
-               double *logZ) {
+int messi_eval(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4, int K16,
+               const double *P16, const double *pi16, const double *logw16, double B, int prec, double *logZ) {
   if (!ds) return 2;
   messi_ctx *ctx = ds->ctx;
   return guard(ctx, [&] {
     MESSI_REQUIRE(P4 && pi4 && logw4 && P16 && pi16 && logw16 && logZ, "null argument");
     MESSI_REQUIRE(B > 0.0, "B must be positive");
+    check_prec(prec);
     MESSI_HIP(hipSetDevice(ctx->device));
-    (void)work(ctx, prec);
-    if (prec == MESSI_FP32)
-      *logZ = eval_impl<float>(ds, K4, P4, pi4, logw4, K16, P16, pi16, logw16, B, false);
-    else
-      *logZ = eval_impl<double>(ds, K4, P4, pi4, logw4, K16, P16, pi16, logw16, B, false);
+    messi_batch *b = self_batch(ds);
+    if (prec == MESSI_FP32) eval_impl<float>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, B, false, logZ);
+    else eval_impl<double>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, B, false, logZ);
   });
 }
 
-int messi_posterior(messi_dataset *ds, int K4, const double *P4, const double *pi4,
-                    const double *logw4, int K16, const double *P16, const double *pi16,
-                    const double *logw16, int prec, double *logZ, double *unpaired,
-                    double *pairprob) {
+int messi_posterior(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4, int K16,
+                    const double *P16, const double *pi16, const double *logw16, int prec, double *logZ,
+                    double *unpaired, double *pairprob) {
   if (!ds) return 2;
   messi_ctx *ctx = ds->ctx;
   return guard(ctx, [&] {
     MESSI_REQUIRE(P4 && pi4 && logw4 && P16 && pi16 && logw16 && logZ, "null argument");
+    check_prec(prec);
     MESSI_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     const int L = ds->L;
     const size_t LL = (size_t)L * L;
+    messi_batch *b = self_batch(ds);
     auto go = [&](auto tag) {
       using T = decltype(tag);
-      InsideWork &w = work(ctx, sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64);
-      *logZ = eval_impl<T>(ds, K4, P4, pi4, logw4, K16, P16, pi16, logw16, 1.0, true);
+      InsideWork &w = member_work(b, 0, sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64);
+      eval_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, 1.0, true, logZ);
       outside_run<T>(s, w, L);
-      record(ds, 4);
-      ds->dtmp.ensure(sizeof(double) * (LL + L));
-      double *dpair = ds->dtmp.as<double>();
+      record(b, 4);
+      w.out3.ensure(sizeof(double) * (LL + L));
+      double *dpair = w.out3.as<double>();
       double *dunp = dpair + LL;
       bpp_run<T>(s, w, L, dunp, dpair);
-      record(ds, 5);
-      if (unpaired)
-        MESSI_HIP(hipMemcpyAsync(unpaired, dunp, sizeof(double) * L, hipMemcpyDeviceToHost, s));
-      if (pairprob)
-        MESSI_HIP(hipMemcpyAsync(pairprob, dpair, sizeof(double) * LL, hipMemcpyDeviceToHost, s));
+      record(b, 5);
+      if (unpaired) MESSI_HIP(hipMemcpyAsync(unpaired, dunp, sizeof(double) * L, hipMemcpyDeviceToHost, s));
+      if (pairprob) MESSI_HIP(hipMemcpyAsync(pairprob, dpair, sizeof(double) * LL, hipMemcpyDeviceToHost, s));
       MESSI_HIP(hipStreamSynchronize(s));
-      collect(ds, 5);
+      collect(b, 5);
     };
     if (prec == MESSI_FP32) go(float{});
     else go(double{});
@@ -739,7 +954,54 @@ int messi_posterior(messi_dataset *ds, int K4, const double *P4, const double *p
 
 int messi_last_timings(const messi_dataset *ds, double *t) {
   if (!ds || !t) return 2;
-  for (int i = 0; i < 5; ++i) t[i] = ds->t[i];
+  for (int i = 0; i < 5; ++i) t[i] = ds->self ? ds->self->t[i] : 0.0;
+  return 0;
+}
+
+int messi_batch_create(messi_ctx *ctx, int n, messi_dataset *const *members, messi_batch **out) {
+  if (!ctx || !out) return 2;
+  *out = nullptr;
+  auto b = std::make_unique<messi_batch>();
+  int rc = guard(ctx, [&] {
+    MESSI_REQUIRE(n >= 1 && members, "a batch needs at least one member");
+    MESSI_HIP(hipSetDevice(ctx->device));
+    b->ctx = ctx;
+    b->n = n;
+    for (int m = 0; m < n; ++m) {
+      MESSI_REQUIRE(members[m] && members[m]->ctx == ctx, "batch members must belong to the context");
+      b->mem.push_back(members[m]);
+    }
+    build_batch(b.get());
+  });
+  if (rc == 0) *out = b.release();
+  return rc;
+}
+
+void messi_batch_destroy(messi_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipStreamSynchronize(b->ctx->stream);
+  delete b;
+}
+
+int messi_batch_eval(messi_batch *b, int K4, const double *P4, const double *pi4, const double *logw4, int K16,
+                     const double *P16, const double *pi16, const double *logw16, double B, int prec,
+                     double *logZ) {
+  if (!b) return 2;
+  messi_ctx *ctx = b->ctx;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(P4 && pi4 && logw4 && P16 && pi16 && logw16 && logZ, "null argument");
+    MESSI_REQUIRE(B > 0.0, "B must be positive");
+    check_prec(prec);
+    MESSI_HIP(hipSetDevice(ctx->device));
+    if (prec == MESSI_FP32) eval_impl<float>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, B, false, logZ);
+    else eval_impl<double>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, B, false, logZ);
+  });
+}
+
+int messi_batch_timings(const messi_batch *b, double *t) {
+  if (!b || !t) return 2;
+  for (int i = 0; i < 5; ++i) t[i] = b->t[i];
   return 0;
 }
 

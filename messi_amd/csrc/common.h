@@ -66,6 +66,12 @@ inline void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
     MESSI_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
+// Device tables of a batch of inside instances.
+struct InsideBatch {
+  DevBuf inst;    // instance descriptors
+  DevBuf prefix;  // per launch t: first workgroup of each instance (n+1 ints)
+};
+
 // Scratch of the inside/outside/bpp phase, per precision.
 struct InsideWork {
   int L = 0;
@@ -81,7 +87,10 @@ struct InsideWork {
   DevBuf lp;                      // bpp log table
   DevBuf red;                     // reductions
   DevBuf out3;                    // double 3 x L x L staging
+  DevBuf inst1;                   // this work's own instance descriptor (batch of one)
+  InsideBatch ib1;                // launch tables when run as a batch of one
 };
+
 
 }  // namespace messi
 
@@ -90,5 +99,6 @@ struct messi_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   bool profiling = false;
+  int tile = 16;  // tile edge of the inside schedule
   messi::InsideWork w64, w32;
 };

@@ -47,72 +47,93 @@ template <class SR> static Rules<SR> make_rules(double B) {
 }
 
 // ---------------------------------------------------------------------------
-// Preparation
+// Instances: one inside problem each (own length and tables); a launch serves all.
 // ---------------------------------------------------------------------------
-__global__ void baseline_kernel(int L, const double *u, double B, double lamconst, int scaled,
-                                double *lam, double *C, double *uhat) {
+template <class V> struct InstDesc {
+  int L, NT;
+  V *Lr, *Lc, *Sr, *Sc, *F, *acc, *ptv;
+  const double *u, *paired;
+  double *lam, *C, *uhat, *z;
+};
+
+// Baseline lambda, its prefix sums C and u - lambda; one workgroup per instance.
+template <class V>
+__global__ void baseline_kernel(const InstDesc<V> *__restrict__ insts, double B, double lamconst, int scaled) {
+  const InstDesc<V> d = insts[blockIdx.x];
+  const int L = d.L;
   __shared__ double part[1024];
   int t = threadIdx.x, nt = blockDim.x;
   int chunk = (L + nt - 1) / nt;
   int b = t * chunk, e = min(L, b + chunk);
   double s = 0.0;
   for (int k = b; k < e; ++k) {
-    double lk = scaled ? (u[k] + lamconst) * B : 0.0;
-    lam[k] = lk;
-    uhat[k] = u[k] - lk;
+    double lk = scaled ? (d.u[k] + lamconst) * B : 0.0;
+    d.lam[k] = lk;
+    d.uhat[k] = d.u[k] - lk;
     s += lk;
   }
   part[t] = s;
   __syncthreads();
-  if (t == 0) {
-    double acc = 0.0;
-    for (int i = 0; i < nt; ++i) {
-      double v = part[i];
-      part[i] = acc;
-      acc += v;
-    }
+  // Hillis-Steele inclusive scan of the chunk sums.
+  for (int o = 1; o < nt; o <<= 1) {
+    double v = t >= o ? part[t - o] : 0.0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
   }
-  __syncthreads();
-  double acc = part[t];
-  if (t == 0) C[0] = 0.0;
+  double acc = t > 0 ? part[t - 1] : 0.0;
+  if (t == 0) d.C[0] = 0.0;
   for (int k = b; k < e; ++k) {
-    acc += lam[k];
-    C[k + 1] = acc;
+    acc += d.lam[k];
+    d.C[k + 1] = acc;
   }
 }
 
-// ptv (row-major) from the Julia column-major paired matrix (upper triangle).
+// ptv (row-major) from the Julia column-major paired matrix (upper triangle);
+// grid (columns, rows, instances).
 template <class SR>
-__global__ void ptv_kernel(int L, const double *paired, const double *lam, double B,
-                           typename SR::V *ptv) {
+__global__ void ptv_kernel(const InstDesc<typename SR::V> *__restrict__ insts, double B) {
+  const InstDesc<typename SR::V> d = insts[blockIdx.z];
+  const int L = d.L;
   int y = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
-  if (y >= L) return;
-  double v = paired[(size_t)y * L + j];
-  double x = (j >= y || v == -HUGE_VAL) ? -HUGE_VAL : v * B - lam[j] - lam[y];
-  ptv[(size_t)j * L + y] = SR::from_log(x);
+  if (y >= L || j >= L) return;
+  double v = d.paired[(size_t)y * L + j];
+  double x = (j >= y || v == -HUGE_VAL) ? -HUGE_VAL : v * B - d.lam[j] - d.lam[y];
+  d.ptv[(size_t)j * L + y] = SR::from_log(x);
 }
 
 // Zero the bulk accumulator; set the diagonal of S and L (F has no s rule).
 template <class SR>
-__global__ void inside_init_kernel(int L, const double *u, const double *lam, double B, double lp1,
-                                   double lp5, typename SR::V *Lr, typename SR::V *Lc,
-                                   typename SR::V *Sr, typename SR::V *Sc, typename SR::V *F,
-                                   typename SR::V *acc) {
+__global__ void inside_init_kernel(const InstDesc<typename SR::V> *__restrict__ insts, double B, double lp1,
+                                   double lp5) {
+  const InstDesc<typename SR::V> d = insts[blockIdx.z];
+  const int L = d.L;
   int y = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
-  if (y >= L) return;
+  if (y >= L || j >= L) return;
   size_t c = (size_t)j * L + y;
-  acc[c] = SR::zero();
+  d.acc[c] = SR::zero();
   if (y == j) {
-    double ub = u[j] * B - lam[j];
-    typename SR::V s = SR::from_log(ub + lp1 * B), l = SR::from_log(ub + lp5 * B);
-    Sr[c] = s;
-    Sc[c] = s;
-    Lr[c] = l;
-    Lc[c] = l;
-    F[c] = SR::zero();
+    double ub = d.u[j] * B - d.lam[j];
+    typename SR::V sv = SR::from_log(ub + lp1 * B), lv = SR::from_log(ub + lp5 * B);
+    d.Sr[c] = sv;
+    d.Sc[c] = sv;
+    d.Lr[c] = lv;
+    d.Lc[c] = lv;
+    d.F[c] = SR::zero();
   }
+}
+
+// log Z (natural, absolute) = S[0, L-1] plus the baseline of the whole span, per instance.
+template <class SR>
+__global__ void zlog_kernel(const InstDesc<typename SR::V> *__restrict__ insts, int n) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const InstDesc<typename SR::V> d = insts[m];
+  double v = SR::to_log(d.Sc[(size_t)(d.L - 1) * d.L]);
+  d.z[0] = v == -HUGE_VAL ? v : v + d.C[d.L];
+  d.z[1] = v;  // baseline-relative
 }
 
 // ---------------------------------------------------------------------------
@@ -251,15 +272,30 @@ __device__ __forceinline__ void tile_product(int L, int I, int K, int J, const t
 
 // One launch of the inside schedule (see the file header).
 template <class SR, int T>
-__global__ __launch_bounds__(T * T) void inside_step_kernel(
-    int L, int NT, int t, typename SR::V *__restrict__ Lr, typename SR::V *__restrict__ Lc,
-    typename SR::V *__restrict__ Sr, typename SR::V *__restrict__ Sc, typename SR::V *__restrict__ F,
-    typename SR::V *__restrict__ acc, const typename SR::V *__restrict__ ptv, Rules<SR> r) {
+__global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
+                                                            const int *__restrict__ prefix, int n, int t,
+                                                            Rules<SR> r) {
   typedef typename SR::V V;
   __shared__ StepLDS<SR, T> sm;
   const int tid = threadIdx.x;
+  // ---- instance of this workgroup: prefix[m] <= blockIdx.x < prefix[m+1]
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= (int)blockIdx.x) lo = mid;
+    else hi = mid;
+  }
+  const InstDesc<V> ins = insts[lo];
+  const int L = ins.L, NT = ins.NT;
+  V *__restrict__ Lr = ins.Lr;
+  V *__restrict__ Lc = ins.Lc;
+  V *__restrict__ Sr = ins.Sr;
+  V *__restrict__ Sc = ins.Sc;
+  V *__restrict__ F = ins.F;
+  V *__restrict__ acc = ins.acc;
+  const V *__restrict__ ptv = ins.ptv;
   // ---- decode the work item
-  int item = blockIdx.x, I, J;
+  int item = (int)blockIdx.x - prefix[lo], I, J;
   bool fin;
   if (item < NT - t) {
     fin = true;
@@ -560,13 +596,6 @@ __global__ void import3_kernel(int L, const double *i3, const double *C, int out
   if (Lc) Lc[cc] = tl;
 }
 
-// log Z (natural, absolute) = S[0, L-1] plus the baseline of the whole span.
-template <class SR>
-__global__ void zlog_kernel(int L, const typename SR::V *Sc, const double *C, double *z) {
-  double v = SR::to_log(Sc[(size_t)(L - 1) * L]);
-  z[0] = v == -HUGE_VAL ? v : v + C[L];
-  z[1] = v;  // baseline-relative
-}
 
 // ---------------------------------------------------------------------------
 // Base-pair posteriors (natural logs in double, then per-row normalisation).
@@ -681,44 +710,104 @@ void inside_alloc(InsideWork &w, int L, bool outside) {
   }
 }
 
+template <class SR>
+static InstDesc<typename SR::V> inst_of(InsideWork &w) {
+  typedef typename SR::V V;
+  InstDesc<V> d;
+  d.L = w.L;
+  d.NT = 0;
+  d.Lr = w.Lr.as<V>();
+  d.Lc = w.Lc.as<V>();
+  d.Sr = w.Sr.as<V>();
+  d.Sc = w.Sc.as<V>();
+  d.F = w.F.as<V>();
+  d.acc = w.acc.as<V>();
+  d.ptv = w.ptv.as<V>();
+  d.u = w.u.as<double>();
+  d.paired = w.paired.as<double>();
+  d.lam = w.lam.as<double>();
+  d.C = w.C.as<double>();
+  d.uhat = w.uhat.as<double>();
+  d.z = w.red.as<double>();
+  return d;
+}
+
+// Baseline and pair term of every instance (u and paired already on the device).
+template <class SR>
+static void prepare_batch(hipStream_t s, const InstDesc<typename SR::V> *d_inst, int n, int Lmax, double B) {
+  double lp[7];
+  kh99_logprobs(lp);
+  hipLaunchKernelGGL(baseline_kernel<typename SR::V>, dim3(n), dim3(1024), 0, s, d_inst, B, lp[0] + lp[5],
+                     (int)SR::scaled);
+  hipLaunchKernelGGL(ptv_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, B);
+}
+
 template <typename T>
 void inside_prepare(hipStream_t s, InsideWork &w, int L, double B) {
   typedef typename SRof<T>::type SR;
+  std::vector<InstDesc<typename SR::V>> d(1, inst_of<SR>(w));
+  upload(w.inst1, d, s);
+  prepare_batch<SR>(s, w.inst1.as<InstDesc<typename SR::V>>(), 1, L, B);
+}
+
+// Inside pass (preparation included) for a batch of instances; each work's red[0]
+// receives its log Z.
+template <typename T>
+void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &ib, double B) {
+  typedef typename SRof<T>::type SR;
+  typedef typename SR::V V;
+  if (n <= 0) return;
+  const int TE = ws[0]->tile == 32 ? 32 : 16;
+  std::vector<InstDesc<V>> d(n);
+  int Lmax = 0, NTmax = 0;
+  for (int m = 0; m < n; ++m) {
+    d[m] = inst_of<SR>(*ws[m]);
+    d[m].NT = (d[m].L + TE - 1) / TE;
+    Lmax = std::max(Lmax, d[m].L);
+    NTmax = std::max(NTmax, d[m].NT);
+  }
+  // Work items of launch t: the tiles (I, I+t) and the ready split-point products.
+  std::vector<int> prefix((size_t)NTmax * (n + 1));
+  std::vector<int> total(NTmax);
+  for (int t = 0; t < NTmax; ++t) {
+    int acc = 0;
+    for (int m = 0; m < n; ++m) {
+      prefix[(size_t)t * (n + 1) + m] = acc;
+      const int NT = d[m].NT;
+      if (t < NT) {
+        int items = NT - t;
+        for (int dist = t + 1; dist <= std::min(2 * t - 2, NT - 1); ++dist) items += NT - dist;
+        acc += items;
+      }
+    }
+    prefix[(size_t)t * (n + 1) + n] = acc;
+    total[t] = acc;
+  }
+  upload(ib.inst, d, s);
+  upload(ib.prefix, prefix, s);
+  const InstDesc<V> *di = ib.inst.as<InstDesc<V>>();
   double lp[7];
   kh99_logprobs(lp);
-  hipLaunchKernelGGL(baseline_kernel, dim3(1), dim3(1024), 0, s, L, w.u.as<double>(), B, lp[0] + lp[5],
-                     (int)SR::scaled, w.lam.as<double>(), w.C.as<double>(), w.uhat.as<double>());
-  hipLaunchKernelGGL(ptv_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, w.paired.as<double>(),
-                     w.lam.as<double>(), B, w.ptv.as<typename SR::V>());
+  Rules<SR> r = make_rules<SR>(B);
+  prepare_batch<SR>(s, di, n, Lmax, B);
+  hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, di, B, lp[1],
+                     lp[5]);
+  for (int t = 0; t < NTmax; ++t) {
+    const int *pf = ib.prefix.as<int>() + (size_t)t * (n + 1);
+    if (TE == 32)
+      hipLaunchKernelGGL((inside_step_kernel<SR, 32>), dim3(total[t]), dim3(32 * 32), 0, s, di, pf, n, t, r);
+    else
+      hipLaunchKernelGGL((inside_step_kernel<SR, 16>), dim3(total[t]), dim3(16 * 16), 0, s, di, pf, n, t, r);
+  }
+  hipLaunchKernelGGL(zlog_kernel<SR>, dim3((n + 63) / 64), dim3(64), 0, s, di, n);
+  MESSI_HIP(hipGetLastError());
 }
 
 template <typename T>
 void inside_run(hipStream_t s, InsideWork &w, int L, double B) {
-  typedef typename SRof<T>::type SR;
-  typedef typename SR::V V;
-  double lp[7];
-  kh99_logprobs(lp);
-  Rules<SR> r = make_rules<SR>(B);
-  hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, w.u.as<double>(),
-                     w.lam.as<double>(), B, lp[1], lp[5], w.Lr.as<V>(), w.Lc.as<V>(), w.Sr.as<V>(),
-                     w.Sc.as<V>(), w.F.as<V>(), w.acc.as<V>());
-  const int TE = w.tile == 32 ? 32 : 16;
-  const int NT = (L + TE - 1) / TE;
-  for (int t = 0; t < NT; ++t) {
-    int items = NT - t;
-    for (int dist = t + 1; dist <= std::min(2 * t - 2, NT - 1); ++dist) items += NT - dist;
-    if (TE == 32)
-      hipLaunchKernelGGL((inside_step_kernel<SR, 32>), dim3(items), dim3(32 * 32), 0, s, L, NT, t, w.Lr.as<V>(),
-                         w.Lc.as<V>(), w.Sr.as<V>(), w.Sc.as<V>(), w.F.as<V>(), w.acc.as<V>(),
-                         (const V *)w.ptv.as<V>(), r);
-    else
-      hipLaunchKernelGGL((inside_step_kernel<SR, 16>), dim3(items), dim3(16 * 16), 0, s, L, NT, t, w.Lr.as<V>(),
-                         w.Lc.as<V>(), w.Sr.as<V>(), w.Sc.as<V>(), w.F.as<V>(), w.acc.as<V>(),
-                         (const V *)w.ptv.as<V>(), r);
-  }
-  hipLaunchKernelGGL(zlog_kernel<SR>, dim3(1), dim3(1), 0, s, L, (const V *)w.Sc.as<V>(),
-                     (const double *)w.C.as<double>(), w.red.as<double>());
-  MESSI_HIP(hipGetLastError());
+  InsideWork *ws[1] = {&w};
+  (void)L;
+  inside_batch_run<T>(s, ws, 1, w.ib1, B);
 }
 
 template <typename T>
@@ -745,8 +834,9 @@ void inside_import(hipStream_t s, InsideWork &w, int L, const double *d_in3) {
   hipLaunchKernelGGL(import3_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, d_in3,
                      (const double *)w.C.as<double>(), 0, w.Sr.as<V>(), w.Sc.as<V>(), w.F.as<V>(), (V *)nullptr,
                      w.Lr.as<V>(), w.Lc.as<V>());
-  hipLaunchKernelGGL(zlog_kernel<SR>, dim3(1), dim3(1), 0, s, L, (const V *)w.Sc.as<V>(),
-                     (const double *)w.C.as<double>(), w.red.as<double>());
+  std::vector<InstDesc<V>> d(1, inst_of<SR>(w));
+  upload(w.inst1, d, s);
+  hipLaunchKernelGGL(zlog_kernel<SR>, dim3(1), dim3(64), 0, s, (const InstDesc<V> *)w.inst1.as<InstDesc<V>>(), 1);
 }
 
 template <typename T>
@@ -808,6 +898,7 @@ void bpp_run(hipStream_t s, InsideWork &w, int L, double *d_unpaired, double *d_
   template void inside_alloc<T>(InsideWork &, int, bool);                               \
   template void inside_prepare<T>(hipStream_t, InsideWork &, int, double);              \
   template void inside_run<T>(hipStream_t, InsideWork &, int, double);                  \
+  template void inside_batch_run<T>(hipStream_t, InsideWork *const *, int, InsideBatch &, double); \
   template double inside_logZ<T>(hipStream_t, InsideWork &, int);                       \
   template void inside_export<T>(hipStream_t, InsideWork &, int, double *);             \
   template void inside_import<T>(hipStream_t, InsideWork &, int, const double *);       \

@@ -88,12 +88,13 @@ __device__ __forceinline__ int frexp_exp(float m) {
 }
 
 // One wavefront per item of 16*NB consecutive entries of one node.
-template <typename T, int NB, bool ROOT>
+template <typename T, int NB>
 __global__ __launch_bounds__(256) void pair_level_kernel(
     const int2 *__restrict__ items, int nitems, const NodeDesc *__restrict__ nodes,
     const int *__restrict__ idx, const uint16_t *__restrict__ leafc, T *__restrict__ part,
-    int *__restrict__ expo, const T *__restrict__ P, const double *__restrict__ pi16, double logw,
-    int first, double *__restrict__ rootacc, double *__restrict__ rootcomp) {
+    int *__restrict__ expo, const T *__restrict__ P, const double *__restrict__ pi16,
+    const double *__restrict__ logws, int first, double *__restrict__ rootacc,
+    double *__restrict__ rootcomp) {
   typedef typename Mfma<T>::v4 v4;
   const int lane = threadIdx.x & 63;
   const int wave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
@@ -109,10 +110,11 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
     al[s] = Pl[col + 16 * Mfma<T>::state(g, s)];
     ar[s] = Pr[col + 16 * Mfma<T>::state(g, s)];
   }
+  const bool ROOT = nd.root != 0;
   double pis[4] = {0, 0, 0, 0};
   if (ROOT) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) pis[r] = pi16[Mfma<T>::state(g, r)];
+    for (int r = 0; r < 4; ++r) pis[r] = pi16[16 * nd.member + Mfma<T>::state(g, r)];
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -177,21 +179,22 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
       s += __shfl_xor(s, 32);
       if (valid && g == 0) {
         const double ll = log(s) + (double)E * 0.69314718055994530942;
-        rootcomp[e] = ll;
-        const double t = logw + ll;
-        rootacc[e] = first ? t : lse_d(rootacc[e], t);
+        const int64_t o = nd.out + e;
+        rootcomp[o] = ll;
+        const double t = logws[nd.member] + ll;
+        rootacc[o] = first ? t : lse_d(rootacc[o], t);
       }
     }
   }
 }
 
 // One thread per entry of a node; 256 entries per item.
-template <typename T, bool ROOT>
+template <typename T>
 __global__ __launch_bounds__(256) void site_level_kernel(
     const int2 *__restrict__ items, int nitems, const NodeDesc *__restrict__ nodes,
     const int *__restrict__ idx, const uint8_t *__restrict__ leafc, T *__restrict__ part,
-    int *__restrict__ expo, const T *__restrict__ P4, const double *__restrict__ pi4, double logw,
-    int first, double *__restrict__ rootacc) {
+    int *__restrict__ expo, const T *__restrict__ P4, const double *__restrict__ pi4,
+    const double *__restrict__ logws, int first, double *__restrict__ rootacc) {
   if ((int)blockIdx.x >= nitems) return;
   const int2 it = items[blockIdx.x];
   const NodeDesc nd = nodes[it.x];
@@ -236,92 +239,83 @@ __global__ __launch_bounds__(256) void site_level_kernel(
     for (int a = 0; a < 4; ++a) v[a] *= sc;
     E += ex;
   }
-  if (!ROOT) {
+  if (!nd.root) {
     store4(part + (size_t)(nd.part + e) * 4, v);
     expo[nd.part + e] = E;
   } else {
     double s = 0.0;
-    for (int a = 0; a < 4; ++a) s += pi4[a] * (double)v[a];
-    const double t = logw + log(s) + (double)E * 0.69314718055994530942;
-    rootacc[e] = first ? t : lse_d(rootacc[e], t);
+    for (int a = 0; a < 4; ++a) s += pi4[4 * nd.member + a] * (double)v[a];
+    const double t = logws[nd.member] + log(s) + (double)E * 0.69314718055994530942;
+    const int64_t o = nd.out + e;
+    rootacc[o] = first ? t : lse_d(rootacc[o], t);
   }
 }
 
-// Site expansion: u[c] = root entry of column c's root pattern.
-__global__ void site_expand_kernel(int L, const int *__restrict__ pid0, const double *__restrict__ acc,
-                                   double *__restrict__ u) {
+// Site expansion: u[c] = root entry of column c's root pattern (one grid row per member).
+__global__ void site_expand_kernel(const ExpandDesc *__restrict__ dd, const double *__restrict__ acc) {
+  const ExpandDesc d = dd[blockIdx.y];
   int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < L) u[c] = acc[pid0[c]];
+  if (c < d.L) d.u[c] = acc[d.rootoff + d.pid0[c]];
 }
 
-// Pair expansion into a symmetric row-major L x L matrix.
-__global__ void pair_expand_kernel(int L, int maxbp, int n0, const int *__restrict__ pid0,
-                                   const int *__restrict__ T0, const uint8_t *__restrict__ mask,
-                                   const double *__restrict__ acc, double *__restrict__ out) {
+// Pair expansion into a symmetric row-major L x L matrix; grid (columns, rows, members).
+__global__ void pair_expand_kernel(const ExpandDesc *__restrict__ dd, const double *__restrict__ acc,
+                                   double *const *__restrict__ outs) {
+  const ExpandDesc d = dd[blockIdx.z];
+  const int L = d.L;
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   int r = blockIdx.y;
-  if (c >= L) return;
+  if (c >= L || r >= L) return;
   int x = min(r, c), y = max(r, c);
   double v = -HUGE_VAL;
-  if (x != y && y - x <= maxbp && (!mask || mask[(size_t)x + (size_t)L * y])) {
-    int k = T0[(size_t)pid0[x] * n0 + pid0[y]];
-    if (k >= 0) v = acc[k];
+  if (x != y && y - x <= d.maxbp && (!d.mask || d.mask[(size_t)x + (size_t)L * y])) {
+    int k = d.T0[(size_t)d.pid0[x] * d.n0 + d.pid0[y]];
+    if (k >= 0) v = acc[d.rootoff + k];
   }
+  double *out = outs ? outs[blockIdx.z] : d.paired;
   out[(size_t)r * L + c] = v;
 }
 
 template <typename T, int NB>
-void launch_pair_level(hipStream_t s, const int2 *items, int nitems, bool root, const NodeDesc *nodes,
-                       const int *idx, const uint16_t *leafc, T *part, int *expo, const T *P,
-                       const double *pi16, double logw, int first, double *rootacc,
-                       double *rootcomp) {
+void launch_pair_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes, const int *idx,
+                       const uint16_t *leafc, T *part, int *expo, const T *P, const double *pi16,
+                       const double *logw, int first, double *rootacc, double *rootcomp) {
   if (nitems <= 0) return;
   int blocks = (nitems + 3) / 4;
-  if (root)
-    hipLaunchKernelGGL((pair_level_kernel<T, NB, true>), dim3(blocks), dim3(256), 0, s, items, nitems,
-                       nodes, idx, leafc, part, expo, P, pi16, logw, first, rootacc, rootcomp);
-  else
-    hipLaunchKernelGGL((pair_level_kernel<T, NB, false>), dim3(blocks), dim3(256), 0, s, items, nitems,
-                       nodes, idx, leafc, part, expo, P, pi16, logw, first, rootacc, rootcomp);
+  hipLaunchKernelGGL((pair_level_kernel<T, NB>), dim3(blocks), dim3(256), 0, s, items, nitems, nodes, idx, leafc,
+                     part, expo, P, pi16, logw, first, rootacc, rootcomp);
 }
 
 template <typename T>
-void launch_site_level(hipStream_t s, const int2 *items, int nitems, bool root, const NodeDesc *nodes,
-                       const int *idx, const uint8_t *leafc, T *part, int *expo, const T *P4,
-                       const double *pi4, double logw, int first, double *rootacc) {
+void launch_site_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes, const int *idx,
+                       const uint8_t *leafc, T *part, int *expo, const T *P4, const double *pi4,
+                       const double *logw, int first, double *rootacc) {
   if (nitems <= 0) return;
-  if (root)
-    hipLaunchKernelGGL((site_level_kernel<T, true>), dim3(nitems), dim3(256), 0, s, items, nitems, nodes,
-                       idx, leafc, part, expo, P4, pi4, logw, first, rootacc);
-  else
-    hipLaunchKernelGGL((site_level_kernel<T, false>), dim3(nitems), dim3(256), 0, s, items, nitems, nodes,
-                       idx, leafc, part, expo, P4, pi4, logw, first, rootacc);
+  hipLaunchKernelGGL((site_level_kernel<T>), dim3(nitems), dim3(256), 0, s, items, nitems, nodes, idx, leafc, part,
+                     expo, P4, pi4, logw, first, rootacc);
 }
 
-void launch_site_expand(hipStream_t s, int L, const int *pid0, const double *acc, double *u) {
-  hipLaunchKernelGGL(site_expand_kernel, dim3((L + 255) / 256), dim3(256), 0, s, L, pid0, acc, u);
+void launch_site_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, const double *acc) {
+  hipLaunchKernelGGL(site_expand_kernel, dim3((Lmax + 255) / 256, n), dim3(256), 0, s, d, acc);
 }
 
-void launch_pair_expand(hipStream_t s, int L, int maxbp, int n0, const int *pid0, const int *T0,
-                        const uint8_t *mask, const double *acc, double *out) {
-  hipLaunchKernelGGL(pair_expand_kernel, dim3((L + 255) / 256, L), dim3(256), 0, s, L, maxbp, n0, pid0,
-                     T0, mask, acc, out);
+void launch_pair_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, const double *acc,
+                        double *const *outs) {
+  hipLaunchKernelGGL(pair_expand_kernel, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d, acc, outs);
 }
 
-template void launch_pair_level<double, PAIR_NB>(hipStream_t, const int2 *, int, bool, const NodeDesc *,
-                                                 const int *, const uint16_t *, double *, int *,
-                                                 const double *, const double *, double, int, double *,
-                                                 double *);
-template void launch_pair_level<float, PAIR_NB>(hipStream_t, const int2 *, int, bool, const NodeDesc *,
-                                                const int *, const uint16_t *, float *, int *,
-                                                const float *, const double *, double, int, double *,
-                                                double *);
-template void launch_site_level<double>(hipStream_t, const int2 *, int, bool, const NodeDesc *,
-                                        const int *, const uint8_t *, double *, int *, const double *,
-                                        const double *, double, int, double *);
-template void launch_site_level<float>(hipStream_t, const int2 *, int, bool, const NodeDesc *,
-                                       const int *, const uint8_t *, float *, int *, const float *,
-                                       const double *, double, int, double *);
+template void launch_pair_level<double, PAIR_NB>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
+                                                 const uint16_t *, double *, int *, const double *, const double *,
+                                                 const double *, int, double *, double *);
+template void launch_pair_level<float, PAIR_NB>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
+                                                const uint16_t *, float *, int *, const float *, const double *,
+                                                const double *, int, double *, double *);
+template void launch_site_level<double>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
+                                        const uint8_t *, double *, int *, const double *, const double *,
+                                        const double *, int, double *);
+template void launch_site_level<float>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
+                                       const uint8_t *, float *, int *, const float *, const double *,
+                                       const double *, int, double *);
 
 // ---------------------------------------------------------------------------
 // Self-test of the MFMA operand layouts the pair kernel relies on.

@@ -49,6 +49,12 @@ SIGNATURES = {
     "messi_posterior": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                                        ctypes.c_int, _dp, _dp, _dp]),
     "messi_last_timings": (ctypes.c_int, [_vp, _dp]),
+    "messi_set_tile": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "messi_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "messi_batch_destroy": (None, [_vp]),
+    "messi_batch_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
+                                        ctypes.c_double, ctypes.c_int, _dp]),
+    "messi_batch_timings": (ctypes.c_int, [_vp, _dp]),
 }
 
 _lib = None
@@ -60,6 +66,9 @@ _shutdown = False
 @atexit.register
 def _close_all():
     global _shutdown
+    for o in list(_live):
+        if isinstance(o, Batch):
+            o.close()
     for o in list(_live):
         if isinstance(o, DeviceDataset):
             o.close()
@@ -132,6 +141,9 @@ class Context:
 
     def selftest(self) -> None:
         self.check(self.lib.messi_selftest(self._h))
+
+    def set_tile(self, tile: int) -> None:
+        self.check(self.lib.messi_set_tile(self._h, int(tile)))
 
     def close(self) -> None:
         if self._h:
@@ -303,3 +315,50 @@ def bpp(inside_t, outside_t, u, paired, prec=FP64, ctx: Optional[Context] = None
     ctx.check(ctx.lib.messi_bpp(ctx.handle, L, ptr(it), ptr(ot), ptr(u), ptr(pm), prec_code(prec), ptr(unp),
                                 ptr(pp)))
     return unp, pp
+
+
+class Batch:
+    """Several (dataset, parameter set) members evaluated together (messi_batch_*)."""
+
+    def __init__(self, members, ctx: Optional[Context] = None):
+        self.ctx = ctx or context()
+        self.members = list(members)            # DeviceDataset objects (may repeat)
+        arr = (_vp * len(self.members))(*[m.handle for m in self.members])
+        h = _vp()
+        self.ctx.check(self.ctx.lib.messi_batch_create(self.ctx.handle, len(self.members), arr, ctypes.byref(h)))
+        self._h = h
+        _live.add(self)
+
+    def eval(self, sites, pairs, B: float = 1.0, prec=FP64) -> np.ndarray:
+        """sites[m] = (P4 (K,nn,4,4), pi4 (K,4), logw (K,)); pairs[m] likewise with 16 states."""
+        n = len(self.members)
+        P4 = np.concatenate([colmajor_P(s[0]).reshape(-1) for s in sites])
+        pi4 = np.ascontiguousarray(np.concatenate([np.asarray(s[1], dtype=np.float64).reshape(-1) for s in sites]))
+        lw4 = np.ascontiguousarray(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1) for s in sites]))
+        P16 = np.concatenate([colmajor_P(p[0]).reshape(-1) for p in pairs])
+        pi16 = np.ascontiguousarray(np.concatenate([np.asarray(p[1], dtype=np.float64).reshape(-1) for p in pairs]))
+        lw16 = np.ascontiguousarray(np.concatenate([np.asarray(p[2], dtype=np.float64).reshape(-1) for p in pairs]))
+        K4 = len(lw4) // n
+        K16 = len(lw16) // n
+        z = np.empty(n)
+        self.ctx.check(self.ctx.lib.messi_batch_eval(self._h, K4, ptr(P4), ptr(pi4), ptr(lw4), K16, ptr(P16),
+                                                     ptr(pi16), ptr(lw16), float(B), prec_code(prec), ptr(z)))
+        return z
+
+    def timings(self) -> np.ndarray:
+        t = np.zeros(5)
+        self.ctx.check(self.ctx.lib.messi_batch_timings(self._h, ptr(t)))
+        return t
+
+    def close(self) -> None:
+        if self._h:
+            self.ctx.lib.messi_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _shutdown:
+            return
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -197,3 +197,75 @@ def test_config2_scale_properties(ctx):
     _say(f"config2 logZ64 {z64:.6f} logZ32 {z32:.6f} rel {abs(z32 - z64) / abs(z64):.2e} bpp err {err:.2e}")
     assert abs(z32 - z64) <= 1e-6 * abs(z64)
     assert err < 1e-4
+
+
+def _perturbed(a, ds, k, seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        v = a.params.copy()
+        v[0:3] = np.maximum(1.0, v[0:3] * np.exp(rng.normal(0, 0.2, 3)))
+        v[3:9] = v[3:9] * np.exp(rng.normal(0, 0.2, 6))
+        out.append(model.getparams(v, ds.tree.branchlengths))
+    return out
+
+
+def _stack_params(p):
+    return hotpath._stack(model.site_components(p)), hotpath._stack(model.pair_components(p))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_same_dataset_matches_single(ctx, prec):
+    _say(f"batch of 5 parameter vectors, 20 x 120, {prec}")
+    a = synth.config1()
+    ds = synth.dataset_of(a)
+    dd = hotpath.device_dataset(ds)
+    ps = _perturbed(a, ds, 5)
+    single = [dd.eval(*_stack_params(p), 1.0, prec) for p in ps]
+    b = lib.Batch([dd] * 5)
+    sp = [_stack_params(p) for p in ps]
+    z = b.eval([x[0] for x in sp], [x[1] for x in sp], 1.0, prec)
+    assert np.allclose(z, single, rtol=1e-12, atol=0)
+    if prec == "fp64":
+        for p, zz in zip(ps[:2], z[:2]):
+            u = oracle.site_loglik(ds, model.site_components(p))
+            M = oracle.pair_loglik(ds, model.pair_components(p))
+            zr = oracle.inside(u, M)[0, 0, -1]
+            assert abs(zz - zr) <= REL64 * abs(zr)
+
+
+def test_batch_uneven_instances(ctx):
+    _say("batch of 6 uneven instances (config4 generator)")
+    als = synth.config4(count=6)
+    dss = [synth.dataset_of(a) for a in als]
+    dds = [hotpath.device_dataset(d) for d in dss]
+    ps = [model.getparams(a.params, d.tree.branchlengths) for a, d in zip(als, dss)]
+    sp = [_stack_params(p) for p in ps]
+    b = lib.Batch(dds)
+    for prec in ("fp64", "fp32"):
+        z = b.eval([x[0] for x in sp], [x[1] for x in sp], 1.0, prec)
+        single = [dd.eval(*x, 1.0, prec) for dd, x in zip(dds, sp)]
+        assert np.allclose(z, single, rtol=1e-12, atol=0), prec
+    # anchor one instance on the oracle
+    d0, p0 = dss[0], ps[0]
+    u = oracle.site_loglik(d0, model.site_components(p0))
+    M = oracle.pair_loglik(d0, model.pair_components(p0))
+    zr = oracle.inside(u, M)[0, 0, -1]
+    z = b.eval([x[0] for x in sp], [x[1] for x in sp], 1.0, "fp64")
+    assert abs(z[0] - zr) <= REL64 * abs(zr)
+
+
+def test_tile32_matches_tile16(ctx):
+    _say("inside tile 32 vs 16")
+    a = synth.generate(10, 300, 21)
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths)
+    dd = hotpath.device_dataset(ds)
+    z16 = [dd.eval(*_stack_params(p), 1.0, prec) for prec in ("fp64", "fp32")]
+    ctx.set_tile(32)
+    try:
+        z32 = [dd.eval(*_stack_params(p), 1.0, prec) for prec in ("fp64", "fp32")]
+    finally:
+        ctx.set_tile(16)
+    assert abs(z32[0] - z16[0]) <= 1e-12 * abs(z16[0])
+    assert abs(z32[1] - z16[1]) <= 1e-6 * abs(z16[1])
