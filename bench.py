@@ -1,13 +1,17 @@
 """Benchmark: full-likelihood evaluations per second at 1,500 nt x 50 sequences.
 
-One step = one full-likelihood evaluation through the C ABI (messi_eval): site
-log-likelihoods (4-state pruning), the all-pairs 16-state pair matrix and the
-KH99 inside partition function, with the alignment and dedup lists resident in
-HBM and the per-branch transition matrices passed in from the host.  Both the
-plain FP64 path and the scaled FP32 path are timed; ``value`` is the FP32 path
-(the FP64 figure and the FP32/FP64 ratio are reported beside it).
+One step = one batched evaluation through the C ABI (messi_batch_eval): for each
+of B parameter vectors of the 50 x 1500 alignment (B independent MCMC chains /
+proposals), the site log-likelihoods (4-state pruning), the all-pairs 16-state
+pair matrix and the KH99 inside partition function.  All members share each
+tree-level launch and each tile-diagonal launch of the inside schedule.  The
+alignment and its dedup lists are resident in HBM; the transition matrices of
+each parameter vector come from the host.  ``value`` is evaluations per second
+of the scaled FP32 path at batch B; the plain FP64 path, and both paths at
+batch 1 (one chain), are reported beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+       python bench.py --instances      # 64 uneven instances: one batch vs 64 calls
 """
 from __future__ import annotations
 
@@ -25,10 +29,23 @@ sys.path.insert(0, ROOT)
 PEAK = {"fp64": 78.6, "fp32": 157.3}   # MI355X dense TFLOP/s (vector == matrix for these types)
 
 
-def _components(ds, params):
+def _components(params):
     from messi_amd import model
     from messi_amd.hotpath import _stack
     return _stack(model.site_components(params)), _stack(model.pair_components(params))
+
+
+def _perturbed(a, ds, k, seed=0):
+    from messi_amd import model
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        v = a.params.copy()
+        if i:
+            v[0:3] = np.maximum(1.0, v[0:3] * np.exp(rng.normal(0, 0.05, 3)))
+            v[3:9] = v[3:9] * np.exp(rng.normal(0, 0.05, 6))
+        out.append(model.getparams(v, ds.tree.branchlengths))
+    return out
 
 
 def cpu_baseline(ds, params, L):
@@ -46,7 +63,6 @@ def cpu_baseline(ds, params, L):
     npairs_sample = sum(min(D, L - 1 - x) for x in range(L))
     npairs = L * (L - 1) // 2
     pair_full = (t2 - t1) * npairs / npairs_sample
-    M = np.where(np.isfinite(M), M, -np.inf)
     t3 = time.perf_counter()
     oracle.inside(u, M)
     t4 = time.perf_counter()
@@ -57,77 +73,125 @@ def cpu_baseline(ds, params, L):
                       f"{(t4 - t0):.1f} s of CPU time"}
 
 
+def time_batch(b, sites, pairs, prec, steps, warmup):
+    from messi_amd import lib
+    ctx = lib.context()
+    ctx.set_profiling(False)
+    for _ in range(warmup):
+        z = b.eval(sites, pairs, 1.0, prec)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        z = b.eval(sites, pairs, 1.0, prec)
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    ctx.set_profiling(True)
+    ph = np.zeros(5)
+    nprof = max(3, min(10, steps))
+    for _ in range(nprof):
+        b.eval(sites, pairs, 1.0, prec)
+        ph += b.timings()
+    ph /= nprof
+    ctx.set_profiling(False)
+    return ms, z, ph
+
+
+def main_instances(args):
+    """64 uneven instances (L ~ U[80,600], N ~ U[10,60]) in one batch vs 64 calls."""
+    from messi_amd import lib, model, synth
+    from messi_amd.hotpath import device_dataset
+    als = synth.config4()
+    dss = [synth.dataset_of(a) for a in als]
+    dds = [device_dataset(d) for d in dss]
+    comps = [_components(model.getparams(a.params, d.tree.branchlengths)) for a, d in zip(als, dss)]
+    sites, pairs = [c[0] for c in comps], [c[1] for c in comps]
+    b = lib.Batch(dds)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        ms_b, zb, ph = time_batch(b, sites, pairs, prec, args.steps, args.warmup)
+        for _ in range(args.warmup):
+            [dd.eval(s, p, 1.0, prec) for dd, s, p in zip(dds, sites, pairs)]
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            zs = [dd.eval(s, p, 1.0, prec) for dd, s, p in zip(dds, sites, pairs)]
+        ms_s = (time.perf_counter() - t0) * 1e3 / args.steps
+        res[prec] = {"batched_evals_per_s": 64e3 / ms_b, "sequential_evals_per_s": 64e3 / ms_s,
+                     "speedup": ms_s / ms_b, "max_rel_diff": float(np.max(np.abs(zb - zs) / np.abs(zs))),
+                     "batched_phase_ms": {"site": ph[0], "pair": ph[1], "inside": ph[2]}}
+    out = {"metric": "total evals/s, 64 uneven instances in one batch vs 64 sequential calls",
+           "value": res["fp32"]["batched_evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "64 instances, L ~ U[80,600], N ~ U[10,60], seed 4",
+                      "L": [d.numcols for d in dss], "N": [d.numseqs for d in dss]},
+           "variants": res}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--prec", default="both", choices=["both", "fp64", "fp32"])
+    ap.add_argument("--instances", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the likelihood path is "
               f"single-GPU in this build", file=sys.stderr)
         sys.exit(2)
+    if args.instances:
+        return main_instances(args)
 
-    from messi_amd import lib, model, synth
+    from messi_amd import lib, synth
     from messi_amd.hotpath import device_dataset
 
     a = synth.config2()
     ds = synth.dataset_of(a)
-    params = model.getparams(a.params, ds.tree.branchlengths)
     L = ds.numcols
-    ctx = lib.context()
+    lib.context()
     dd = device_dataset(ds)
     stats = dd.stats()
-    site, pair = _components(ds, params)
-    precs = ["fp64", "fp32"] if args.prec == "both" else [args.prec]
+    ps = _perturbed(a, ds, args.batch)
+    comps = [_components(p) for p in ps]
+    sites, pairs = [c[0] for c in comps], [c[1] for c in comps]
+    bB = lib.Batch([dd] * args.batch)
+    b1 = lib.Batch([dd])
     res = {}
-    for prec in precs:
-        ctx.set_profiling(False)
-        for _ in range(args.warmup):
-            z = dd.eval(site, pair, 1.0, prec)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            z = dd.eval(site, pair, 1.0, prec)
-        t1 = time.perf_counter()
-        ms = (t1 - t0) * 1e3 / args.steps
-        # Phase device times from HIP events on the library's stream.
-        ctx.set_profiling(True)
-        ph = np.zeros(5)
-        nprof = max(3, min(10, args.steps))
-        for _ in range(nprof):
-            dd.eval(site, pair, 1.0, prec)
-            ph += dd.timings()
-        ph /= nprof
-        ctx.set_profiling(False)
-        res[prec] = {"ms_per_step": ms, "evals_per_s": 1e3 / ms, "logZ": z,
-                     "phase_ms": {"site": ph[0], "pair": ph[1], "inside": ph[2]}}
+    for prec in ("fp64", "fp32"):
+        ms1, z1, ph1 = time_batch(b1, sites[:1], pairs[:1], prec, args.steps, args.warmup)
+        msB, zB, phB = time_batch(bB, sites, pairs, prec, args.steps, args.warmup)
+        res[prec] = {"batch1": {"ms_per_eval": ms1, "evals_per_s": 1e3 / ms1, "logZ": float(z1[0]),
+                                "phase_ms": {"site": ph1[0], "pair": ph1[1], "inside": ph1[2]}},
+                     f"batch{args.batch}": {"ms_per_step": msB, "evals_per_s": args.batch * 1e3 / msB,
+                                            "phase_ms": {"site": phB[0], "pair": phB[1], "inside": phB[2]}},
+                     "logZ_batch": zB.tolist()}
     T_in = (L ** 3 - L) / 6
-    main_prec = "fp32" if "fp32" in res else "fp64"
-    r = res[main_prec]
-    inside_ms = r["phase_ms"]["inside"]
-    achieved = 2 * T_in / (inside_ms * 1e-3) / 1e12
-    roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK[main_prec], "unit": "TFLOP/s",
-            "frac": achieved / PEAK[main_prec], "traffic": None,
-            "kernel": "inside_diag_kernel (inside phase, 2 flops per bifurcation term, "
-                      f"{int(T_in)} terms per eval)"}
+    r = res["fp32"][f"batch{args.batch}"]
+    achieved = args.batch * 2 * T_in / (r["phase_ms"]["inside"] * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
+            "frac": achieved / PEAK["fp32"], "traffic": None,
+            "kernel": f"inside_step_kernel<XF,16> (inside phase of a batch of {args.batch}, 2 flops per "
+                      f"bifurcation term, {int(T_in)} terms per evaluation)"}
     out = {
         "metric": "likelihood evals/s at 1500nt x 50seq, FP32 vs FP64",
-        "value": r["evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+        "value": args.batch * 1e3 / r["ms_per_step"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32" if main_prec == "fp32" else "f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded Yule tree, GTR/MuSE simulation, 40% paired, 5% gaps)",
-        "config": {"workload": "config2: 50 sequences x 1500 nt, K=1, full-span pairs and inside",
-                   "L": L, "nseq": ds.numseqs, "sum_U": stats["sum_U"], "root_entries": stats["root_entries"]},
+        "config": {"workload": f"config2: 50 sequences x 1500 nt, K=1, full-span pairs and inside; "
+                               f"{args.batch} parameter vectors per step (one per chain)",
+                   "L": L, "nseq": ds.numseqs, "batch": args.batch, "sum_U": stats["sum_U"],
+                   "root_entries": stats["root_entries"]},
         "roofline": roof,
         "variants": res,
+        "fp32_over_fp64": {"batch1": res["fp32"]["batch1"]["evals_per_s"] / res["fp64"]["batch1"]["evals_per_s"],
+                           f"batch{args.batch}": res["fp32"][f"batch{args.batch}"]["evals_per_s"] /
+                           res["fp64"][f"batch{args.batch}"]["evals_per_s"]},
+        "fp32_logZ_rel_err": abs(res["fp32"]["batch1"]["logZ"] - res["fp64"]["batch1"]["logZ"]) /
+        abs(res["fp64"]["batch1"]["logZ"]),
     }
-    if "fp64" in res and "fp32" in res:
-        out["fp32_over_fp64"] = res["fp32"]["evals_per_s"] / res["fp64"]["evals_per_s"]
-        out["fp32_logZ_rel_err"] = abs(res["fp32"]["logZ"] - res["fp64"]["logZ"]) / abs(res["fp64"]["logZ"])
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ds, params, L)
+        out["cpu_baseline"] = cpu_baseline(ds, ps[0], L)
     print(json.dumps(out), flush=True)
 
 
