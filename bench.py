@@ -12,6 +12,7 @@ batch 1 (one chain), are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
        python bench.py --instances      # 64 uneven instances: one batch vs 64 calls
+       python bench.py --mcmc           # learned MCMC proposal vs random walk (ESS/s)
 """
 from __future__ import annotations
 
@@ -126,6 +127,44 @@ def main_instances(args):
     print(json.dumps(out), flush=True)
 
 
+def main_mcmc(args):
+    """Learned proposal vs adaptive random walk on the 50 x 1500 alignment:
+    16 chains, adaptive random-walk burn-in, network trained on the device on the
+    second half of the burn-in, then learned / random-walk / long random-walk
+    chains from the same states.  ESS/s counts each chain's wall time (host
+    model construction, device evaluation, network inference)."""
+    from messi_amd import lib, mcmc, synth
+
+    a = synth.config2()
+    ds = synth.dataset_of(a)
+    rng = np.random.default_rng(args.seed)
+    t0 = time.perf_counter()
+    post = mcmc.Posterior(ds, a.params, args.chains, prec=args.prec)
+    net = lib.Mlp(mcmc.DIM, 128, seed=args.seed)
+    th0 = post.theta_of(a.params) + rng.normal(0, 0.05, (args.chains, mcmc.DIM))
+
+    def log(*m):
+        print("[mcmc]", *m, f"({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+
+    res = mcmc.run_experiment(post, th0, net, rng, burn=args.burn, steps=args.mcmc_steps,
+                              long_steps=args.long_steps, train_steps=args.train_steps, log=log,
+                              mode_logp=lambda th: post(th, prec="fp64"))
+    res.pop("traces")
+    z = np.abs(np.array(res["mean_z_learned_vs_long"]))
+    out = {"metric": "MCMC ESS/s, learned proposal vs adaptive random walk (min ESS over 12 parameters)",
+           "value": res["ess_per_s_gain_min"], "unit": "x (learned ESS/s / random-walk ESS/s)", "n_gpus": 1,
+           "higher_is_better": True, "dtype": args.prec.replace("fp", "f"), "data": "synthetic",
+           "config": {"workload": f"50 sequences x 1500 nt, 12 parameters, {args.chains} chains, burn-in "
+                                  f"{args.burn}, {args.mcmc_steps} steps learned and random walk, long chain "
+                                  f"{args.long_steps}, network 12-128-128-24 trained {args.train_steps} Adam steps"},
+           "means_within_2_mcse": int(np.sum(z <= 2.0)), "means_within_3_mcse": int(np.sum(z <= 3.0)),
+           "evals": post.evals, "device_eval_seconds": post.eval_seconds,
+           "param_names": list(mcmc.NAMES), "result": res}
+    net.close()
+    post.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +173,14 @@ def main():
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--instances", action="store_true")
+    ap.add_argument("--mcmc", action="store_true")
+    ap.add_argument("--chains", type=int, default=16)
+    ap.add_argument("--burn", type=int, default=2000)
+    ap.add_argument("--mcmc-steps", type=int, default=1250)
+    ap.add_argument("--long-steps", type=int, default=4000)
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--prec", default="fp32")
+    ap.add_argument("--seed", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the likelihood path is "
@@ -141,6 +188,8 @@ def main():
         sys.exit(2)
     if args.instances:
         return main_instances(args)
+    if args.mcmc:
+        return main_mcmc(args)
 
     from messi_amd import lib, synth
     from messi_amd.hotpath import device_dataset

@@ -34,6 +34,7 @@ extern "C" {
 typedef struct messi_ctx messi_ctx;
 typedef struct messi_dataset messi_dataset;
 typedef struct messi_batch messi_batch;
+typedef struct messi_mlp messi_mlp;
 
 /* Context and error handling.  Replaces the per-call CuContext/CuModuleFile setup
  * of src/FelsensteinCUDA.jl:240-253 and src/InsideCUDA.jl:96-104. */
@@ -123,6 +124,32 @@ int messi_batch_eval(messi_batch *b, int K4, const double *P4, const double *pi4
                      int K16, const double *P16, const double *pi16, const double *logw16, double B,
                      int prec, double *logZ);
 int messi_batch_timings(const messi_batch *b, double *t);
+
+/* Learned MCMC proposal network: din inputs -> tanh(dh) -> tanh(dh) -> 2*din
+ * outputs (mean shift mu, then log-scale ell, of a diagonal Gaussian proposal
+ * z' ~ N(z + mu(z), diag exp(2 ell(z)))).  Products are bf16 MFMA with FP32
+ * accumulation; weights and Adam state are FP32.  Training minimises the mean
+ * negative log density  sum_d ell_d + ((t_d - z_d - mu_d) exp(-ell_d))^2 / 2
+ * over rows (z, t).  Row-major host arrays: x and t are n x din, out n x 2*din.
+ * The flat parameter vector (messi_mlp_num_params floats) holds W1 (D0 x dh),
+ * b1, W2 (dh x dh), b2, W3 (dh x D3), b3 with D0 = din and D3 = 2*din rounded
+ * up to multiples of 16 (padding entries are zero and never updated).
+ * No counterpart in the reference: its MCMC proposals are random walks and
+ * adaptive Metropolis (src/MCMC.jl:393-449, 1306-1633); this network replaces
+ * those moves with an MH-corrected learned proposal. */
+int messi_mlp_create(messi_ctx *ctx, int din, int dh, uint64_t seed, messi_mlp **out);
+void messi_mlp_destroy(messi_mlp *m);
+int messi_mlp_num_params(const messi_mlp *m, int64_t *n);
+int messi_mlp_get_params(messi_mlp *m, float *out);
+/* Replaces the weights and restarts the optimiser (Adam moments and step count zeroed). */
+int messi_mlp_set_params(messi_mlp *m, const float *in);
+int messi_mlp_forward(messi_mlp *m, int n, const float *x, float *out);
+/* Loss and gradient (flat, same layout as the parameters; grad may be NULL). */
+int messi_mlp_loss_grad(messi_mlp *m, int n, const float *x, const float *t, float *loss, float *grad);
+/* One AdamW step (beta1 0.9, beta2 0.999, eps 1e-8, decoupled weight decay
+ * w -= lr * weight_decay * w) on the loss over the n rows. */
+int messi_mlp_train_step(messi_mlp *m, int n, const float *x, const float *t, float lr, float weight_decay,
+                         float *loss);
 
 /* Device time in milliseconds of the phases of the last messi_eval /
  * messi_posterior call on this dataset: t[0] site, t[1] pair, t[2] inside,

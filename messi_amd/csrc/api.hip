@@ -734,6 +734,7 @@ int messi_selftest(messi_ctx *ctx) {
     MESSI_HIP(hipSetDevice(ctx->device));
     MESSI_REQUIRE(mfma_probe<double>(ctx->stream), "FP64 MFMA 16x16x4 layout check failed");
     MESSI_REQUIRE(mfma_probe<float>(ctx->stream), "FP32 MFMA 16x16x4 layout check failed");
+    MESSI_REQUIRE(mfma_bf16_probe(ctx->stream), "bf16 MFMA 16x16x16 layout check failed");
   });
 }
 

@@ -47,6 +47,7 @@ void launch_site_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, con
 void launch_pair_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, const double *acc,
                         double *const *outs);
 template <typename T> bool mfma_probe(hipStream_t s);
+bool mfma_bf16_probe(hipStream_t s);
 
 template <typename T> void inside_alloc(InsideWork &w, int L, bool outside);
 template <typename T> void inside_prepare(hipStream_t s, InsideWork &w, int L, double B);

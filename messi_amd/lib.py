@@ -22,6 +22,7 @@ FP64 = 0
 FP32 = 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _ip = ctypes.POINTER(ctypes.c_int)
 _u8 = ctypes.POINTER(ctypes.c_uint8)
 _i64 = ctypes.POINTER(ctypes.c_int64)
@@ -55,6 +56,14 @@ SIGNATURES = {
     "messi_batch_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                                         ctypes.c_double, ctypes.c_int, _dp]),
     "messi_batch_timings": (ctypes.c_int, [_vp, _dp]),
+    "messi_mlp_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "messi_mlp_destroy": (None, [_vp]),
+    "messi_mlp_num_params": (ctypes.c_int, [_vp, _i64]),
+    "messi_mlp_get_params": (ctypes.c_int, [_vp, _fp]),
+    "messi_mlp_set_params": (ctypes.c_int, [_vp, _fp]),
+    "messi_mlp_forward": (ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp]),
+    "messi_mlp_loss_grad": (ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp, _fp, _fp]),
+    "messi_mlp_train_step": (ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp, ctypes.c_float, ctypes.c_float, _fp]),
 }
 
 _lib = None
@@ -70,7 +79,7 @@ def _close_all():
         if isinstance(o, Batch):
             o.close()
     for o in list(_live):
-        if isinstance(o, DeviceDataset):
+        if isinstance(o, (DeviceDataset, Mlp)):
             o.close()
     for o in list(_live):
         o.close()
@@ -345,6 +354,24 @@ class Batch:
                                                      ptr(pi16), ptr(lw16), float(B), prec_code(prec), ptr(z)))
         return z
 
+    def eval_stacked(self, P4, pi4, lw4, P16, pi16, lw16, B: float = 1.0, prec=FP64) -> np.ndarray:
+        """Stacked inputs: P4 (n, K4, nn, 4, 4), pi4 (n, K4, 4), lw4 (n, K4) and the
+        16-state arrays likewise; every member must have nn nodes."""
+        n = len(self.members)
+        P4 = colmajor_P(P4)
+        P16 = colmajor_P(P16)
+        if P4.shape[0] != n or P16.shape[0] != n:
+            raise ValueError(f"expected {n} members, got {P4.shape[0]} and {P16.shape[0]}")
+        pi4 = np.ascontiguousarray(pi4, dtype=np.float64)
+        lw4 = np.ascontiguousarray(lw4, dtype=np.float64)
+        pi16 = np.ascontiguousarray(pi16, dtype=np.float64)
+        lw16 = np.ascontiguousarray(lw16, dtype=np.float64)
+        K4, K16 = lw4.shape[1], lw16.shape[1]
+        z = np.empty(n)
+        self.ctx.check(self.ctx.lib.messi_batch_eval(self._h, K4, ptr(P4), ptr(pi4), ptr(lw4), K16, ptr(P16),
+                                                     ptr(pi16), ptr(lw16), float(B), prec_code(prec), ptr(z)))
+        return z
+
     def timings(self) -> np.ndarray:
         t = np.zeros(5)
         self.ctx.check(self.ctx.lib.messi_batch_timings(self._h, ptr(t)))
@@ -353,6 +380,78 @@ class Batch:
     def close(self) -> None:
         if self._h:
             self.ctx.lib.messi_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _shutdown:
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mlp:
+    """Proposal network trained and run on the device (messi_mlp_*)."""
+
+    def __init__(self, din: int, dh: int = 128, seed: int = 0, ctx: Optional[Context] = None):
+        self.ctx = ctx or context()
+        self.din, self.dh = int(din), int(dh)
+        h = _vp()
+        self.ctx.check(self.ctx.lib.messi_mlp_create(self.ctx.handle, self.din, self.dh, int(seed),
+                                                     ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_int64()
+        self.ctx.check(self.ctx.lib.messi_mlp_num_params(self._h, ctypes.byref(n)))
+        self.num_params = int(n.value)
+        _live.add(self)
+
+    def _rows(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.ndim != 2 or x.shape[1] != self.din:
+            raise ValueError(f"expected rows of {self.din} values, got shape {x.shape}")
+        return x
+
+    def get_params(self) -> np.ndarray:
+        w = np.empty(self.num_params, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.messi_mlp_get_params(self._h, ptr(w, _fp)))
+        return w
+
+    def set_params(self, w) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.shape != (self.num_params,):
+            raise ValueError(f"expected {self.num_params} parameters, got shape {w.shape}")
+        self.ctx.check(self.ctx.lib.messi_mlp_set_params(self._h, ptr(w, _fp)))
+
+    def forward(self, x) -> np.ndarray:
+        """Rows of inputs -> rows of (mu, ell), shape (n, 2*din)."""
+        x = self._rows(x)
+        out = np.empty((x.shape[0], 2 * self.din), dtype=np.float32)
+        self.ctx.check(self.ctx.lib.messi_mlp_forward(self._h, x.shape[0], ptr(x, _fp), ptr(out, _fp)))
+        return out
+
+    def loss_grad(self, x, t):
+        x, t = self._rows(x), self._rows(t)
+        if x.shape != t.shape:
+            raise ValueError("inputs and targets differ in shape")
+        loss = ctypes.c_float()
+        g = np.empty(self.num_params, dtype=np.float32)
+        self.ctx.check(self.ctx.lib.messi_mlp_loss_grad(self._h, x.shape[0], ptr(x, _fp), ptr(t, _fp),
+                                                        ctypes.byref(loss), ptr(g, _fp)))
+        return float(loss.value), g
+
+    def train_step(self, x, t, lr: float = 1e-3, weight_decay: float = 0.0) -> float:
+        x, t = self._rows(x), self._rows(t)
+        if x.shape != t.shape:
+            raise ValueError("inputs and targets differ in shape")
+        loss = ctypes.c_float()
+        self.ctx.check(self.ctx.lib.messi_mlp_train_step(self._h, x.shape[0], ptr(x, _fp), ptr(t, _fp),
+                                                         float(lr), float(weight_decay), ctypes.byref(loss)))
+        return float(loss.value)
+
+    def close(self) -> None:
+        if self._h:
+            self.ctx.lib.messi_mlp_destroy(self._h)
             self._h = None
 
     def __del__(self):
