@@ -13,6 +13,7 @@ batch 1 (one chain), are reported beside it.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
        python bench.py --instances      # 64 uneven instances: one batch vs 64 calls
        python bench.py --mcmc           # learned MCMC proposal vs random walk (ESS/s)
+       python bench.py --genome         # 30 x 9000 nt posterior, full span and D=500
 """
 from __future__ import annotations
 
@@ -127,6 +128,80 @@ def main_instances(args):
     print(json.dumps(out), flush=True)
 
 
+def _device_memory():
+    """(free, total) bytes of device 0 from the HIP runtime."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+        free, total = ctypes.c_size_t(), ctypes.c_size_t()
+        if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) == 0:
+            return int(free.value), int(total.value)
+    except OSError:
+        pass
+    return None, None
+
+
+def main_genome(args):
+    """30 sequences x 9000 nt: posterior (site, pair, inside, outside, bpp) at full
+    span and at maximum base-pair distance 500, FP64 and FP32; errors of the FP32
+    path against FP64, phase times and device memory."""
+    from messi_amd import lib, model, synth
+
+    t0 = time.perf_counter()
+
+    def log(*m):
+        print("[genome]", *m, f"({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+
+    a = synth.config3()
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths)
+    site, pair = _components(p)
+    ctx = lib.context()
+    free0, total = _device_memory()
+    out = {"metric": "genome-scale posterior, 30 x 9000 nt: FP32 vs FP64 error and time", "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "data": "synthetic (seeded Yule tree, 30% paired, helices <= 12)",
+           "config": {"workload": "config3: 30 sequences x 9000 nt, posterior at full span and at D=500",
+                      "L": ds.numcols, "nseq": ds.numseqs}, "runs": {}}
+    for D in args.genome_spans:
+        tb = time.perf_counter()
+        dd = lib.DeviceDataset(ds, D)
+        build_s = time.perf_counter() - tb
+        st = dd.stats()
+        log(f"D={D}: dataset built in {build_s:.1f} s, stats {st}")
+        r = {"dataset_build_s": build_s, "stats": st}
+        res = {}
+        for prec in ("fp64", "fp32"):
+            ctx.set_profiling(False)
+            tw = time.perf_counter()
+            dd.posterior(site, pair, prec)            # allocates the tables
+            first_s = time.perf_counter() - tw
+            ctx.set_profiling(True)
+            tw = time.perf_counter()
+            z, unp, pp = dd.posterior(site, pair, prec)
+            wall = time.perf_counter() - tw
+            ph = dd.timings()
+            ctx.set_profiling(False)
+            free, _ = _device_memory()
+            res[prec] = (z, unp, pp)
+            r[prec] = {"logZ": z, "posterior_s": wall, "first_call_s": first_s,
+                       "phase_s": {k: float(v) / 1e3 for k, v in zip(("site", "pair", "inside", "outside", "bpp"), ph)},
+                       "device_bytes_in_use": (total - free) if free is not None else None}
+            log(f"D={D} {prec}: logZ {z:.6f}, posterior {wall:.2f} s, phases (ms) {np.round(ph, 1).tolist()}")
+        z64, u64, p64 = res["fp64"]
+        z32, u32, p32 = res["fp32"]
+        r["fp32_logZ_rel_err"] = abs(z32 - z64) / abs(z64)
+        r["fp32_bpp_max_abs_err"] = float(max(np.max(np.abs(u32 - u64)), np.max(np.abs(p32 - p64))))
+        r["fp64_row_sum_max_dev"] = float(np.max(np.abs(u64 + p64.sum(axis=1) - 1.0)))
+        log(f"D={D}: fp32 logZ rel err {r['fp32_logZ_rel_err']:.2e}, bpp max abs err {r['fp32_bpp_max_abs_err']:.2e}")
+        out["runs"][str(D)] = r
+        del res, z64, u64, p64, z32, u32, p32
+        dd.close()
+    out["device_total_bytes"] = total
+    first = out["runs"][str(args.genome_spans[0])]
+    out["value"] = first["fp32"]["posterior_s"]
+    print(json.dumps(out), flush=True)
+
+
 def main_mcmc(args):
     """Learned proposal vs adaptive random walk on the 50 x 1500 alignment:
     16 chains, adaptive random-walk burn-in, network trained on the device on the
@@ -174,6 +249,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--instances", action="store_true")
     ap.add_argument("--mcmc", action="store_true")
+    ap.add_argument("--genome", action="store_true")
+    ap.add_argument("--genome-spans", type=int, nargs="+", default=[1000000, 500])
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--burn", type=int, default=2000)
     ap.add_argument("--mcmc-steps", type=int, default=1250)
@@ -190,6 +267,8 @@ def main():
         return main_instances(args)
     if args.mcmc:
         return main_mcmc(args)
+    if args.genome:
+        return main_genome(args)
 
     from messi_amd import lib, synth
     from messi_amd.hotpath import device_dataset
@@ -239,6 +318,11 @@ def main():
         "fp32_logZ_rel_err": abs(res["fp32"]["batch1"]["logZ"] - res["fp64"]["batch1"]["logZ"]) /
         abs(res["fp64"]["batch1"]["logZ"]),
     }
+    # posterior (outside + base-pair probabilities) of the first parameter vector in both paths
+    post = {prec: dd.posterior(sites[0], pairs[0], prec) for prec in ("fp64", "fp32")}
+    (z64, u64, p64), (z32, u32, p32) = post["fp64"], post["fp32"]
+    out["fp32_posterior_logZ_rel_err"] = abs(z32 - z64) / abs(z64)
+    out["fp32_bpp_max_abs_err"] = float(max(np.max(np.abs(u32 - u64)), np.max(np.abs(p32 - p64))))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ds, ps[0], L)
     print(json.dumps(out), flush=True)

@@ -660,6 +660,9 @@ void eval_impl(messi_batch *b, int K4, const double *P4, const double *pi4, cons
   for (int m = 0; m < b->n; ++m) {
     ws[m] = &member_work(b, m, prec);
     ws[m]->tile = ctx->tile;
+    // pair scores beyond the distance bound are -inf (pair_expand writes them so)
+    const int D = b->mem[m]->maxbp, Lm = b->mem[m]->L;
+    ws[m]->band = D < Lm - 1 ? D : -1;
     inside_alloc<T>(*ws[m], b->mem[m]->L, keep_outside);
   }
   if (ctx->profiling) b->ev.make();
@@ -812,6 +815,22 @@ int messi_pair_loglik(messi_dataset *ds, int K, const double *P16, const double 
   });
 }
 
+// Largest y - j (j < y) with a finite pair score in a column-major L x L matrix
+// (0 when there is none), or -1 when it reaches the full span.  The inside and
+// outside passes skip the terms that a smaller band makes -inf.
+static int pairing_band(int L, const double *paired) {
+  int band = 0;
+  for (int y = 1; y < L; ++y) {
+    const double *col = paired + (size_t)y * L;
+    for (int j = 0; j < y - band; ++j)
+      if (col[j] != -HUGE_VAL) {
+        band = y - j;
+        break;
+      }
+  }
+  return band >= L - 1 ? -1 : band;
+}
+
 int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, double B, int zonly, int prec,
                  double *Z, double *in3) {
   if (!ctx) return 2;
@@ -823,6 +842,7 @@ int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, d
     hipStream_t s = ctx->stream;
     InsideWork &w = work(ctx, prec);
     w.tile = ctx->tile;
+    w.band = pairing_band(L, paired);
     const size_t LL = (size_t)L * L;
     auto go = [&](auto tag) {
       using T = decltype(tag);
@@ -851,6 +871,7 @@ int messi_outside(messi_ctx *ctx, int L, const double *in3, const double *u, con
     MESSI_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     InsideWork &w = work(ctx, prec);
+    w.band = pairing_band(L, paired);
     const size_t LL = (size_t)L * L;
     auto go = [&](auto tag) {
       using T = decltype(tag);
@@ -861,7 +882,7 @@ int messi_outside(messi_ctx *ctx, int L, const double *in3, const double *u, con
       MESSI_HIP(hipMemcpyAsync(w.out3.p, in3, sizeof(double) * 3 * LL, hipMemcpyHostToDevice, s));
       inside_prepare<T>(s, w, L, 1.0);
       inside_import<T>(s, w, L, w.out3.as<double>());
-      outside_run<T>(s, w, L);
+      outside_run<T>(s, w, L, true);
       outside_export<T>(s, w, L, w.out3.as<double>());
       MESSI_HIP(hipMemcpyAsync(out3, w.out3.p, sizeof(double) * 3 * LL, hipMemcpyDeviceToHost, s));
       MESSI_HIP(hipStreamSynchronize(s));
@@ -936,7 +957,7 @@ int messi_posterior(messi_dataset *ds, int K4, const double *P4, const double *p
       using T = decltype(tag);
       InsideWork &w = member_work(b, 0, sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64);
       eval_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, 1.0, true, logZ);
-      outside_run<T>(s, w, L);
+      outside_run<T>(s, w, L, false);
       record(b, 4);
       w.out3.ensure(sizeof(double) * (LL + L));
       double *dpair = w.out3.as<double>();

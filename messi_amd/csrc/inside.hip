@@ -51,6 +51,7 @@ template <class SR> static Rules<SR> make_rules(double B) {
 // ---------------------------------------------------------------------------
 template <class V> struct InstDesc {
   int L, NT;
+  int KB;  // largest tile offset K - I whose L tile (I, K) can hold a finite value
   V *Lr, *Lc, *Sr, *Sc, *F, *acc, *ptv;
   const double *u, *paired;
   double *lam, *C, *uhat, *z;
@@ -325,9 +326,10 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   __syncthreads();
   // ---- split-point products that became ready with tile-diagonal t-1
   if (t >= 2) {
+    // L[j, h] is -inf for h - j beyond the pairing band: such L tiles contribute nothing.
     const int K1 = I + t - 1, K2 = J - (t - 1);
-    tile_product<SR, T>(L, I, K1, J, Lr, Sc, sm);
-    if (K2 != K1) tile_product<SR, T>(L, I, K2, J, Lr, Sc, sm);
+    if (K1 - I <= ins.KB) tile_product<SR, T>(L, I, K1, J, Lr, Sc, sm);
+    if (K2 != K1 && K2 - I <= ins.KB) tile_product<SR, T>(L, I, K2, J, Lr, Sc, sm);
   }
   if (!fin) {
     for (int idx = tid; idx < T * T; idx += T * T) {
@@ -472,7 +474,7 @@ __global__ __launch_bounds__(256) void outside_diag_kernel(
     int L, int b, int G, const typename SR::V *__restrict__ iSr, const typename SR::V *__restrict__ iLc,
     typename SR::V *__restrict__ oSr, typename SR::V *__restrict__ oSc, typename SR::V *__restrict__ oFr,
     typename SR::V *__restrict__ oFc, typename SR::V *__restrict__ oL, const typename SR::V *__restrict__ ptv,
-    Rules<SR> r) {
+    Rules<SR> r, int band, int lband) {
   typedef typename SR::V V;
   typedef typename SR::K K;
   typedef typename SR::S S;
@@ -485,14 +487,19 @@ __global__ __launch_bounds__(256) void outside_diag_kernel(
   const V *iSrow = iSr + (size_t)(y + 1 < L ? y + 1 : y) * L;
   const V *oScol = oSc + (size_t)y * L, *oFcol = oFc + (size_t)y * L;
   const V *iLcol = iLc + (size_t)(j >= 1 ? j - 1 : 0) * L;
+  // With a pairing band: L[k, j-1] is -inf for j-1-k > band.  L's outside is
+  // consumed only through pair terms, so beyond lband (the band, when the
+  // caller does not need the L outside table) it is left at zero.
+  const int kend = (lband >= 0 && b > lband) ? y + 1 : L;
+  const int k0 = band >= 0 ? max(0, j - 1 - band) : 0;
   K krS = SR::kmin(), krF = SR::kmin(), klS = SR::kmin(), klF = SR::kmin();
   if (valid) {
-    for (int k = y + 1 + sub; k < L; k += G) {
+    for (int k = y + 1 + sub; k < kend; k += G) {
       V is = iSrow[k];
       krS = SR::kmax(krS, SR::key(oSrow[k], is));
       krF = SR::kmax(krF, SR::key(oFrow[k], is));
     }
-    for (int k = sub; k < j; k += G) {
+    for (int k = k0 + sub; k < j; k += G) {
       V il = iLcol[k];
       klS = SR::kmax(klS, SR::key(oScol[k], il));
       klF = SR::kmax(klF, SR::key(oFcol[k], il));
@@ -504,12 +511,12 @@ __global__ __launch_bounds__(256) void outside_diag_kernel(
   klF = grp_kmax<SR>(klF, G);
   S srS = 0, srF = 0, slS = 0, slF = 0;
   if (valid) {
-    for (int k = y + 1 + sub; k < L; k += G) {
+    for (int k = y + 1 + sub; k < kend; k += G) {
       V is = iSrow[k];
       if (krS != SR::kmin()) srS += SR::contrib(oSrow[k], is, krS);
       if (krF != SR::kmin()) srF += SR::contrib(oFrow[k], is, krF);
     }
-    for (int k = sub; k < j; k += G) {
+    for (int k = k0 + sub; k < j; k += G) {
       V il = iLcol[k];
       if (klS != SR::kmin()) slS += SR::contrib(oScol[k], il, klS);
       if (klF != SR::kmin()) slF += SR::contrib(oFcol[k], il, klF);
@@ -716,6 +723,7 @@ static InstDesc<typename SR::V> inst_of(InsideWork &w) {
   InstDesc<V> d;
   d.L = w.L;
   d.NT = 0;
+  d.KB = 0;
   d.Lr = w.Lr.as<V>();
   d.Lc = w.Lc.as<V>();
   d.Sr = w.Sr.as<V>();
@@ -763,20 +771,27 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
   for (int m = 0; m < n; ++m) {
     d[m] = inst_of<SR>(*ws[m]);
     d[m].NT = (d[m].L + TE - 1) / TE;
+    // tile (I, K) holds an L cell with h - j <= band iff (K - I - 1) T + 1 <= band
+    const int band = ws[m]->band;
+    d[m].KB = (band < 0 || band >= d[m].L) ? d[m].NT : (band == 0 ? 0 : (band - 1) / TE + 1);
     Lmax = std::max(Lmax, d[m].L);
     NTmax = std::max(NTmax, d[m].NT);
   }
-  // Work items of launch t: the tiles (I, I+t) and the ready split-point products.
+  // Work items of launch t: the tiles (I, I+t) and the ready split-point products
+  // (I, K) x (K, J) with K - I = t - 1 or J - K = t - 1; a bulk tile (I, I+dist)
+  // is scheduled only when one of its two products lies within the band.
   std::vector<int> prefix((size_t)NTmax * (n + 1));
   std::vector<int> total(NTmax);
   for (int t = 0; t < NTmax; ++t) {
     int acc = 0;
     for (int m = 0; m < n; ++m) {
       prefix[(size_t)t * (n + 1) + m] = acc;
-      const int NT = d[m].NT;
+      const int NT = d[m].NT, KB = d[m].KB;
       if (t < NT) {
         int items = NT - t;
-        for (int dist = t + 1; dist <= std::min(2 * t - 2, NT - 1); ++dist) items += NT - dist;
+        int dmax = std::min(2 * t - 2, NT - 1);
+        if (t - 1 > KB) dmax = std::min(dmax, t - 1 + KB);
+        for (int dist = t + 1; dist <= dmax; ++dist) items += NT - dist;
         acc += items;
       }
     }
@@ -840,19 +855,23 @@ void inside_import(hipStream_t s, InsideWork &w, int L, const double *d_in3) {
 }
 
 template <typename T>
-void outside_run(hipStream_t s, InsideWork &w, int L) {
+void outside_run(hipStream_t s, InsideWork &w, int L, bool full_L) {
   typedef typename SRof<T>::type SR;
   typedef typename SR::V V;
   Rules<SR> r = make_rules<SR>(1.0);
   hipLaunchKernelGGL(outside_init_kernel<SR>, dim3(1), dim3(1), 0, s, L, w.oSr.as<V>(), w.oSc.as<V>(),
                      w.oFr.as<V>(), w.oFc.as<V>(), w.oL.as<V>());
+  const int band = (w.band >= 0 && w.band < L) ? w.band : -1;
+  const int lband = full_L ? -1 : band;
   for (int b = L - 2; b >= 0; --b) {
-    int G = lanes_for(L - b);
+    int terms = L - b;
+    if (band >= 0) terms = (lband >= 0 && b > lband) ? band + 1 : std::max(L - b, band + 1);
+    int G = lanes_for(terms);
     long threads = (long)(L - b) * G;
     int blocks = (int)((threads + 255) / 256);
     hipLaunchKernelGGL(outside_diag_kernel<SR>, dim3(blocks), dim3(256), 0, s, L, b, G, (const V *)w.Sr.as<V>(),
                        (const V *)w.Lc.as<V>(), w.oSr.as<V>(), w.oSc.as<V>(), w.oFr.as<V>(), w.oFc.as<V>(),
-                       w.oL.as<V>(), (const V *)w.ptv.as<V>(), r);
+                       w.oL.as<V>(), (const V *)w.ptv.as<V>(), r, band, lband);
   }
   MESSI_HIP(hipGetLastError());
 }
@@ -902,7 +921,7 @@ void bpp_run(hipStream_t s, InsideWork &w, int L, double *d_unpaired, double *d_
   template double inside_logZ<T>(hipStream_t, InsideWork &, int);                       \
   template void inside_export<T>(hipStream_t, InsideWork &, int, double *);             \
   template void inside_import<T>(hipStream_t, InsideWork &, int, const double *);       \
-  template void outside_run<T>(hipStream_t, InsideWork &, int);                         \
+  template void outside_run<T>(hipStream_t, InsideWork &, int, bool);                   \
   template void outside_export<T>(hipStream_t, InsideWork &, int, double *);            \
   template void outside_import<T>(hipStream_t, InsideWork &, int, const double *);      \
   template void bpp_run<T>(hipStream_t, InsideWork &, int, double *, double *);

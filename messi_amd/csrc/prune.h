@@ -57,7 +57,8 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
 template <typename T> double inside_logZ(hipStream_t s, InsideWork &w, int L);
 template <typename T> void inside_export(hipStream_t s, InsideWork &w, int L, double *d_in3);
 template <typename T> void inside_import(hipStream_t s, InsideWork &w, int L, const double *d_in3);
-template <typename T> void outside_run(hipStream_t s, InsideWork &w, int L);
+// full_L: also the L outside table beyond the pairing band (the bpp needs it only within)
+template <typename T> void outside_run(hipStream_t s, InsideWork &w, int L, bool full_L);
 template <typename T> void outside_export(hipStream_t s, InsideWork &w, int L, double *d_out3);
 template <typename T> void outside_import(hipStream_t s, InsideWork &w, int L, const double *d_out3);
 template <typename T> void bpp_run(hipStream_t s, InsideWork &w, int L, double *d_unpaired, double *d_pair);

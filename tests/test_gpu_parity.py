@@ -269,3 +269,35 @@ def test_tile32_matches_tile16(ctx):
         ctx.set_tile(16)
     assert abs(z32[0] - z16[0]) <= 1e-12 * abs(z16[0])
     assert abs(z32[1] - z16[1]) <= 1e-6 * abs(z16[1])
+
+
+@pytest.mark.parametrize("tile", [16, 32])
+@pytest.mark.parametrize("maxbp", [0, 5, 17, 40])
+def test_pairing_band_posterior(ctx, tile, maxbp):
+    """With a distance bound the inside pass skips out-of-band split-point tiles
+    and the outside pass the out-of-band sums; results equal the oracle's full
+    recursions on the same bounded pair matrix."""
+    _say(f"band posterior maxbp={maxbp} tile={tile}")
+    a = synth.generate(12, 150, 31)
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths)
+    u = oracle.site_loglik(ds, model.site_components(p))
+    M = oracle.pair_loglik(ds, model.pair_components(p), maxbp=maxbp)
+    ins = oracle.inside(u, M)
+    out = oracle.outside(ins, u, M)
+    res, pair = oracle.bpp(ins, out, u, M)
+    zr = ins[0, 0, -1]
+    ctx.set_tile(tile)
+    try:
+        z, r2, p2 = hotpath.posterior(ds, p, maxbasepairdistance=maxbp)
+        assert abs(z - zr) <= REL64 * abs(zr)
+        assert np.max(np.abs(r2 - res)) < BPP64 and np.max(np.abs(p2 - pair)) < BPP64
+        z32, r32, p32 = hotpath.posterior(ds, p, prec="fp32", maxbasepairdistance=maxbp)
+        assert abs(z32 - zr) <= 1e-6 * abs(zr)
+        assert max(np.max(np.abs(r32 - res)), np.max(np.abs(p32 - pair))) < 1e-5
+        # standalone entry points detect the band from the pair matrix
+        t = hotpath.computeinsideKH99(u, M, 1.0, False)
+        assert _rel(t, ins) < 1e-11
+        assert _rel(hotpath.computeoutsideKH99(ins, u, M), out) < 1e-10
+    finally:
+        ctx.set_tile(16)
