@@ -124,6 +124,15 @@ int messi_batch_eval(messi_batch *b, int K4, const double *P4, const double *pi4
                      int K16, const double *P16, const double *pi16, const double *logw16, double B,
                      int prec, double *logZ);
 int messi_batch_timings(const messi_batch *b, double *t);
+/* messi_batch_eval in two steps: messi_batch_load copies the mixture
+ * components (same arguments and layouts as messi_batch_eval) into device
+ * memory, converted to the precision prec; messi_batch_eval_loaded evaluates
+ * every member from them, as often as wanted, with no host-to-device transfer
+ * besides its launch parameters.  An evaluation in the other precision needs a
+ * new load.  No counterpart in the reference. */
+int messi_batch_load(messi_batch *b, int K4, const double *P4, const double *pi4, const double *logw4,
+                     int K16, const double *P16, const double *pi16, const double *logw16, int prec);
+int messi_batch_eval_loaded(messi_batch *b, double B, int prec, double *logZ);
 
 /* Learned MCMC proposal network: din inputs -> tanh(dh) -> tanh(dh) -> 2*din
  * outputs (mean shift mu, then log-scale ell, of a diagonal Gaussian proposal

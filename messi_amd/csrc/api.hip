@@ -83,6 +83,37 @@ struct messi_dataset {
   ~messi_dataset();
 };
 
+// Pinned host staging buffer (grow-only).
+struct HostBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void ensure(size_t n) {
+    if (n <= bytes) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    MESSI_HIP(hipHostMalloc(&p, n < 256 ? 256 : n, hipHostMallocDefault));
+    bytes = n < 256 ? 256 : n;
+  }
+  template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Mixture components resident on the device, component-major: the matrix of
+// component k for global node g is P + (k * nn_total + g) * dim2, member m's pi
+// and log weight are pi + (k * n + m) * pidim and logw[k * n + m].
+struct Components {
+  DevBuf P, pi, logw;
+  HostBuf stage;
+  int K = 0;
+  int prec = -1;  // element type of P (MESSI_FP64 / MESSI_FP32), -1 when nothing is loaded
+};
+
 struct messi_batch {
   messi_ctx *ctx = nullptr;
   int n = 0, Hmax = 0;
@@ -98,7 +129,8 @@ struct messi_batch {
   int64_t nn_total = 0, site_entries = 0, pair_entries = 0, site_root_total = 0, pair_root_total = 0;
   int Lmax = 0;
   DevBuf site_part, site_expo, site_acc, pair_part, pair_expo, pair_acc, pair_comp;
-  DevBuf dP, dpi, dlogw, dexp, douts;
+  Components c4, c16;  // loaded mixture components (site, pair)
+  DevBuf dexp, douts;
   std::vector<std::unique_ptr<InsideWork>> w64, w32;
   InsideBatch ib;
   Events ev;
@@ -518,26 +550,41 @@ messi_batch *self_batch(messi_dataset *ds) {
 
 // Member-major input (member m: K x nn_m x dim^2) -> component-major device arena.
 template <typename T>
-void upload_components(messi_batch *b, int K, int dim2, const double *P, const double *pi, int pidim,
-                       const double *logw) {
+void load_components(messi_batch *b, Components &c, int K, int dim2, const double *P, const double *pi,
+                     int pidim, const double *logw, int prec) {
   hipStream_t s = b->ctx->stream;
   const int n = b->n;
-  std::vector<T> Ph((size_t)K * b->nn_total * dim2);
-  std::vector<double> pih((size_t)K * n * pidim), lwh((size_t)K * n);
+  const size_t nP = (size_t)K * b->nn_total * dim2, nq = (size_t)K * n * pidim, nw = (size_t)K * n;
+  const size_t bP = nP * sizeof(T), bq = nq * sizeof(double), bw = nw * sizeof(double);
+  MESSI_HIP(hipStreamSynchronize(s));  // the staging buffer may still feed an earlier copy
+  c.stage.ensure(bP + bq + bw);
+  T *Ph = c.stage.as<T>();
+  double *pih = reinterpret_cast<double *>(c.stage.as<char>() + bP);
+  double *lwh = pih + nq;
   size_t src = 0;
   for (int m = 0; m < n; ++m) {
     const int nn = b->mem[m]->nn;
     for (int k = 0; k < K; ++k) {
-      T *dst = Ph.data() + ((size_t)k * b->nn_total + b->node_base[m]) * dim2;
+      T *dst = Ph + ((size_t)k * b->nn_total + b->node_base[m]) * dim2;
       for (size_t i = 0; i < (size_t)nn * dim2; ++i) dst[i] = (T)P[src + i];
       src += (size_t)nn * dim2;
       for (int a = 0; a < pidim; ++a) pih[((size_t)k * n + m) * pidim + a] = pi[((size_t)m * K + k) * pidim + a];
       lwh[(size_t)k * n + m] = logw[(size_t)m * K + k];
     }
   }
-  upload(b->dP, Ph, s);
-  upload(b->dpi, pih, s);
-  upload(b->dlogw, lwh, s);
+  c.P.ensure(bP);
+  c.pi.ensure(bq);
+  c.logw.ensure(bw);
+  MESSI_HIP(hipMemcpyAsync(c.P.p, Ph, bP, hipMemcpyHostToDevice, s));
+  MESSI_HIP(hipMemcpyAsync(c.pi.p, pih, bq, hipMemcpyHostToDevice, s));
+  MESSI_HIP(hipMemcpyAsync(c.logw.p, lwh, bw, hipMemcpyHostToDevice, s));
+  c.K = K;
+  c.prec = prec;
+}
+
+void require_loaded(const Components &c, int prec, const char *what) {
+  MESSI_REQUIRE(c.prec >= 0, std::string("no ") + what + " components loaded");
+  MESSI_REQUIRE(c.prec == prec, std::string(what) + " components were loaded for the other precision");
 }
 
 InsideWork &member_work(messi_batch *b, int m, int prec) {
@@ -563,10 +610,10 @@ void upload_expand(messi_batch *b, int prec, bool pairs) {
 }
 
 template <typename T>
-void run_site(messi_batch *b, int K, const double *P4, const double *pi4, const double *logw, int prec) {
+void run_site(messi_batch *b, int prec) {
   hipStream_t s = b->ctx->stream;
-  MESSI_REQUIRE(K >= 1, "need at least one site component");
-  upload_components<T>(b, K, 16, P4, pi4, 4, logw);
+  require_loaded(b->c4, prec, "site");
+  const int K = b->c4.K;
   b->site_part.ensure(std::max<int64_t>(1, b->site_entries) * 4 * sizeof(T));
   b->site_expo.ensure(std::max<int64_t>(1, b->site_entries) * sizeof(int));
   b->site_acc.ensure(std::max<int64_t>(1, b->site_root_total) * sizeof(double));
@@ -576,8 +623,9 @@ void run_site(messi_batch *b, int K, const double *P4, const double *pi4, const 
       int lo = b->site_lv[h - 1], hi = b->site_lv[h];
       launch_site_level<T>(s, b->site_items.as<int2>() + lo, hi - lo, b->site_nodes.as<NodeDesc>(), b->site_idx,
                            b->site_leaf, b->site_part.as<T>(), b->site_expo.as<int>(),
-                           b->dP.as<T>() + (size_t)k * b->nn_total * 16, b->dpi.as<double>() + (size_t)k * b->n * 4,
-                           b->dlogw.as<double>() + (size_t)k * b->n, k == 0, b->site_acc.as<double>());
+                           b->c4.P.as<T>() + (size_t)k * b->nn_total * 16,
+                           b->c4.pi.as<double>() + (size_t)k * b->n * 4, b->c4.logw.as<double>() + (size_t)k * b->n,
+                           k == 0, b->site_acc.as<double>());
     }
   upload_expand(b, prec, false);
   launch_site_expand(s, b->n, b->Lmax, b->dexp.as<ExpandDesc>(), b->site_acc.as<double>());
@@ -585,11 +633,10 @@ void run_site(messi_batch *b, int K, const double *P4, const double *pi4, const 
 }
 
 template <typename T>
-void run_pair(messi_batch *b, int K, const double *P16, const double *pi16, const double *logw, int prec,
-              double *h_percomp) {
+void run_pair(messi_batch *b, int prec, double *h_percomp) {
   hipStream_t s = b->ctx->stream;
-  MESSI_REQUIRE(K >= 1, "need at least one pair component");
-  upload_components<T>(b, K, 256, P16, pi16, 16, logw);
+  require_loaded(b->c16, prec, "pair");
+  const int K = b->c16.K;
   b->pair_part.ensure(std::max<int64_t>(1, b->pair_entries) * 16 * sizeof(T));
   b->pair_expo.ensure(std::max<int64_t>(1, b->pair_entries) * sizeof(int));
   b->pair_acc.ensure(std::max<int64_t>(1, b->pair_root_total) * sizeof(double));
@@ -602,9 +649,9 @@ void run_pair(messi_batch *b, int K, const double *P16, const double *pi16, cons
       int lo = b->pair_lv[h - 1], hi = b->pair_lv[h];
       launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
                                     b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), b->pair_expo.as<int>(),
-                                    b->dP.as<T>() + (size_t)k * b->nn_total * 256,
-                                    b->dpi.as<double>() + (size_t)k * b->n * 16,
-                                    b->dlogw.as<double>() + (size_t)k * b->n, k == 0, b->pair_acc.as<double>(),
+                                    b->c16.P.as<T>() + (size_t)k * b->nn_total * 256,
+                                    b->c16.pi.as<double>() + (size_t)k * b->n * 16,
+                                    b->c16.logw.as<double>() + (size_t)k * b->n, k == 0, b->pair_acc.as<double>(),
                                     b->pair_comp.as<double>());
     }
     if (h_percomp) {
@@ -650,9 +697,17 @@ void collect(messi_batch *b, int upto) {
 
 // Site + pair + inside for every member; logZ[m] on return.
 template <typename T>
-void eval_impl(messi_batch *b, int K4, const double *P4, const double *pi4, const double *logw4, int K16,
-               const double *P16, const double *pi16, const double *logw16, double B, bool keep_outside,
-               double *logZ) {
+void load_impl(messi_batch *b, int K4, const double *P4, const double *pi4, const double *logw4, int K16,
+               const double *P16, const double *pi16, const double *logw16) {
+  MESSI_REQUIRE(K4 >= 1 && K16 >= 1, "need at least one site and one pair component");
+  const int prec = sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64;
+  load_components<T>(b, b->c4, K4, 16, P4, pi4, 4, logw4, prec);
+  load_components<T>(b, b->c16, K16, 256, P16, pi16, 16, logw16, prec);
+}
+
+// Site + pair + inside for every member from the loaded components; logZ[m] on return.
+template <typename T>
+void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
   messi_ctx *ctx = b->ctx;
   hipStream_t s = ctx->stream;
   const int prec = sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64;
@@ -667,9 +722,9 @@ void eval_impl(messi_batch *b, int K4, const double *P4, const double *pi4, cons
   }
   if (ctx->profiling) b->ev.make();
   record(b, 0);
-  run_site<T>(b, K4, P4, pi4, logw4, prec);
+  run_site<T>(b, prec);
   record(b, 1);
-  run_pair<T>(b, K16, P16, pi16, logw16, prec, nullptr);
+  run_pair<T>(b, prec, nullptr);
   record(b, 2);
   inside_batch_run<T>(s, ws.data(), b->n, b->ib, B);
   record(b, 3);
@@ -788,9 +843,15 @@ int messi_site_loglik(messi_dataset *ds, int K, const double *P4, const double *
     MESSI_REQUIRE(P4 && pi4 && logw && u_out, "null argument");
     check_prec(prec);
     MESSI_HIP(hipSetDevice(ctx->device));
+    MESSI_REQUIRE(K >= 1, "need at least one site component");
     messi_batch *b = self_batch(ds);
-    if (prec == MESSI_FP32) run_site<float>(b, K, P4, pi4, logw, prec);
-    else run_site<double>(b, K, P4, pi4, logw, prec);
+    if (prec == MESSI_FP32) {
+      load_components<float>(b, b->c4, K, 16, P4, pi4, 4, logw, prec);
+      run_site<float>(b, prec);
+    } else {
+      load_components<double>(b, b->c4, K, 16, P4, pi4, 4, logw, prec);
+      run_site<double>(b, prec);
+    }
     InsideWork &w = member_work(b, 0, prec);
     MESSI_HIP(hipMemcpyAsync(u_out, w.u.p, sizeof(double) * ds->L, hipMemcpyDeviceToHost, ctx->stream));
     MESSI_HIP(hipStreamSynchronize(ctx->stream));
@@ -805,9 +866,15 @@ int messi_pair_loglik(messi_dataset *ds, int K, const double *P16, const double 
     MESSI_REQUIRE(P16 && pi16 && logw && out, "null argument");
     check_prec(prec);
     MESSI_HIP(hipSetDevice(ctx->device));
+    MESSI_REQUIRE(K >= 1, "need at least one pair component");
     messi_batch *b = self_batch(ds);
-    if (prec == MESSI_FP32) run_pair<float>(b, K, P16, pi16, logw, prec, percomp);
-    else run_pair<double>(b, K, P16, pi16, logw, prec, percomp);
+    if (prec == MESSI_FP32) {
+      load_components<float>(b, b->c16, K, 256, P16, pi16, 16, logw, prec);
+      run_pair<float>(b, prec, percomp);
+    } else {
+      load_components<double>(b, b->c16, K, 256, P16, pi16, 16, logw, prec);
+      run_pair<double>(b, prec, percomp);
+    }
     InsideWork &w = member_work(b, 0, prec);
     MESSI_HIP(hipMemcpyAsync(out, w.paired.p, sizeof(double) * ds->L * ds->L, hipMemcpyDeviceToHost,
                              ctx->stream));
@@ -935,8 +1002,13 @@ int messi_eval(messi_dataset *ds, int K4, const double *P4, const double *pi4, c
     check_prec(prec);
     MESSI_HIP(hipSetDevice(ctx->device));
     messi_batch *b = self_batch(ds);
-    if (prec == MESSI_FP32) eval_impl<float>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, B, false, logZ);
-    else eval_impl<double>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, B, false, logZ);
+    if (prec == MESSI_FP32) {
+      load_impl<float>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+      eval_impl<float>(b, B, false, logZ);
+    } else {
+      load_impl<double>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+      eval_impl<double>(b, B, false, logZ);
+    }
   });
 }
 
@@ -956,7 +1028,8 @@ int messi_posterior(messi_dataset *ds, int K4, const double *P4, const double *p
     auto go = [&](auto tag) {
       using T = decltype(tag);
       InsideWork &w = member_work(b, 0, sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64);
-      eval_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, 1.0, true, logZ);
+      load_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+      eval_impl<T>(b, 1.0, true, logZ);
       outside_run<T>(s, w, L, false);
       record(b, 4);
       w.out3.ensure(sizeof(double) * (LL + L));
@@ -1016,8 +1089,40 @@ int messi_batch_eval(messi_batch *b, int K4, const double *P4, const double *pi4
     MESSI_REQUIRE(B > 0.0, "B must be positive");
     check_prec(prec);
     MESSI_HIP(hipSetDevice(ctx->device));
-    if (prec == MESSI_FP32) eval_impl<float>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, B, false, logZ);
-    else eval_impl<double>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16, B, false, logZ);
+    if (prec == MESSI_FP32) {
+      load_impl<float>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+      eval_impl<float>(b, B, false, logZ);
+    } else {
+      load_impl<double>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+      eval_impl<double>(b, B, false, logZ);
+    }
+  });
+}
+
+int messi_batch_load(messi_batch *b, int K4, const double *P4, const double *pi4, const double *logw4, int K16,
+                     const double *P16, const double *pi16, const double *logw16, int prec) {
+  if (!b) return 2;
+  messi_ctx *ctx = b->ctx;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(P4 && pi4 && logw4 && P16 && pi16 && logw16, "null argument");
+    check_prec(prec);
+    MESSI_HIP(hipSetDevice(ctx->device));
+    if (prec == MESSI_FP32) load_impl<float>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+    else load_impl<double>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+    MESSI_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int messi_batch_eval_loaded(messi_batch *b, double B, int prec, double *logZ) {
+  if (!b) return 2;
+  messi_ctx *ctx = b->ctx;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(logZ, "null argument");
+    MESSI_REQUIRE(B > 0.0, "B must be positive");
+    check_prec(prec);
+    MESSI_HIP(hipSetDevice(ctx->device));
+    if (prec == MESSI_FP32) eval_impl<float>(b, B, false, logZ);
+    else eval_impl<double>(b, B, false, logZ);
   });
 }
 

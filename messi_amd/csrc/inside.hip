@@ -186,49 +186,51 @@ __device__ __forceinline__ XFv exact_cell_xf(const XFv (*A)[T + 1], const XFv (*
   return XF::finish(km, s);
 }
 
-// acc (+)= sum over split points h in tile K of L[IT+a, h] * S[h+1, JT+b]
+// Operands of one split-point product, one element of each tile per thread:
+// a = L[IT + tid/T, KT + tid%T], b = S[KT + tid%T + 1, JT + tid/T].
+template <class SR> struct Operands {
+  typename SR::V a, b;
+};
+
 template <class SR, int T>
-__device__ __forceinline__ void tile_product(int L, int I, int K, int J, const typename SR::V *__restrict__ Lr,
-                             const typename SR::V *__restrict__ Sc, StepLDS<SR, T> &sm) {
+__device__ __forceinline__ Operands<SR> fetch_operands(int L, int I, int K, int J,
+                                                       const typename SR::V *__restrict__ Lr,
+                                                       const typename SR::V *__restrict__ Sc) {
+  const int tid = threadIdx.x, r = tid / T, c = tid % T;
+  const int j = I * T + r, h = K * T + c;
+  const int y = J * T + r, sr = K * T + c + 1;
+  Operands<SR> o;
+  o.a = (j < L && h < L) ? Lr[(size_t)j * L + h] : SR::zero();
+  o.b = (y < L && sr < L) ? Sc[(size_t)y * L + sr] : SR::zero();
+  return o;
+}
+
+// acc (+)= sum over split points h in tile K of L[IT+a, h] * S[h+1, JT+b], from
+// operands already fetched into registers.
+template <class SR, int T>
+__device__ __forceinline__ void tile_product(const Operands<SR> &o, StepLDS<SR, T> &sm) {
   typedef typename SR::V V;
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < T * T; idx += T * T) {
-    int a = idx / T, c = idx % T;
-    int j = I * T + a, h = K * T + c;
-    sm.u.p.A[a][c] = (j < L && h < L) ? Lr[(size_t)j * L + h] : SR::zero();
-  }
-  for (int idx = tid; idx < T * T; idx += T * T) {
-    int b = idx / T, c = idx % T;
-    int y = J * T + b, r = K * T + c + 1;
-    sm.u.p.B[c][b] = (y < L && r < L) ? Sc[(size_t)y * L + r] : SR::zero();
-  }
-  __syncthreads();
+  const int tid = threadIdx.x, r = tid / T, c = tid % T;
+  sm.u.p.A[r][c] = o.a;
+  sm.u.p.B[c][r] = o.b;
   if constexpr (SR::scaled) {
-    // Exponent-aligned float tiles, then one MFMA 16x16x4 f32 chain per 16x16 block.
-    if (tid < T) {
-      int m = XF::EZ;
-      for (int c = 0; c < T; ++c) {
-        V v = sm.u.p.A[tid][c];
-        if (v.m != 0.0f) m = max(m, v.e);
-      }
-      sm.rA[tid] = m;
-    } else if (tid >= 64 && tid < 64 + T) {
-      int b = tid - 64, m = XF::EZ;
-      for (int c = 0; c < T; ++c) {
-        V v = sm.u.p.B[c][b];
-        if (v.m != 0.0f) m = max(m, v.e);
-      }
-      sm.cB[b] = m;
+    // Row maxima of A's exponents and column maxima of B's (over the split
+    // points c, held by T consecutive lanes), then exponent-aligned floats.
+    int ea = o.a.m != 0.0f ? o.a.e : XF::EZ;
+    int eb = o.b.m != 0.0f ? o.b.e : XF::EZ;
+#pragma unroll
+    for (int off = T / 2; off > 0; off >>= 1) {
+      ea = max(ea, __shfl_xor(ea, off));
+      eb = max(eb, __shfl_xor(eb, off));
+    }
+    sm.u.p.Ap[r][c] = o.a.m == 0.0f ? 0.0f : ldexpf(o.a.m, max(o.a.e - ea, -200));
+    sm.u.p.Bp[c][r] = o.b.m == 0.0f ? 0.0f : ldexpf(o.b.m, max(o.b.e - eb, -200));
+    if (c == 0) {
+      sm.rA[r] = ea;
+      sm.cB[r] = eb;
     }
     __syncthreads();
-    for (int idx = tid; idx < T * T; idx += T * T) {
-      int a = idx / T, c = idx % T;
-      V v = sm.u.p.A[a][c];
-      sm.u.p.Ap[a][c] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - sm.rA[a], -200));
-      V w = sm.u.p.B[a][c];  // row a of B = split point, column c = target column
-      sm.u.p.Bp[a][c] = w.m == 0.0f ? 0.0f : ldexpf(w.m, max(w.e - sm.cB[c], -200));
-    }
-    __syncthreads();
+    // One MFMA 16x16x4 f32 chain per 16x16 block.
     typedef float v4f __attribute__((ext_vector_type(4)));
     const int wave = tid >> 6, lane = tid & 63;
     constexpr int NB = T / 16;
@@ -242,12 +244,12 @@ __device__ __forceinline__ void tile_product(int L, int I, int K, int J, const t
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int a = bi * 16 + 4 * (lane >> 4) + r, b = bj * 16 + (lane & 15);
+      for (int q = 0; q < 4; ++q) {
+        const int a = bi * 16 + 4 * (lane >> 4) + q, b = bj * 16 + (lane & 15);
         const int ra = sm.rA[a], cb = sm.cB[b];
         V contrib = SR::zero();
         if (ra != XF::EZ && cb != XF::EZ) {
-          float cv = acc[r];
+          float cv = acc[q];
           if (cv >= 0x1p-100f) {
             contrib = XF::norm(cv, ra + cb);
           } else {
@@ -258,22 +260,29 @@ __device__ __forceinline__ void tile_product(int L, int I, int K, int J, const t
       }
     }
   } else {
+    __syncthreads();
     for (int idx = tid; idx < T * T; idx += T * T) {
       int a = idx / T, b = idx % T;
       typename SR::K km = SR::kmin();
-      for (int c = 0; c < T; ++c) km = SR::kmax(km, SR::key(sm.u.p.A[a][c], sm.u.p.B[c][b]));
-      typename SR::S s = 0;
+      for (int k = 0; k < T; ++k) km = SR::kmax(km, SR::key(sm.u.p.A[a][k], sm.u.p.B[k][b]));
+      typename SR::S sum = 0;
       if (km != SR::kmin())
-        for (int c = 0; c < T; ++c) s += SR::contrib(sm.u.p.A[a][c], sm.u.p.B[c][b], km);
-      sm.cA[a][b] = SR::add(sm.cA[a][b], SR::finish(km, s));
+        for (int k = 0; k < T; ++k) sum += SR::contrib(sm.u.p.A[a][k], sm.u.p.B[k][b], km);
+      sm.cA[a][b] = SR::add(sm.cA[a][b], SR::finish(km, sum));
     }
   }
   __syncthreads();
 }
 
+// Bulk items of launch t: tiles at distance d = t+1 .. dmax, NT - d of them per
+// distance; C(d) counts the items of distances t+1 .. d.
+__device__ __forceinline__ int bulk_count(int NT, int t, int d) {
+  return (d - t) * NT - (d * (d + 1) - t * (t + 1)) / 2;
+}
+
 // One launch of the inside schedule (see the file header).
 template <class SR, int T>
-__global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
+__global__ __launch_bounds__(T * T, T == 16 ? 8 : 1) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
                                                             const int *__restrict__ prefix, int n, int t,
                                                             Rules<SR> r) {
   typedef typename SR::V V;
@@ -304,33 +313,35 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     J = item + t;
   } else {
     fin = false;
-    int idx = item - (NT - t);
-    int dist = t + 1;
-    while (idx >= NT - dist) {
-      idx -= NT - dist;
-      ++dist;
+    const int idx = item - (NT - t);
+    int lo = t + 1, hi = NT - 1;  // smallest distance d with bulk_count(d) > idx
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (bulk_count(NT, t, mid) > idx) hi = mid;
+      else lo = mid + 1;
     }
-    I = idx;
-    J = idx + dist;
+    I = idx - bulk_count(NT, t, lo - 1);
+    J = I + lo;
   }
   const int IT = I * T, JT = J * T;
-  // ---- bulk accumulator of the target tile
-  if (t >= 2 || !fin) {
-    for (int idx = tid; idx < T * T; idx += T * T) {
-      int a = idx / T, b = idx % T, j = IT + a, y = JT + b;
-      sm.cA[a][b] = (j < L && y < L) ? acc[(size_t)j * L + y] : SR::zero();
-    }
-  } else {
-    for (int idx = tid; idx < T * T; idx += T * T) sm.cA[idx / T][idx % T] = SR::zero();
+  // ---- split-point products that became ready with tile-diagonal t-1: every
+  // global load (accumulator and both products' operands) is issued up front.
+  // L[j, h] is -inf for h - j beyond the pairing band: such L tiles contribute nothing.
+  const int K1 = I + t - 1, K2 = J - (t - 1);
+  const bool do1 = t >= 2 && K1 - I <= ins.KB;
+  const bool do2 = t >= 2 && K2 != K1 && K2 - I <= ins.KB;
+  V accv = SR::zero();
+  {
+    const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
+    if ((t >= 2 || !fin) && j < L && y < L) accv = acc[(size_t)j * L + y];
   }
+  Operands<SR> o1, o2;
+  if (do1) o1 = fetch_operands<SR, T>(L, I, K1, J, Lr, Sc);
+  if (do2) o2 = fetch_operands<SR, T>(L, I, K2, J, Lr, Sc);
+  sm.cA[tid / T][tid % T] = accv;
+  if (do1) tile_product<SR, T>(o1, sm);
+  if (do2) tile_product<SR, T>(o2, sm);
   __syncthreads();
-  // ---- split-point products that became ready with tile-diagonal t-1
-  if (t >= 2) {
-    // L[j, h] is -inf for h - j beyond the pairing band: such L tiles contribute nothing.
-    const int K1 = I + t - 1, K2 = J - (t - 1);
-    if (K1 - I <= ins.KB) tile_product<SR, T>(L, I, K1, J, Lr, Sc, sm);
-    if (K2 != K1 && K2 - I <= ins.KB) tile_product<SR, T>(L, I, K2, J, Lr, Sc, sm);
-  }
   if (!fin) {
     for (int idx = tid; idx < T * T; idx += T * T) {
       int a = idx / T, b = idx % T, j = IT + a, y = JT + b;

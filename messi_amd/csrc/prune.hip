@@ -116,42 +116,55 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
 #pragma unroll
     for (int r = 0; r < 4; ++r) pis[r] = pi16[16 * nd.member + Mfma<T>::state(g, r)];
   }
+  // Every load of the NB blocks is issued before the first store: part is both
+  // read (children) and written (this node), so loads after a store could not
+  // be hoisted above it.
+  int cl[NB], cr[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int64_t e = (int64_t)it.y + b * 16 + col;
+    const int64_t ec = e < nd.U ? e : nd.U - 1;
+    cl[b] = idx[nd.idx_l + ec];
+    cr[b] = idx[nd.idx_r + ec];
+  }
+  T bl[NB][4], br[NB][4];
+  int el[NB], er[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    el[b] = 0;
+    er[b] = 0;
+    if (nd.leaf_l) {
+      const unsigned cc = leafc[nd.cpart_l + cl[b]];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int a = Mfma<T>::state(g, s);
+        bl[b][s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
+      }
+    } else {
+      load4(part + (size_t)(nd.cpart_l + cl[b]) * 16 + 4 * g, bl[b]);
+      el[b] = expo[nd.cpart_l + cl[b]];
+    }
+    if (nd.leaf_r) {
+      const unsigned cc = leafc[nd.cpart_r + cr[b]];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int a = Mfma<T>::state(g, s);
+        br[b][s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
+      }
+    } else {
+      load4(part + (size_t)(nd.cpart_r + cr[b]) * 16 + 4 * g, br[b]);
+      er[b] = expo[nd.cpart_r + cr[b]];
+    }
+  }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int64_t e = (int64_t)it.y + b * 16 + col;
     const bool valid = e < nd.U;
-    const int64_t ec = valid ? e : nd.U - 1;
-    T bl[4], br[4];
-    int el = 0, er = 0;
-    const int cl = idx[nd.idx_l + ec];
-    const int cr = idx[nd.idx_r + ec];
-    if (nd.leaf_l) {
-      const unsigned cc = leafc[nd.cpart_l + cl];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int a = Mfma<T>::state(g, s);
-        bl[s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
-      }
-    } else {
-      load4(part + (size_t)(nd.cpart_l + cl) * 16 + 4 * g, bl);
-      el = expo[nd.cpart_l + cl];
-    }
-    if (nd.leaf_r) {
-      const unsigned cc = leafc[nd.cpart_r + cr];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int a = Mfma<T>::state(g, s);
-        br[s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
-      }
-    } else {
-      load4(part + (size_t)(nd.cpart_r + cr) * 16 + 4 * g, br);
-      er = expo[nd.cpart_r + cr];
-    }
     v4 ml = {0, 0, 0, 0}, mr = {0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      ml = Mfma<T>::op(al[s], bl[s], ml);
-      mr = Mfma<T>::op(ar[s], br[s], mr);
+      ml = Mfma<T>::op(al[s], bl[b][s], ml);
+      mr = Mfma<T>::op(ar[s], br[b][s], mr);
     }
     T v[4];
 #pragma unroll
@@ -159,7 +172,7 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
     T m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
     m = fmax(m, __shfl_xor(m, 16));
     m = fmax(m, __shfl_xor(m, 32));
-    int E = el + er;
+    int E = el[b] + er[b];
     if (m > (T)0) {
       const int ex = frexp_exp(m);
       const T sc = ldexp((T)1, -ex);
@@ -187,7 +200,6 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
     }
   }
 }
-
 // One thread per entry of a node; 256 entries per item.
 template <typename T>
 __global__ __launch_bounds__(256) void site_level_kernel(

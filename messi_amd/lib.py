@@ -16,7 +16,8 @@ from typing import Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libmessi.so")
+# MESSI_LIBRARY selects another build of the same library (for comparing builds).
+LIBPATH = os.environ.get("MESSI_LIBRARY") or os.path.join(HERE, "libmessi.so")
 
 FP64 = 0
 FP32 = 1
@@ -56,6 +57,8 @@ SIGNATURES = {
     "messi_batch_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                                         ctypes.c_double, ctypes.c_int, _dp]),
     "messi_batch_timings": (ctypes.c_int, [_vp, _dp]),
+    "messi_batch_load": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
+    "messi_batch_eval_loaded": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp]),
     "messi_mlp_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "messi_mlp_destroy": (None, [_vp]),
     "messi_mlp_num_params": (ctypes.c_int, [_vp, _i64]),
@@ -337,6 +340,28 @@ class Batch:
         self.ctx.check(self.ctx.lib.messi_batch_create(self.ctx.handle, len(self.members), arr, ctypes.byref(h)))
         self._h = h
         _live.add(self)
+
+    def _inputs(self, sites, pairs):
+        n = len(self.members)
+        P4 = np.concatenate([colmajor_P(s[0]).reshape(-1) for s in sites])
+        pi4 = np.ascontiguousarray(np.concatenate([np.asarray(s[1], dtype=np.float64).reshape(-1) for s in sites]))
+        lw4 = np.ascontiguousarray(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1) for s in sites]))
+        P16 = np.concatenate([colmajor_P(p[0]).reshape(-1) for p in pairs])
+        pi16 = np.ascontiguousarray(np.concatenate([np.asarray(p[1], dtype=np.float64).reshape(-1) for p in pairs]))
+        lw16 = np.ascontiguousarray(np.concatenate([np.asarray(p[2], dtype=np.float64).reshape(-1) for p in pairs]))
+        return P4, pi4, lw4, len(lw4) // n, P16, pi16, lw16, len(lw16) // n
+
+    def load(self, sites, pairs, prec=FP64) -> None:
+        """Copy the members' mixture components to the device (messi_batch_load)."""
+        P4, pi4, lw4, K4, P16, pi16, lw16, K16 = self._inputs(sites, pairs)
+        self.ctx.check(self.ctx.lib.messi_batch_load(self._h, K4, ptr(P4), ptr(pi4), ptr(lw4), K16, ptr(P16),
+                                                     ptr(pi16), ptr(lw16), prec_code(prec)))
+
+    def eval_loaded(self, B: float = 1.0, prec=FP64) -> np.ndarray:
+        """log Z of every member from the components of the last load."""
+        z = np.empty(len(self.members))
+        self.ctx.check(self.ctx.lib.messi_batch_eval_loaded(self._h, float(B), prec_code(prec), ptr(z)))
+        return z
 
     def eval(self, sites, pairs, B: float = 1.0, prec=FP64) -> np.ndarray:
         """sites[m] = (P4 (K,nn,4,4), pi4 (K,4), logw (K,)); pairs[m] likewise with 16 states."""

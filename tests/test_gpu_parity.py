@@ -301,3 +301,26 @@ def test_pairing_band_posterior(ctx, tile, maxbp):
         assert _rel(hotpath.computeoutsideKH99(ins, u, M), out) < 1e-10
     finally:
         ctx.set_tile(16)
+
+
+def test_batch_load_then_eval_loaded(ctx):
+    """messi_batch_load + messi_batch_eval_loaded equal messi_batch_eval; a
+    precision other than the loaded one is an error."""
+    _say("batch load / eval_loaded")
+    a = synth.config1()
+    ds = synth.dataset_of(a)
+    dd = hotpath.device_dataset(ds)
+    ps = [model.getparams(a.params * (1 + 0.02 * i), ds.tree.branchlengths) for i in range(3)]
+    comps = [_stack_params(p) for p in ps]
+    b = lib.Batch([dd] * 3)
+    try:
+        for prec in ("fp64", "fp32"):
+            want = b.eval([c[0] for c in comps], [c[1] for c in comps], 1.0, prec)
+            b.load([c[0] for c in comps], [c[1] for c in comps], prec)
+            z1 = b.eval_loaded(1.0, prec)
+            z2 = b.eval_loaded(1.0, prec)
+            assert np.array_equal(z1, want) and np.array_equal(z2, want)
+        with pytest.raises(lib.MessiError):
+            b.eval_loaded(1.0, "fp64")      # fp32 components are loaded
+    finally:
+        b.close()
