@@ -130,7 +130,8 @@ struct messi_batch {
   int Lmax = 0;
   DevBuf site_part, site_expo, site_acc, pair_part, pair_expo, pair_acc, pair_comp;
   Components c4, c16;  // loaded mixture components (site, pair)
-  DevBuf dexp, douts;
+  DevBuf dexp_site, dexp_pair, douts;
+  std::vector<ExpandDesc> hexp_site, hexp_pair;  // what dexp_site / dexp_pair hold
   std::vector<std::unique_ptr<InsideWork>> w64, w32;
   InsideBatch ib;
   Events ev;
@@ -591,7 +592,9 @@ InsideWork &member_work(messi_batch *b, int m, int prec) {
   return prec == MESSI_FP32 ? *b->w32[m] : *b->w64[m];
 }
 
-void upload_expand(messi_batch *b, int prec, bool pairs) {
+// Expansion descriptors of the site (pairs = false) or pair root entries; the
+// device copy is refreshed only when a buffer it points to has moved.
+const ExpandDesc *upload_expand(messi_batch *b, int prec, bool pairs) {
   std::vector<ExpandDesc> e(b->n);
   for (int m = 0; m < b->n; ++m) {
     messi_dataset *d = b->mem[m];
@@ -606,7 +609,13 @@ void upload_expand(messi_batch *b, int prec, bool pairs) {
     e[m].u = w.u.as<double>();
     e[m].paired = w.paired.as<double>();
   }
-  upload(b->dexp, e, b->ctx->stream);
+  std::vector<ExpandDesc> &h = pairs ? b->hexp_pair : b->hexp_site;
+  DevBuf &d = pairs ? b->dexp_pair : b->dexp_site;
+  if (h.size() != e.size() || std::memcmp(h.data(), e.data(), e.size() * sizeof(ExpandDesc)) != 0) {
+    upload(d, e, b->ctx->stream);
+    h = e;
+  }
+  return d.as<ExpandDesc>();
 }
 
 template <typename T>
@@ -627,13 +636,15 @@ void run_site(messi_batch *b, int prec) {
                            b->c4.pi.as<double>() + (size_t)k * b->n * 4, b->c4.logw.as<double>() + (size_t)k * b->n,
                            k == 0, b->site_acc.as<double>());
     }
-  upload_expand(b, prec, false);
-  launch_site_expand(s, b->n, b->Lmax, b->dexp.as<ExpandDesc>(), b->site_acc.as<double>());
+  const ExpandDesc *dexp = upload_expand(b, prec, false);
+  launch_site_expand(s, b->n, b->Lmax, dexp, b->site_acc.as<double>());
   MESSI_HIP(hipGetLastError());
 }
 
 template <typename T>
-void run_pair(messi_batch *b, int prec, double *h_percomp) {
+// expand: write each member's L x L pair matrix (the evaluation path instead
+// builds the inside pair term straight from the root entries).
+void run_pair(messi_batch *b, int prec, double *h_percomp, bool expand) {
   hipStream_t s = b->ctx->stream;
   require_loaded(b->c16, prec, "pair");
   const int K = b->c16.K;
@@ -641,9 +652,10 @@ void run_pair(messi_batch *b, int prec, double *h_percomp) {
   b->pair_expo.ensure(std::max<int64_t>(1, b->pair_entries) * sizeof(int));
   b->pair_acc.ensure(std::max<int64_t>(1, b->pair_root_total) * sizeof(double));
   b->pair_comp.ensure(std::max<int64_t>(1, b->pair_root_total) * sizeof(double));
-  for (int m = 0; m < b->n; ++m)
-    member_work(b, m, prec).paired.ensure(sizeof(double) * b->mem[m]->L * b->mem[m]->L);
-  upload_expand(b, prec, true);
+  if (expand || h_percomp)
+    for (int m = 0; m < b->n; ++m)
+      member_work(b, m, prec).paired.ensure(sizeof(double) * b->mem[m]->L * b->mem[m]->L);
+  const ExpandDesc *dexp = upload_expand(b, prec, true);
   for (int k = 0; k < K; ++k) {
     for (int h = 1; h <= b->Hmax; ++h) {
       int lo = b->pair_lv[h - 1], hi = b->pair_lv[h];
@@ -662,12 +674,12 @@ void run_pair(messi_batch *b, int prec, double *h_percomp) {
       double *o = w.out3.as<double>();
       std::vector<double *> outs(1, o);
       upload(b->douts, outs, s);
-      launch_pair_expand(s, 1, L, b->dexp.as<ExpandDesc>(), b->pair_comp.as<double>(), b->douts.as<double *>());
+      launch_pair_expand(s, 1, L, dexp, b->pair_comp.as<double>(), b->douts.as<double *>());
       MESSI_HIP(hipMemcpyAsync(h_percomp + (size_t)k * L * L, o, sizeof(double) * L * L, hipMemcpyDeviceToHost, s));
       MESSI_HIP(hipStreamSynchronize(s));
     }
   }
-  launch_pair_expand(s, b->n, b->Lmax, b->dexp.as<ExpandDesc>(), b->pair_acc.as<double>(), nullptr);
+  if (expand) launch_pair_expand(s, b->n, b->Lmax, dexp, b->pair_acc.as<double>(), nullptr);
   MESSI_HIP(hipGetLastError());
 }
 
@@ -724,9 +736,9 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
   record(b, 0);
   run_site<T>(b, prec);
   record(b, 1);
-  run_pair<T>(b, prec, nullptr);
+  run_pair<T>(b, prec, nullptr, false);
   record(b, 2);
-  inside_batch_run<T>(s, ws.data(), b->n, b->ib, B);
+  inside_batch_run<T>(s, ws.data(), b->n, b->ib, B, b->dexp_pair.as<ExpandDesc>(), b->pair_acc.as<double>());
   record(b, 3);
   std::vector<double> z(2 * b->n);
   for (int m = 0; m < b->n; ++m)
@@ -870,10 +882,10 @@ int messi_pair_loglik(messi_dataset *ds, int K, const double *P16, const double 
     messi_batch *b = self_batch(ds);
     if (prec == MESSI_FP32) {
       load_components<float>(b, b->c16, K, 256, P16, pi16, 16, logw, prec);
-      run_pair<float>(b, prec, percomp);
+      run_pair<float>(b, prec, percomp, true);
     } else {
       load_components<double>(b, b->c16, K, 256, P16, pi16, 16, logw, prec);
-      run_pair<double>(b, prec, percomp);
+      run_pair<double>(b, prec, percomp, true);
     }
     InsideWork &w = member_work(b, 0, prec);
     MESSI_HIP(hipMemcpyAsync(out, w.paired.p, sizeof(double) * ds->L * ds->L, hipMemcpyDeviceToHost,

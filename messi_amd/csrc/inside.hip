@@ -104,6 +104,27 @@ __global__ void ptv_kernel(const InstDesc<typename SR::V> *__restrict__ insts, d
   d.ptv[(size_t)j * L + y] = SR::from_log(x);
 }
 
+// The same pair term gathered straight from the pair pruning's root entries
+// (no L x L pair matrix): the pair (j, y) maps to root entry
+// T0[pid0[j], pid0[y]] when y - j <= maxbp and the mask allows it.
+template <class SR>
+__global__ void ptv_root_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
+                                const ExpandDesc *__restrict__ exps, const double *__restrict__ rootacc, double B) {
+  const InstDesc<typename SR::V> d = insts[blockIdx.z];
+  const ExpandDesc e = exps[blockIdx.z];
+  const int L = d.L;
+  int y = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (y >= L || j >= L) return;
+  double x = -HUGE_VAL;
+  if (j < y && y - j <= e.maxbp && (!e.mask || e.mask[(size_t)j + (size_t)L * y])) {
+    const int k = e.T0[(size_t)e.pid0[j] * e.n0 + e.pid0[y]];
+    const double v = k >= 0 ? rootacc[e.rootoff + k] : -HUGE_VAL;
+    if (v != -HUGE_VAL) x = v * B - d.lam[j] - d.lam[y];
+  }
+  d.ptv[(size_t)j * L + y] = SR::from_log(x);
+}
+
 // Zero the bulk accumulator; set the diagonal of S and L (F has no s rule).
 template <class SR>
 __global__ void inside_init_kernel(const InstDesc<typename SR::V> *__restrict__ insts, double B, double lp1,
@@ -753,12 +774,17 @@ static InstDesc<typename SR::V> inst_of(InsideWork &w) {
 
 // Baseline and pair term of every instance (u and paired already on the device).
 template <class SR>
-static void prepare_batch(hipStream_t s, const InstDesc<typename SR::V> *d_inst, int n, int Lmax, double B) {
+static void prepare_batch(hipStream_t s, const InstDesc<typename SR::V> *d_inst, int n, int Lmax, double B,
+                          const ExpandDesc *dexp = nullptr, const double *rootacc = nullptr) {
   double lp[7];
   kh99_logprobs(lp);
   hipLaunchKernelGGL(baseline_kernel<typename SR::V>, dim3(n), dim3(1024), 0, s, d_inst, B, lp[0] + lp[5],
                      (int)SR::scaled);
-  hipLaunchKernelGGL(ptv_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, B);
+  if (dexp)
+    hipLaunchKernelGGL(ptv_root_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, dexp,
+                       rootacc, B);
+  else
+    hipLaunchKernelGGL(ptv_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, B);
 }
 
 template <typename T>
@@ -772,7 +798,8 @@ void inside_prepare(hipStream_t s, InsideWork &w, int L, double B) {
 // Inside pass (preparation included) for a batch of instances; each work's red[0]
 // receives its log Z.
 template <typename T>
-void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &ib, double B) {
+void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &ib, double B,
+                      const ExpandDesc *dexp, const double *rootacc) {
   typedef typename SRof<T>::type SR;
   typedef typename SR::V V;
   if (n <= 0) return;
@@ -815,7 +842,7 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
   double lp[7];
   kh99_logprobs(lp);
   Rules<SR> r = make_rules<SR>(B);
-  prepare_batch<SR>(s, di, n, Lmax, B);
+  prepare_batch<SR>(s, di, n, Lmax, B, dexp, rootacc);
   hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, di, B, lp[1],
                      lp[5]);
   for (int t = 0; t < NTmax; ++t) {
@@ -928,7 +955,8 @@ void bpp_run(hipStream_t s, InsideWork &w, int L, double *d_unpaired, double *d_
   template void inside_alloc<T>(InsideWork &, int, bool);                               \
   template void inside_prepare<T>(hipStream_t, InsideWork &, int, double);              \
   template void inside_run<T>(hipStream_t, InsideWork &, int, double);                  \
-  template void inside_batch_run<T>(hipStream_t, InsideWork *const *, int, InsideBatch &, double); \
+  template void inside_batch_run<T>(hipStream_t, InsideWork *const *, int, InsideBatch &, double,    \
+                                    const ExpandDesc *, const double *);                       \
   template double inside_logZ<T>(hipStream_t, InsideWork &, int);                       \
   template void inside_export<T>(hipStream_t, InsideWork &, int, double *);             \
   template void inside_import<T>(hipStream_t, InsideWork &, int, const double *);       \
