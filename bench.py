@@ -107,7 +107,8 @@ def time_batch(b, sites, pairs, prec, steps, warmup):
 
 
 def main_instances(args):
-    """64 uneven instances (L ~ U[80,600], N ~ U[10,60]) in one batch vs 64 calls."""
+    """64 uneven instances (L ~ U[80,600], N ~ U[10,60]) in one batch vs 64 calls,
+    with the inside tile edges chosen by the autotuner for the batch."""
     from messi_amd import lib, model, synth
     from messi_amd.hotpath import device_dataset
     als = synth.config4()
@@ -115,21 +116,36 @@ def main_instances(args):
     dds = [device_dataset(d) for d in dss]
     comps = [_components(model.getparams(a.params, d.tree.branchlengths)) for a, d in zip(als, dss)]
     sites, pairs = [c[0] for c in comps], [c[1] for c in comps]
+    from messi_amd import autotune
     b = lib.Batch(dds)
+    ctx = lib.context()
     res = {}
     for prec in ("fp64", "fp32"):
+        # schedule from the autotuner (timed on this batch, outside the timed region)
+        b.load(sites, pairs, prec)
+        tuned = autotune.tune_batch(b, prec)
         ms_b, zb, ph, ms_bh = time_batch(b, sites, pairs, prec, args.steps, args.warmup)
+        tile_of = autotune.tiles_for([d.numcols for d in dss], tuned["threshold"])
+
+        def sequential():
+            out = []
+            for dd, s, p, t in zip(dds, sites, pairs, tile_of):
+                ctx.set_tile(t)
+                out.append(dd.eval(s, p, 1.0, prec))
+            ctx.set_tile(16)
+            return out
         for _ in range(args.warmup):
-            [dd.eval(s, p, 1.0, prec) for dd, s, p in zip(dds, sites, pairs)]
+            sequential()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            zs = [dd.eval(s, p, 1.0, prec) for dd, s, p in zip(dds, sites, pairs)]
+            zs = sequential()
         ms_s = (time.perf_counter() - t0) * 1e3 / args.steps
         res[prec] = {"batched_evals_per_s": 64e3 / ms_b, "batched_evals_per_s_host_inputs": 64e3 / ms_bh,
                      "sequential_evals_per_s": 64e3 / ms_s,
                      "speedup_host_inputs": ms_s / ms_bh,
                      "max_rel_diff": float(np.max(np.abs(zb - zs) / np.abs(zs))),
-                     "batched_phase_ms": {"site": ph[0], "pair": ph[1], "inside": ph[2]}}
+                     "batched_phase_ms": {"site": ph[0], "pair": ph[1], "inside": ph[2]},
+                     "autotuner": {"tile32_from_length": tuned["threshold"], "candidates_ms": tuned["candidates"]}}
     out = {"metric": "total evals/s, 64 uneven instances in one batch vs 64 sequential calls",
            "value": res["fp32"]["batched_evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "higher_is_better": True, "dtype": "f32", "data": "synthetic",

@@ -133,6 +133,11 @@ int messi_batch_timings(const messi_batch *b, double *t);
 int messi_batch_load(messi_batch *b, int K4, const double *P4, const double *pi4, const double *logw4,
                      int K16, const double *P16, const double *pi16, const double *logw16, int prec);
 int messi_batch_eval_loaded(messi_batch *b, double B, int prec, double *logZ);
+/* Inside tile edge per member (n values, each 16, 32, or 0 for the context's
+ * setting; NULL resets all to 0).  Members of one edge share one launch
+ * sequence.  Set by the autotuner (messi_amd/autotune.py).  No counterpart in
+ * the reference. */
+int messi_batch_set_tiles(messi_batch *b, const int *tiles);
 
 /* Learned MCMC proposal network: din inputs -> tanh(dh) -> tanh(dh) -> 2*din
  * outputs (mean shift mu, then log-scale ell, of a diagonal Gaussian proposal

@@ -133,7 +133,9 @@ struct messi_batch {
   DevBuf dexp_site, dexp_pair, douts;
   std::vector<ExpandDesc> hexp_site, hexp_pair;  // what dexp_site / dexp_pair hold
   std::vector<std::unique_ptr<InsideWork>> w64, w32;
-  InsideBatch ib;
+  InsideBatch ib, ib32;     // launch tables of the tile-16 and tile-32 groups
+  std::vector<int> tiles;   // per-member inside tile edge (0: the context's)
+  DevBuf zall;              // gathered log Z of all members (2 per member)
   Events ev;
   double t[5] = {0, 0, 0, 0, 0};
 };
@@ -724,9 +726,11 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
   hipStream_t s = ctx->stream;
   const int prec = sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64;
   std::vector<InsideWork *> ws(b->n);
+  b->zall.ensure(sizeof(double) * 2 * b->n);
   for (int m = 0; m < b->n; ++m) {
     ws[m] = &member_work(b, m, prec);
-    ws[m]->tile = ctx->tile;
+    ws[m]->tile = (m < (int)b->tiles.size() && b->tiles[m]) ? b->tiles[m] : ctx->tile;
+    ws[m]->zout = b->zall.as<double>() + 2 * m;
     // pair scores beyond the distance bound are -inf (pair_expand writes them so)
     const int D = b->mem[m]->maxbp, Lm = b->mem[m]->L;
     ws[m]->band = D < Lm - 1 ? D : -1;
@@ -738,11 +742,24 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
   record(b, 1);
   run_pair<T>(b, prec, nullptr, false);
   record(b, 2);
-  inside_batch_run<T>(s, ws.data(), b->n, b->ib, B, b->dexp_pair.as<ExpandDesc>(), b->pair_acc.as<double>());
+  // one launch sequence per tile edge, each over its members
+  for (int TE : {16, 32}) {
+    std::vector<InsideWork *> g;
+    for (int m = 0; m < b->n; ++m)
+      if (ws[m]->tile == TE) {
+        ws[m]->exp = b->dexp_pair.as<ExpandDesc>() + m;
+        g.push_back(ws[m]);
+      }
+    if (!g.empty())
+      inside_batch_run<T>(s, g.data(), (int)g.size(), TE == 32 ? b->ib32 : b->ib, B, b->pair_acc.as<double>());
+  }
+  for (int m = 0; m < b->n; ++m) {
+    ws[m]->exp = nullptr;
+    ws[m]->zout = nullptr;
+  }
   record(b, 3);
   std::vector<double> z(2 * b->n);
-  for (int m = 0; m < b->n; ++m)
-    MESSI_HIP(hipMemcpyAsync(&z[2 * m], ws[m]->red.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  MESSI_HIP(hipMemcpyAsync(z.data(), b->zall.p, sizeof(double) * 2 * b->n, hipMemcpyDeviceToHost, s));
   MESSI_HIP(hipStreamSynchronize(s));
   for (int m = 0; m < b->n; ++m) logZ[m] = z[2 * m];
   if (!keep_outside) collect(b, 3);
@@ -1135,6 +1152,19 @@ int messi_batch_eval_loaded(messi_batch *b, double B, int prec, double *logZ) {
     MESSI_HIP(hipSetDevice(ctx->device));
     if (prec == MESSI_FP32) eval_impl<float>(b, B, false, logZ);
     else eval_impl<double>(b, B, false, logZ);
+  });
+}
+
+int messi_batch_set_tiles(messi_batch *b, const int *tiles) {
+  if (!b) return 2;
+  return guard(b->ctx, [&] {
+    if (!tiles) {
+      b->tiles.clear();
+      return;
+    }
+    for (int m = 0; m < b->n; ++m)
+      MESSI_REQUIRE(tiles[m] == 0 || tiles[m] == 16 || tiles[m] == 32, "tile must be 0, 16 or 32");
+    b->tiles.assign(tiles, tiles + b->n);
   });
 }
 

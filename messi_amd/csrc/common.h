@@ -77,6 +77,8 @@ struct InsideWork {
   int L = 0;
   int tile = 16;                  // tile edge of the inside schedule (16 or 32)
   int band = -1;                  // paired[j,y] is -inf for y - j > band (-1: no such bound)
+  const void *exp = nullptr;      // pair-root expansion descriptor of this member (device), or null
+  double *zout = nullptr;         // where log Z (absolute, baseline-relative) is also written (device)
   DevBuf Lr, Lc, Sr, Sc, F;       // inside tables (row- and column-major copies)
   DevBuf acc;                     // bulk split-point accumulator of the tiled inside
   DevBuf oSr, oSc, oFr, oFc, oL;  // outside tables

@@ -55,6 +55,8 @@ template <class V> struct InstDesc {
   V *Lr, *Lc, *Sr, *Sc, *F, *acc, *ptv;
   const double *u, *paired;
   double *lam, *C, *uhat, *z;
+  const ExpandDesc *exp;  // pair term from the pair root entries when non-null
+  double *zout;           // second destination of z (the batch's gathered log Z), or null
 };
 
 // Baseline lambda, its prefix sums C and u - lambda; one workgroup per instance.
@@ -109,9 +111,9 @@ __global__ void ptv_kernel(const InstDesc<typename SR::V> *__restrict__ insts, d
 // T0[pid0[j], pid0[y]] when y - j <= maxbp and the mask allows it.
 template <class SR>
 __global__ void ptv_root_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
-                                const ExpandDesc *__restrict__ exps, const double *__restrict__ rootacc, double B) {
+                                const double *__restrict__ rootacc, double B) {
   const InstDesc<typename SR::V> d = insts[blockIdx.z];
-  const ExpandDesc e = exps[blockIdx.z];
+  const ExpandDesc e = *d.exp;
   const int L = d.L;
   int y = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
@@ -156,6 +158,10 @@ __global__ void zlog_kernel(const InstDesc<typename SR::V> *__restrict__ insts, 
   double v = SR::to_log(d.Sc[(size_t)(d.L - 1) * d.L]);
   d.z[0] = v == -HUGE_VAL ? v : v + d.C[d.L];
   d.z[1] = v;  // baseline-relative
+  if (d.zout) {
+    d.zout[0] = d.z[0];
+    d.zout[1] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -769,20 +775,21 @@ static InstDesc<typename SR::V> inst_of(InsideWork &w) {
   d.C = w.C.as<double>();
   d.uhat = w.uhat.as<double>();
   d.z = w.red.as<double>();
+  d.exp = static_cast<const ExpandDesc *>(w.exp);
+  d.zout = w.zout;
   return d;
 }
 
 // Baseline and pair term of every instance (u and paired already on the device).
 template <class SR>
 static void prepare_batch(hipStream_t s, const InstDesc<typename SR::V> *d_inst, int n, int Lmax, double B,
-                          const ExpandDesc *dexp = nullptr, const double *rootacc = nullptr) {
+                          const double *rootacc = nullptr) {
   double lp[7];
   kh99_logprobs(lp);
   hipLaunchKernelGGL(baseline_kernel<typename SR::V>, dim3(n), dim3(1024), 0, s, d_inst, B, lp[0] + lp[5],
                      (int)SR::scaled);
-  if (dexp)
-    hipLaunchKernelGGL(ptv_root_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, dexp,
-                       rootacc, B);
+  if (rootacc)
+    hipLaunchKernelGGL(ptv_root_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, rootacc, B);
   else
     hipLaunchKernelGGL(ptv_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, B);
 }
@@ -799,7 +806,7 @@ void inside_prepare(hipStream_t s, InsideWork &w, int L, double B) {
 // receives its log Z.
 template <typename T>
 void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &ib, double B,
-                      const ExpandDesc *dexp, const double *rootacc) {
+                      const double *rootacc) {
   typedef typename SRof<T>::type SR;
   typedef typename SR::V V;
   if (n <= 0) return;
@@ -842,7 +849,9 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
   double lp[7];
   kh99_logprobs(lp);
   Rules<SR> r = make_rules<SR>(B);
-  prepare_batch<SR>(s, di, n, Lmax, B, dexp, rootacc);
+  for (int m = 0; m < n; ++m)
+    MESSI_REQUIRE((rootacc != nullptr) == (ws[m]->exp != nullptr), "pair term source differs between members");
+  prepare_batch<SR>(s, di, n, Lmax, B, rootacc);
   hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, di, B, lp[1],
                      lp[5]);
   for (int t = 0; t < NTmax; ++t) {
@@ -956,7 +965,7 @@ void bpp_run(hipStream_t s, InsideWork &w, int L, double *d_unpaired, double *d_
   template void inside_prepare<T>(hipStream_t, InsideWork &, int, double);              \
   template void inside_run<T>(hipStream_t, InsideWork &, int, double);                  \
   template void inside_batch_run<T>(hipStream_t, InsideWork *const *, int, InsideBatch &, double,    \
-                                    const ExpandDesc *, const double *);                       \
+                                    const double *);                                           \
   template double inside_logZ<T>(hipStream_t, InsideWork &, int);                       \
   template void inside_export<T>(hipStream_t, InsideWork &, int, double *);             \
   template void inside_import<T>(hipStream_t, InsideWork &, int, const double *);       \

@@ -52,11 +52,12 @@ bool mfma_bf16_probe(hipStream_t s);
 template <typename T> void inside_alloc(InsideWork &w, int L, bool outside);
 template <typename T> void inside_prepare(hipStream_t s, InsideWork &w, int L, double B);
 template <typename T> void inside_run(hipStream_t s, InsideWork &w, int L, double B);
-// With dexp / rootacc the pair term of member m is gathered from its pair root
-// entries (dexp[m]) instead of being read from its L x L pair matrix.
+// With rootacc the pair term of each member is gathered from its pair root
+// entries (its work's exp descriptor) instead of being read from its L x L pair
+// matrix.  All members take the tile edge of ws[0].
 template <typename T>
 void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &ib, double B,
-                      const ExpandDesc *dexp = nullptr, const double *rootacc = nullptr);
+                      const double *rootacc = nullptr);
 template <typename T> double inside_logZ(hipStream_t s, InsideWork &w, int L);
 template <typename T> void inside_export(hipStream_t s, InsideWork &w, int L, double *d_in3);
 template <typename T> void inside_import(hipStream_t s, InsideWork &w, int L, const double *d_in3);

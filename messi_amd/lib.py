@@ -59,6 +59,7 @@ SIGNATURES = {
     "messi_batch_timings": (ctypes.c_int, [_vp, _dp]),
     "messi_batch_load": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
     "messi_batch_eval_loaded": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp]),
+    "messi_batch_set_tiles": (ctypes.c_int, [_vp, _ip]),
     "messi_mlp_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "messi_mlp_destroy": (None, [_vp]),
     "messi_mlp_num_params": (ctypes.c_int, [_vp, _i64]),
@@ -356,6 +357,16 @@ class Batch:
         P4, pi4, lw4, K4, P16, pi16, lw16, K16 = self._inputs(sites, pairs)
         self.ctx.check(self.ctx.lib.messi_batch_load(self._h, K4, ptr(P4), ptr(pi4), ptr(lw4), K16, ptr(P16),
                                                      ptr(pi16), ptr(lw16), prec_code(prec)))
+
+    def set_tiles(self, tiles) -> None:
+        """Inside tile edge per member (16, 32, or 0 for the context's); None resets."""
+        if tiles is None:
+            self.ctx.check(self.ctx.lib.messi_batch_set_tiles(self._h, None))
+            return
+        t = np.ascontiguousarray(tiles, dtype=np.int32)
+        if t.shape != (len(self.members),):
+            raise ValueError(f"expected {len(self.members)} tiles, got shape {t.shape}")
+        self.ctx.check(self.ctx.lib.messi_batch_set_tiles(self._h, ptr(t, _ip)))
 
     def eval_loaded(self, B: float = 1.0, prec=FP64) -> np.ndarray:
         """log Z of every member from the components of the last load."""

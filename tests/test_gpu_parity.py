@@ -324,3 +324,29 @@ def test_batch_load_then_eval_loaded(ctx):
             b.eval_loaded(1.0, "fp64")      # fp32 components are loaded
     finally:
         b.close()
+
+
+def test_batch_mixed_tiles_and_autotuner(ctx):
+    """Members split over tile-16 and tile-32 launch sequences give the same log Z
+    as one sequence; the autotuner leaves a valid assignment behind."""
+    from messi_amd import autotune
+    _say("batch with mixed tiles + autotuner")
+    als = [synth.generate(8 + 3 * i, 90 + 70 * i, 50 + i) for i in range(5)]
+    dss = [synth.dataset_of(a) for a in als]
+    dds = [hotpath.device_dataset(d) for d in dss]
+    comps = [_stack_params(model.getparams(a.params, d.tree.branchlengths)) for a, d in zip(als, dss)]
+    b = lib.Batch(dds)
+    try:
+        for prec, tol in (("fp64", 1e-12), ("fp32", 1e-6)):
+            b.load([c[0] for c in comps], [c[1] for c in comps], prec)
+            b.set_tiles(None)
+            ref = b.eval_loaded(1.0, prec)
+            b.set_tiles([16, 32, 16, 32, 32])
+            z = b.eval_loaded(1.0, prec)
+            assert np.max(np.abs(z - ref) / np.abs(ref)) < tol
+            ch = autotune.tune_batch(b, prec, reps=1)
+            assert len(ch["candidates"]) >= 2
+            z2 = b.eval_loaded(1.0, prec)
+            assert np.max(np.abs(z2 - ref) / np.abs(ref)) < tol
+    finally:
+        b.close()
