@@ -253,6 +253,7 @@ def main_mcmc(args):
                               mode_logp=lambda th: post(th, prec="fp64"))
     res.pop("traces")
     z = np.abs(np.array(res["mean_z_learned_vs_long"]))
+    z_ess = np.abs(np.array(res["mean_z_learned_vs_long_ess_only"]))
     out = {"metric": "MCMC ESS/s, learned proposal vs adaptive random walk (min ESS over 12 parameters)",
            "value": res["ess_per_s_gain_min"], "unit": "x (learned ESS/s / random-walk ESS/s)", "n_gpus": 1,
            "higher_is_better": True, "dtype": args.prec.replace("fp", "f"), "data": "synthetic",
@@ -260,6 +261,8 @@ def main_mcmc(args):
                                   f"{args.burn}, {args.mcmc_steps} steps learned and random walk, long chain "
                                   f"{args.long_steps}, network 12-128-128-24 trained {args.train_steps} Adam steps"},
            "means_within_2_mcse": int(np.sum(z <= 2.0)), "means_within_3_mcse": int(np.sum(z <= 3.0)),
+           "means_within_2_mcse_ess_only": int(np.sum(z_ess <= 2.0)),
+           "max_split_rhat": {k: float(np.max(v)) for k, v in res["split_rhat"].items()},
            "evals": post.evals, "device_eval_seconds": post.eval_seconds,
            "param_names": list(mcmc.NAMES), "result": res}
     net.close()
@@ -284,7 +287,7 @@ def main():
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--burn", type=int, default=2000)
     ap.add_argument("--mcmc-steps", type=int, default=1250)
-    ap.add_argument("--long-steps", type=int, default=4000)
+    ap.add_argument("--long-steps", type=int, default=12000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--prec", default="fp32")
     ap.add_argument("--seed", type=int, default=5)

@@ -81,7 +81,10 @@ struct InsideWork {
   double *zout = nullptr;         // where log Z (absolute, baseline-relative) is also written (device)
   DevBuf Lr, Lc, Sr, Sc, F;       // inside tables (row- and column-major copies)
   DevBuf acc;                     // bulk split-point accumulator of the tiled inside
-  DevBuf oSr, oSc, oFr, oFc, oL;  // outside tables
+  DevBuf oSr, oSc, oFr, oL;       // outside tables
+  DevBuf oWr, oWc, oaL, oaS;      // outside: parent weight W (row/column-major), accumulators
+  DevBuf orng;                    // outside launch tables
+  int orng_key[4] = {0, 0, 0, 0};   // (L, tile, band, lband) the launch tables were built for
   DevBuf ptv;                     // scaled pair term per cell
   DevBuf lam, C;                  // per-position baseline and its prefix sums (double)
   DevBuf uhat;                    // u - lambda (double)

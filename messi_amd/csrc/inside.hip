@@ -22,6 +22,8 @@
 // operand tiles are final, either inside the workgroup that finalises (I,J) or
 // in a workgroup of its own.  A launch therefore carries the cell wavefronts
 // of one tile-diagonal plus the split-point products that just became ready.
+#include <algorithm>
+
 #include "common.h"
 #include "prune.h"
 #include "sr.h"
@@ -181,6 +183,16 @@ __device__ __forceinline__ typename SR::S grp_sum(typename SR::S s, int G) {
 // ---------------------------------------------------------------------------
 // Shared memory of one step workgroup.
 // ---------------------------------------------------------------------------
+// Staging of one T x T tile product dst[a][b] (+)= sum_c A[a][c] * B[c][b].
+template <class SR, int T> struct ProdLDS {
+  typedef typename SR::V V;
+  V A[T][T + 1];
+  V B[T][T + 1];
+  float Ap[T][T + 1];  // exponent-aligned mantissas (scaled representation)
+  float Bp[T][T + 1];
+  int rA[T], cB[T];    // row / column exponent maxima
+};
+
 template <class SR, int T> struct StepLDS {
   typedef typename SR::V V;
   V Ld[T][T + 1];   // L of the diagonal tile (I, I)
@@ -190,15 +202,9 @@ template <class SR, int T> struct StepLDS {
   union {
     struct {
       V cS[T][T + 1], cF[T][T + 1], cL[T][T + 1];
-    } w;              // the tile being finalised
-    struct {
-      V A[T][T + 1];  // L[IT+a, KT+c]
-      V B[T][T + 1];  // S[KT+c+1, JT+b]
-      float Ap[T][T + 1];
-      float Bp[T][T + 1];
-    } p;              // a split-point product
+    } w;                  // the tile being finalised
+    ProdLDS<SR, T> p;     // a split-point product
   } u;
-  int rA[T], cB[T];
   V Sb[T], Fbot[T], Fleft[T];
 };
 
@@ -232,14 +238,34 @@ __device__ __forceinline__ Operands<SR> fetch_operands(int L, int I, int K, int 
   return o;
 }
 
-// acc (+)= sum over split points h in tile K of L[IT+a, h] * S[h+1, JT+b], from
-// operands already fetched into registers.
+// General operand fetch: a = A[(ar0 + r) * L + ac0 + c] and b = B[(br0 + r) * L +
+// bc0 + c] (r = tid / T, c = tid % T), each only where it names a cell of the
+// upper triangle: row <= column in a row-major table (up == true), column <=
+// row index in a column-major one (up == false); zero elsewhere.
 template <class SR, int T>
-__device__ __forceinline__ void tile_product(const Operands<SR> &o, StepLDS<SR, T> &sm) {
+__device__ __forceinline__ Operands<SR> fetch_general(int L, const typename SR::V *__restrict__ A, int ar0,
+                                                      int ac0, bool aup, const typename SR::V *__restrict__ B,
+                                                      int br0, int bc0, bool bup) {
+  const int tid = threadIdx.x, r = tid / T, c = tid % T;
+  const int ar = ar0 + r, ac = ac0 + c, br = br0 + r, bc = bc0 + c;
+  Operands<SR> o;
+  o.a = (ar >= 0 && ac >= 0 && ar < L && ac < L && (aup ? ar <= ac : ac <= ar)) ? A[(size_t)ar * L + ac]
+                                                                                 : SR::zero();
+  o.b = (br >= 0 && bc >= 0 && br < L && bc < L && (bup ? br <= bc : bc <= br)) ? B[(size_t)br * L + bc]
+                                                                                 : SR::zero();
+  return o;
+}
+
+// dst[a][b] (+)= sum_c A[a][c] * B[c][b] with A[a][c] = o.a of thread a*T+c and
+// B[c][b] = o.b of thread b*T+c (inside: L[IT+a, h] * S[h+1, JT+b] over h in tile
+// K), from operands already fetched into registers.
+template <class SR, int T>
+__device__ __forceinline__ void tile_product(const Operands<SR> &o, ProdLDS<SR, T> &pp,
+                                             typename SR::V (*dst)[T + 1]) {
   typedef typename SR::V V;
   const int tid = threadIdx.x, r = tid / T, c = tid % T;
-  sm.u.p.A[r][c] = o.a;
-  sm.u.p.B[c][r] = o.b;
+  pp.A[r][c] = o.a;
+  pp.B[c][r] = o.b;
   if constexpr (SR::scaled) {
     // Row maxima of A's exponents and column maxima of B's (over the split
     // points c, held by T consecutive lanes), then exponent-aligned floats.
@@ -250,11 +276,11 @@ __device__ __forceinline__ void tile_product(const Operands<SR> &o, StepLDS<SR, 
       ea = max(ea, __shfl_xor(ea, off));
       eb = max(eb, __shfl_xor(eb, off));
     }
-    sm.u.p.Ap[r][c] = o.a.m == 0.0f ? 0.0f : ldexpf(o.a.m, max(o.a.e - ea, -200));
-    sm.u.p.Bp[c][r] = o.b.m == 0.0f ? 0.0f : ldexpf(o.b.m, max(o.b.e - eb, -200));
+    pp.Ap[r][c] = o.a.m == 0.0f ? 0.0f : ldexpf(o.a.m, max(o.a.e - ea, -200));
+    pp.Bp[c][r] = o.b.m == 0.0f ? 0.0f : ldexpf(o.b.m, max(o.b.e - eb, -200));
     if (c == 0) {
-      sm.rA[r] = ea;
-      sm.cB[r] = eb;
+      pp.rA[r] = ea;
+      pp.cB[r] = eb;
     }
     __syncthreads();
     // One MFMA 16x16x4 f32 chain per 16x16 block.
@@ -266,24 +292,24 @@ __device__ __forceinline__ void tile_product(const Operands<SR> &o, StepLDS<SR, 
       v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int s = 0; s < T / 4; ++s) {
-        float av = sm.u.p.Ap[bi * 16 + (lane & 15)][4 * s + (lane >> 4)];
-        float bv = sm.u.p.Bp[4 * s + (lane >> 4)][bj * 16 + (lane & 15)];
+        float av = pp.Ap[bi * 16 + (lane & 15)][4 * s + (lane >> 4)];
+        float bv = pp.Bp[4 * s + (lane >> 4)][bj * 16 + (lane & 15)];
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int a = bi * 16 + 4 * (lane >> 4) + q, b = bj * 16 + (lane & 15);
-        const int ra = sm.rA[a], cb = sm.cB[b];
+        const int ra = pp.rA[a], cb = pp.cB[b];
         V contrib = SR::zero();
         if (ra != XF::EZ && cb != XF::EZ) {
           float cv = acc[q];
           if (cv >= 0x1p-100f) {
             contrib = XF::norm(cv, ra + cb);
           } else {
-            contrib = exact_cell_xf<T>(sm.u.p.A, sm.u.p.B, a, b);
+            contrib = exact_cell_xf<T>(pp.A, pp.B, a, b);
           }
         }
-        sm.cA[a][b] = SR::add(sm.cA[a][b], contrib);
+        dst[a][b] = SR::add(dst[a][b], contrib);
       }
     }
   } else {
@@ -291,11 +317,11 @@ __device__ __forceinline__ void tile_product(const Operands<SR> &o, StepLDS<SR, 
     for (int idx = tid; idx < T * T; idx += T * T) {
       int a = idx / T, b = idx % T;
       typename SR::K km = SR::kmin();
-      for (int k = 0; k < T; ++k) km = SR::kmax(km, SR::key(sm.u.p.A[a][k], sm.u.p.B[k][b]));
+      for (int k = 0; k < T; ++k) km = SR::kmax(km, SR::key(pp.A[a][k], pp.B[k][b]));
       typename SR::S sum = 0;
       if (km != SR::kmin())
-        for (int k = 0; k < T; ++k) sum += SR::contrib(sm.u.p.A[a][k], sm.u.p.B[k][b], km);
-      sm.cA[a][b] = SR::add(sm.cA[a][b], SR::finish(km, sum));
+        for (int k = 0; k < T; ++k) sum += SR::contrib(pp.A[a][k], pp.B[k][b], km);
+      dst[a][b] = SR::add(dst[a][b], SR::finish(km, sum));
     }
   }
   __syncthreads();
@@ -366,8 +392,8 @@ __global__ __launch_bounds__(T * T, T == 16 ? 8 : 1) void inside_step_kernel(con
   if (do1) o1 = fetch_operands<SR, T>(L, I, K1, J, Lr, Sc);
   if (do2) o2 = fetch_operands<SR, T>(L, I, K2, J, Lr, Sc);
   sm.cA[tid / T][tid % T] = accv;
-  if (do1) tile_product<SR, T>(o1, sm);
-  if (do2) tile_product<SR, T>(o2, sm);
+  if (do1) tile_product<SR, T>(o1, sm.u.p, sm.cA);
+  if (do2) tile_product<SR, T>(o2, sm.u.p, sm.cA);
   __syncthreads();
   if (!fin) {
     for (int idx = tid; idx < T * T; idx += T * T) {
@@ -505,94 +531,216 @@ __global__ __launch_bounds__(T * T, T == 16 ? 8 : 1) void inside_step_kernel(con
 }
 
 // ---------------------------------------------------------------------------
-// Outside, one anti-diagonal b (descending), G lanes per cell.
+// Tiled outside.  With W = p(S->LS) oS + p(F->LS) oF (the weight of a cell as
+// the parent of a bifurcation),
+//   oL[j,y] = sum_{k>y} W[j,k] iS[y+1,k]        (L is the left child)
+//   oS[j,y] = sum_{k<j} W[k,y] iL[k,j-1]        (S is the right child)
+//   oF[j,y] = ptv[j-1,y+1] Q[j-1,y+1],  Q = p2 oS + p3 oF + p6 oL.
+// Tiles are finalised by descending tile distance d = J - I (launch u finalises
+// d = NT-1-u).  The sums over k outside the tile's own row / column tiles are
+// tile products: row products W(I, J+m) x iS(rows JT+1.., cols (J+m)T..)^T into
+// the L accumulator, column products iL(rows (I-m)T.., cols IT-1..)^T x W(I-m, J)
+// into the S accumulator; the pair with offset m becomes ready in launch
+// NT - d - m (its W tile has distance d + m).  m = 1 runs in the finalising
+// workgroup, m >= 2 in bulk workgroups.  The remaining terms (k inside tile J,
+// resp. I) are pulled by an in-LDS wavefront over the tile's cells in
+// descending span order, at most one row and one column term per step.
 // ---------------------------------------------------------------------------
-template <class SR>
-__global__ __launch_bounds__(256) void outside_diag_kernel(
-    int L, int b, int G, const typename SR::V *__restrict__ iSr, const typename SR::V *__restrict__ iLc,
-    typename SR::V *__restrict__ oSr, typename SR::V *__restrict__ oSc, typename SR::V *__restrict__ oFr,
-    typename SR::V *__restrict__ oFc, typename SR::V *__restrict__ oL, const typename SR::V *__restrict__ ptv,
-    Rules<SR> r, int band, int lband) {
+struct OutDesc {
+  int L, NT, band, lband;
+  const void *iSr, *iLc, *ptv;              // inside S (row-major), L (column-major), pair term
+  void *oSr, *oSc, *oFr, *oL, *Wr, *Wc;     // outside tables and W (row- and column-major)
+  void *accL, *accS;                        // bulk accumulators (row-major)
+};
+
+// Per launch u and tile distance d < NT-1-u: the bulk tiles (I, I+d) with a
+// ready product are I in [0, rend) and [cbeg, cend); cum counts the items of
+// the smaller distances.
+struct OutRange {
+  int cum, rend, cbeg, cend;
+};
+
+__host__ __device__ inline bool out_row_ok(int d, int T, int lband) {
+  return lband < 0 || (d == 0 ? 0 : d * T - (T - 1)) <= lband;
+}
+__host__ __device__ inline bool out_col_ok(int m, int T, int band) { return band < 0 || (m - 1) * T <= band; }
+
+template <class SR, int T> struct OutLDS {
   typedef typename SR::V V;
-  typedef typename SR::K K;
-  typedef typename SR::S S;
-  long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  int cell = (int)(tid / G), sub = (int)(tid % G);
-  bool valid = cell < L - b;
-  int j = valid ? cell : 0;
-  int y = j + b;
-  const V *oSrow = oSr + (size_t)j * L, *oFrow = oFr + (size_t)j * L;
-  const V *iSrow = iSr + (size_t)(y + 1 < L ? y + 1 : y) * L;
-  const V *oScol = oSc + (size_t)y * L, *oFcol = oFc + (size_t)y * L;
-  const V *iLcol = iLc + (size_t)(j >= 1 ? j - 1 : 0) * L;
-  // With a pairing band: L[k, j-1] is -inf for j-1-k > band.  L's outside is
-  // consumed only through pair terms, so beyond lband (the band, when the
-  // caller does not need the L outside table) it is left at zero.
-  const int kend = (lband >= 0 && b > lband) ? y + 1 : L;
-  const int k0 = band >= 0 ? max(0, j - 1 - band) : 0;
-  K krS = SR::kmin(), krF = SR::kmin(), klS = SR::kmin(), klF = SR::kmin();
-  if (valid) {
-    for (int k = y + 1 + sub; k < kend; k += G) {
-      V is = iSrow[k];
-      krS = SR::kmax(krS, SR::key(oSrow[k], is));
-      krF = SR::kmax(krF, SR::key(oFrow[k], is));
+  union {
+    struct {
+      V aL[T][T + 1], aS[T][T + 1];
+      ProdLDS<SR, T> p;
+    } pr;  // product phase
+    struct {
+      V W[T][T + 1], Q[T][T + 1];
+      V iSd[T][T + 1];  // iS[JT+b+1, JT+c] as [b][c]
+      V iLd[T][T + 1];  // iL[IT+c, IT+a-1] as [c][a]
+      V cP[T][T + 1];   // ptv[j-1, y+1]
+    } w;   // wavefront phase
+  } u;
+  V Qtop[T], Qright[T];
+};
+
+template <class SR, int T>
+__global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const OutRange *__restrict__ ranges,
+                                                             int u, Rules<SR> r) {
+  typedef typename SR::V V;
+  __shared__ OutLDS<SR, T> sm;
+  const int tid = threadIdx.x, L = od.L, NT = od.NT;
+  const int du = NT - 1 - u;
+  const V *__restrict__ iSr = static_cast<const V *>(od.iSr);
+  const V *__restrict__ iLc = static_cast<const V *>(od.iLc);
+  const V *__restrict__ ptv = static_cast<const V *>(od.ptv);
+  V *__restrict__ oSr = static_cast<V *>(od.oSr);
+  V *__restrict__ oSc = static_cast<V *>(od.oSc);
+  V *__restrict__ oFr = static_cast<V *>(od.oFr);
+  V *__restrict__ oL = static_cast<V *>(od.oL);
+  V *__restrict__ Wr = static_cast<V *>(od.Wr);
+  V *__restrict__ Wc = static_cast<V *>(od.Wc);
+  V *__restrict__ accL = static_cast<V *>(od.accL);
+  V *__restrict__ accS = static_cast<V *>(od.accS);
+  // ---- decode: u+1 finalising tiles first, then the bulk tiles by distance
+  int I, d, m;
+  bool fin;
+  const int item = blockIdx.x;
+  if (item <= u) {
+    fin = true;
+    I = item;
+    d = du;
+    m = 1;
+  } else {
+    fin = false;
+    const int idx = item - (u + 1);
+    const OutRange *rg = ranges + (size_t)u * NT;
+    int lo = 0, hi = du - 1;  // largest distance with cum <= idx
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rg[mid].cum <= idx) lo = mid;
+      else hi = mid - 1;
     }
-    for (int k = k0 + sub; k < j; k += G) {
-      V il = iLcol[k];
-      klS = SR::kmax(klS, SR::key(oScol[k], il));
-      klF = SR::kmax(klF, SR::key(oFcol[k], il));
-    }
+    d = lo;
+    const int i = idx - rg[d].cum;
+    I = i < rg[d].rend ? i : rg[d].cbeg + (i - rg[d].rend);
+    m = du + 1 - d;
   }
-  krS = grp_kmax<SR>(krS, G);
-  krF = grp_kmax<SR>(krF, G);
-  klS = grp_kmax<SR>(klS, G);
-  klF = grp_kmax<SR>(klF, G);
-  S srS = 0, srF = 0, slS = 0, slF = 0;
-  if (valid) {
-    for (int k = y + 1 + sub; k < kend; k += G) {
-      V is = iSrow[k];
-      if (krS != SR::kmin()) srS += SR::contrib(oSrow[k], is, krS);
-      if (krF != SR::kmin()) srF += SR::contrib(oFrow[k], is, krF);
+  const int J = I + d, IT = I * T, JT = J * T;
+  const bool dorow = out_row_ok(d, T, od.lband) && J + m <= NT - 1;
+  const bool docol = out_col_ok(m, T, od.band) && I - m >= 0;
+  // ---- accumulators and the ready products (all global loads issued first)
+  const int a0 = tid / T, b0 = tid % T, j0 = IT + a0, y0 = JT + b0;
+  const bool cell0 = j0 < L && y0 < L;
+  V vL = SR::zero(), vS = SR::zero();
+  if (cell0 && (fin || dorow)) vL = accL[(size_t)j0 * L + y0];
+  if (cell0 && (fin || docol)) vS = accS[(size_t)j0 * L + y0];
+  Operands<SR> orow, ocol;
+  if (dorow) orow = fetch_general<SR, T>(L, Wr, IT, (J + m) * T, true, iSr, JT + 1, (J + m) * T, true);
+  if (docol) ocol = fetch_general<SR, T>(L, iLc, IT - 1, (I - m) * T, false, Wc, JT, (I - m) * T, false);
+  sm.u.pr.aL[a0][b0] = vL;
+  sm.u.pr.aS[a0][b0] = vS;
+  if (dorow) tile_product<SR, T>(orow, sm.u.pr.p, sm.u.pr.aL);
+  if (docol) tile_product<SR, T>(ocol, sm.u.pr.p, sm.u.pr.aS);
+  __syncthreads();
+  if (!fin) {
+    if (cell0) {
+      if (dorow) accL[(size_t)j0 * L + y0] = sm.u.pr.aL[a0][b0];
+      if (docol) accS[(size_t)j0 * L + y0] = sm.u.pr.aS[a0][b0];
     }
-    for (int k = k0 + sub; k < j; k += G) {
-      V il = iLcol[k];
-      if (klS != SR::kmin()) slS += SR::contrib(oScol[k], il, klS);
-      if (klF != SR::kmin()) slF += SR::contrib(oFcol[k], il, klF);
-    }
+    return;
   }
-  srS = grp_sum<SR>(srS, G);
-  srF = grp_sum<SR>(srF, G);
-  slS = grp_sum<SR>(slS, G);
-  slF = grp_sum<SR>(slF, G);
-  if (valid && sub == 0) {
-    // S->LS then F->LS: L gets the right extensions, S the left ones.
-    V vL = SR::add(SR::mulc(SR::finish(krS, srS), r.p[0]), SR::mulc(SR::finish(krF, srF), r.p[4]));
-    V vS = SR::add(SR::mulc(SR::finish(klS, slS), r.p[0]), SR::mulc(SR::finish(klF, slF), r.p[4]));
-    V vF = SR::zero();
-    if (j >= 1 && y + 1 < L) {
-      size_t pc = (size_t)(j - 1) * L + (y + 1);
-      V pt = ptv[pc];
-      V t = SR::add(SR::add(SR::mulc(oSr[pc], r.p[2]), SR::mulc(oFr[pc], r.p[3])), SR::mulc(oL[pc], r.p[6]));
-      vF = SR::mul(pt, t);
+  // ---- finalise tile (I, J)
+  const int a = a0, b = b0, j = j0, y = y0;
+  typename SR::Acc acL = SR::acc(sm.u.pr.aL[a][b]);
+  typename SR::Acc acS = SR::acc(sm.u.pr.aS[a][b]);
+  __syncthreads();  // the product staging is reused below
+  {
+    const int rr = JT + a + 1, cc = JT + b;  // iSd[a][b] = iS[JT+a+1, JT+b]
+    sm.u.w.iSd[a][b] = (rr < L && cc < L && rr <= cc) ? iSr[(size_t)rr * L + cc] : SR::zero();
+    const int lr = IT + a, lc = IT + b - 1;  // iLd[a][b] = iL[IT+a, IT+b-1]
+    sm.u.w.iLd[a][b] = (lc >= 0 && lr < L && lc < L && lr <= lc) ? iLc[(size_t)lc * L + lr] : SR::zero();
+    sm.u.w.cP[a][b] = (j >= 1 && j <= y && y + 1 < L) ? ptv[(size_t)(j - 1) * L + (y + 1)] : SR::zero();
+    sm.u.w.W[a][b] = SR::zero();
+    sm.u.w.Q[a][b] = SR::zero();
+  }
+  if (tid < T) {
+    // Q on the row above the tile (IT-1, JT+b+1) and the column right of it (IT+b-1, JT+T)
+    const int bq = tid;
+    V qt = SR::zero(), qr = SR::zero();
+    const int tr = IT - 1, tc = JT + bq + 1;
+    if (tr >= 0 && tc < L) {
+      const size_t c = (size_t)tr * L + tc;
+      qt = SR::add(SR::add(SR::mulc(oSr[c], r.p[2]), SR::mulc(oFr[c], r.p[3])), SR::mulc(oL[c], r.p[6]));
     }
-    size_t rc = (size_t)j * L + y, cc = (size_t)y * L + j;
-    oSr[rc] = vS;
-    oSc[cc] = vS;
-    oFr[rc] = vF;
-    oFc[cc] = vF;
-    oL[rc] = vL;
+    const int rr = IT + bq - 1, rc = JT + T;
+    if (bq >= 1 && rr < L && rc < L) {
+      const size_t c = (size_t)rr * L + rc;
+      qr = SR::add(SR::add(SR::mulc(oSr[c], r.p[2]), SR::mulc(oFr[c], r.p[3])), SR::mulc(oL[c], r.p[6]));
+    }
+    sm.Qtop[bq] = qt;
+    sm.Qright[bq] = qr;
+  }
+  __syncthreads();
+  const bool mine = j < L && y < L && j <= y;
+  const bool lkeep = out_row_ok(d, T, od.lband);
+  const int own = (T - 1 - b) + a;
+  V fS = SR::zero(), fF = SR::zero(), fL = SR::zero();
+  for (int st = 0; st <= 2 * T - 2; ++st) {
+    const bool act = mine && st >= 1 && st <= own, fn = mine && st == own;
+    V w1 = SR::zero(), s1 = SR::zero(), l2 = SR::zero(), w2 = SR::zero();
+    if (act) {
+      const int bs = T - st + a;  // row producer (a, bs), finalised at step st-1
+      if (bs > b && bs < T) {
+        w1 = sm.u.w.W[a][bs];
+        s1 = sm.u.w.iSd[b][bs];
+      }
+      const int as = st - T + b;  // column producer (as, b), finalised at step st-1
+      if (as >= 0 && as < a) {
+        l2 = sm.u.w.iLd[as][a];
+        w2 = sm.u.w.W[as][b];
+      }
+      SR::acc_add2(acL, SR::mul_raw(w1, s1), SR::zero());
+      SR::acc_add2(acS, SR::mul_raw(l2, w2), SR::zero());
+    }
+    if (fn) {
+      fL = lkeep ? SR::acc_value(acL) : SR::zero();
+      fS = SR::acc_value(acS);
+      V q = a >= 1 && b + 1 < T ? sm.u.w.Q[a - 1][b + 1] : (a == 0 ? sm.Qtop[b] : sm.Qright[a]);
+      fF = SR::mul(sm.u.w.cP[a][b], q);
+      if (j == 0 && y == L - 1) fS = SR::from_log(0.0);  // the start symbol
+      sm.u.w.W[a][b] = SR::add(SR::mulc(fS, r.p[0]), SR::mulc(fF, r.p[4]));
+      sm.u.w.Q[a][b] = SR::add(SR::add(SR::mulc(fS, r.p[2]), SR::mulc(fF, r.p[3])), SR::mulc(fL, r.p[6]));
+    }
+    __syncthreads();
+  }
+  // ---- write back: row-major from registers, column-major through LDS
+  if (mine) {
+    const size_t rc = (size_t)j * L + y;
+    oSr[rc] = fS;
+    oFr[rc] = fF;
+    oL[rc] = fL;
+    Wr[rc] = sm.u.w.W[a][b];
+  }
+  sm.u.w.Q[a][b] = fS;
+  __syncthreads();
+  {
+    const int bb = tid / T, aa = tid % T, jj = IT + aa, yy = JT + bb;
+    if (jj < L && yy < L && jj <= yy) {
+      const size_t cc = (size_t)yy * L + jj;
+      oSc[cc] = sm.u.w.Q[aa][bb];
+      Wc[cc] = sm.u.w.W[aa][bb];
+    }
   }
 }
 
+// Zero the outside accumulators (upper triangle).
 template <class SR>
-__global__ void outside_init_kernel(int L, typename SR::V *oSr, typename SR::V *oSc, typename SR::V *oFr,
-                                    typename SR::V *oFc, typename SR::V *oL) {
-  size_t rc = (size_t)(L - 1), cc = (size_t)(L - 1) * L;
-  oSr[rc] = SR::from_log(0.0);
-  oSc[cc] = SR::from_log(0.0);
-  oFr[rc] = SR::zero();
-  oFc[cc] = SR::zero();
-  oL[rc] = SR::zero();
+__global__ void outside_tiled_init_kernel(int L, typename SR::V *accL, typename SR::V *accS) {
+  int y = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (y >= L || j > y) return;
+  const size_t c = (size_t)j * L + y;
+  accL[c] = SR::zero();
+  accS[c] = SR::zero();
 }
 
 // ---------------------------------------------------------------------------
@@ -717,16 +865,6 @@ template <typename T> struct SRof;
 template <> struct SRof<double> { typedef LogSR type; };
 template <> struct SRof<float> { typedef XF type; };
 
-static inline int pow2ceil(int x) {
-  int g = 1;
-  while (g < x) g <<= 1;
-  return g;
-}
-static inline int lanes_for(int terms) {
-  int g = pow2ceil(terms / 8 > 1 ? terms / 8 : 1);
-  return g > 64 ? 64 : g;
-}
-
 template <typename T>
 void inside_alloc(InsideWork &w, int L, bool outside) {
   typedef typename SRof<T>::type SR;
@@ -749,8 +887,11 @@ void inside_alloc(InsideWork &w, int L, bool outside) {
     w.oSr.ensure(n);
     w.oSc.ensure(n);
     w.oFr.ensure(n);
-    w.oFc.ensure(n);
     w.oL.ensure(n);
+    w.oWr.ensure(n);
+    w.oWc.ensure(n);
+    w.oaL.ensure(n);
+    w.oaS.ensure(n);
     w.lp.ensure(sizeof(double) * L * L);
   }
 }
@@ -906,19 +1047,62 @@ void outside_run(hipStream_t s, InsideWork &w, int L, bool full_L) {
   typedef typename SRof<T>::type SR;
   typedef typename SR::V V;
   Rules<SR> r = make_rules<SR>(1.0);
-  hipLaunchKernelGGL(outside_init_kernel<SR>, dim3(1), dim3(1), 0, s, L, w.oSr.as<V>(), w.oSc.as<V>(),
-                     w.oFr.as<V>(), w.oFc.as<V>(), w.oL.as<V>());
+  const int TE = w.tile == 32 ? 32 : 16, NT = (L + TE - 1) / TE;
   const int band = (w.band >= 0 && w.band < L) ? w.band : -1;
   const int lband = full_L ? -1 : band;
-  for (int b = L - 2; b >= 0; --b) {
-    int terms = L - b;
-    if (band >= 0) terms = (lband >= 0 && b > lband) ? band + 1 : std::max(L - b, band + 1);
-    int G = lanes_for(terms);
-    long threads = (long)(L - b) * G;
-    int blocks = (int)((threads + 255) / 256);
-    hipLaunchKernelGGL(outside_diag_kernel<SR>, dim3(blocks), dim3(256), 0, s, L, b, G, (const V *)w.Sr.as<V>(),
-                       (const V *)w.Lc.as<V>(), w.oSr.as<V>(), w.oSc.as<V>(), w.oFr.as<V>(), w.oFc.as<V>(),
-                       w.oL.as<V>(), (const V *)w.ptv.as<V>(), r, band, lband);
+  // launch tables: the bulk tile ranges of every launch (host, cached per shape)
+  std::vector<int> total(NT);
+  const int key[4] = {L, TE, band, lband};
+  const bool fresh = std::equal(key, key + 4, w.orng_key);
+  std::vector<OutRange> rg(fresh ? 0 : (size_t)NT * NT);
+  for (int u = 0; u < NT; ++u) {
+    const int du = NT - 1 - u;
+    int cum = 0;
+    for (int d = 0; d < du; ++d) {
+      const int m = du + 1 - d;
+      int rend = out_row_ok(d, TE, lband) ? u : 0;
+      int cbeg = rend, cend = rend;
+      if (out_col_ok(m, TE, band) && m <= NT - 1 - d) {
+        cbeg = m;
+        cend = NT - d;
+        if (cbeg <= rend) {  // the two ranges meet: one range
+          rend = cend;
+          cbeg = cend;
+        }
+      }
+      if (!fresh) rg[(size_t)u * NT + d] = OutRange{cum, rend, cbeg, cend};
+      cum += rend + (cend - cbeg);
+    }
+    total[u] = (u + 1) + cum;
+  }
+  if (!fresh) {
+    upload(w.orng, rg, s);
+    std::copy(key, key + 4, w.orng_key);
+  }
+  OutDesc od;
+  od.L = L;
+  od.NT = NT;
+  od.band = band;
+  od.lband = lband;
+  od.iSr = w.Sr.p;
+  od.iLc = w.Lc.p;
+  od.ptv = w.ptv.p;
+  od.oSr = w.oSr.p;
+  od.oSc = w.oSc.p;
+  od.oFr = w.oFr.p;
+  od.oL = w.oL.p;
+  od.Wr = w.oWr.p;
+  od.Wc = w.oWc.p;
+  od.accL = w.oaL.p;
+  od.accS = w.oaS.p;
+  hipLaunchKernelGGL(outside_tiled_init_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, w.oaL.as<V>(),
+                     w.oaS.as<V>());
+  const OutRange *dr = w.orng.as<OutRange>();
+  for (int u = 0; u < NT; ++u) {
+    if (TE == 32)
+      hipLaunchKernelGGL((outside_step_kernel<SR, 32>), dim3(total[u]), dim3(32 * 32), 0, s, od, dr, u, r);
+    else
+      hipLaunchKernelGGL((outside_step_kernel<SR, 16>), dim3(total[u]), dim3(16 * 16), 0, s, od, dr, u, r);
   }
   MESSI_HIP(hipGetLastError());
 }
@@ -938,7 +1122,7 @@ void outside_import(hipStream_t s, InsideWork &w, int L, const double *d_out3) {
   typedef typename SR::V V;
   hipLaunchKernelGGL(import3_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, d_out3,
                      (const double *)w.C.as<double>(), 1, w.oSr.as<V>(), w.oSc.as<V>(), w.oFr.as<V>(),
-                     w.oFc.as<V>(), w.oL.as<V>(), (V *)nullptr);
+                     nullptr, w.oL.as<V>(), (V *)nullptr);
 }
 
 template <typename T>

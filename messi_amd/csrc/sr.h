@@ -62,6 +62,7 @@ struct LogSR {
       a.s += exp(t - a.m);
     }
   }
+  static __device__ __forceinline__ V acc_value(const Acc &a) { return a.s > 0.0 ? a.m + log(a.s) : -HUGE_VAL; }
   // Add two terms at once; the three exponentials are independent.
   static __device__ __forceinline__ void acc_add2(Acc &a, V t1, V t2) {
     const double m = fmax(a.m, fmax(t1, t2));
@@ -160,6 +161,7 @@ struct XF {
   // aligned to the larger exponent and added; normalised once when finalised.
   typedef V Acc;
   static __device__ __forceinline__ Acc acc(V v) { return v; }
+  static __device__ __forceinline__ V acc_value(const Acc &a) { return norm(a.m, a.e); }
   static __device__ __forceinline__ V mul_raw(V a, V b) { return V{a.m * b.m, a.e + b.e}; }
   static __device__ __forceinline__ void acc_add(Acc &a, V t) {
     const int ne = max(a.e, t.e);

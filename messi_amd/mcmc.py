@@ -526,18 +526,48 @@ def ess(samples) -> np.ndarray:
     return out
 
 
+def split_rhat(samples) -> np.ndarray:
+    """Split potential scale reduction factor of each parameter (samples:
+    (T, nchains, d)); Gelman et al., Bayesian Data Analysis 3rd ed., (11.4)."""
+    x = np.asarray(samples, dtype=np.float64)
+    h = x.shape[0] // 2
+    y = np.concatenate([x[:h], x[h:2 * h]], axis=1)
+    W = y.var(axis=0, ddof=1).mean(axis=0)
+    Bv = h * y.mean(axis=0).var(axis=0, ddof=1)
+    return np.sqrt(((h - 1) / h * W + Bv / h) / W)
+
+
 def integrated_time(samples) -> float:
     """Largest integrated autocorrelation time over the parameters."""
     x = np.asarray(samples)
     return float(np.max(x.shape[0] * x.shape[1] / ess(x)))
 
 
-def compare_means(a, ess_a, b, ess_b):
-    """z-scores of the difference of posterior means of two sample sets
-    (each (T, nchains, d)) with Monte Carlo standard errors sd / sqrt(ESS)."""
+def mcse(samples, ess_values) -> np.ndarray:
+    """Monte Carlo standard error of each posterior mean (samples: (T, nchains, d)):
+    the larger of sd / sqrt(ESS) and the spread of the chain means over
+    sqrt(nchains).  The second treats the chains as independent replicates and
+    stays honest when chains are stuck in different modes (split R-hat well
+    above 1), where autocorrelation-based ESS estimates are too optimistic."""
+    x = np.asarray(samples, dtype=np.float64)
+    T, m, d = x.shape
+    se_ess = x.reshape(-1, d).std(axis=0) / np.sqrt(np.asarray(ess_values, dtype=np.float64))
+    if m < 2:
+        return se_ess
+    se_rep = x.mean(axis=0).std(axis=0, ddof=1) / np.sqrt(m)
+    return np.maximum(se_ess, se_rep)
+
+
+def compare_means(a, ess_a, b, ess_b, conservative: bool = True):
+    """z-scores of the difference of posterior means of two sample sets (each
+    (T, nchains, d)); with conservative=False the standard errors are
+    sd / sqrt(ESS) only."""
     fa = np.asarray(a).reshape(-1, np.shape(a)[-1])
     fb = np.asarray(b).reshape(-1, np.shape(b)[-1])
-    se = np.sqrt(fa.var(axis=0) / ess_a + fb.var(axis=0) / ess_b)
+    if conservative:
+        se = np.sqrt(mcse(a, ess_a) ** 2 + mcse(b, ess_b) ** 2)
+    else:
+        se = np.sqrt(fa.var(axis=0) / ess_a + fb.var(axis=0) / ess_b)
     return (fa.mean(axis=0) - fb.mean(axis=0)) / se
 
 
@@ -634,6 +664,7 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     e_l, e_r, e_long = ess(tl.samples), ess(tr.samples), ess(tlong.samples)
     z = compare_means(tl.samples, e_l, tlong.samples, e_long)
     z_rw = compare_means(tr.samples, e_r, tlong.samples, e_long)
+    z_ess = compare_means(tl.samples, e_l, tlong.samples, e_long, conservative=False)
     res = {
         "nchains": n, "burn": burn, "steps": steps, "long_steps": long_steps,
         "train_seconds": train_s, "learned_burn_seconds": learned_burn_s, "trials": trials,
@@ -649,7 +680,11 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
                        "mean": tlong.samples.reshape(-1, d).mean(axis=0).tolist(),
                        "sd": tlong.samples.reshape(-1, d).std(axis=0).tolist()},
         "learned_mean": tl.samples.reshape(-1, d).mean(axis=0).tolist(),
+        "split_rhat": {"learned": split_rhat(tl.samples).tolist(), "random_walk": split_rhat(tr.samples).tolist(),
+                       "long": split_rhat(tlong.samples).tolist()},
         "mean_z_learned_vs_long": z.tolist(), "max_abs_z_learned": float(np.max(np.abs(z))),
+        "mean_z_learned_vs_long_ess_only": z_ess.tolist(),
+        "mcse": {"learned": mcse(tl.samples, e_l).tolist(), "long": mcse(tlong.samples, e_long).tolist()},
         "mean_z_rw_vs_long": z_rw.tolist(), "max_abs_z_rw": float(np.max(np.abs(z_rw))),
     }
     res["ess_per_s_gain_min"] = res["learned"]["min_ess_per_s"] / res["random_walk"]["min_ess_per_s"]

@@ -256,3 +256,28 @@ def test_laplace_recovers_gaussian_mean_and_covariance():
     m, cov = mcmc.laplace(counted, mean + 0.05, 8, log=lambda *a: None)
     assert np.allclose(m, mean, atol=1e-8)
     assert np.allclose(cov, C, rtol=1e-4, atol=1e-9)
+
+
+def test_split_rhat_detects_unmixed_chains():
+    rng = np.random.default_rng(9)
+    good = rng.normal(0, 1, (2000, 4, 1))
+    assert abs(mcmc.split_rhat(good)[0] - 1.0) < 0.01
+    bad = good + np.array([0.0, 0.0, 3.0, 3.0])[None, :, None]
+    assert mcmc.split_rhat(bad)[0] > 1.5
+
+
+def test_mcse_covers_chains_stuck_in_modes():
+    """Chains that never leave their mode: the standard error is at least the
+    spread of the chain means over sqrt(nchains), and at least sd / sqrt(ESS)."""
+    rng = np.random.default_rng(10)
+    T, m = 1000, 16
+    centre = np.where(np.arange(m) < 4, -2.0, 1.0)
+    x = (centre[None, :] + 0.3 * rng.standard_normal((T, m)))[:, :, None]
+    e = mcmc.ess(x)
+    se = mcmc.mcse(x, e)[0]
+    se_ess = x.reshape(-1).std() / np.sqrt(e[0])
+    se_rep = x.mean(axis=0)[:, 0].std(ddof=1) / np.sqrt(m)
+    assert se == pytest.approx(max(se_ess, se_rep), rel=1e-12)
+    # and with well-mixed chains the replicate term does not dominate
+    y = rng.standard_normal((T, m, 1))
+    assert mcmc.mcse(y, mcmc.ess(y))[0] == pytest.approx(y.std() / np.sqrt(mcmc.ess(y)[0]), rel=0.5)
