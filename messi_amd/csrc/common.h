@@ -37,6 +37,17 @@ inline void kh99_logprobs(double lp[7]) {
   lp[6] = std::log(0.105397);
 }
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin by workgroup
+// id, and each XCD has its own L2.  This permutation gives XCD k a contiguous
+// chunk of the logical work items (in their natural order), so items that share
+// operands run on the same XCD and close in time.  (On another XCD count it is
+// still a permutation.)
+__device__ __forceinline__ int xcd_chunk_id(int x, int n) {
+  constexpr int NX = 8;
+  const int k = x % NX, q = n / NX, r = n % NX;
+  return k * q + (k < r ? k : r) + x / NX;
+}
+
 // Device buffer that only grows.
 struct DevBuf {
   void *p = nullptr;

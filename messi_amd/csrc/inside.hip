@@ -183,145 +183,185 @@ __device__ __forceinline__ typename SR::S grp_sum(typename SR::S s, int G) {
 // ---------------------------------------------------------------------------
 // Shared memory of one step workgroup.
 // ---------------------------------------------------------------------------
-// Staging of one T x T tile product dst[a][b] (+)= sum_c A[a][c] * B[c][b].
-template <class SR, int T> struct ProdLDS {
+// Staging of one T x T tile product sum_c A[a][c] * B[c][b].  Log values are
+// staged as they are; scaled values as exponent-aligned float mantissas with
+// their row / column exponent maxima, plus the MFMA result.
+template <class SR, int T, bool S = SR::scaled> struct ProdLDS;
+template <class SR, int T> struct ProdLDS<SR, T, false> {
   typedef typename SR::V V;
   V A[T][T + 1];
   V B[T][T + 1];
-  float Ap[T][T + 1];  // exponent-aligned mantissas (scaled representation)
+};
+template <class SR, int T> struct ProdLDS<SR, T, true> {
+  XFv A[T][T + 1];     // the operands as they are (for the exact fallback)
+  XFv B[T][T + 1];
+  union {
+    float Ap[T][T + 1];  // exponent-aligned mantissas ...
+    float R[T][T + 1];   // ... overwritten by the MFMA result of the same wavefront
+  };
   float Bp[T][T + 1];
-  int rA[T], cB[T];    // row / column exponent maxima
+  int rA[T], cB[T];
 };
 
 template <class SR, int T> struct StepLDS {
   typedef typename SR::V V;
   V Ld[T][T + 1];   // L of the diagonal tile (I, I)
   V Sd[T][T + 1];   // S of the diagonal tile (J, J)
-  V cA[T][T + 1];   // bulk accumulator of the target tile
   V cP[T][T + 1];   // pair term of the target tile
   union {
     struct {
       V cS[T][T + 1], cF[T][T + 1], cL[T][T + 1];
     } w;                  // the tile being finalised
-    ProdLDS<SR, T> p;     // a split-point product
+    ProdLDS<SR, T> p[2];  // the (at most two) split-point products of a launch
   } u;
   V Sb[T], Fbot[T], Fleft[T];
 };
 
-// Exact exponent-aligned split-point sum of one cell of a staged product (the
-// rare case where aligning a whole row/column lost the significant terms).
-template <int T>
-__device__ __forceinline__ XFv exact_cell_xf(const XFv (*A)[T + 1], const XFv (*B)[T + 1], int a, int b) {
-  int km = XF::kmin();
-  for (int c = 0; c < T; ++c) km = max(km, XF::key(A[a][c], B[c][b]));
-  float s = 0.0f;
-  for (int c = 0; c < T; ++c) s += XF::contrib(A[a][c], B[c][b], km);
-  return XF::finish(km, s);
+// Where the two operand tiles of a product live: element (r, c) of A is
+// A[(ar0 + r) * L + ac0 + c] and element (c, r) of B is B[(br0 + r) * L + bc0 + c],
+// each only where it names a cell of the upper triangle (row <= column in a
+// row-major table: up == true; column <= row index in a column-major one: up ==
+// false), zero elsewhere.
+template <class SR> struct OpSrc {
+  const typename SR::V *A;
+  int ar0, ac0;
+  bool aup;
+  const typename SR::V *B;
+  int br0, bc0;
+  bool bup;
+};
+
+template <class SR>
+__device__ __forceinline__ typename SR::V op_a(const OpSrc<SR> &s, int L, int r, int c) {
+  const int ar = s.ar0 + r, ac = s.ac0 + c;
+  return (ar >= 0 && ac >= 0 && ar < L && ac < L && (s.aup ? ar <= ac : ac <= ar)) ? s.A[(size_t)ar * L + ac]
+                                                                                 : SR::zero();
+}
+template <class SR>
+__device__ __forceinline__ typename SR::V op_b(const OpSrc<SR> &s, int L, int r, int c) {
+  const int br = s.br0 + r, bc = s.bc0 + c;
+  return (br >= 0 && bc >= 0 && br < L && bc < L && (s.bup ? br <= bc : bc <= br)) ? s.B[(size_t)br * L + bc]
+                                                                                 : SR::zero();
 }
 
-// Operands of one split-point product, one element of each tile per thread:
-// a = L[IT + tid/T, KT + tid%T], b = S[KT + tid%T + 1, JT + tid/T].
+// Operands of one product, one element of each tile per thread (r = tid / T,
+// c = tid % T): a = A(r, c), b = B(c, r).
 template <class SR> struct Operands {
   typename SR::V a, b;
 };
 
 template <class SR, int T>
-__device__ __forceinline__ Operands<SR> fetch_operands(int L, int I, int K, int J,
-                                                       const typename SR::V *__restrict__ Lr,
-                                                       const typename SR::V *__restrict__ Sc) {
+__device__ __forceinline__ Operands<SR> fetch_operands(const OpSrc<SR> &s, int L) {
   const int tid = threadIdx.x, r = tid / T, c = tid % T;
-  const int j = I * T + r, h = K * T + c;
-  const int y = J * T + r, sr = K * T + c + 1;
   Operands<SR> o;
-  o.a = (j < L && h < L) ? Lr[(size_t)j * L + h] : SR::zero();
-  o.b = (y < L && sr < L) ? Sc[(size_t)y * L + sr] : SR::zero();
+  o.a = op_a(s, L, r, c);
+  o.b = op_b(s, L, r, c);
   return o;
 }
 
-// General operand fetch: a = A[(ar0 + r) * L + ac0 + c] and b = B[(br0 + r) * L +
-// bc0 + c] (r = tid / T, c = tid % T), each only where it names a cell of the
-// upper triangle: row <= column in a row-major table (up == true), column <=
-// row index in a column-major one (up == false); zero elsewhere.
+// Inside split-point product (I,K) x (K,J): L[IT+a, KT+c] * S[KT+c+1, JT+b].
 template <class SR, int T>
-__device__ __forceinline__ Operands<SR> fetch_general(int L, const typename SR::V *__restrict__ A, int ar0,
-                                                      int ac0, bool aup, const typename SR::V *__restrict__ B,
-                                                      int br0, int bc0, bool bup) {
-  const int tid = threadIdx.x, r = tid / T, c = tid % T;
-  const int ar = ar0 + r, ac = ac0 + c, br = br0 + r, bc = bc0 + c;
-  Operands<SR> o;
-  o.a = (ar >= 0 && ac >= 0 && ar < L && ac < L && (aup ? ar <= ac : ac <= ar)) ? A[(size_t)ar * L + ac]
-                                                                                 : SR::zero();
-  o.b = (br >= 0 && bc >= 0 && br < L && bc < L && (bup ? br <= bc : bc <= br)) ? B[(size_t)br * L + bc]
-                                                                                 : SR::zero();
-  return o;
+__device__ __forceinline__ OpSrc<SR> inside_src(int I, int K, int J, const typename SR::V *Lr,
+                                               const typename SR::V *Sc) {
+  return OpSrc<SR>{Lr, I * T, K * T, true, Sc, J * T, K * T + 1, false};
 }
 
-// dst[a][b] (+)= sum_c A[a][c] * B[c][b] with A[a][c] = o.a of thread a*T+c and
-// B[c][b] = o.b of thread b*T+c (inside: L[IT+a, h] * S[h+1, JT+b] over h in tile
-// K), from operands already fetched into registers.
+// Exact sum over c of A[a][c] * B[c][b] of a staged product (the rare cell whose
+// exponent-aligned sum lost the significant terms).
+template <int T>
+__device__ __forceinline__ XFv exact_cell_xf(const XFv (*A)[T + 1], const XFv (*B)[T + 1], int a, int b) {
+  int km = XF::kmin();
+  for (int c = 0; c < T; ++c) km = max(km, XF::key(A[a][c], B[c][b]));
+  float sum = 0.0f;
+  for (int c = 0; c < T; ++c) sum += XF::contrib(A[a][c], B[c][b], km);
+  return XF::finish(km, sum);
+}
+
+// acc0 (+)= product 0 and acc1 (+)= product 1 (each if do_k) for this thread's
+// cell (a, b) = (tid / T, tid % T), from operands already fetched into registers.
+// Both products are staged together; the MFMAs of the two run on different
+// wavefronts; each thread then folds its own cell.  Ends with a barrier.
 template <class SR, int T>
-__device__ __forceinline__ void tile_product(const Operands<SR> &o, ProdLDS<SR, T> &pp,
-                                             typename SR::V (*dst)[T + 1]) {
+__device__ __forceinline__ void tile_products(const Operands<SR> &o0, bool do0, const Operands<SR> &o1, bool do1,
+                                              ProdLDS<SR, T> *pp, typename SR::V &acc0, typename SR::V &acc1) {
   typedef typename SR::V V;
   const int tid = threadIdx.x, r = tid / T, c = tid % T;
-  pp.A[r][c] = o.a;
-  pp.B[c][r] = o.b;
   if constexpr (SR::scaled) {
-    // Row maxima of A's exponents and column maxima of B's (over the split
-    // points c, held by T consecutive lanes), then exponent-aligned floats.
-    int ea = o.a.m != 0.0f ? o.a.e : XF::EZ;
-    int eb = o.b.m != 0.0f ? o.b.e : XF::EZ;
 #pragma unroll
-    for (int off = T / 2; off > 0; off >>= 1) {
-      ea = max(ea, __shfl_xor(ea, off));
-      eb = max(eb, __shfl_xor(eb, off));
-    }
-    pp.Ap[r][c] = o.a.m == 0.0f ? 0.0f : ldexpf(o.a.m, max(o.a.e - ea, -200));
-    pp.Bp[c][r] = o.b.m == 0.0f ? 0.0f : ldexpf(o.b.m, max(o.b.e - eb, -200));
-    if (c == 0) {
-      pp.rA[r] = ea;
-      pp.cB[r] = eb;
+    for (int k = 0; k < 2; ++k) {
+      const Operands<SR> &o = k ? o1 : o0;
+      if (!(k ? do1 : do0)) continue;
+      // Row maxima of A's exponents and column maxima of B's (over the split
+      // points c, held by T consecutive lanes), then exponent-aligned floats.
+      int ea = o.a.m != 0.0f ? o.a.e : XF::EZ;
+      int eb = o.b.m != 0.0f ? o.b.e : XF::EZ;
+#pragma unroll
+      for (int off = T / 2; off > 0; off >>= 1) {
+        ea = max(ea, __shfl_xor(ea, off));
+        eb = max(eb, __shfl_xor(eb, off));
+      }
+      pp[k].A[r][c] = o.a;
+      pp[k].B[c][r] = o.b;
+      pp[k].Ap[r][c] = o.a.m == 0.0f ? 0.0f : ldexpf(o.a.m, max(o.a.e - ea, -200));
+      pp[k].Bp[c][r] = o.b.m == 0.0f ? 0.0f : ldexpf(o.b.m, max(o.b.e - eb, -200));
+      if (c == 0) {
+        pp[k].rA[r] = ea;
+        pp[k].cB[r] = eb;
+      }
     }
     __syncthreads();
-    // One MFMA 16x16x4 f32 chain per 16x16 block.
+    // One MFMA 16x16x4 f32 chain per 16x16 block; product k's blocks on waves
+    // k * NB^2 .. (k + 1) * NB^2 - 1.
     typedef float v4f __attribute__((ext_vector_type(4)));
     const int wave = tid >> 6, lane = tid & 63;
-    constexpr int NB = T / 16;
-    for (int blk = wave; blk < NB * NB; blk += T * T / 64) {
-      const int bi = blk / NB, bj = blk % NB;
-      v4f acc = {0.f, 0.f, 0.f, 0.f};
+    constexpr int NB = T / 16, NB2 = NB * NB;
+    if (wave < 2 * NB2) {
+      const int k = wave / NB2, blk = wave % NB2;
+      if (k ? do1 : do0) {
+        const int bi = blk / NB, bj = blk % NB;
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int s = 0; s < T / 4; ++s) {
-        float av = pp.Ap[bi * 16 + (lane & 15)][4 * s + (lane >> 4)];
-        float bv = pp.Bp[4 * s + (lane >> 4)][bj * 16 + (lane & 15)];
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int a = bi * 16 + 4 * (lane >> 4) + q, b = bj * 16 + (lane & 15);
-        const int ra = pp.rA[a], cb = pp.cB[b];
-        V contrib = SR::zero();
-        if (ra != XF::EZ && cb != XF::EZ) {
-          float cv = acc[q];
-          if (cv >= 0x1p-100f) {
-            contrib = XF::norm(cv, ra + cb);
-          } else {
-            contrib = exact_cell_xf<T>(pp.A, pp.B, a, b);
-          }
+        for (int q = 0; q < T / 4; ++q) {
+          float av = pp[k].Ap[bi * 16 + (lane & 15)][4 * q + (lane >> 4)];
+          float bv = pp[k].Bp[4 * q + (lane >> 4)][bj * 16 + (lane & 15)];
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
         }
-        dst[a][b] = SR::add(dst[a][b], contrib);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pp[k].R[bi * 16 + 4 * (lane >> 4) + q][bj * 16 + (lane & 15)] = acc[q];
       }
     }
-  } else {
     __syncthreads();
-    for (int idx = tid; idx < T * T; idx += T * T) {
-      int a = idx / T, b = idx % T;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!(k ? do1 : do0)) continue;
+      const int ra = pp[k].rA[r], cb = pp[k].cB[c];
+      V contrib = SR::zero();
+      if (ra != XF::EZ && cb != XF::EZ) {
+        const float cv = pp[k].R[r][c];
+        contrib = cv >= 0x1p-100f ? XF::norm(cv, ra + cb) : exact_cell_xf<T>(pp[k].A, pp[k].B, r, c);
+      }
+      V &acc = k ? acc1 : acc0;
+      acc = SR::add(acc, contrib);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!(k ? do1 : do0)) continue;
+      const Operands<SR> &o = k ? o1 : o0;
+      pp[k].A[r][c] = o.a;
+      pp[k].B[c][r] = o.b;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!(k ? do1 : do0)) continue;
       typename SR::K km = SR::kmin();
-      for (int k = 0; k < T; ++k) km = SR::kmax(km, SR::key(pp.A[a][k], pp.B[k][b]));
+      for (int q = 0; q < T; ++q) km = SR::kmax(km, SR::key(pp[k].A[r][q], pp[k].B[q][c]));
       typename SR::S sum = 0;
       if (km != SR::kmin())
-        for (int k = 0; k < T; ++k) sum += SR::contrib(pp.A[a][k], pp.B[k][b], km);
-      dst[a][b] = SR::add(dst[a][b], SR::finish(km, sum));
+        for (int q = 0; q < T; ++q) sum += SR::contrib(pp[k].A[r][q], pp[k].B[q][c], km);
+      V &acc = k ? acc1 : acc0;
+      acc = SR::add(acc, SR::finish(km, sum));
     }
   }
   __syncthreads();
@@ -335,17 +375,18 @@ __device__ __forceinline__ int bulk_count(int NT, int t, int d) {
 
 // One launch of the inside schedule (see the file header).
 template <class SR, int T>
-__global__ __launch_bounds__(T * T, T == 16 ? 8 : 1) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
+__global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
                                                             const int *__restrict__ prefix, int n, int t,
                                                             Rules<SR> r) {
   typedef typename SR::V V;
   __shared__ StepLDS<SR, T> sm;
   const int tid = threadIdx.x;
-  // ---- instance of this workgroup: prefix[m] <= blockIdx.x < prefix[m+1]
+  const int bx = xcd_chunk_id(blockIdx.x, gridDim.x);
+  // ---- instance of this workgroup: prefix[m] <= bx < prefix[m+1]
   int lo = 0, hi = n;
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
-    if (prefix[mid] <= (int)blockIdx.x) lo = mid;
+    if (prefix[mid] <= bx) lo = mid;
     else hi = mid;
   }
   const InstDesc<V> ins = insts[lo];
@@ -358,7 +399,7 @@ __global__ __launch_bounds__(T * T, T == 16 ? 8 : 1) void inside_step_kernel(con
   V *__restrict__ acc = ins.acc;
   const V *__restrict__ ptv = ins.ptv;
   // ---- decode the work item
-  int item = (int)blockIdx.x - prefix[lo], I, J;
+  int item = bx - prefix[lo], I, J;
   bool fin;
   if (item < NT - t) {
     fin = true;
@@ -388,18 +429,18 @@ __global__ __launch_bounds__(T * T, T == 16 ? 8 : 1) void inside_step_kernel(con
     const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
     if ((t >= 2 || !fin) && j < L && y < L) accv = acc[(size_t)j * L + y];
   }
+  const OpSrc<SR> s1 = inside_src<SR, T>(I, K1, J, Lr, Sc), s2 = inside_src<SR, T>(I, K2, J, Lr, Sc);
   Operands<SR> o1, o2;
-  if (do1) o1 = fetch_operands<SR, T>(L, I, K1, J, Lr, Sc);
-  if (do2) o2 = fetch_operands<SR, T>(L, I, K2, J, Lr, Sc);
-  sm.cA[tid / T][tid % T] = accv;
-  if (do1) tile_product<SR, T>(o1, sm.u.p, sm.cA);
-  if (do2) tile_product<SR, T>(o2, sm.u.p, sm.cA);
-  __syncthreads();
+  if (do1) o1 = fetch_operands<SR, T>(s1, L);
+  if (do2) o2 = fetch_operands<SR, T>(s2, L);
+  if (do1 || do2) {
+    V acc2 = SR::zero();
+    tile_products<SR, T>(o1, do1, o2, do2, sm.u.p, accv, acc2);
+    accv = SR::add(accv, acc2);
+  }
   if (!fin) {
-    for (int idx = tid; idx < T * T; idx += T * T) {
-      int a = idx / T, b = idx % T, j = IT + a, y = JT + b;
-      if (j < L && y < L) acc[(size_t)j * L + y] = sm.cA[a][b];
-    }
+    const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
+    if (j < L && y < L) acc[(size_t)j * L + y] = accv;
     return;
   }
   // ---- finalise tile (I, J): stage operands
@@ -441,7 +482,7 @@ __global__ __launch_bounds__(T * T, T == 16 ? 8 : 1) void inside_step_kernel(con
   const int j = IT + a, y = JT + b;
   const bool mine = j < L && y < L && (t > 0 || a < b);
   const typename SR::FinRules rk = SR::fin_rules(r.p[6], r.lin[0], r.lin[2], r.lin[3], r.lin[4], r.lin[6]);
-  typename SR::Acc ac = SR::acc(sm.cA[a][b]);
+  typename SR::Acc ac = SR::acc(accv);
   if (t == 0) {
     const int own = b - a;
     for (int st = 1; st < T; ++st) {
@@ -568,10 +609,7 @@ __host__ __device__ inline bool out_col_ok(int m, int T, int band) { return band
 template <class SR, int T> struct OutLDS {
   typedef typename SR::V V;
   union {
-    struct {
-      V aL[T][T + 1], aS[T][T + 1];
-      ProdLDS<SR, T> p;
-    } pr;  // product phase
+    ProdLDS<SR, T> p[2];  // product phase
     struct {
       V W[T][T + 1], Q[T][T + 1];
       V iSd[T][T + 1];  // iS[JT+b+1, JT+c] as [b][c]
@@ -603,7 +641,7 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   // ---- decode: u+1 finalising tiles first, then the bulk tiles by distance
   int I, d, m;
   bool fin;
-  const int item = blockIdx.x;
+  const int item = xcd_chunk_id(blockIdx.x, gridDim.x);
   if (item <= u) {
     fin = true;
     I = item;
@@ -633,26 +671,23 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   V vL = SR::zero(), vS = SR::zero();
   if (cell0 && (fin || dorow)) vL = accL[(size_t)j0 * L + y0];
   if (cell0 && (fin || docol)) vS = accS[(size_t)j0 * L + y0];
+  const OpSrc<SR> srow{Wr, IT, (J + m) * T, true, iSr, JT + 1, (J + m) * T, true};
+  const OpSrc<SR> scol{iLc, IT - 1, (I - m) * T, false, Wc, JT, (I - m) * T, false};
   Operands<SR> orow, ocol;
-  if (dorow) orow = fetch_general<SR, T>(L, Wr, IT, (J + m) * T, true, iSr, JT + 1, (J + m) * T, true);
-  if (docol) ocol = fetch_general<SR, T>(L, iLc, IT - 1, (I - m) * T, false, Wc, JT, (I - m) * T, false);
-  sm.u.pr.aL[a0][b0] = vL;
-  sm.u.pr.aS[a0][b0] = vS;
-  if (dorow) tile_product<SR, T>(orow, sm.u.pr.p, sm.u.pr.aL);
-  if (docol) tile_product<SR, T>(ocol, sm.u.pr.p, sm.u.pr.aS);
-  __syncthreads();
+  if (dorow) orow = fetch_operands<SR, T>(srow, L);
+  if (docol) ocol = fetch_operands<SR, T>(scol, L);
+  if (dorow || docol) tile_products<SR, T>(orow, dorow, ocol, docol, sm.u.p, vL, vS);
   if (!fin) {
     if (cell0) {
-      if (dorow) accL[(size_t)j0 * L + y0] = sm.u.pr.aL[a0][b0];
-      if (docol) accS[(size_t)j0 * L + y0] = sm.u.pr.aS[a0][b0];
+      if (dorow) accL[(size_t)j0 * L + y0] = vL;
+      if (docol) accS[(size_t)j0 * L + y0] = vS;
     }
     return;
   }
   // ---- finalise tile (I, J)
   const int a = a0, b = b0, j = j0, y = y0;
-  typename SR::Acc acL = SR::acc(sm.u.pr.aL[a][b]);
-  typename SR::Acc acS = SR::acc(sm.u.pr.aS[a][b]);
-  __syncthreads();  // the product staging is reused below
+  typename SR::Acc acL = SR::acc(vL);
+  typename SR::Acc acS = SR::acc(vS);
   {
     const int rr = JT + a + 1, cc = JT + b;  // iSd[a][b] = iS[JT+a+1, JT+b]
     sm.u.w.iSd[a][b] = (rr < L && cc < L && rr <= cc) ? iSr[(size_t)rr * L + cc] : SR::zero();

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
     double *__restrict__ rootcomp) {
   typedef typename Mfma<T>::v4 v4;
   const int lane = threadIdx.x & 63;
-  const int wave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+  const int wave = (int)((xcd_chunk_id(blockIdx.x, gridDim.x) * (unsigned)blockDim.x + threadIdx.x) >> 6);
   if (wave >= nitems) return;  // uniform per wavefront
   const int2 it = items[wave];
   const NodeDesc nd = nodes[it.x];
