@@ -352,6 +352,16 @@ def main():
                            "frac": achieved / PEAK["fp32"], "traffic": None,
                            "kernel": f"inside_step_kernel<XF,16> (inside phase of a batch of {args.batch}, 2 flops "
                                      f"per bifurcation term, {int(T_in)} terms per evaluation)"}
+        # HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE passes of this same workload
+        # (copy-calibrated); only quoted for the shape the counters were taken on
+        tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                          "inside_xf16_config2_batch16_traffic.json")
+        if args.batch == 16 and args.tile in (0, 16) and os.path.exists(tf):
+            with open(tf) as f:
+                tj = json.load(f)
+            out["roofline"]["traffic"] = tj["hbm_bytes_per_launch"]
+            out["roofline"]["traffic_note"] = ("bytes per launch, mean over the 94 launches of one batch-16 "
+                                               "evaluation, from profiles/" + os.path.basename(tf))
     if "fp32" in res and "fp64" in res:
         out["fp32_over_fp64"] = {k: res["fp32"][k]["evals_per_s"] / res["fp64"][k]["evals_per_s"]
                                  for k in ("batch1", f"batch{args.batch}")}
