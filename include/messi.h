@@ -110,6 +110,33 @@ int messi_posterior(messi_dataset *ds, int K4, const double *P4, const double *p
                     const double *logw16, int prec, double *logZ, double *unpaired,
                     double *pairprob);
 
+/* Maximum-expected-accuracy consensus structure from base-pair posteriors
+ * (getPosteriorDecodingConsensusStructure, src/RNATools.jl:76-85): the fill of
+ * computeposteriordecodingcuda (src/InsideCUDA.jl:183-215, kernel
+ * src/cuda/inside.cu:286-321) and posteriordecodingtraceback (src/RNATools.jl:44-74).
+ * unpaired (L) and pairprob (L x L, column-major) are as messi_bpp returns them.
+ * prec MESSI_FP64 follows iterativeposteriordecoding (src/RNATools.jl:6-42) with
+ * its arguments in the order of its signature; MESSI_FP32 follows the CUDA kernel
+ * (float inputs, pairprob read at i*L + k, alpha*p + E fused).  paired (L)
+ * receives the 1-based partner of each position, 0 when unpaired; emat (double)
+ * and smat (-1, -2 or the 1-based split point) receive the L x L tables,
+ * column-major, zero on and below the diagonal, or are NULL.  1 <= L <= 16383. */
+int messi_mea(messi_ctx *ctx, int L, const double *unpaired, const double *pairprob, double alpha, int prec,
+              int *paired, double *emat, int *smat);
+
+/* Milliseconds of the last MEA call with profiling on: host upload, fill,
+ * traceback (the upload is 0 after messi_posterior_consensus).  No counterpart in
+ * the reference. */
+int messi_mea_timings(messi_ctx *ctx, double *t);
+
+/* Evaluation, outside, base-pair posteriors and MEA consensus on the device with
+ * only log Z and the pairing returned (the processmax path, src/MESSI.jl:1097-1115).
+ * prec selects the likelihood path, mea_prec the decoding. */
+int messi_posterior_consensus(messi_dataset *ds, int K4, const double *P4, const double *pi4,
+                              const double *logw4, int K16, const double *P16, const double *pi16,
+                              const double *logw16, int prec, int mea_prec, double alpha, double *logZ,
+                              int *paired);
+
 /* Batch of members, each a dataset (the same dataset may appear several times,
  * e.g. one member per MCMC chain or proposal).  messi_batch_eval evaluates all
  * members together: each tree level and each tile-diagonal of the inside

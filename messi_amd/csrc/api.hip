@@ -767,6 +767,52 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
 
 }  // namespace
 
+namespace {
+
+// Events of one MEA call (profiling on): [0] start, [1] after the upload, [2] after
+// the fill, [3] after the traceback.
+struct MeaEvents {
+  hipEvent_t e[4] = {nullptr, nullptr, nullptr, nullptr};
+  double ms[3] = {0, 0, 0};
+};
+MeaEvents &mea_events(messi_ctx *ctx) {
+  static MeaEvents ev[64];
+  MeaEvents &m = ev[ctx->device & 63];
+  if (!m.e[0])
+    for (auto &e : m.e) MESSI_HIP(hipEventCreate(&e));
+  return m;
+}
+
+template <typename V>
+void mea_timed(messi_ctx *ctx, int L, const double *d_single, const double *d_pair, double alpha, int *d_paired) {
+  hipStream_t s = ctx->stream;
+  if (ctx->profiling) {
+    MeaEvents &m = mea_events(ctx);
+    MESSI_HIP(hipEventRecord(m.e[1], s));
+    mea_run<V>(s, ctx->mea, L, d_single, d_pair, alpha, nullptr);
+    MESSI_HIP(hipEventRecord(m.e[2], s));
+    mea_run_traceback(s, ctx->mea, L, d_paired);
+    MESSI_HIP(hipEventRecord(m.e[3], s));
+  } else {
+    mea_run<V>(s, ctx->mea, L, d_single, d_pair, alpha, d_paired);
+  }
+}
+
+void mea_collect(messi_ctx *ctx, bool upload) {
+  if (!ctx->profiling) return;
+  MeaEvents &m = mea_events(ctx);
+  MESSI_HIP(hipEventSynchronize(m.e[3]));
+  float a = 0.f, b = 0.f, c = 0.f;
+  if (upload) MESSI_HIP(hipEventElapsedTime(&a, m.e[0], m.e[1]));
+  MESSI_HIP(hipEventElapsedTime(&b, m.e[1], m.e[2]));
+  MESSI_HIP(hipEventElapsedTime(&c, m.e[2], m.e[3]));
+  m.ms[0] = upload ? a : 0.0;
+  m.ms[1] = b;
+  m.ms[2] = c;
+}
+
+}  // namespace
+
 extern "C" {
 
 int messi_ctx_create(int device, messi_ctx **out) {
@@ -1070,6 +1116,92 @@ int messi_posterior(messi_dataset *ds, int K4, const double *P4, const double *p
       if (pairprob) MESSI_HIP(hipMemcpyAsync(pairprob, dpair, sizeof(double) * LL, hipMemcpyDeviceToHost, s));
       MESSI_HIP(hipStreamSynchronize(s));
       collect(b, 5);
+    };
+    if (prec == MESSI_FP32) go(float{});
+    else go(double{});
+  });
+}
+
+int messi_mea(messi_ctx *ctx, int L, const double *unpaired, const double *pairprob, double alpha, int prec,
+              int *paired, double *emat, int *smat) {
+  if (!ctx) return 2;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(L >= 1 && L <= MEA_MAX_L, "MEA decoding supports 1 <= L <= 16383");
+    MESSI_REQUIRE(unpaired && pairprob && paired, "null argument");
+    MESSI_REQUIRE(std::isfinite(alpha), "alpha must be finite");
+    check_prec(prec);
+    MESSI_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    MeaWork &w = ctx->mea;
+    const size_t LL = (size_t)L * L;
+    w.single.ensure(sizeof(double) * L);
+    w.pair.ensure(sizeof(double) * LL);
+    w.paired.ensure(sizeof(int) * L);
+    if (ctx->profiling) MESSI_HIP(hipEventRecord(mea_events(ctx).e[0], s));
+    MESSI_HIP(hipMemcpyAsync(w.single.p, unpaired, sizeof(double) * L, hipMemcpyHostToDevice, s));
+    MESSI_HIP(hipMemcpyAsync(w.pair.p, pairprob, sizeof(double) * LL, hipMemcpyHostToDevice, s));
+    if (prec == MESSI_FP32)
+      mea_timed<float>(ctx, L, w.single.as<double>(), w.pair.as<double>(), alpha, w.paired.as<int>());
+    else
+      mea_timed<double>(ctx, L, w.single.as<double>(), w.pair.as<double>(), alpha, w.paired.as<int>());
+    MESSI_HIP(hipMemcpyAsync(paired, w.paired.p, sizeof(int) * L, hipMemcpyDeviceToHost, s));
+    if (emat || smat) {
+      DevBuf de, ds;
+      if (emat) de.ensure(sizeof(double) * LL);
+      if (smat) ds.ensure(sizeof(int) * LL);
+      mea_export(s, w, L, emat ? de.as<double>() : nullptr, smat ? ds.as<int>() : nullptr);
+      if (emat) MESSI_HIP(hipMemcpyAsync(emat, de.p, sizeof(double) * LL, hipMemcpyDeviceToHost, s));
+      if (smat) MESSI_HIP(hipMemcpyAsync(smat, ds.p, sizeof(int) * LL, hipMemcpyDeviceToHost, s));
+      MESSI_HIP(hipStreamSynchronize(s));
+    }
+    MESSI_HIP(hipStreamSynchronize(s));
+    mea_collect(ctx, true);
+  });
+}
+
+int messi_mea_timings(messi_ctx *ctx, double *t) {
+  if (!ctx || !t) return 2;
+  const MeaEvents &m = mea_events(ctx);
+  for (int q = 0; q < 3; ++q) t[q] = m.ms[q];
+  return 0;
+}
+
+int messi_posterior_consensus(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
+                              int K16, const double *P16, const double *pi16, const double *logw16, int prec,
+                              int mea_prec, double alpha, double *logZ, int *paired) {
+  if (!ds) return 2;
+  messi_ctx *ctx = ds->ctx;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(P4 && pi4 && logw4 && P16 && pi16 && logw16 && logZ && paired, "null argument");
+    MESSI_REQUIRE(std::isfinite(alpha), "alpha must be finite");
+    check_prec(prec);
+    check_prec(mea_prec);
+    const int L = ds->L;
+    MESSI_REQUIRE(L <= MEA_MAX_L, "MEA decoding supports L <= 16383");
+    MESSI_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t LL = (size_t)L * L;
+    messi_batch *b = self_batch(ds);
+    auto go = [&](auto tag) {
+      using T = decltype(tag);
+      InsideWork &w = member_work(b, 0, sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64);
+      load_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+      eval_impl<T>(b, 1.0, true, logZ);
+      outside_run<T>(s, w, L, false);
+      record(b, 4);
+      w.out3.ensure(sizeof(double) * (LL + L));
+      double *dpair = w.out3.as<double>();
+      double *dunp = dpair + LL;
+      bpp_run<T>(s, w, L, dunp, dpair);
+      record(b, 5);
+      collect(b, 5);
+      ctx->mea.paired.ensure(sizeof(int) * L);
+      if (ctx->profiling) MESSI_HIP(hipEventRecord(mea_events(ctx).e[0], s));
+      if (mea_prec == MESSI_FP32) mea_timed<float>(ctx, L, dunp, dpair, alpha, ctx->mea.paired.as<int>());
+      else mea_timed<double>(ctx, L, dunp, dpair, alpha, ctx->mea.paired.as<int>());
+      MESSI_HIP(hipMemcpyAsync(paired, ctx->mea.paired.p, sizeof(int) * L, hipMemcpyDeviceToHost, s));
+      MESSI_HIP(hipStreamSynchronize(s));
+      mea_collect(ctx, false);
     };
     if (prec == MESSI_FP32) go(float{});
     else go(double{});

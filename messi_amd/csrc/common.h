@@ -108,6 +108,15 @@ struct InsideWork {
   InsideBatch ib1;                // launch tables when run as a batch of one
 };
 
+// Tables of MEA decoding (L x L, cells i < j; E and U column-major, S and the
+// split-point accumulators row-major).
+constexpr int MEA_MAX_L = 16383;  // the traceback stack (4 bytes per entry) fits 64 KiB of LDS
+struct MeaWork {
+  int L = 0;
+  bool fp32 = false;
+  DevBuf E, U, S, accV, accK;
+  DevBuf single, pair, paired;  // staging of host inputs and the result
+};
 
 }  // namespace messi
 
@@ -118,4 +127,5 @@ struct messi_ctx {
   bool profiling = false;
   int tile = 16;  // tile edge of the inside schedule
   messi::InsideWork w64, w32;
+  messi::MeaWork mea;
 };

@@ -66,5 +66,14 @@ template <typename T> void outside_run(hipStream_t s, InsideWork &w, int L, bool
 template <typename T> void outside_export(hipStream_t s, InsideWork &w, int L, double *d_out3);
 template <typename T> void outside_import(hipStream_t s, InsideWork &w, int L, const double *d_out3);
 template <typename T> void bpp_run(hipStream_t s, InsideWork &w, int L, double *d_unpaired, double *d_pair);
+// MEA decoding of (single, pair) on the device; d_paired (optional) receives the
+// traceback (1-based partners, 0 unpaired).  V = double: FP64 semantics, float: FP32.
+template <typename V>
+void mea_run(hipStream_t s, MeaWork &w, int L, const double *d_single, const double *d_pair, double alpha,
+             int *d_paired);
+// Traceback of the last mea_run's S table into d_paired.
+void mea_run_traceback(hipStream_t s, MeaWork &w, int L, int *d_paired);
+// E (as double) and S of the last mea_run, column-major, zero on and below the diagonal.
+void mea_export(hipStream_t s, MeaWork &w, int L, double *d_E, int *d_S);
 
 }  // namespace messi

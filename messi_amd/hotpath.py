@@ -11,6 +11,10 @@ Function here                    Reference
 ``computeoutsideKH99``           src/InsideCUDA.jl:55-61
 ``computebasepairprobs``         src/InsideOutsideAlgorithm.jl:486-533
 ``computestructureintegratedlikelihood``  src/FARCE.jl:2322-2345
+``getPosteriorDecodingConsensusStructure``  src/RNATools.jl:76-85
+``computeposteriordecodingcuda``  src/InsideCUDA.jl:183-215
+``iterativeposteriordecoding``   src/RNATools.jl:6-42
+``consensus``                    src/MESSI.jl:1097-1115 (posteriors and decoding in one device pass)
 ===============================  ==========================================================
 
 ``prec`` selects the plain FP64 path (default) or the scaled FP32 path.
@@ -131,3 +135,37 @@ def posterior(dataset, params: ModelParameters, maxbasepairdistance: int = 10000
     """(log Z, unpaired probabilities, pair probabilities) in one device pass."""
     dd = device_dataset(dataset, maxbasepairdistance)
     return dd.posterior(_stack(site_components(params)), _stack(pair_components(params)), prec)
+
+
+def getPosteriorDecodingConsensusStructure(pairedposteriorprobs, unpairedposteriorprobs, usecuda: bool = False,
+                                           alpha: float = 2.0) -> np.ndarray:
+    """MEA consensus pairing: 1-based partner per position, 0 when unpaired.
+
+    Both settings run on the device.  usecuda=True keeps the semantics of the
+    reference's CUDA branch (computeposteriordecodingcuda: FP32, the posterior
+    matrix read at i*L + k, alpha*p + E fused); usecuda=False those of its CPU
+    branch's iterativeposteriordecoding (FP64), called with the arguments in the
+    order of that function's signature (the reference's CPU branch passes them
+    swapped, src/RNATools.jl:83 versus :6, and fails with a MethodError)."""
+    return lib.mea(unpairedposteriorprobs, pairedposteriorprobs, alpha, lib.FP32 if usecuda else lib.FP64)
+
+
+def computeposteriordecodingcuda(singleprobsin, pairprobsin, alpha: float = 2.0):
+    """(E, S) tables of the FP32 decoding, L x L (S: -1, -2 or the 1-based split)."""
+    _, E, S = lib.mea(singleprobsin, pairprobsin, alpha, lib.FP32, tables=True)
+    return E, S
+
+
+def iterativeposteriordecoding(singleprobs, pairprobs, alpha: float = 2.0):
+    """(E, S) tables of the FP64 decoding, computed on the device."""
+    _, E, S = lib.mea(singleprobs, pairprobs, alpha, lib.FP64, tables=True)
+    return E, S
+
+
+def consensus(dataset, params: ModelParameters, maxbasepairdistance: int = 1000000, prec=lib.FP64,
+              usecuda: bool = True, alpha: float = 2.0):
+    """(log Z, MEA consensus pairing) with the posteriors kept on the device;
+    usecuda selects the decoding semantics as in getPosteriorDecodingConsensusStructure."""
+    dd = device_dataset(dataset, maxbasepairdistance)
+    return dd.posterior_consensus(_stack(site_components(params)), _stack(pair_components(params)), prec,
+                                  lib.FP32 if usecuda else lib.FP64, alpha)

@@ -51,6 +51,10 @@ SIGNATURES = {
     "messi_posterior": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                                        ctypes.c_int, _dp, _dp, _dp]),
     "messi_last_timings": (ctypes.c_int, [_vp, _dp]),
+    "messi_mea": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp, _ip]),
+    "messi_mea_timings": (ctypes.c_int, [_vp, _dp]),
+    "messi_posterior_consensus": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip]),
     "messi_set_tile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "messi_batch_destroy": (None, [_vp]),
@@ -279,6 +283,24 @@ class DeviceDataset:
                                                     ptr(unp), ptr(pp)))
         return z.value, unp, pp
 
+    def posterior_consensus(self, site, pair, prec=FP64, mea_prec=FP64, alpha: float = 2.0):
+        """log Z and the MEA consensus pairing (1-based partners, 0 unpaired), with
+        the posteriors kept on the device."""
+        P4, pi4, lw4 = site
+        P16, pi16, lw16 = pair
+        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
+        K4, K16 = P4c.shape[0], P16c.shape[0]
+        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
+        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
+        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
+        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        z = ctypes.c_double()
+        paired = np.empty(self.L, dtype=np.intc)
+        self.ctx.check(self.ctx.lib.messi_posterior_consensus(
+            self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c), ptr(pi16), ptr(lw16), prec_code(prec),
+            prec_code(mea_prec), float(alpha), ctypes.byref(z), paired.ctypes.data_as(_ip)))
+        return z.value, paired.astype(np.int64)
+
     def close(self) -> None:
         if self._h:
             self.ctx.lib.messi_dataset_destroy(self._h)
@@ -328,6 +350,35 @@ def bpp(inside_t, outside_t, u, paired, prec=FP64, ctx: Optional[Context] = None
     ctx.check(ctx.lib.messi_bpp(ctx.handle, L, ptr(it), ptr(ot), ptr(u), ptr(pm), prec_code(prec), ptr(unp),
                                 ptr(pp)))
     return unp, pp
+
+
+def mea(unpaired, pairprob, alpha: float = 2.0, prec=FP64, tables: bool = False,
+        ctx: Optional[Context] = None):
+    """MEA consensus pairing (1-based partners, 0 unpaired) from base-pair
+    posteriors; with tables, also the E (float64) and S (int) tables, L x L."""
+    ctx = ctx or context()
+    u = np.ascontiguousarray(unpaired, dtype=np.float64)
+    L = len(u)
+    pm = np.asfortranarray(pairprob, dtype=np.float64)
+    if pm.shape != (L, L):
+        raise ValueError(f"pairprob must be {L} x {L}")
+    paired = np.empty(L, dtype=np.intc)
+    E = np.empty((L, L), order="F") if tables else None
+    S = np.empty((L, L), dtype=np.intc, order="F") if tables else None
+    ctx.check(ctx.lib.messi_mea(ctx.handle, L, ptr(u), ptr(pm), float(alpha), prec_code(prec),
+                                paired.ctypes.data_as(_ip), ptr(E),
+                                S.ctypes.data_as(_ip) if tables else None))
+    if tables:
+        return paired.astype(np.int64), E, S.astype(np.int64)
+    return paired.astype(np.int64)
+
+
+def mea_timings(ctx: Optional[Context] = None) -> np.ndarray:
+    """[upload, fill, traceback] milliseconds of the last MEA call (profiling on)."""
+    ctx = ctx or context()
+    t = np.zeros(3)
+    ctx.check(ctx.lib.messi_mea_timings(ctx.handle, ptr(t)))
+    return t
 
 
 class Batch:

@@ -29,7 +29,7 @@ _u8 = ctypes.POINTER(ctypes.c_uint8)
 
 def build() -> str:
     os.makedirs(os.path.dirname(_LIB), exist_ok=True)
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fno-fast-math", "-o", _LIB, _SRC, "-lm"]
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fno-fast-math", "-ffp-contract=off", "-o", _LIB, _SRC, "-lm"]
     subprocess.run(cmd, check=True)
     return _LIB
 
@@ -47,6 +47,8 @@ def lib():
         L.oracle_inside.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, _dp]
         L.oracle_outside.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
         L.oracle_bpp.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_mea.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _ip]
+        L.oracle_mea_traceback.argtypes = [ctypes.c_int, _ip, _ip]
         _lib = L
     return _lib
 
@@ -134,3 +136,25 @@ def bpp(inside_t, outside_t, u, paired):
     pr = np.empty(L * L)
     lib().oracle_bpp(L, _p(it), _p(ot), _p(u), _p(pm), _p(res), _p(pr))
     return res, pr.reshape(L, L, order="F")
+
+
+def mea(single, pair, alpha: float = 2.0, prec: str = "fp64"):
+    """MEA fill: (E, S) as L x L arrays (E float64, S int: -1, -2 or the 1-based
+    split point; 0 on and below the diagonal).  ``pair`` is the L x L posterior
+    matrix as computebasepairprobs returns it (Julia indexing pair[i, k])."""
+    single = np.ascontiguousarray(single, dtype=np.float64)
+    L = len(single)
+    pm = np.asfortranarray(pair, dtype=np.float64).reshape(-1, order="F")
+    E = np.empty(L * L)
+    S = np.empty(L * L, dtype=np.intc)
+    lib().oracle_mea(L, _p(single), _p(pm), float(alpha), 0 if prec == "fp64" else 1, _p(E), _p(S, _ip))
+    return E.reshape(L, L, order="F"), S.reshape(L, L, order="F").astype(np.int64)
+
+
+def mea_traceback(S) -> np.ndarray:
+    """Pairing from an S table: 1-based partner per position, 0 when unpaired."""
+    L = S.shape[0]
+    s = np.asfortranarray(S, dtype=np.intc).reshape(-1, order="F")
+    out = np.empty(L, dtype=np.intc)
+    lib().oracle_mea_traceback(L, _p(s, _ip), _p(out, _ip))
+    return out.astype(np.int64)

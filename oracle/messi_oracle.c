@@ -16,6 +16,9 @@
  *   oracle_inside        -- computeinsideparallel/computeblock (src/InsideOutsideAlgorithm.jl:648-749)
  *   oracle_outside       -- computeoutside (src/InsideOutsideAlgorithm.jl:447-483)
  *   oracle_bpp           -- computebasepairprobs (src/InsideOutsideAlgorithm.jl:486-533)
+ *   oracle_mea           -- iterativeposteriordecoding (src/RNATools.jl:6-42), FP64; the kernel
+ *                           posteriordecoding (src/cuda/inside.cu:286-321), FP32
+ *   oracle_mea_traceback -- posteriordecodingtraceback (src/RNATools.jl:44-74)
  *   lse                  -- CommonUtils.logsumexp (src/CommonUtils.jl:145-155)
  *
  * Conventions: trees are flat pre-order arrays with the root at index 0,
@@ -251,4 +254,139 @@ void oracle_bpp(int L, const double *in, const double *out, const double *u, con
     res[x] /= norm;
     for (int y = 0; y < L; ++y) pair[x + (size_t)L * y] /= norm;
   }
+}
+
+/* Maximum-expected-accuracy (posterior) decoding, fill step.
+ *
+ * prec 0 (FP64) follows iterativeposteriordecoding (src/RNATools.jl:6-42): the
+ * pair weight of (i, k) is pairprobs[i, k] (column-major element i + L*k), and
+ * v = (alpha * p + E[i+1,k-1]) + E[k+1,j] with every operation rounded.
+ * prec 1 (FP32) follows the kernel posteriordecoding (src/cuda/inside.cu:286-321)
+ * as driven by computeposteriordecodingcuda (src/InsideCUDA.jl:183-215): inputs
+ * converted to float, the device reads element i*L + k of the column-major matrix
+ * (pairprobs[k, i]), and nvcc's default contraction fuses alpha * p + E into one
+ * fused multiply-add, then adds E[k+1,j].
+ * Both: E[i,i] = 0 (the diagonal is never set), e1 = single[i] + E[i+1,j],
+ * e2 = the k = j pair term with E[i+1,j-1], e3 = the first maximal k in i < k < j
+ * (strict >), E = e1, replaced by e2 and then e3 only when strictly greater
+ * (the Julia loop's findmax agrees for inputs without NaN or negative zero; the
+ * CUDA kernel starts its maximum at -1e10 rather than -inf, which only matters
+ * for terms below -1e10, never for probabilities).
+ * S[i,j] (column-major int) = -1 (e1), -2 (e2) or the 1-based k of e3.
+ * E (L x L, column-major) receives the table as double. */
+static double mea_pair_term(int prec, double alpha, const double *pair, int L, int i, int k, double e) {
+  if (prec == 0) {
+    double ap = alpha * pair[i + (size_t)L * k];
+    return ap + e;
+  }
+  float p = (float)pair[k + (size_t)L * i];
+  return (double)fmaf((float)alpha, p, (float)e);
+}
+
+void oracle_mea(int L, const double *single, const double *pair, double alpha, int prec, double *E, int *S) {
+  for (size_t q = 0; q < (size_t)L * L; ++q) {
+    E[q] = 0.0;
+    S[q] = 0;
+  }
+#define EM(r, c) ((r) > (c) ? 0.0 : E[(r) + (size_t)L * (c)])
+  for (int d = 1; d < L; ++d) {
+    for (int i = 0; i + d < L; ++i) {
+      int j = i + d;
+      double val, best = -INFINITY;
+      int bestk = -1, s;
+      if (prec == 0) {
+        double e1 = single[i] + EM(i + 1, j);
+        double e2 = mea_pair_term(0, alpha, pair, L, i, j, EM(i + 1, j - 1));
+        for (int k = i + 1; k <= j - 1; ++k) {
+          double v = mea_pair_term(0, alpha, pair, L, i, k, EM(i + 1, k - 1)) + EM(k + 1, j);
+          if (v > best) {
+            best = v;
+            bestk = k;
+          }
+        }
+        val = e1;
+        s = -1;
+        if (e2 > val) {
+          val = e2;
+          s = -2;
+        }
+        if (best > val) {
+          val = best;
+          s = bestk + 1;
+        }
+      } else {
+        float e1 = (float)single[i] + (float)EM(i + 1, j);
+        float e2 = (float)mea_pair_term(1, alpha, pair, L, i, j, EM(i + 1, j - 1));
+        float bf = -INFINITY;
+        for (int k = i + 1; k <= j - 1; ++k) {
+          float v = (float)mea_pair_term(1, alpha, pair, L, i, k, EM(i + 1, k - 1)) + (float)EM(k + 1, j);
+          if (v > bf) {
+            bf = v;
+            bestk = k;
+          }
+        }
+        float vf = e1;
+        s = -1;
+        if (e2 > vf) {
+          vf = e2;
+          s = -2;
+        }
+        if (bf > vf) {
+          vf = bf;
+          s = bestk + 1;
+        }
+        val = vf;
+      }
+      E[i + (size_t)L * j] = val;
+      S[i + (size_t)L * j] = s;
+    }
+  }
+#undef EM
+}
+
+/* Traceback of an S table into a pairing, posteriordecodingtraceback
+ * (src/RNATools.jl:44-74): paired[i] = 1-based partner, 0 when unpaired. */
+void oracle_mea_traceback(int L, const int *S, int *paired) {
+  for (int i = 0; i < L; ++i) paired[i] = 0;
+  if (L < 1) return;
+  int *st = (int *)malloc(sizeof(int) * 2 * ((size_t)L + 1));
+  int top = 0;
+  st[0] = 0;
+  st[1] = L - 1;
+  top = 1;
+  while (top > 0) {
+    --top;
+    int i = st[2 * top], j = st[2 * top + 1];
+    int k = S[i + (size_t)L * j];
+    if (k == -1) {
+      st[2 * top] = i + 1;
+      st[2 * top + 1] = j;
+      ++top;
+    } else if (k == -2) {
+      paired[i] = j + 1;
+      paired[j] = i + 1;
+      if (i + 1 < j - 1) {
+        st[2 * top] = i + 1;
+        st[2 * top + 1] = j - 1;
+        ++top;
+      }
+    } else if (k > 0) {
+      int kk = k - 1;
+      if (i < kk) {
+        paired[i] = kk + 1;
+        paired[kk] = i + 1;
+      }
+      if (i + 1 <= kk - 1) {
+        st[2 * top] = i + 1;
+        st[2 * top + 1] = kk - 1;
+        ++top;
+      }
+      if (kk + 1 <= j) {
+        st[2 * top] = kk + 1;
+        st[2 * top + 1] = j;
+        ++top;
+      }
+    }
+  }
+  free(st);
 }
