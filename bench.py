@@ -16,6 +16,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu
        python bench.py --instances      # 64 uneven instances: one batch vs 64 calls
        python bench.py --mcmc           # learned MCMC proposal vs random walk (ESS/s)
        python bench.py --genome         # 30 x 9000 nt posterior, full span and D=500
+       python bench.py --mea            # MEA consensus decoding at L = 1500 and 9000
 """
 from __future__ import annotations
 
@@ -229,6 +230,92 @@ def main_genome(args):
     print(json.dumps(out), flush=True)
 
 
+def _structure_mixture(L, n, rng):
+    """Posteriors (getpairprobs, src/RNATools.jl:87-102) of n samples of one random
+    nested structure (35 % of sites paired, helices of 4-8 bp), each sample keeping
+    every pair with probability 0.75."""
+    from messi_amd import synth
+    partner = synth._structure(L, 0.35, 4, 8, rng)
+    single = np.zeros(L)
+    pair = np.zeros((L, L), order="F")
+    opens = np.flatnonzero(partner > np.arange(L))
+    for _ in range(n):
+        keep = opens[rng.random(len(opens)) < 0.75]
+        pair[keep, partner[keep]] += 1.0
+        pair[partner[keep], keep] += 1.0
+        u = np.ones(L)
+        u[keep] = 0.0
+        u[partner[keep]] = 0.0
+        single += u
+    return single / n, pair / n
+
+
+def main_mea(args):
+    """MEA consensus decoding (fill + traceback) of L x L posteriors at L = 1500
+    and 9000, FP64 (the Julia loop's semantics) and FP32 (the CUDA kernel's), with
+    the posteriors resident in HBM (device event times) and with the host upload."""
+    from messi_amd import lib
+    ctx = lib.context()
+    rng = np.random.default_rng(args.seed)
+    runs = {}
+    for L in args.mea_lengths:
+        single, pair = _structure_mixture(L, 8, rng)
+        terms = L * (L - 1) * (L - 2) // 6     # (i, k, j) with i < k < j
+        r = {}
+        paired = {}
+        for prec in ("fp64", "fp32"):
+            ctx.set_profiling(False)
+            for _ in range(args.warmup):
+                lib.mea(single, pair, 2.0, prec)
+            ctx.set_profiling(True)
+            fill, tb, wall = [], [], []
+            for _ in range(args.steps):
+                t0 = time.perf_counter()
+                paired[prec] = lib.mea(single, pair, 2.0, prec)
+                wall.append(time.perf_counter() - t0)
+                t = lib.mea_timings()
+                fill.append(t[1])
+                tb.append(t[2])
+            ctx.set_profiling(False)
+            f, b, w = float(np.mean(fill)), float(np.mean(tb)), float(np.mean(wall)) * 1e3
+            peak = PEAK[prec]
+            ach = 2 * terms / (f * 1e-3) / 1e12
+            r[prec] = {"fill_ms": f, "traceback_ms": b, "decodings_per_s": 1e3 / (f + b),
+                       "decodings_per_s_host_inputs": 1e3 / w, "wall_ms_host_inputs": w,
+                       "fill_tflops_equiv": ach, "fill_frac_of_vector_peak": ach / peak,
+                       "pairs": int(np.count_nonzero(paired[prec]) // 2)}
+            print(f"[mea] L={L} {prec}: fill {f:.3f} ms, traceback {b:.3f} ms, host-input wall {w:.2f} ms",
+                  file=sys.stderr, flush=True)
+        r["fp32_fp64_same_pairing"] = bool(np.array_equal(paired["fp32"], paired["fp64"]))
+        runs[str(L)] = r
+    L0 = str(args.mea_lengths[0])
+    lead = runs[L0]["fp32"]
+    out = {"metric": f"MEA consensus decodings/s at L={L0} (fill + traceback, posteriors resident)",
+           "value": lead["decodings_per_s"], "unit": "decodings/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": lead["fill_ms"] + lead["traceback_ms"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic posteriors: 8 samples of one random nested structure, each pair kept with p=0.75",
+           "config": {"workload": f"MEA decoding, alpha 2, L in {args.mea_lengths}"},
+           "roofline": {"bound": "vector", "achieved": lead["fill_tflops_equiv"], "peak": PEAK["fp32"],
+                        "unit": "TFLOP/s", "frac": lead["fill_frac_of_vector_peak"], "traffic": None,
+                        "kernel": "mea_step_kernel<float,16>: 2 ops (add, max) per (i, k, j) term"},
+           "runs": runs}
+    if not args.no_cpu:
+        import oracle
+        L = int(L0)
+        single, pair = _structure_mixture(L, 8, np.random.default_rng(args.seed))
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < 10.0 or n == 0:
+            _, S = oracle.mea(single, pair, 2.0, "fp64")
+            oracle.mea_traceback(S)
+            n += 1
+        dt = (time.perf_counter() - t0) / n
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "decodings/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} full FP64 decodings at L={L} (oracle fill + traceback)"}
+    print(json.dumps(out), flush=True)
+
+
 def main_mcmc(args):
     """Learned proposal vs adaptive random walk on the 50 x 1500 alignment:
     16 chains, adaptive random-walk burn-in, network trained on the device on the
@@ -280,6 +367,8 @@ def main():
     ap.add_argument("--instances", action="store_true")
     ap.add_argument("--mcmc", action="store_true")
     ap.add_argument("--genome", action="store_true")
+    ap.add_argument("--mea", action="store_true")
+    ap.add_argument("--mea-lengths", type=int, nargs="+", default=[1500, 9000])
     ap.add_argument("--tile", type=int, default=0, help="inside tile edge 16 or 32 (default: library's)")
     ap.add_argument("--precs", nargs="+", default=["fp64", "fp32"], help="precisions to time (profiling runs)")
     ap.add_argument("--no-posterior", action="store_true", help="skip the posterior error check (profiling runs)")
@@ -305,6 +394,8 @@ def main():
         return main_mcmc(args)
     if args.genome:
         return main_genome(args)
+    if args.mea:
+        return main_mea(args)
 
     from messi_amd import lib, synth
     from messi_amd.hotpath import device_dataset

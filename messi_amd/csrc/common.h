@@ -110,7 +110,7 @@ struct InsideWork {
 
 // Tables of MEA decoding (L x L, cells i < j; E and U column-major, S and the
 // split-point accumulators row-major).
-constexpr int MEA_MAX_L = 16383;  // the traceback stack (4 bytes per entry) fits 64 KiB of LDS
+constexpr int MEA_MAX_L = 16383;  // the traceback stack (L/2 + 1 words) and pairing (L halfwords) fit 64 KiB of LDS
 struct MeaWork {
   int L = 0;
   bool fp32 = false;

@@ -273,46 +273,63 @@ __global__ __launch_bounds__(T * T) void mea_step_kernel(MeaDesc<V> d, int t) {
   }
 }
 
-// Traceback by one wavefront.  The stack of pending intervals (packed i | j << 16)
-// is in LDS; the walk reads S through an 8 x 8 window (rows i..i+7, columns
-// j-7..j) held one entry per lane, so runs of unpaired positions and stacked
-// pairs cost one global load per eight steps.  Every lane runs the same control
-// flow; lane 0 writes the pairs.
+// Traceback by one wavefront.  The stack of pending intervals (packed i | j << 16;
+// they are disjoint and at least two long, so at most L/2 are pending) and the
+// pairing (16-bit partners) are in LDS; the pairing goes to global memory at the
+// end, so no global store is outstanding while the walk waits on its loads.  S
+// is read through a 16 x 16 window (rows i..i+15, columns j-15..j) held four
+// entries per lane, so runs of unpaired positions and stacked pairs cost one
+// round of global loads per sixteen steps; a step reads its entry with
+// v_readlane into a scalar register.  Every lane runs the same control flow.
 __global__ __launch_bounds__(64) void mea_traceback_kernel(int L, const int *__restrict__ S, int *__restrict__ paired) {
-  extern __shared__ unsigned stk[];
+  extern __shared__ unsigned smem[];
+  constexpr int W = 16;
   const int lane = threadIdx.x;
+  unsigned *stk = smem;                                         // L/2 + 1 entries
+  unsigned short *pr = (unsigned short *)(smem + (L / 2 + 1));  // L entries
+  for (int x = lane; x < L; x += 64) pr[x] = 0;
+  __syncthreads();
   int top = 0;
   if (L >= 2) {
     stk[0] = 0u | ((unsigned)(L - 1) << 16);
     top = 1;
   }
-  int wi = -1000, wj = -1000, wv = 0;
+  int wi = -1000, wj = -1000;
+  int w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // window entry lane + 64 q: row (lane + 64 q) / W
   while (top > 0) {
     --top;
-    const unsigned e = stk[top];
+    // every lane holds the same interval: keep the walk's state in scalar registers
+    const unsigned e = (unsigned)__builtin_amdgcn_readfirstlane((int)stk[top]);
     int i = (int)(e & 0xffffu), j = (int)(e >> 16);
     while (i < j) {
-      if (!(i >= wi && i <= wi + 7 && j <= wj && j >= wj - 7)) {
+      if (!(i >= wi && i <= wi + W - 1 && j <= wj && j >= wj - (W - 1))) {
         wi = i;
         wj = j;
-        const int r = wi + (lane >> 3), c = wj - 7 + (lane & 7);
-        wv = (r >= 0 && r < c && c < L) ? S[(size_t)r * L + c] : 0;
+        const int c = wj - (W - 1) + (lane % W);
+        const int r0 = wi + lane / W;
+        auto at = [&](int r) { return (r < c && c < L) ? S[(size_t)r * L + c] : 0; };
+        w0 = at(r0);
+        w1 = at(r0 + 4);
+        w2 = at(r0 + 8);
+        w3 = at(r0 + 12);
       }
-      const int s = __shfl(wv, (i - wi) * 8 + (j - (wj - 7)));
+      const int idx = (i - wi) * W + (j - (wj - (W - 1)));
+      const int q = idx >> 6;
+      const int s = __builtin_amdgcn_readlane(q == 0 ? w0 : q == 1 ? w1 : q == 2 ? w2 : w3, idx & 63);
       if (s == -1) {
         i += 1;
       } else if (s == -2) {
         if (lane == 0) {
-          paired[i] = j + 1;
-          paired[j] = i + 1;
+          pr[i] = (unsigned short)(j + 1);
+          pr[j] = (unsigned short)(i + 1);
         }
         i += 1;
         j -= 1;
       } else if (s > 0) {
         const int k = s - 1;
         if (lane == 0) {
-          paired[i] = k + 1;
-          paired[k] = i + 1;
+          pr[i] = (unsigned short)(k + 1);
+          pr[k] = (unsigned short)(i + 1);
         }
         if (k + 1 < j) stk[top++] = (unsigned)(k + 1) | ((unsigned)j << 16);
         i += 1;
@@ -322,6 +339,8 @@ __global__ __launch_bounds__(64) void mea_traceback_kernel(int L, const int *__r
       }
     }
   }
+  __syncthreads();
+  for (int x = lane; x < L; x += 64) paired[x] = pr[x];
 }
 
 template <typename V>
@@ -381,9 +400,8 @@ void mea_run(hipStream_t s, MeaWork &w, int L, const double *d_single, const dou
 
 void mea_run_traceback(hipStream_t s, MeaWork &w, int L, int *d_paired) {
   MESSI_REQUIRE(w.L == L && L >= 1 && L <= MEA_MAX_L, "no MEA table of this length");
-  MESSI_HIP(hipMemsetAsync(d_paired, 0, sizeof(int) * L, s));
-  hipLaunchKernelGGL(mea_traceback_kernel, dim3(1), dim3(64), sizeof(unsigned) * (L + 1), s, L,
-                     (const int *)w.S.as<int>(), d_paired);
+  const size_t lds = sizeof(unsigned) * (L / 2 + 1) + sizeof(unsigned short) * L;  // <= 64 KiB for L <= 16383
+  hipLaunchKernelGGL(mea_traceback_kernel, dim3(1), dim3(64), lds, s, L, (const int *)w.S.as<int>(), d_paired);
   MESSI_HIP(hipGetLastError());
 }
 
