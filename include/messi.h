@@ -137,6 +137,25 @@ int messi_posterior_consensus(messi_dataset *ds, int K4, const double *P4, const
                               const double *logw16, int prec, int mea_prec, double alpha, double *logZ,
                               int *paired);
 
+/* Stochastic tracebacks: nsamples KH99 structures drawn from an inside table, as
+ * samplestructure (src/InsideOutsideAlgorithm.jl:623-627) draws one from (S, 1, L).
+ * in3 (3 x L x L), paired (L x L) and unpaired (L) as for messi_outside; prec
+ * selects the table the draws read (FP64 log values or the scaled FP32 copy).
+ * Interval (symbol, x, y) of sample k uses the uniform keyed by (seed, k, symbol,
+ * x, y) (SplitMix64 finaliser), not the reference's MersenneTwister stream.
+ * structures (nsamples x L, sample-major) receives 1-based partners, 0 when
+ * unpaired; logprior (nsamples, may be NULL) the KH99 log prior of each structure
+ * (calculateKH99prior, src/InsideOutsideAlgorithm.jl:86-144).  1 <= L <= 16383. */
+int messi_sample_structures(messi_ctx *ctx, int L, const double *in3, const double *paired, const double *unpaired,
+                            int prec, int nsamples, uint64_t seed, double B, int *structures, double *logprior);
+
+/* Evaluation and sampling without the inside table leaving the device (the MCMC
+ * step's structure draw, src/MCMC.jl:1736-1746, for many samples at once). */
+int messi_sample_posterior(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
+                           int K16, const double *P16, const double *pi16, const double *logw16, int prec,
+                           int nsamples, uint64_t seed, double B, double *logZ, int *structures,
+                           double *logprior);
+
 /* Batch of members, each a dataset (the same dataset may appear several times,
  * e.g. one member per MCMC chain or proposal).  messi_batch_eval evaluates all
  * members together: each tree level and each tile-diagonal of the inside

@@ -1208,6 +1208,79 @@ int messi_posterior_consensus(messi_dataset *ds, int K4, const double *P4, const
   });
 }
 
+int messi_sample_structures(messi_ctx *ctx, int L, const double *in3, const double *paired, const double *unpaired,
+                            int prec, int nsamples, uint64_t seed, double B, int *structures, double *logprior) {
+  if (!ctx) return 2;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(L >= 1 && L <= SAMPLE_MAX_L, "structure sampling supports 1 <= L <= 16383");
+    MESSI_REQUIRE(in3 && paired && unpaired && structures && nsamples >= 0, "bad argument");
+    MESSI_REQUIRE(std::isfinite(B), "B must be finite");
+    check_prec(prec);
+    MESSI_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    InsideWork &w = work(ctx, prec);
+    const size_t LL = (size_t)L * L;
+    SampleWork &sw = ctx->samp;
+    sw.out.ensure(sizeof(int) * (size_t)nsamples * L);
+    sw.prior.ensure(sizeof(double) * (size_t)nsamples);
+    auto go = [&](auto tag) {
+      using T = decltype(tag);
+      inside_alloc<T>(w, L, false);
+      w.out3.ensure(sizeof(double) * 3 * LL);
+      MESSI_HIP(hipMemcpyAsync(w.u.p, unpaired, sizeof(double) * L, hipMemcpyHostToDevice, s));
+      MESSI_HIP(hipMemcpyAsync(w.paired.p, paired, sizeof(double) * LL, hipMemcpyHostToDevice, s));
+      inside_prepare<T>(s, w, L, 1.0);
+      MESSI_HIP(hipMemcpyAsync(w.out3.p, in3, sizeof(double) * 3 * LL, hipMemcpyHostToDevice, s));
+      inside_import<T>(s, w, L, w.out3.as<double>());
+      sample_run<T>(s, w, L, nsamples, seed, 0, B, sw.out.as<int>(), sw.prior.as<double>());
+    };
+    if (prec == MESSI_FP32) go(float{});
+    else go(double{});
+    if (nsamples > 0) {
+      MESSI_HIP(hipMemcpyAsync(structures, sw.out.p, sizeof(int) * (size_t)nsamples * L, hipMemcpyDeviceToHost, s));
+      if (logprior)
+        MESSI_HIP(hipMemcpyAsync(logprior, sw.prior.p, sizeof(double) * nsamples, hipMemcpyDeviceToHost, s));
+    }
+    MESSI_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int messi_sample_posterior(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
+                           int K16, const double *P16, const double *pi16, const double *logw16, int prec,
+                           int nsamples, uint64_t seed, double B, double *logZ, int *structures, double *logprior) {
+  if (!ds) return 2;
+  messi_ctx *ctx = ds->ctx;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(P4 && pi4 && logw4 && P16 && pi16 && logw16 && logZ && structures && nsamples >= 0,
+                  "bad argument");
+    MESSI_REQUIRE(std::isfinite(B), "B must be finite");
+    check_prec(prec);
+    const int L = ds->L;
+    MESSI_REQUIRE(L <= SAMPLE_MAX_L, "structure sampling supports L <= 16383");
+    MESSI_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    messi_batch *b = self_batch(ds);
+    SampleWork &sw = ctx->samp;
+    sw.out.ensure(sizeof(int) * (size_t)nsamples * L);
+    sw.prior.ensure(sizeof(double) * (size_t)nsamples);
+    auto go = [&](auto tag) {
+      using T = decltype(tag);
+      InsideWork &w = member_work(b, 0, sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64);
+      load_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+      eval_impl<T>(b, 1.0, true, logZ);
+      sample_run<T>(s, w, L, nsamples, seed, 0, B, sw.out.as<int>(), sw.prior.as<double>());
+    };
+    if (prec == MESSI_FP32) go(float{});
+    else go(double{});
+    if (nsamples > 0) {
+      MESSI_HIP(hipMemcpyAsync(structures, sw.out.p, sizeof(int) * (size_t)nsamples * L, hipMemcpyDeviceToHost, s));
+      if (logprior)
+        MESSI_HIP(hipMemcpyAsync(logprior, sw.prior.p, sizeof(double) * nsamples, hipMemcpyDeviceToHost, s));
+    }
+    MESSI_HIP(hipStreamSynchronize(s));
+  });
+}
+
 int messi_last_timings(const messi_dataset *ds, double *t) {
   if (!ds || !t) return 2;
   for (int i = 0; i < 5; ++i) t[i] = ds->self ? ds->self->t[i] : 0.0;

@@ -118,6 +118,11 @@ struct MeaWork {
   DevBuf single, pair, paired;  // staging of host inputs and the result
 };
 
+// Outputs of structure sampling.
+struct SampleWork {
+  DevBuf out, prior;
+};
+
 }  // namespace messi
 
 struct messi_ctx {
@@ -128,4 +133,5 @@ struct messi_ctx {
   int tile = 16;  // tile edge of the inside schedule
   messi::InsideWork w64, w32;
   messi::MeaWork mea;
+  messi::SampleWork samp;
 };

@@ -896,9 +896,6 @@ __global__ void bpp_write_kernel(int L, const double *lp, const double *res, con
 // ---------------------------------------------------------------------------
 // Host side (T = double -> LogSR, T = float -> XF)
 // ---------------------------------------------------------------------------
-template <typename T> struct SRof;
-template <> struct SRof<double> { typedef LogSR type; };
-template <> struct SRof<float> { typedef XF type; };
 
 template <typename T>
 void inside_alloc(InsideWork &w, int L, bool outside) {

@@ -1,0 +1,203 @@
+// Stochastic traceback of KH99 structures from an inside table:
+// samplestructure / samplestructurehelper (src/InsideOutsideAlgorithm.jl:570-629),
+// one wavefront per sample, many samples per launch.
+//
+// From (S, 0, L-1), an interval (symbol, x, y) with x < y chooses among the
+// bifurcations h = x..y-1 of its type-3 rule (S->LS or F->LS: weight
+// p L[x,h] S[h+1,y]) and, last, the pairing of its type-2 rule (p paired[x,y]
+// F[x+1,y-1]), with probability proportional to weight^B; x == y is a terminal.
+// The 64 lanes score 64 consecutive split points at a time from one row of L
+// (row-major) and one column of S (column-major copy): a pass for the maximum,
+// a pass for the total, and a chunked prefix-sum search for the first
+// cumulative weight >= u * total.  The uniform u of an interval is keyed by
+// (seed, sample, symbol, x, y) with the SplitMix64 finaliser (Steele, Lea and
+// Flood 2014), so the draw does not depend on the visiting order.  In the
+// scaled FP32 tables every candidate of an interval carries the same baseline
+// factor exp(-sum of lambda over [x,y]), which cancels.
+//
+// The KH99 log prior of the sampled structure (calculateKH99prior,
+// src/InsideOutsideAlgorithm.jl:86-144) is accumulated along the sampled
+// derivation: the log probability of each rule used.
+#include "prune.h"
+#include "sr.h"
+
+namespace messi {
+
+namespace {
+
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__device__ inline double keyed_uniform(uint64_t seed, int sample, int sym, int x, int y) {
+  const uint64_t key = mix64(seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(sample + 1));
+  const uint64_t z = mix64(key ^ (((uint64_t)(uint32_t)x << 32) | (uint64_t)(uint32_t)y) ^ ((uint64_t)sym << 62));
+  return (double)((z >> 11) + 1) * 0x1.0p-53;  // (0, 1]
+}
+
+template <class SR> struct SampleArgs {
+  typedef typename SR::V V;
+  int L, n, first;
+  uint64_t seed;
+  double B;
+  const V *Lr, *Sc, *F, *ptv;
+  double lp[7];   // KH99 natural-log rule probabilities
+  int *out;       // n x L, 1-based partners, 0 unpaired
+  double *prior;  // n
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double wave_incl_scan(double v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const double t = __shfl_up(v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+constexpr unsigned SYM_S = 0, SYM_F = 1, SYM_L = 2;
+
+template <class SR>
+__global__ __launch_bounds__(64) void sample_kernel(SampleArgs<SR> a) {
+  extern __shared__ unsigned smem[];
+  const int L = a.L, lane = threadIdx.x;
+  const int sample = a.first + blockIdx.x;
+  unsigned *stk = smem;                                         // L/2 + 1 pending L intervals
+  unsigned short *pr = (unsigned short *)(smem + (L / 2 + 1));  // the pairing
+  for (int q = lane; q < L; q += 64) pr[q] = 0;
+  __syncthreads();
+  const double B = a.B;
+  double prior = 0.0;
+  int top = 0;
+  int sym = SYM_S, x = 0, y = L - 1;
+  bool have = L >= 1;
+  while (true) {
+    if (!have) {
+      if (top == 0) break;
+      --top;
+      const unsigned e = (unsigned)__builtin_amdgcn_readfirstlane((int)stk[top]);
+      sym = (int)(e >> 28);
+      x = (int)((e >> 14) & 0x3fffu);
+      y = (int)(e & 0x3fffu);
+    }
+    have = false;
+    if (x > y) continue;
+    if (x == y) {  // terminal: S -> s or L -> s (F has none)
+      prior += sym == SYM_S ? a.lp[1] : sym == SYM_L ? a.lp[5] : 0.0;
+      continue;
+    }
+    const double t3 = sym == SYM_S ? a.lp[0] : a.lp[4];
+    const double t2 = sym == SYM_S ? a.lp[2] : sym == SYM_F ? a.lp[3] : a.lp[6];
+    const int nb = sym == SYM_L ? 0 : y - x;  // bifurcation candidates h = x .. y-1
+    const typename SR::V *Lrow = a.Lr + (size_t)x * L + x;
+    const typename SR::V *Scol = a.Sc + (size_t)y * L + x + 1;
+    const double fin = x + 1 <= y - 1 ? SR::to_log(a.F[(size_t)(x + 1) * L + (y - 1)]) : -HUGE_VAL;
+    const double lpair = ((t2 + SR::to_log(a.ptv[(size_t)x * L + y])) + fin) * B;
+    // pass 1: the maximum
+    double m = lpair;
+    for (int base = 0; base < nb; base += 64) {
+      const int k = base + lane;
+      if (k < nb) m = fmax(m, ((t3 + SR::to_log(Lrow[k])) + SR::to_log(Scol[k])) * B);
+    }
+    m = wave_max(m);
+    int choice = nb;  // index of the chosen candidate (nb = the pairing)
+    if (m != -HUGE_VAL) {
+      // pass 2: the total
+      double part = 0.0;
+      for (int base = 0; base < nb; base += 64) {
+        const int k = base + lane;
+        if (k < nb) part += exp(((t3 + SR::to_log(Lrow[k])) + SR::to_log(Scol[k])) * B - m);
+      }
+      const double wpair = exp(lpair - m);
+      const double tot = wave_sum(part) + wpair;
+      const double target = keyed_uniform(a.seed, sample, sym, x, y) * tot;
+      // pass 3: the first cumulative weight >= target
+      double run = 0.0;
+      int lastpos = wpair > 0.0 ? nb : -1;
+      bool found = false;
+      for (int base = 0; base < nb; base += 64) {
+        const int k = base + lane;
+        const double w = k < nb ? exp(((t3 + SR::to_log(Lrow[k])) + SR::to_log(Scol[k])) * B - m) : 0.0;
+        const double incl = wave_incl_scan(w, lane);
+        const uint64_t pos = __ballot(w > 0.0);
+        if (pos && lastpos < nb) lastpos = base + 63 - __clzll(pos);
+        const uint64_t hit = __ballot(w > 0.0 && run + incl >= target);
+        if (hit) {
+          choice = base + __ffsll((unsigned long long)hit) - 1;
+          found = true;
+          break;
+        }
+        run += __shfl(incl, 63);
+      }
+      if (!found && !(wpair > 0.0 && run + wpair >= target)) choice = lastpos >= 0 ? lastpos : nb;
+    }
+    if (choice == nb) {  // pairing; then F over the interior
+      prior += t2;
+      if (lane == 0) {
+        pr[x] = (unsigned short)(y + 1);
+        pr[y] = (unsigned short)(x + 1);
+      }
+      sym = SYM_F;
+      x = x + 1;
+      y = y - 1;
+      have = true;
+    } else {             // bifurcation at h: L over [x, h], S over [h+1, y]
+      const int h = x + choice;
+      prior += t3;
+      if (h == x) prior += a.lp[5];  // L -> s
+      else stk[top++] = (SYM_L << 28) | ((unsigned)x << 14) | (unsigned)h;
+      sym = SYM_S;
+      x = h + 1;
+      have = true;
+    }
+  }
+  __syncthreads();
+  int *o = a.out + (size_t)blockIdx.x * L;
+  for (int q = lane; q < L; q += 64) o[q] = pr[q];
+  if (lane == 0) a.prior[blockIdx.x] = prior;
+}
+
+}  // namespace
+
+template <typename T>
+void sample_run(hipStream_t s, InsideWork &w, int L, int n, uint64_t seed, int first, double B, int *d_out,
+                double *d_prior) {
+  typedef typename SRof<T>::type SR;
+  typedef typename SR::V V;
+  MESSI_REQUIRE(L >= 1 && L <= SAMPLE_MAX_L, "structure sampling supports 1 <= L <= 16383");
+  MESSI_REQUIRE(n >= 0, "negative sample count");
+  if (n == 0) return;
+  SampleArgs<SR> a;
+  a.L = L;
+  a.n = n;
+  a.first = first;
+  a.seed = seed;
+  a.B = B;
+  a.Lr = w.Lr.as<V>();
+  a.Sc = w.Sc.as<V>();
+  a.F = w.F.as<V>();
+  a.ptv = w.ptv.as<V>();
+  kh99_logprobs(a.lp);
+  a.out = d_out;
+  a.prior = d_prior;
+  const size_t lds = sizeof(unsigned) * (L / 2 + 1) + sizeof(unsigned short) * L;  // <= 64 KiB for L <= 16383
+  hipLaunchKernelGGL(sample_kernel<SR>, dim3(n), dim3(64), lds, s, a);
+  MESSI_HIP(hipGetLastError());
+}
+
+template void sample_run<double>(hipStream_t, InsideWork &, int, int, uint64_t, int, double, int *, double *);
+template void sample_run<float>(hipStream_t, InsideWork &, int, int, uint64_t, int, double, int *, double *);
+
+}  // namespace messi

@@ -190,4 +190,9 @@ struct XF {
   }
 };
 
+// Representation of each storage precision: double -> LogSR, float -> XF.
+template <typename T> struct SRof;
+template <> struct SRof<double> { typedef LogSR type; };
+template <> struct SRof<float> { typedef XF type; };
+
 }  // namespace messi

@@ -15,6 +15,9 @@ Function here                    Reference
 ``computeposteriordecodingcuda``  src/InsideCUDA.jl:183-215
 ``iterativeposteriordecoding``   src/RNATools.jl:6-42
 ``consensus``                    src/MESSI.jl:1097-1115 (posteriors and decoding in one device pass)
+``samplestructure``              src/InsideOutsideAlgorithm.jl:623-627
+``samplestructures``             the same, many samples per device launch
+``calculateKH99prior``           src/InsideOutsideAlgorithm.jl:86-144 (host, for structures given by the caller)
 ===============================  ==========================================================
 
 ``prec`` selects the plain FP64 path (default) or the scaled FP32 path.
@@ -169,3 +172,58 @@ def consensus(dataset, params: ModelParameters, maxbasepairdistance: int = 10000
     dd = device_dataset(dataset, maxbasepairdistance)
     return dd.posterior_consensus(_stack(site_components(params)), _stack(pair_components(params)), prec,
                                   lib.FP32 if usecuda else lib.FP64, alpha)
+
+
+def samplestructures(inside, pairedlogprobs, unpairedlogprobs, nsamples: int, seed: int = 0, B: float = 1.0,
+                     prec=lib.FP64):
+    """nsamples KH99 structures from (S, 1, L) of an inside table: (structures
+    (nsamples x L, 1-based partners), KH99 log priors)."""
+    return lib.sample_structures(inside, pairedlogprobs, unpairedlogprobs, nsamples, seed, B, prec)
+
+
+def samplestructure(seed: int, inside, pairedlogprobs, unpairedlogprobs, x: int, y: int, paired,
+                    grammar: KH99 = None, B: float = 1.0, prec=lib.FP64) -> float:
+    """One stochastic traceback written into ``paired`` (in place, 1-based partners);
+    returns the structure's KH99 log prior.  The reference takes an RNG; here the
+    draw is keyed by ``seed``.  Only the whole sequence (x = 1, y = L) is supported,
+    as in every reference call."""
+    L = len(unpairedlogprobs)
+    if (x, y) != (1, L):
+        raise ValueError("samplestructure samples the whole sequence only (x = 1, y = L)")
+    S, pri = lib.sample_structures(inside, pairedlogprobs, unpairedlogprobs, 1, seed, B, prec)
+    paired[:] = S[0]
+    return float(pri[0])
+
+
+def calculateKH99prior(pairedsites) -> float:
+    """KH99 log prior of a structure (1-based partners, 0 unpaired): the log
+    probability of its unique derivation."""
+    import math
+    p = [int(v) for v in pairedsites]
+    L = len(p)
+    r = {("S", "sS"): math.log(0.868534 * 0.894603), ("S", "dFdS"): math.log(0.868534 * 0.105397),
+         ("S", "s"): math.log(0.117609877998), ("S", "dFd"): math.log(0.013856122002),
+         ("F", "dFd"): math.log(0.787640), ("F", "sS"): math.log(0.21236 * 0.894603),
+         ("F", "dFdS"): math.log(0.21236 * 0.105397)}
+    ll = 0.0
+    stack = [("S", 1, L)] if L >= 1 else []
+    while stack:
+        sym, s, e = stack.pop()
+        if s < 1 or s > L:
+            continue
+        k = p[s - 1]
+        if k == 0:
+            if s == e and sym == "S":
+                ll += r[("S", "s")]
+            elif s < e:
+                ll += r[(sym, "sS")]
+                stack.append(("S", s + 1, e))
+        elif k > s:
+            if k == e:
+                ll += r[(sym, "dFd")]
+                stack.append(("F", s + 1, k - 1))
+            else:
+                ll += r[(sym, "dFdS")]
+                stack.append(("F", s + 1, k - 1))
+                stack.append(("S", k + 1, e))
+    return ll

@@ -53,6 +53,11 @@ SIGNATURES = {
     "messi_last_timings": (ctypes.c_int, [_vp, _dp]),
     "messi_mea": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp, _ip]),
     "messi_mea_timings": (ctypes.c_int, [_vp, _dp]),
+    "messi_sample_structures": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_uint64, ctypes.c_double, _ip, _dp]),
+    "messi_sample_posterior": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
+                                              ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, _dp,
+                                              _ip, _dp]),
     "messi_posterior_consensus": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip]),
     "messi_set_tile": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -301,6 +306,26 @@ class DeviceDataset:
             prec_code(mea_prec), float(alpha), ctypes.byref(z), paired.ctypes.data_as(_ip)))
         return z.value, paired.astype(np.int64)
 
+    def sample_posterior(self, site, pair, nsamples: int, seed: int = 0, B: float = 1.0, prec=FP64):
+        """log Z, then nsamples structures (nsamples x L, 1-based partners) drawn from
+        the evaluation's inside table on the device, and their KH99 log priors."""
+        P4, pi4, lw4 = site
+        P16, pi16, lw16 = pair
+        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
+        K4, K16 = P4c.shape[0], P16c.shape[0]
+        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
+        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
+        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
+        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        z = ctypes.c_double()
+        out = np.empty((nsamples, self.L), dtype=np.intc)
+        pri = np.empty(nsamples)
+        self.ctx.check(self.ctx.lib.messi_sample_posterior(
+            self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c), ptr(pi16), ptr(lw16), prec_code(prec),
+            int(nsamples), int(seed) & 0xFFFFFFFFFFFFFFFF, float(B), ctypes.byref(z), out.ctypes.data_as(_ip),
+            ptr(pri)))
+        return z.value, out.astype(np.int64), pri
+
     def close(self) -> None:
         if self._h:
             self.ctx.lib.messi_dataset_destroy(self._h)
@@ -371,6 +396,25 @@ def mea(unpaired, pairprob, alpha: float = 2.0, prec=FP64, tables: bool = False,
     if tables:
         return paired.astype(np.int64), E, S.astype(np.int64)
     return paired.astype(np.int64)
+
+
+def sample_structures(inside_t, paired, unpaired, nsamples: int, seed: int = 0, B: float = 1.0, prec=FP64,
+                      ctx: Optional[Context] = None):
+    """nsamples structures (nsamples x L, 1-based partners, 0 unpaired) drawn from a
+    3 x L x L inside table, and their KH99 log priors."""
+    ctx = ctx or context()
+    u = np.ascontiguousarray(unpaired, dtype=np.float64)
+    L = len(u)
+    it = np.asfortranarray(inside_t, dtype=np.float64)
+    pm = np.asfortranarray(paired, dtype=np.float64)
+    if it.shape != (3, L, L) or pm.shape != (L, L):
+        raise ValueError("inside must be 3 x L x L and paired L x L")
+    out = np.empty((nsamples, L), dtype=np.intc)
+    pri = np.empty(nsamples)
+    ctx.check(ctx.lib.messi_sample_structures(ctx.handle, L, ptr(it), ptr(pm), ptr(u), prec_code(prec), int(nsamples),
+                                              int(seed) & 0xFFFFFFFFFFFFFFFF, float(B), out.ctypes.data_as(_ip),
+                                              ptr(pri)))
+    return out.astype(np.int64), pri
 
 
 def mea_timings(ctx: Optional[Context] = None) -> np.ndarray:

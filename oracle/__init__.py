@@ -49,6 +49,13 @@ def lib():
         L.oracle_bpp.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_mea.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _ip]
         L.oracle_mea_traceback.argtypes = [ctypes.c_int, _ip, _ip]
+        L.oracle_sample_uniform.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oracle_sample_uniform.restype = ctypes.c_double
+        L.oracle_sample_structure.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_uint64,
+                                              ctypes.c_int, _ip]
+        L.oracle_sample_structure.restype = ctypes.c_double
+        L.oracle_kh99_prior.argtypes = [ctypes.c_int, _ip]
+        L.oracle_kh99_prior.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -158,3 +165,29 @@ def mea_traceback(S) -> np.ndarray:
     out = np.empty(L, dtype=np.intc)
     lib().oracle_mea_traceback(L, _p(s, _ip), _p(out, _ip))
     return out.astype(np.int64)
+
+
+def sample_structures(inside_t, paired, nsamples: int, seed: int = 0, B: float = 1.0, first: int = 0):
+    """Stochastic tracebacks from (S, 1, L): (partners (nsamples x L, 1-based, 0
+    unpaired), KH99 log priors (nsamples)).  Sample s draws its uniforms from
+    (seed, first + s)."""
+    L = inside_t.shape[1]
+    it = np.asfortranarray(inside_t, dtype=np.float64).reshape(-1, order="F")
+    pm = np.asfortranarray(paired, dtype=np.float64).reshape(-1, order="F")
+    out = np.empty((nsamples, L), dtype=np.intc)
+    pri = np.empty(nsamples)
+    f = lib().oracle_sample_structure
+    for k in range(nsamples):
+        row = out[k]
+        pri[k] = f(L, _p(it), _p(pm), float(B), int(seed), first + k, row.ctypes.data_as(_ip))
+    return out.astype(np.int64), pri
+
+
+def kh99_prior(partner) -> float:
+    """calculateKH99prior of a structure given as 1-based partners (0 unpaired)."""
+    p = np.ascontiguousarray(partner, dtype=np.intc)
+    return float(lib().oracle_kh99_prior(len(p), _p(p, _ip)))
+
+
+def sample_uniform(seed: int, sample: int, sym: int, x: int, y: int) -> float:
+    return float(lib().oracle_sample_uniform(int(seed), sample, sym, x, y))

@@ -19,6 +19,9 @@
  *   oracle_mea           -- iterativeposteriordecoding (src/RNATools.jl:6-42), FP64; the kernel
  *                           posteriordecoding (src/cuda/inside.cu:286-321), FP32
  *   oracle_mea_traceback -- posteriordecodingtraceback (src/RNATools.jl:44-74)
+ *   oracle_sample_structure -- samplestructure / samplestructurehelper
+ *                           (src/InsideOutsideAlgorithm.jl:570-629), keyed uniforms
+ *   oracle_kh99_prior    -- calculateKH99prior (src/InsideOutsideAlgorithm.jl:86-144)
  *   lse                  -- CommonUtils.logsumexp (src/CommonUtils.jl:145-155)
  *
  * Conventions: trees are flat pre-order arrays with the root at index 0,
@@ -389,4 +392,151 @@ void oracle_mea_traceback(int L, const int *S, int *paired) {
     }
   }
   free(st);
+}
+
+/* Keyed uniform in (0, 1] for the stochastic traceback: one draw per interval
+ * (symbol, x, y) of a sample, so the result does not depend on the order in
+ * which intervals are visited.  The mixer is the SplitMix64 finaliser (Steele,
+ * Lea and Flood, "Fast splittable pseudorandom number generators", 2014).  The
+ * reference draws from Julia's MersenneTwister instead; its stream is not
+ * reproduced. */
+static uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+double oracle_sample_uniform(uint64_t seed, int sample, int sym, int x, int y) {
+  uint64_t key = mix64(seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(sample + 1));
+  uint64_t z = mix64(key ^ (((uint64_t)(uint32_t)x << 32) | (uint64_t)(uint32_t)y) ^ ((uint64_t)sym << 62));
+  return (double)((z >> 11) + 1) * 0x1.0p-53;
+}
+
+/* Stochastic traceback of one structure, samplestructure / samplestructurehelper
+ * (src/InsideOutsideAlgorithm.jl:570-629) from (S, 0, L-1): for an interval
+ * (symbol, x, y), x < y, the candidates are the bifurcations h = x..y-1 of the
+ * type-3 rule of the symbol (S->LS, F->LS: logprob + L[x,h] + S[h+1,y]) and then
+ * the pairing of its type-2 rule (logprob + paired[x,y] + F[x+1,y-1]), all times B;
+ * a candidate is drawn with probability exp(l - logsumexp(l)) by the first
+ * normalised cumulative sum >= u (CommonUtils.sample, src/CommonUtils.jl:129-142).
+ * in: 3 x L x L inside table (S=0, F=1, L=2); paired: column-major pair log-likelihoods.
+ * partner receives 1-based partners (0 unpaired); the return value is the
+ * structure's KH99 log prior (calculateKH99prior of the result). */
+double oracle_kh99_prior(int L, const int *partner);
+
+double oracle_sample_structure(int L, const double *in, const double *paired, double B, uint64_t seed, int sample,
+                               int *partner) {
+  const double lp[7] = {log(P_S_LS), log(P_S_s), log(P_S_dFd), log(P_F_dFd),
+                        log(P_F_LS), log(P_L_s), log(P_L_dFd)};
+  for (int i = 0; i < L; ++i) partner[i] = 0;
+  if (L < 1) return 0.0;
+  int *st = (int *)malloc(sizeof(int) * 3 * (2 * (size_t)L + 2));
+  double *ll = (double *)malloc(sizeof(double) * ((size_t)L + 1));
+  int top = 0;
+  st[0] = 0;
+  st[1] = 0;
+  st[2] = L - 1;
+  top = 1;
+  while (top > 0) {
+    --top;
+    int sym = st[3 * top], x = st[3 * top + 1], y = st[3 * top + 2];
+    if (x >= y) continue;
+    int n = 0;
+    double t3 = sym == 0 ? lp[0] : lp[4];
+    if (sym != 2)
+      for (int h = x; h <= y - 1; ++h) ll[n++] = (t3 + in[IDX3(2, x, h)] + in[IDX3(0, h + 1, y)]) * B;
+    double t2 = sym == 0 ? lp[2] : sym == 1 ? lp[3] : lp[6];
+    double f = (x + 1 <= y - 1) ? in[IDX3(1, x + 1, y - 1)] : -INFINITY;
+    ll[n++] = ((t2 + paired[x + (size_t)L * y]) + f) * B;
+    double s = -INFINITY;
+    for (int k = 0; k < n; ++k) s = lse(s, ll[k]);
+    double tot = 0.0;
+    for (int k = 0; k < n; ++k) {
+      ll[k] = exp(ll[k] - s);
+      tot += ll[k];
+    }
+    double u = oracle_sample_uniform(seed, sample, sym, x, y);
+    double cum = 0.0;
+    int r = n - 1;
+    for (int k = 0; k < n; ++k) {
+      cum += ll[k] / tot;
+      if (cum >= u) {
+        r = k;
+        break;
+      }
+    }
+    if (r == n - 1) {  /* the pairing */
+      partner[x] = y + 1;
+      partner[y] = x + 1;
+      st[3 * top] = 1;
+      st[3 * top + 1] = x + 1;
+      st[3 * top + 2] = y - 1;
+      ++top;
+    } else {           /* bifurcation at h = x + r: L (x, h), then S (h+1, y) popped first */
+      int h = x + r;
+      st[3 * top] = 2;
+      st[3 * top + 1] = x;
+      st[3 * top + 2] = h;
+      ++top;
+      st[3 * top] = 0;
+      st[3 * top + 1] = h + 1;
+      st[3 * top + 2] = y;
+      ++top;
+    }
+  }
+  free(st);
+  free(ll);
+  return oracle_kh99_prior(L, partner);
+}
+
+/* calculateKH99prior (src/InsideOutsideAlgorithm.jl:86-144): log probability of a
+ * structure's KH99 derivation with the composite rules S -> sS | dFdS | s | dFd,
+ * F -> dFd | sS | dFdS (S -> LS, F -> LS times L -> s or L -> dFd). */
+double oracle_kh99_prior(int L, const int *partner) {
+  if (L < 1) return 0.0;
+  const double r_S_sS = log(P_S_LS * P_L_s), r_S_dFdS = log(P_S_LS * P_L_dFd), r_S_s = log(P_S_s),
+               r_S_dFd = log(P_S_dFd), r_F_dFd = log(P_F_dFd), r_F_sS = log(P_F_LS * P_L_s),
+               r_F_dFdS = log(P_F_LS * P_L_dFd);
+  int *st = (int *)malloc(sizeof(int) * 3 * (2 * (size_t)L + 2));
+  int top = 1;
+  st[0] = 0;
+  st[1] = 0;
+  st[2] = L - 1;
+  double ll = 0.0;
+  while (top > 0) {
+    --top;
+    int sym = st[3 * top], s = st[3 * top + 1], e = st[3 * top + 2];
+    if (s < 0 || s >= L) continue;
+    int p = partner[s] - 1;  /* 0-based partner, -1 unpaired */
+    if (partner[s] == 0) {
+      if (s == e) {
+        if (sym == 0) ll += r_S_s;          /* F has no single-position rule */
+      } else if (s < e) {
+        ll += sym == 0 ? r_S_sS : r_F_sS;
+        st[3 * top] = 0;
+        st[3 * top + 1] = s + 1;
+        st[3 * top + 2] = e;
+        ++top;
+      }
+    } else if (p > s) {
+      if (p == e) {
+        ll += sym == 0 ? r_S_dFd : r_F_dFd;
+        st[3 * top] = 1;
+        st[3 * top + 1] = s + 1;
+        st[3 * top + 2] = p - 1;
+        ++top;
+      } else {
+        ll += sym == 0 ? r_S_dFdS : r_F_dFdS;
+        st[3 * top] = 1;
+        st[3 * top + 1] = s + 1;
+        st[3 * top + 2] = p - 1;
+        ++top;
+        st[3 * top] = 0;
+        st[3 * top + 1] = p + 1;
+        st[3 * top + 2] = e;
+        ++top;
+      }
+    }
+  }
+  free(st);
+  return ll;
 }
