@@ -17,6 +17,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu
        python bench.py --mcmc           # learned MCMC proposal vs random walk (ESS/s)
        python bench.py --genome         # 30 x 9000 nt posterior, full span and D=500
        python bench.py --mea            # MEA consensus decoding at L = 1500 and 9000
+       python bench.py --sample         # structures sampled from the inside table (1024 per evaluation)
 """
 from __future__ import annotations
 
@@ -316,6 +317,61 @@ def main_mea(args):
     print(json.dumps(out), flush=True)
 
 
+def main_sample(args):
+    """Structure sampling (stochastic traceback) on the 50 x 1500 alignment: one
+    step = one evaluation plus args.samples structures drawn from its inside
+    table on the device; the sampling time is the step time minus the
+    evaluation-only step time.  FP64 and FP32 tables."""
+    from messi_amd import lib, model, synth
+    from messi_amd.hotpath import device_dataset
+    a = synth.config2()
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths)
+    site, pair = _components(p)
+    dd = device_dataset(ds)
+    L = ds.numcols
+    n = args.samples
+    res = {}
+    for prec in ("fp64", "fp32"):
+        for _ in range(args.warmup):
+            dd.sample_posterior(site, pair, n, seed=1, prec=prec)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            dd.sample_posterior(site, pair, 0, seed=k, prec=prec)
+        ev = (time.perf_counter() - t0) * 1e3 / args.steps
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            z, S, pri = dd.sample_posterior(site, pair, n, seed=k, prec=prec)
+        tot = (time.perf_counter() - t0) * 1e3 / args.steps
+        smp = max(tot - ev, 1e-6)
+        res[prec] = {"eval_ms": ev, "eval_plus_sampling_ms": tot, "sampling_ms": smp,
+                     "structures_per_s": n * 1e3 / smp, "mean_pairs": float((S > 0).sum(1).mean() / 2),
+                     "mean_logprior": float(np.mean(pri))}
+        print(f"[sample] {prec}: eval {ev:.2f} ms, eval + {n} samples {tot:.2f} ms", file=sys.stderr, flush=True)
+    out = {"metric": f"structures sampled per second at 1500nt x 50seq ({n} per evaluation)",
+           "value": res["fp32"]["structures_per_s"], "unit": "structures/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res["fp32"]["sampling_ms"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded Yule tree, GTR/MuSE simulation, 40% paired, 5% gaps)",
+           "config": {"workload": f"config2: 50 sequences x 1500 nt; {n} stochastic tracebacks per evaluation",
+                      "L": L, "samples": n}, "variants": res}
+    if not args.no_cpu:
+        import oracle
+        u = oracle.site_loglik(ds, model.site_components(p))
+        M = oracle.pair_loglik(ds, model.pair_components(p), maxbp=L)
+        ins = oracle.inside(u, M)
+        t0 = time.perf_counter()
+        k = 0
+        while time.perf_counter() - t0 < 10.0 or k == 0:
+            oracle.sample_structures(ins, M, 8, seed=5, first=8 * k)
+            k += 1
+        dt = (time.perf_counter() - t0) / (8 * k)
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "structures/s", "cores": 1, "kind": "port",
+                               "sample": f"{8 * k} oracle tracebacks at L={L} from the oracle's FP64 inside table "
+                                         f"(the table itself not timed)"}
+    print(json.dumps(out), flush=True)
+
+
 def main_mcmc(args):
     """Learned proposal vs adaptive random walk on the 50 x 1500 alignment:
     16 chains, adaptive random-walk burn-in, network trained on the device on the
@@ -368,6 +424,8 @@ def main():
     ap.add_argument("--mcmc", action="store_true")
     ap.add_argument("--genome", action="store_true")
     ap.add_argument("--mea", action="store_true")
+    ap.add_argument("--sample", action="store_true")
+    ap.add_argument("--samples", type=int, default=1024)
     ap.add_argument("--mea-lengths", type=int, nargs="+", default=[1500, 9000])
     ap.add_argument("--tile", type=int, default=0, help="inside tile edge 16 or 32 (default: library's)")
     ap.add_argument("--precs", nargs="+", default=["fp64", "fp32"], help="precisions to time (profiling runs)")
@@ -396,6 +454,8 @@ def main():
         return main_genome(args)
     if args.mea:
         return main_mea(args)
+    if args.sample:
+        return main_sample(args)
 
     from messi_amd import lib, synth
     from messi_amd.hotpath import device_dataset
