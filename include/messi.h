@@ -156,6 +156,18 @@ int messi_sample_posterior(messi_dataset *ds, int K4, const double *P4, const do
                            int nsamples, uint64_t seed, double B, double *logZ, int *structures,
                            double *logprior);
 
+/* Fixed-structure log-likelihood: the sum of the site log-likelihoods of the
+ * unpaired columns and of the pair log-likelihoods of the pairs of the structure
+ * paired (L, 1-based partners, 0 unpaired); parts (2, may be NULL) receives the
+ * two sums.  Replaces felsensteincudapaired (src/FelsensteinCUDA.jl:721-912) with
+ * the unpaired sum of computemodellikelihood's GPU branch (src/FARCE.jl:2479-2487);
+ * the reference's leftnumpairs offset (src/FelsensteinCUDA.jl:858-859) is not
+ * carried over.  Every pair must be in the dataset's pair list (distance bound and
+ * mask): a dataset created with the structure's pairs as its mask prunes only those. */
+int messi_structure_loglik(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
+                           int K16, const double *P16, const double *pi16, const double *logw16,
+                           const int *paired, int prec, double *loglik, double *parts);
+
 /* Batch of members, each a dataset (the same dataset may appear several times,
  * e.g. one member per MCMC chain or proposal).  messi_batch_eval evaluates all
  * members together: each tree level and each tile-diagonal of the inside

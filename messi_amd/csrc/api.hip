@@ -78,6 +78,7 @@ struct messi_dataset {
   int64_t site_idx_n = 0, pair_idx_n = 0, site_leaf_n = 0, pair_leaf_n = 0;
   DevBuf d_site_idx, d_site_leaf, d_pair_idx, d_pair_leaf;
   DevBuf d_pid0, d_T0, d_mask;
+  DevBuf d_partner, d_sums;     // fixed-structure likelihood: the structure and its sums
   bool has_mask = false;
   messi_batch *self = nullptr;  // batch of one, created on first use
   ~messi_dataset();
@@ -1205,6 +1206,49 @@ int messi_posterior_consensus(messi_dataset *ds, int K4, const double *P4, const
     };
     if (prec == MESSI_FP32) go(float{});
     else go(double{});
+  });
+}
+
+int messi_structure_loglik(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
+                           int K16, const double *P16, const double *pi16, const double *logw16,
+                           const int *paired, int prec, double *loglik, double *parts) {
+  if (!ds) return 2;
+  messi_ctx *ctx = ds->ctx;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(P4 && pi4 && logw4 && P16 && pi16 && logw16 && paired && loglik, "null argument");
+    check_prec(prec);
+    const int L = ds->L;
+    for (int x = 0; x < L; ++x) {
+      const int p = paired[x];
+      MESSI_REQUIRE(p >= 0 && p <= L && p != x + 1, "paired must hold 1-based partners (0 unpaired)");
+      MESSI_REQUIRE(p == 0 || paired[p - 1] == x + 1, "paired is not symmetric");
+    }
+    MESSI_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    messi_batch *b = self_batch(ds);
+    ds->d_partner.ensure(sizeof(int) * L);
+    ds->d_sums.ensure(sizeof(double) * 4);
+    MESSI_HIP(hipMemcpyAsync(ds->d_partner.p, paired, sizeof(int) * L, hipMemcpyHostToDevice, s));
+    auto go = [&](auto tag) {
+      using T = decltype(tag);
+      load_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+      run_site<T>(b, prec);
+      run_pair<T>(b, prec, nullptr, false);
+      const ExpandDesc *dexp = upload_expand(b, prec, true);
+      launch_structure_sum(s, dexp, b->pair_acc.as<double>(), ds->d_partner.as<int>(), ds->d_sums.as<double>());
+      MESSI_HIP(hipGetLastError());
+    };
+    if (prec == MESSI_FP32) go(float{});
+    else go(double{});
+    double h[3];
+    MESSI_HIP(hipMemcpyAsync(h, ds->d_sums.p, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    MESSI_HIP(hipStreamSynchronize(s));
+    MESSI_REQUIRE(h[2] == 0.0, "the dataset's pair list lacks a pair of the structure (distance bound or mask)");
+    *loglik = h[0] + h[1];
+    if (parts) {
+      parts[0] = h[0];
+      parts[1] = h[1];
+    }
   });
 }
 

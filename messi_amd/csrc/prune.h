@@ -46,6 +46,9 @@ struct ExpandDesc {
 void launch_site_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, const double *acc);
 void launch_pair_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, const double *acc,
                         double *const *outs);
+// Fixed-structure sums of one member (descriptor d, pair root values acc):
+// out[0] unpaired site log-likelihoods, out[1] pair log-likelihoods, out[2] missing pairs.
+void launch_structure_sum(hipStream_t s, const ExpandDesc *d, const double *acc, const int *partner, double *out);
 template <typename T> bool mfma_probe(hipStream_t s);
 bool mfma_bf16_probe(hipStream_t s);
 

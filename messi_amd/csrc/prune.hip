@@ -288,6 +288,56 @@ __global__ void pair_expand_kernel(const ExpandDesc *__restrict__ dd, const doub
   out[(size_t)r * L + c] = v;
 }
 
+// Fixed-structure log-likelihood of one member from its site values (d.u) and its
+// pair root entries: out[0] = sum of u over unpaired columns, out[1] = sum of the
+// pair log-likelihoods of the structure's pairs, out[2] = number of pairs that the
+// member's pair list does not hold.  partner is 1-based (0 unpaired).  One
+// workgroup; the fixed per-thread order and tree make the sums deterministic.
+__global__ __launch_bounds__(1024) void structure_sum_kernel(const ExpandDesc *__restrict__ dd,
+                                                             const double *__restrict__ acc,
+                                                             const int *__restrict__ partner, double *out) {
+  const ExpandDesc d = *dd;
+  const int L = d.L, t = threadIdx.x;
+  __shared__ double su[1024], sp[1024];
+  __shared__ int sm[1024];
+  double a = 0.0, b = 0.0;
+  int miss = 0;
+  for (int x = t; x < L; x += 1024) {
+    const int p = partner[x];
+    if (p == 0) {
+      a += d.u[x];
+    } else if (p - 1 > x) {
+      const int y = p - 1;
+      int k = -1;
+      if (y - x <= d.maxbp && (!d.mask || d.mask[(size_t)x + (size_t)L * y]))
+        k = d.T0[(size_t)d.pid0[x] * d.n0 + d.pid0[y]];
+      if (k >= 0) b += acc[d.rootoff + k];
+      else ++miss;
+    }
+  }
+  su[t] = a;
+  sp[t] = b;
+  sm[t] = miss;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if (t < o) {
+      su[t] += su[t + o];
+      sp[t] += sp[t + o];
+      sm[t] += sm[t + o];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = su[0];
+    out[1] = sp[0];
+    out[2] = (double)sm[0];
+  }
+}
+
+void launch_structure_sum(hipStream_t s, const ExpandDesc *d, const double *acc, const int *partner, double *out) {
+  hipLaunchKernelGGL(structure_sum_kernel, dim3(1), dim3(1024), 0, s, d, acc, partner, out);
+}
+
 template <typename T, int NB>
 void launch_pair_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes, const int *idx,
                        const uint16_t *leafc, T *part, int *expo, const T *P, const double *pi16,

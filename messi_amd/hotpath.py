@@ -15,6 +15,8 @@ Function here                    Reference
 ``computeposteriordecodingcuda``  src/InsideCUDA.jl:183-215
 ``iterativeposteriordecoding``   src/RNATools.jl:6-42
 ``consensus``                    src/MESSI.jl:1097-1115 (posteriors and decoding in one device pass)
+``felsensteincudapaired``        src/FelsensteinCUDA.jl:721-912 (without its leftnumpairs offset)
+``computemodellikelihood``       src/FARCE.jl:2439-2515 (fixed structure, site rates integrated)
 ``samplestructure``              src/InsideOutsideAlgorithm.jl:623-627
 ``samplestructures``             the same, many samples per device launch
 ``calculateKH99prior``           src/InsideOutsideAlgorithm.jl:86-144 (host, for structures given by the caller)
@@ -227,3 +229,44 @@ def calculateKH99prior(pairedsites) -> float:
                 stack.append(("F", s + 1, k - 1))
                 stack.append(("S", k + 1, e))
     return ll
+
+
+def structure_dataset(dataset, paired) -> "lib.DeviceDataset":
+    """Device dataset whose pair list is exactly the pairs of one structure (the
+    reference's getindexdict(dataset, paired), src/FelsensteinCUDA.jl:77-142);
+    cached on the dataset by the structure's content."""
+    p = np.asarray(paired, dtype=np.int64)
+    key = hash(p.tobytes())
+    cache = dataset.__dict__.setdefault("_device_structure", {})
+    if key not in cache:
+        L = len(p)
+        mask = np.zeros((L, L), dtype=np.uint8, order="F")
+        x = np.flatnonzero(p > np.arange(1, L + 1))
+        mask[x, p[x] - 1] = 1
+        if len(cache) > 8:
+            cache.clear()
+        cache[key] = lib.DeviceDataset(dataset, L, mask)
+    return cache[key]
+
+
+def felsensteincudapaired(dataset, paired, params: ModelParameters, museparams=None, prec=lib.FP64) -> float:
+    """Sum of the pair log-likelihoods of the structure's pairs (1-based partners)."""
+    dd = structure_dataset(dataset, paired)
+    _, (_, pairsum) = dd.structure_loglik(_stack(site_components(params)), _stack(pair_components(params)),
+                                         paired, prec)
+    return pairsum
+
+
+def computemodellikelihood(dataset, params: ModelParameters, paired, samplebranchlengths: bool = False,
+                           useprior: bool = True, fixLambdaGT: bool = False, prec=lib.FP64) -> float:
+    """Fixed-structure log posterior: prior + unpaired site log-likelihoods + pair
+    log-likelihoods of the structure, in one device call (the site rates are
+    integrated, as in the reference's GPU branch)."""
+    p = params
+    if fixLambdaGT:
+        p = params.copy()
+        p.lambdaGT = 1.0
+        p.lambdaratesGT = np.array([0.0])
+    dd = structure_dataset(dataset, paired)
+    ll, _ = dd.structure_loglik(_stack(site_components(p)), _stack(pair_components(p)), paired, prec)
+    return (computeprior(p, samplebranchlengths, fixLambdaGT) if useprior else 0.0) + ll

@@ -53,6 +53,8 @@ SIGNATURES = {
     "messi_last_timings": (ctypes.c_int, [_vp, _dp]),
     "messi_mea": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp, _ip]),
     "messi_mea_timings": (ctypes.c_int, [_vp, _dp]),
+    "messi_structure_loglik": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _ip,
+                                              ctypes.c_int, _dp, _dp]),
     "messi_sample_structures": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_uint64, ctypes.c_double, _ip, _dp]),
     "messi_sample_posterior": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
@@ -305,6 +307,27 @@ class DeviceDataset:
             self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c), ptr(pi16), ptr(lw16), prec_code(prec),
             prec_code(mea_prec), float(alpha), ctypes.byref(z), paired.ctypes.data_as(_ip)))
         return z.value, paired.astype(np.int64)
+
+    def structure_loglik(self, site, pair, paired, prec=FP64):
+        """(log-likelihood of the fixed structure, (unpaired sum, pair sum)); paired
+        holds 1-based partners, 0 unpaired."""
+        P4, pi4, lw4 = site
+        P16, pi16, lw16 = pair
+        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
+        K4, K16 = P4c.shape[0], P16c.shape[0]
+        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
+        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
+        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
+        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        pa = np.ascontiguousarray(paired, dtype=np.intc)
+        if pa.shape != (self.L,):
+            raise ValueError(f"paired must have length {self.L}")
+        ll = ctypes.c_double()
+        parts = np.zeros(2)
+        self.ctx.check(self.ctx.lib.messi_structure_loglik(
+            self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c), ptr(pi16), ptr(lw16),
+            pa.ctypes.data_as(_ip), prec_code(prec), ctypes.byref(ll), ptr(parts)))
+        return ll.value, (float(parts[0]), float(parts[1]))
 
     def sample_posterior(self, site, pair, nsamples: int, seed: int = 0, B: float = 1.0, prec=FP64):
         """log Z, then nsamples structures (nsamples x L, 1-based partners) drawn from
