@@ -18,6 +18,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu
        python bench.py --genome         # 30 x 9000 nt posterior, full span and D=500
        python bench.py --mea            # MEA consensus decoding at L = 1500 and 9000
        python bench.py --sample         # structures sampled from the inside table (1024 per evaluation)
+       python bench.py --structure      # fixed-structure log-likelihood (true structure)
 """
 from __future__ import annotations
 
@@ -372,6 +373,55 @@ def main_sample(args):
     print(json.dumps(out), flush=True)
 
 
+def main_structure(args):
+    """Fixed-structure log-likelihood on the 50 x 1500 alignment with its true
+    structure: one step = site pruning of every column, pair pruning of the
+    structure's pairs only (a structure-masked pair list) and the sums, with the
+    transition matrices copied from the host each call."""
+    from messi_amd import hotpath, lib, model, synth
+    a = synth.config2()
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths)
+    site, pair = _components(p)
+    true = np.asarray(a.pairedsites)
+    paired = np.where(true >= 0, true + 1, 0)
+    dd = hotpath.structure_dataset(ds, paired)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        for _ in range(args.warmup):
+            dd.structure_loglik(site, pair, paired, prec)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ll, parts = dd.structure_loglik(site, pair, paired, prec)
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        res[prec] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms, "loglik": ll, "parts": parts}
+    out = {"metric": "fixed-structure log-likelihood evals/s at 1500nt x 50seq",
+           "value": res["fp32"]["evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_eval"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded Yule tree, GTR/MuSE simulation, 40% paired, 5% gaps); its true structure",
+           "config": {"workload": "config2: 50 sequences x 1500 nt, fixed structure "
+                                  f"({int(np.count_nonzero(paired) // 2)} pairs), host inputs each call",
+                      "L": ds.numcols, "pairs": int(np.count_nonzero(paired) // 2)},
+           "variants": res,
+           "fp32_rel_err": abs(res["fp32"]["loglik"] - res["fp64"]["loglik"]) / abs(res["fp64"]["loglik"]),
+           "dataset_stats": dd.stats()}
+    if not args.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        u = oracle.site_loglik(ds, model.site_components(p))
+        mask = np.zeros((ds.numcols, ds.numcols), dtype=bool)
+        x = np.flatnonzero(paired > np.arange(1, ds.numcols + 1))
+        mask[x, paired[x] - 1] = True
+        M = oracle.pair_loglik(ds, model.pair_components(p), mask=mask)
+        ll = float(np.sum(u[paired == 0]) + np.sum(M[x, paired[x] - 1]))
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "evals/s", "cores": 1, "kind": "port",
+                               "sample": f"one full oracle evaluation (all sites, the {len(x)} structure pairs)",
+                               "loglik": ll}
+    print(json.dumps(out), flush=True)
+
+
 def main_mcmc(args):
     """Learned proposal vs adaptive random walk on the 50 x 1500 alignment:
     16 chains, adaptive random-walk burn-in, network trained on the device on the
@@ -425,6 +475,7 @@ def main():
     ap.add_argument("--genome", action="store_true")
     ap.add_argument("--mea", action="store_true")
     ap.add_argument("--sample", action="store_true")
+    ap.add_argument("--structure", action="store_true")
     ap.add_argument("--samples", type=int, default=1024)
     ap.add_argument("--mea-lengths", type=int, nargs="+", default=[1500, 9000])
     ap.add_argument("--tile", type=int, default=0, help="inside tile edge 16 or 32 (default: library's)")
@@ -456,6 +507,8 @@ def main():
         return main_mea(args)
     if args.sample:
         return main_sample(args)
+    if args.structure:
+        return main_structure(args)
 
     from messi_amd import lib, synth
     from messi_amd.hotpath import device_dataset

@@ -168,6 +168,39 @@ int messi_structure_loglik(messi_dataset *ds, int K4, const double *P4, const do
                            int K16, const double *P16, const double *pi16, const double *logw16,
                            const int *paired, int prec, double *loglik, double *parts);
 
+/* ---- Input formats (host only, no device context) ----
+ * Alignment handle from a FASTA file and the first line of a Newick file
+ * (getalignmentfromfastaandnewick, src/AlignmentIO.jl:18-48; the Dataset
+ * constructor, src/Dataset.jl:126-199): state masks, gap frequency and observed
+ * base frequencies (case-sensitive ACGTU, src/Dataset.jl:135-152), the tree
+ * parsed (src/Tree.jl:114-171), rooted at its first child (roottree,
+ * src/Tree.jl:45-55) and numbered in pre-order (annotatetree, src/Tree.jl:190-199).
+ * Failures return non-zero with the message in messi_io_error(). */
+typedef struct messi_alignment messi_alignment;
+const char *messi_io_error(void);
+int messi_alignment_read(const char *fasta_path, const char *newick_path, messi_alignment **out);
+void messi_alignment_destroy(messi_alignment *a);
+int messi_alignment_info(const messi_alignment *a, int *nseq, int *ncol, int *nnodes);
+/* data: (nseq, ncol, 4) 0/1, column-major; codes: nseq x ncol state masks,
+ * row-major; gapfrequency: ncol; obsfreqs: 4.  Any may be NULL. */
+int messi_alignment_data(const messi_alignment *a, double *data, uint8_t *codes, double *gapfrequency,
+                         double *obsfreqs);
+/* children (2 per node, -1 for leaves), seqindex (alignment row of a leaf, -1
+ * for internal nodes), branch lengths (root 0), pre-order with the root at 0. */
+int messi_alignment_tree(const messi_alignment *a, int *children, int *seqindex, double *branchlengths);
+/* FASTA name of alignment row `row` (NUL-terminated, truncated to cap; 3 if truncated). */
+int messi_alignment_name(const messi_alignment *a, int row, char *buf, int cap);
+/* The dataset of an alignment handle, as messi_dataset_create builds it. */
+int messi_dataset_from_alignment(messi_ctx *ctx, const messi_alignment *a, int maxbasepairdistance,
+                                 const uint8_t *mask, messi_dataset **out);
+/* readctfile (src/RNATools.jl:165-193): *L receives the length from the header;
+ * with paired non-NULL (cap >= L) the 1-based partners, and with sequence
+ * non-NULL (cap + 1 bytes) the sequence column. */
+int messi_read_ct(const char *path, int cap, int *L, int *paired, char *sequence);
+/* parsedbn (src/RNATools.jl:109-140): brackets ( < A B C D E close with ) > a b c d e;
+ * a closing bracket pairs only with the innermost open bracket of its kind. */
+int messi_parse_dbn(const char *dbn, int cap, int *L, int *paired);
+
 /* Batch of members, each a dataset (the same dataset may appear several times,
  * e.g. one member per MCMC chain or proposal).  messi_batch_eval evaluates all
  * members together: each tree level and each tile-diagonal of the inside

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmessi.so")
-SOURCES = ["api.hip", "prune.hip", "inside.hip", "mlp.hip", "mea.hip", "sample.hip"]
+SOURCES = ["api.hip", "prune.hip", "inside.hip", "mlp.hip", "mea.hip", "sample.hip", "io.cpp"]
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",

@@ -69,7 +69,7 @@ class Dataset:
         if len(set(len(s) for s in sequences)) != 1:
             raise ValueError("aligned sequences must have equal length")
         self.names = list(names)
-        self.sequences = [s.upper() for s in sequences]
+        self.sequences = list(sequences)
         self.numseqs = len(sequences)
         self.numcols = len(sequences[0])
         self.codes = encode(self.sequences)

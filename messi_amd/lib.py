@@ -53,6 +53,16 @@ SIGNATURES = {
     "messi_last_timings": (ctypes.c_int, [_vp, _dp]),
     "messi_mea": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, _ip, _dp, _ip]),
     "messi_mea_timings": (ctypes.c_int, [_vp, _dp]),
+    "messi_io_error": (ctypes.c_char_p, []),
+    "messi_alignment_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "messi_alignment_destroy": (None, [_vp]),
+    "messi_alignment_info": (ctypes.c_int, [_vp, _ip, _ip, _ip]),
+    "messi_alignment_data": (ctypes.c_int, [_vp, _dp, _u8, _dp, _dp]),
+    "messi_alignment_tree": (ctypes.c_int, [_vp, _ip, _ip, _dp]),
+    "messi_alignment_name": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
+    "messi_dataset_from_alignment": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _u8, ctypes.POINTER(_vp)]),
+    "messi_read_ct": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _ip, _ip, ctypes.c_char_p]),
+    "messi_parse_dbn": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _ip, _ip]),
     "messi_structure_loglik": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _ip,
                                               ctypes.c_int, _dp, _dp]),
     "messi_sample_structures": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
@@ -219,6 +229,17 @@ class DeviceDataset:
             self.maxbp, ptr(m, _u8), ctypes.byref(h)))
         self._h = h
         _live.add(self)
+
+    @classmethod
+    def _adopt(cls, ctx, h, L: int, nnodes: int, maxbp: int) -> "DeviceDataset":
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.L = L
+        self.nnodes = nnodes
+        self.maxbp = maxbp
+        self._h = h
+        _live.add(self)
+        return self
 
     @property
     def handle(self):
@@ -446,6 +467,89 @@ def mea_timings(ctx: Optional[Context] = None) -> np.ndarray:
     t = np.zeros(3)
     ctx.check(ctx.lib.messi_mea_timings(ctx.handle, ptr(t)))
     return t
+
+
+def _io_check(rc: int) -> None:
+    if rc != 0:
+        msg = load().messi_io_error()
+        raise MessiError(msg.decode() if msg else f"error {rc}")
+
+
+class Alignment:
+    """FASTA alignment + Newick tree parsed by the library (messi_alignment_*): state
+    masks, gap and base frequencies, the rooted pre-order tree."""
+
+    def __init__(self, fasta_path: str, newick_path: str):
+        L = load()
+        h = _vp()
+        _io_check(L.messi_alignment_read(os.fsencode(fasta_path), os.fsencode(newick_path), ctypes.byref(h)))
+        self._h = h
+        n, c, m = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.messi_alignment_info(h, ctypes.byref(n), ctypes.byref(c), ctypes.byref(m))
+        self.numseqs, self.numcols, self.numnodes = n.value, c.value, m.value
+        self.codes = np.empty((self.numseqs, self.numcols), dtype=np.uint8)
+        self.gapfrequency = np.empty(self.numcols)
+        self.obsfreqs = np.empty(4)
+        L.messi_alignment_data(h, None, ptr(self.codes, _u8), ptr(self.gapfrequency), ptr(self.obsfreqs))
+        self.children = np.empty((self.numnodes, 2), dtype=np.intc)
+        self.seqindex = np.empty(self.numnodes, dtype=np.intc)
+        self.branchlengths = np.empty(self.numnodes)
+        L.messi_alignment_tree(h, ptr(self.children, _ip), ptr(self.seqindex, _ip), ptr(self.branchlengths))
+        buf = ctypes.create_string_buffer(4096)
+        self.names = []
+        for r in range(self.numseqs):
+            L.messi_alignment_name(h, r, buf, 4096)
+            self.names.append(buf.value.decode())
+
+    def data(self) -> np.ndarray:
+        """(nseq, ncol, 4) 0/1 state indicators (the reference's Dataset.data)."""
+        out = np.empty((self.numseqs, self.numcols, 4), order="F")
+        load().messi_alignment_data(self._h, ptr(out), None, None, None)
+        return out
+
+    def device_dataset(self, maxbasepairdistance: int = 1000000, mask: Optional[np.ndarray] = None,
+                       ctx: Optional[Context] = None) -> "DeviceDataset":
+        ctx = ctx or context()
+        m = None if mask is None else np.asfortranarray(mask > 0, dtype=np.uint8)
+        maxbp = int(min(maxbasepairdistance, 2**31 - 1))
+        h = _vp()
+        ctx.check(ctx.lib.messi_dataset_from_alignment(ctx.handle, self._h, maxbp, ptr(m, _u8), ctypes.byref(h)))
+        return DeviceDataset._adopt(ctx, h, self.numcols, self.numnodes, maxbp)
+
+    def close(self) -> None:
+        if self._h:
+            load().messi_alignment_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _shutdown:
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_ct(path: str):
+    """(sequence, 1-based partners) of a .ct file (readctfile)."""
+    L = load()
+    n = ctypes.c_int()
+    _io_check(L.messi_read_ct(os.fsencode(path), 0, ctypes.byref(n), None, None))
+    paired = np.empty(n.value, dtype=np.intc)
+    seq = ctypes.create_string_buffer(n.value + 1)
+    _io_check(L.messi_read_ct(os.fsencode(path), n.value, ctypes.byref(n), paired.ctypes.data_as(_ip), seq))
+    return seq.value.decode(), paired.astype(np.int64)
+
+
+def parse_dbn(dbn: str) -> np.ndarray:
+    """1-based partners of a dot-bracket string (parsedbn)."""
+    L = load()
+    n = ctypes.c_int()
+    b = dbn.encode()
+    _io_check(L.messi_parse_dbn(b, 0, ctypes.byref(n), None))
+    paired = np.empty(n.value, dtype=np.intc)
+    _io_check(L.messi_parse_dbn(b, n.value, ctypes.byref(n), paired.ctypes.data_as(_ip)))
+    return paired.astype(np.int64)
 
 
 class Batch:

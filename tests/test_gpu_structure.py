@@ -87,3 +87,23 @@ def test_missing_and_malformed_pairs_are_rejected(c1):
     with pytest.raises(lib.MessiError, match="symmetric"):
         hotpath.device_dataset(c1["ds"]).structure_loglik(site, pair, bad)
     near.close()
+
+
+def test_dataset_from_files_evaluates_like_the_python_dataset(c1, tmp_path):
+    """FASTA + Newick parsed by the library give the same device dataset."""
+    _say("dataset from files")
+    ds = c1["ds"]
+    fa = tmp_path / "c1.fa"
+    fa.write_text(ds.to_fasta())
+    nw = tmp_path / "c1.nwk"
+    nw.write_text(ds.newick + "\n")
+    al = lib.Alignment(str(fa), str(nw))
+    p = model.getparams(synth.config1().params, al.branchlengths)
+    site, pair = _comps(p)
+    d1 = al.device_dataset()
+    d2 = hotpath.device_dataset(ds)
+    assert d1.stats() == d2.stats()
+    for prec in ("fp64", "fp32"):
+        assert d1.eval(site, pair, 1.0, prec) == d2.eval(site, pair, 1.0, prec)
+    d1.close()
+    al.close()
