@@ -197,7 +197,7 @@ template <class SR, int T> struct ProdLDS<SR, T, true> {
   XFv B[T][T + 1];
   union {
     float Ap[T][T + 1];  // exponent-aligned mantissas ...
-    float R[T][T + 1];   // ... overwritten by the MFMA result of the same wavefront
+    float R[T][T + 1];   // ... overwritten by the MFMA results once all are computed
   };
   float Bp[T][T + 1];
   int rA[T], cB[T];
@@ -315,20 +315,25 @@ __device__ __forceinline__ void tile_products(const Operands<SR> &o0, bool do0, 
     typedef float v4f __attribute__((ext_vector_type(4)));
     const int wave = tid >> 6, lane = tid & 63;
     constexpr int NB = T / 16, NB2 = NB * NB;
-    if (wave < 2 * NB2) {
-      const int k = wave / NB2, blk = wave % NB2;
-      if (k ? do1 : do0) {
-        const int bi = blk / NB, bj = blk % NB;
-        v4f acc = {0.f, 0.f, 0.f, 0.f};
+    const int k = wave / NB2, blk = wave % NB2;
+    const bool mf = wave < 2 * NB2 && (k == 0 ? do0 : do1);
+    const int bi = blk / NB, bj = blk % NB;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    if (mf) {
 #pragma unroll
-        for (int q = 0; q < T / 4; ++q) {
-          float av = pp[k].Ap[bi * 16 + (lane & 15)][4 * q + (lane >> 4)];
-          float bv = pp[k].Bp[4 * q + (lane >> 4)][bj * 16 + (lane & 15)];
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) pp[k].R[bi * 16 + 4 * (lane >> 4) + q][bj * 16 + (lane & 15)] = acc[q];
+      for (int q = 0; q < T / 4; ++q) {
+        float av = pp[k].Ap[bi * 16 + (lane & 15)][4 * q + (lane >> 4)];
+        float bv = pp[k].Bp[4 * q + (lane >> 4)][bj * 16 + (lane & 15)];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
       }
+    }
+    // R shares storage with Ap: with several wavefronts per product (T = 32),
+    // block (bi, bj)'s result overwrites mantissas that block (bi, bj') still
+    // reads, so no result is stored before every wavefront has read its operands.
+    if constexpr (NB > 1) __syncthreads();
+    if (mf) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) pp[k].R[bi * 16 + 4 * (lane >> 4) + q][bj * 16 + (lane & 15)] = acc[q];
     }
     __syncthreads();
 #pragma unroll

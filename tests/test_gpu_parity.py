@@ -342,8 +342,9 @@ def test_batch_mixed_tiles_and_autotuner(ctx):
             b.set_tiles(None)
             ref = b.eval_loaded(1.0, prec)
             b.set_tiles([16, 32, 16, 32, 32])
-            z = b.eval_loaded(1.0, prec)
-            assert np.max(np.abs(z - ref) / np.abs(ref)) < tol
+            for _ in range(3):   # each tile-32 evaluation, not only the first
+                z = b.eval_loaded(1.0, prec)
+                assert np.max(np.abs(z - ref) / np.abs(ref)) < tol
             ch = autotune.tune_batch(b, prec, reps=1)
             assert len(ch["candidates"]) >= 2
             z2 = b.eval_loaded(1.0, prec)
