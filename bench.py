@@ -485,7 +485,7 @@ def main():
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--burn", type=int, default=2000)
     ap.add_argument("--mcmc-steps", type=int, default=1250)
-    ap.add_argument("--long-steps", type=int, default=12000)
+    ap.add_argument("--long-steps", type=int, default=30000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--prec", default="fp32")
     ap.add_argument("--seed", type=int, default=5)

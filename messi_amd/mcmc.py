@@ -34,6 +34,7 @@ squared jump distance.
 """
 from __future__ import annotations
 
+import math
 import time
 from dataclasses import dataclass, field
 from typing import Callable, Optional
@@ -256,7 +257,8 @@ class Trace:
 
 
 def rw_chain(logp: Callable, theta, lp, steps: int, chol, scale: float, rng,
-             adapt_until: int = 0, target_accept: float = 0.234) -> Trace:
+             adapt_until: int = 0, target_accept: float = 0.234,
+             progress: Optional[Callable[[int], None]] = None) -> Trace:
     """Random-walk Metropolis: theta' = theta + scale * chol @ eps.
 
     With ``adapt_until > 0`` (burn-in only) the proposal adapts during the first
@@ -283,6 +285,8 @@ def rw_chain(logp: Callable, theta, lp, steps: int, chol, scale: float, rng,
         theta = np.where(acc[:, None], prop, theta)
         lp = np.where(acc, lpp, lp)
         S[it], LP[it], A[it] = theta, lp, acc
+        if progress is not None and (it + 1) % 2000 == 0:
+            progress(it + 1)
         if it < adapt_until:
             log_scale += (acc.mean() - target_accept) * 2.0 / np.sqrt(it + 10.0)
             if (it + 1) % 50 == 0 and it + 1 >= 4 * d:
@@ -380,16 +384,88 @@ class Whitening:
         return self.m + np.asarray(z) @ self.L.T
 
 
+class Mixture:
+    """Mixture of multivariate Student-t components in whitened coordinates: an
+    independence proposal that can jump between the modes the burn-in visited.
+    Fitted by EM (Gaussian responsibilities, k-means++ starts) to burn-in
+    samples; each component's scale is inflated so that the proposal is wider
+    than the fitted modes.  logq and sample are exact for the mixture itself,
+    so a Metropolis-Hastings move with it is exact whatever the fit quality."""
+
+    def __init__(self, w, means, chols, nu: float = 7.0):
+        self.w = np.asarray(w, dtype=np.float64)
+        self.means = np.asarray(means, dtype=np.float64)
+        self.chols = np.asarray(chols, dtype=np.float64)
+        self.nu = float(nu)
+        self.inv = np.linalg.inv(self.chols)
+        self.logdet = np.sum(np.log(np.abs(np.diagonal(self.chols, axis1=1, axis2=2))), axis=1)
+
+    @classmethod
+    def fit(cls, z, k: int, rng, iters: int = 150, inflate: float = 1.3, nu: float = 7.0) -> "Mixture":
+        z = np.asarray(z, dtype=np.float64)
+        n, d = z.shape
+        k = max(1, min(k, n // (4 * d)))
+        # k-means++ starts
+        cent = [z[rng.integers(n)]]
+        for _ in range(1, k):
+            d2 = np.min(((z[:, None, :] - np.array(cent)[None]) ** 2).sum(-1), axis=1)
+            cent.append(z[rng.choice(n, p=d2 / d2.sum())])
+        means = np.array(cent)
+        covs = np.repeat(np.cov(z.T)[None] + 1e-3 * np.eye(d), k, axis=0)
+        w = np.full(k, 1.0 / k)
+        for _ in range(iters):
+            ll = np.empty((n, k))
+            for j in range(k):
+                Lc = np.linalg.cholesky(covs[j])
+                r = np.linalg.solve(Lc, (z - means[j]).T).T
+                ll[:, j] = np.log(w[j]) - np.sum(np.log(np.diag(Lc))) - 0.5 * np.sum(r * r, axis=1)
+            ll -= ll.max(axis=1, keepdims=True)
+            g = np.exp(ll)
+            g /= g.sum(axis=1, keepdims=True)
+            nk = g.sum(axis=0) + 1e-9
+            w = nk / n
+            means = (g.T @ z) / nk[:, None]
+            for j in range(k):
+                r = z - means[j]
+                covs[j] = (g[:, j, None] * r).T @ r / nk[j] + 1e-3 * np.eye(d)
+        chols = np.array([np.linalg.cholesky(c) * inflate for c in covs])
+        return cls(w, means, chols, nu)
+
+    def logq(self, z) -> np.ndarray:
+        z = np.atleast_2d(np.asarray(z, dtype=np.float64))
+        d = z.shape[1]
+        nu = self.nu
+        c = math.lgamma(0.5 * (nu + d)) - math.lgamma(0.5 * nu) - 0.5 * d * math.log(nu * math.pi)
+        out = np.empty((z.shape[0], len(self.w)))
+        for j in range(len(self.w)):
+            r = (z - self.means[j]) @ self.inv[j].T
+            q = np.sum(r * r, axis=1)
+            out[:, j] = np.log(self.w[j]) + c - self.logdet[j] - 0.5 * (nu + d) * np.log1p(q / nu)
+        m = out.max(axis=1, keepdims=True)
+        return (m + np.log(np.exp(out - m).sum(axis=1, keepdims=True)))[:, 0]
+
+    def sample(self, n: int, rng) -> np.ndarray:
+        d = self.means.shape[1]
+        comp = rng.choice(len(self.w), size=n, p=self.w / self.w.sum())
+        g = rng.standard_normal((n, d))
+        scale = np.sqrt(self.nu / rng.chisquare(self.nu, n))
+        y = np.einsum("nij,nj->ni", self.chols[comp], g) * scale[:, None]
+        return self.means[comp] + y
+
+
 def _gauss_logq(x, mean, ell):
     r = (x - mean) * np.exp(-ell)
     return -np.sum(ell + 0.5 * r * r, axis=-1)
 
 
 def learned_chain(logp: Callable, theta, lp, steps: int, net, wh: Whitening, rng, rw_chol, rw_scale: float,
-                  rw_prob: float = 0.1, kappa: float = 1.0) -> Trace:
+                  rw_prob: float = 0.1, kappa: float = 1.0, mix: Optional[Mixture] = None,
+                  ind_prob: float = 0.0) -> Trace:
     """MH with the learned proposal z' ~ N(z + mu(z), diag (kappa exp(ell(z)))^2);
-    each chain takes a random-walk step instead with probability ``rw_prob``
-    (both kernels leave the target invariant, so the random mixture does too)."""
+    each chain takes a random-walk step instead with probability ``rw_prob``, and
+    with probability ``ind_prob`` an independence step from the fitted mixture
+    ``mix`` (every kernel leaves the target invariant, so the random mixture of
+    kernels does too).  Trace.kind: 0 random walk, 1 learned, 2 independence."""
     theta = np.array(theta, dtype=np.float64)
     lp = np.array(lp, dtype=np.float64)
     n, d = theta.shape
@@ -399,7 +475,9 @@ def learned_chain(logp: Callable, theta, lp, steps: int, net, wh: Whitening, rng
     K = np.zeros((steps, n), dtype=np.int8)
     t0 = time.perf_counter()
     for it in range(steps):
-        use_rw = rng.random(n) < rw_prob
+        u = rng.random(n)
+        use_rw = u < rw_prob
+        use_ind = (u >= rw_prob) & (u < rw_prob + ind_prob) if mix is not None else np.zeros(n, dtype=bool)
         z = wh.z(theta)
         out = net.forward(z.astype(np.float32)).astype(np.float64)
         mu, ell = out[:, :d], np.clip(out[:, d:], -8.0, 4.0) + np.log(kappa)
@@ -407,6 +485,11 @@ def learned_chain(logp: Callable, theta, lp, steps: int, net, wh: Whitening, rng
         outp = net.forward(zp.astype(np.float32)).astype(np.float64)
         mup, ellp = outp[:, :d], np.clip(outp[:, d:], -8.0, 4.0) + np.log(kappa)
         log_ratio = _gauss_logq(z, zp + mup, ellp) - _gauss_logq(zp, z + mu, ell)
+        if mix is not None and use_ind.any():
+            zi = mix.sample(n, rng)
+            li = mix.logq(z) - mix.logq(zi)      # q(z) / q(z'): the whitening's Jacobian cancels
+            zp = np.where(use_ind[:, None], zi, zp)
+            log_ratio = np.where(use_ind, li, log_ratio)
         prop = wh.theta(zp)
         rwp = theta + rw_scale * (rng.standard_normal((n, d)) @ np.asarray(rw_chol).T)
         prop = np.where(use_rw[:, None], rwp, prop)
@@ -415,7 +498,8 @@ def learned_chain(logp: Callable, theta, lp, steps: int, net, wh: Whitening, rng
         acc = np.log(rng.random(n)) < lpp - lp + log_ratio
         theta = np.where(acc[:, None], prop, theta)
         lp = np.where(acc, lpp, lp)
-        S[it], LP[it], A[it], K[it] = theta, lp, acc, (~use_rw).astype(np.int8)
+        kind = np.where(use_rw, 0, np.where(use_ind, 2, 1)).astype(np.int8)
+        S[it], LP[it], A[it], K[it] = theta, lp, acc, kind
     return Trace(S, LP, A, K, time.perf_counter() - t0)
 
 
@@ -581,7 +665,8 @@ def _esjd(samples, wh: Whitening) -> float:
 def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: int = 1250,
                    long_steps: int = 4000, train_steps: int = 1500, rw_prob: float = 0.1,
                    noises=(0.8, 1.1), kappas=(1.3, 1.6), log: Callable = print,
-                   laplace_init: bool = True, mode_logp: Optional[Callable] = None) -> dict:
+                   laplace_init: bool = True, mode_logp: Optional[Callable] = None,
+                   ind_prob: float = 0.15, mix_components: int = 4) -> dict:
     """Learned proposal against the adaptive random walk, each with a burn-in of
     ``burn`` steps from the same start, then ``steps`` production steps each and
     a long random-walk chain as the judge of the posterior means.
@@ -591,9 +676,11 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     and the learned proposal runs once per scale factor kappa, the settings
     being scored by their expected squared jump distance; a last training on
     all burn-in samples at the best noise level fixes the network, and the best
-    kappa is kept for production.  Random-walk method: adaptive random walk for
-    all ``burn`` steps.  Production kernels are fixed (no adaptation), so each
-    production chain is an exact MH chain.
+    kappa is kept for production.  A mixture of Student-t components fitted to
+    the whitened burn-in samples gives independence moves (probability
+    ``ind_prob``) that jump between the modes the burn-in visited.  Random-walk
+    method: adaptive random walk for all ``burn`` steps.  Production kernels are
+    fixed (no adaptation), so each production chain is an exact MH chain.
     """
     theta0 = np.asarray(theta0, dtype=np.float64)
     n, d = theta0.shape
@@ -646,6 +733,10 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     train_s += time.perf_counter() - t0
     log(f"final training: noise {best['noise']}, kappa {best['kappa']}, pool {pool.shape[0]} x {n}, "
         f"held-out loss {tinfo['best_val_loss']:.3f} at step {tinfo['best_step']}")
+    mix = None
+    if ind_prob > 0.0:
+        mix = Mixture.fit(wh.z(pool).reshape(-1, d), mix_components, rng)
+        log(f"independence mixture: {len(mix.w)} components, weights {np.round(mix.w, 3).tolist()}")
     learned_burn_s = time.perf_counter() - t_start
     # random-walk method: the rest of its own burn-in
     trb = rw_chain(logp, tb.samples[-1], tb.logp[-1], burn - burn_rw, rw_chol, rw_scale, rng,
@@ -654,12 +745,15 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     log(f"burn-in (random walk, rest): {burn - burn_rw} steps, acceptance {trb.acceptance:.3f}")
 
     tl = learned_chain(logp, theta, lp, steps, net, wh, rng, rw_chol, rw_scale, rw_prob=rw_prob,
-                       kappa=best["kappa"])
+                       kappa=best["kappa"], mix=mix, ind_prob=ind_prob if mix is not None else 0.0)
     learned_acc = float(tl.accepted[tl.kind == 1].mean()) if (tl.kind == 1).any() else 0.0
-    log(f"learned chain: {steps} steps, acceptance {learned_acc:.3f} (learned moves), {tl.seconds:.1f} s")
+    ind_acc = float(tl.accepted[tl.kind == 2].mean()) if (tl.kind == 2).any() else 0.0
+    log(f"learned chain: {steps} steps, acceptance {learned_acc:.3f} (learned moves), {ind_acc:.3f} "
+        f"(independence moves), {tl.seconds:.1f} s")
     tr = rw_chain(logp, trb.samples[-1], trb.logp[-1], steps, rw_chol2, rw_scale2, rng)
     log(f"random-walk chain: {steps} steps, acceptance {tr.acceptance:.3f}, {tr.seconds:.1f} s")
-    tlong = rw_chain(logp, trb.samples[-1], trb.logp[-1], long_steps, rw_chol2, rw_scale2, rng)
+    tlong = rw_chain(logp, trb.samples[-1], trb.logp[-1], long_steps, rw_chol2, rw_scale2, rng,
+                     progress=lambda i: log(f"long random-walk chain: {i} of {long_steps} steps"))
     log(f"long random-walk chain: {long_steps} steps, {tlong.seconds:.1f} s")
     e_l, e_r, e_long = ess(tl.samples), ess(tr.samples), ess(tlong.samples)
     z = compare_means(tl.samples, e_l, tlong.samples, e_long)
@@ -670,6 +764,7 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
         "train_seconds": train_s, "learned_burn_seconds": learned_burn_s, "trials": trials,
         "chosen": {"noise": best["noise"], "kappa": best["kappa"]},
         "learned": {"seconds": tl.seconds, "acceptance_learned_moves": learned_acc,
+                    "acceptance_independence_moves": ind_acc, "independence_prob": ind_prob if mix else 0.0,
                     "acceptance": tl.acceptance, "ess": e_l.tolist(), "min_ess": float(e_l.min()),
                     "min_ess_per_s": float(e_l.min() / tl.seconds),
                     "median_ess_per_s": float(np.median(e_l) / tl.seconds)},

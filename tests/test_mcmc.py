@@ -281,3 +281,73 @@ def test_mcse_covers_chains_stuck_in_modes():
     # and with well-mixed chains the replicate term does not dominate
     y = rng.standard_normal((T, m, 1))
     assert mcmc.mcse(y, mcmc.ess(y))[0] == pytest.approx(y.std() / np.sqrt(mcmc.ess(y)[0]), rel=0.5)
+
+
+def _bimodal_logp(th):
+    """Two Gaussian modes in 2-d, weights 0.3 (at -3) and 0.7 (at +3) on x0."""
+    th = np.atleast_2d(th)
+    a = np.log(0.3) - 0.5 * ((th[:, 0] + 3.0) / 0.5) ** 2
+    b = np.log(0.7) - 0.5 * ((th[:, 0] - 3.0) / 0.5) ** 2
+    m = np.maximum(a, b)
+    return m + np.log(np.exp(a - m) + np.exp(b - m)) - 0.5 * (th[:, 1] / 0.7) ** 2
+
+
+def _bimodal_pool(rng, n):
+    mode = rng.random(n) < 0.3
+    x0 = np.where(mode, rng.normal(-3.0, 0.5, n), rng.normal(3.0, 0.5, n))
+    return np.stack([x0, rng.normal(0.0, 0.7, n)], axis=1)
+
+
+def test_mixture_fit_logq_and_sampling_agree():
+    rng = np.random.default_rng(8)
+    z = _bimodal_pool(rng, 6000)
+    mix = mcmc.Mixture.fit(z, 3, rng)
+    # the density integrates to one (importance estimate with the mixture's own draws)
+    y = mix.sample(200000, rng)
+    assert np.all(np.isfinite(mix.logq(y)))
+    # normalisation: E_q[1] by a wide Gaussian importance sample
+    g = rng.normal(0, 6.0, (400000, 2))
+    lg = -np.sum(0.5 * (g / 6.0) ** 2, axis=1) - 2 * np.log(6.0) - np.log(2 * np.pi)
+    est = np.mean(np.exp(mix.logq(g) - lg))
+    assert abs(est - 1.0) < 0.02
+    # both modes are covered by its draws
+    assert 0.15 < np.mean(y[:, 0] < 0) < 0.45
+
+
+def test_independence_moves_fix_the_mode_weights():
+    """Chains started in one mode: with independence moves from a deliberately
+    mis-weighted mixture the mode weights are right (the ratio corrects the
+    weights); local moves alone never leave the starting mode."""
+    rng = np.random.default_rng(9)
+    n, d = 32, 2
+    wh = mcmc.Whitening(np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0], [1.0, 1.0]]) - 0.5)
+    wh.m[:] = 0.0
+    wh.L = np.eye(2)
+    wh.Li = np.eye(2)
+    mix = mcmc.Mixture(np.array([0.8, 0.2]), np.array([[-3.0, 0.0], [3.0, 0.0]]),
+                       np.array([np.eye(2) * 0.8, np.eye(2) * 0.8]))
+    net = FixedNet(d, 0.0, 0.3)
+    th0 = np.tile([3.0, 0.0], (n, 1)) + rng.normal(0, 0.2, (n, d))
+    tr = mcmc.learned_chain(_bimodal_logp, th0, _bimodal_logp(th0), 3000, net, wh, rng, np.eye(d) * 0.3, 1.0,
+                            rw_prob=0.1, mix=mix, ind_prob=0.3)
+    frac = np.mean(tr.samples[500:, :, 0] < 0)
+    assert abs(frac - 0.3) < 0.03, frac
+    assert (tr.kind == 2).any() and tr.accepted[tr.kind == 2].mean() > 0.2
+    local = mcmc.learned_chain(_bimodal_logp, th0, _bimodal_logp(th0), 1500, net, wh, rng, np.eye(d) * 0.3, 1.0,
+                               rw_prob=0.1)
+    assert np.mean(local.samples[:, :, 0] < 0) < 0.01
+
+    # control: the same independence draws without the ratio q(z)/q(z') settle on
+    # target x proposal (mode at -3 weighted 0.3 x 0.8 against 0.7 x 0.2)
+    th = th0.copy()
+    lp = _bimodal_logp(th)
+    fr = []
+    for it in range(3000):
+        prop = mix.sample(n, rng)
+        lpp = _bimodal_logp(prop)
+        acc = np.log(rng.random(n)) < lpp - lp
+        th = np.where(acc[:, None], prop, th)
+        lp = np.where(acc, lpp, lp)
+        if it >= 500:
+            fr.append(np.mean(th[:, 0] < 0))
+    assert np.mean(fr) > 0.5, np.mean(fr)
