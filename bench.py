@@ -443,8 +443,13 @@ def main_mcmc(args):
 
     res = mcmc.run_experiment(post, th0, net, rng, burn=args.burn, steps=args.mcmc_steps,
                               long_steps=args.long_steps, train_steps=args.train_steps, log=log,
-                              mode_logp=lambda th: post(th, prec="fp64"))
-    res.pop("traces")
+                              mode_logp=lambda th: post(th, prec="fp64"), ind_prob=args.ind_prob,
+                              mix_components=args.mix_components, mix_flatten=args.mix_flatten)
+    tr = res.pop("traces")
+    if args.save_traces:
+        np.savez_compressed(args.save_traces, learned=tr["learned"].samples, learned_kind=tr["learned"].kind,
+                            learned_acc=tr["learned"].accepted, random_walk=tr["random_walk"].samples,
+                            long=tr["long"].samples[::10], burn=tr["burn"].samples)
     z = np.abs(np.array(res["mean_z_learned_vs_long"]))
     z_ess = np.abs(np.array(res["mean_z_learned_vs_long_ess_only"]))
     out = {"metric": "MCMC ESS/s, learned proposal vs adaptive random walk (min ESS over 12 parameters)",
@@ -484,7 +489,11 @@ def main():
     ap.add_argument("--genome-spans", type=int, nargs="+", default=[1000000, 500])
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--burn", type=int, default=2000)
-    ap.add_argument("--mcmc-steps", type=int, default=1250)
+    ap.add_argument("--mcmc-steps", type=int, default=2000)
+    ap.add_argument("--ind-prob", type=float, default=0.25)
+    ap.add_argument("--save-traces", default="", help="write the chains to this .npz (MCMC mode)")
+    ap.add_argument("--mix-components", type=int, default=6)
+    ap.add_argument("--mix-flatten", type=float, default=0.5)
     ap.add_argument("--long-steps", type=int, default=30000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--prec", default="fp32")

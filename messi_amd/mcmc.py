@@ -431,6 +431,12 @@ class Mixture:
         chols = np.array([np.linalg.cholesky(c) * inflate for c in covs])
         return cls(w, means, chols, nu)
 
+    def flattened(self, a: float) -> "Mixture":
+        """The same components with weights (1 - a) w + a / K: rarely visited modes
+        keep enough proposal mass to be entered and left."""
+        k = len(self.w)
+        return Mixture((1.0 - a) * self.w + a / k, self.means, self.chols, self.nu)
+
     def logq(self, z) -> np.ndarray:
         z = np.atleast_2d(np.asarray(z, dtype=np.float64))
         d = z.shape[1]
@@ -666,7 +672,8 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
                    long_steps: int = 4000, train_steps: int = 1500, rw_prob: float = 0.1,
                    noises=(0.8, 1.1), kappas=(1.3, 1.6), log: Callable = print,
                    laplace_init: bool = True, mode_logp: Optional[Callable] = None,
-                   ind_prob: float = 0.15, mix_components: int = 4) -> dict:
+                   ind_prob: float = 0.15, mix_components: int = 4, mix_inflate: float = 1.3,
+                   mix_nu: float = 7.0, mix_flatten: float = 0.5) -> dict:
     """Learned proposal against the adaptive random walk, each with a burn-in of
     ``burn`` steps from the same start, then ``steps`` production steps each and
     a long random-walk chain as the judge of the posterior means.
@@ -735,8 +742,19 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
         f"held-out loss {tinfo['best_val_loss']:.3f} at step {tinfo['best_step']}")
     mix = None
     if ind_prob > 0.0:
-        mix = Mixture.fit(wh.z(pool).reshape(-1, d), mix_components, rng)
-        log(f"independence mixture: {len(mix.w)} components, weights {np.round(mix.w, 3).tolist()}")
+        # fit, explore with the independence moves (still burn-in), refit on
+        # everything: modes the random-walk burn-in barely visited get samples
+        mix = Mixture.fit(wh.z(pool).reshape(-1, d), mix_components, rng, inflate=mix_inflate,
+                          nu=mix_nu).flattened(mix_flatten)
+        te = learned_chain(logp, theta, lp, per_trial, net, wh, rng, rw_chol, rw_scale, rw_prob=rw_prob,
+                           kappa=best["kappa"], mix=mix, ind_prob=max(ind_prob, 0.5))
+        segs.append(te.samples)
+        theta, lp = te.samples[-1], te.logp[-1]
+        explore = np.concatenate(segs[1:], axis=0)
+        mix = Mixture.fit(wh.z(explore).reshape(-1, d), mix_components, rng, inflate=mix_inflate,
+                          nu=mix_nu).flattened(mix_flatten)
+        log(f"independence mixture: {len(mix.w)} components, weights {np.round(mix.w, 3).tolist()} "
+            f"(exploration acceptance {te.accepted[te.kind == 2].mean() if (te.kind == 2).any() else 0.0:.3f})")
     learned_burn_s = time.perf_counter() - t_start
     # random-walk method: the rest of its own burn-in
     trb = rw_chain(logp, tb.samples[-1], tb.logp[-1], burn - burn_rw, rw_chol, rw_scale, rng,
