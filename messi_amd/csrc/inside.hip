@@ -438,6 +438,30 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   Operands<SR> o1, o2;
   if (do1) o1 = fetch_operands<SR, T>(s1, L);
   if (do2) o2 = fetch_operands<SR, T>(s2, L);
+  // ---- finalising workgroup: stage the tile's inputs now, while the product
+  // operands are in flight (these LDS arrays are not shared with the products)
+  if (fin) {
+    for (int idx = tid; idx < T * T; idx += T * T) {
+      int a = idx / T, b = idx % T;
+      int j = IT + a, y = JT + b;
+      int jd = IT + b;  // column index inside the diagonal tile (I, I)
+      sm.Ld[a][b] = (j < L && jd < L && jd >= j) ? Lr[(size_t)j * L + jd] : SR::zero();
+      sm.cP[a][b] = (j < L && y < L) ? ptv[(size_t)j * L + y] : SR::zero();
+    }
+    for (int idx = tid; idx < T * T; idx += T * T) {
+      int b = idx / T, c = idx % T;  // S[JT+c, JT+b], c fastest along the column
+      int y = JT + b, rr = JT + c;
+      sm.Sd[c][b] = (y < L && rr <= y) ? Sc[(size_t)y * L + rr] : SR::zero();
+    }
+    if (tid < T) {
+      int b = tid, y = JT + b, rb = (I + 1) * T;
+      sm.Sb[b] = (t >= 1 && y < L && rb <= y) ? Sc[(size_t)y * L + rb] : SR::zero();
+      int yc = JT + b - 1;
+      sm.Fbot[b] = (t >= 1 && rb < L && yc < L && rb <= yc) ? F[(size_t)rb * L + yc] : SR::zero();
+      int ja = IT + b + 1, yl = JT - 1;
+      sm.Fleft[b] = (t >= 1 && ja < L && yl >= 0 && ja <= yl) ? F[(size_t)ja * L + yl] : SR::zero();
+    }
+  }
   if (do1 || do2) {
     V acc2 = SR::zero();
     tile_products<SR, T>(o1, do1, o2, do2, sm.u.p, accv, acc2);
@@ -447,27 +471,6 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
     if (j < L && y < L) acc[(size_t)j * L + y] = accv;
     return;
-  }
-  // ---- finalise tile (I, J): stage operands
-  for (int idx = tid; idx < T * T; idx += T * T) {
-    int a = idx / T, b = idx % T;
-    int j = IT + a, y = JT + b;
-    int jd = IT + b;  // column index inside the diagonal tile (I, I)
-    sm.Ld[a][b] = (j < L && jd < L && jd >= j) ? Lr[(size_t)j * L + jd] : SR::zero();
-    sm.cP[a][b] = (j < L && y < L) ? ptv[(size_t)j * L + y] : SR::zero();
-  }
-  for (int idx = tid; idx < T * T; idx += T * T) {
-    int b = idx / T, c = idx % T;  // S[JT+c, JT+b], c fastest along the column
-    int y = JT + b, rr = JT + c;
-    sm.Sd[c][b] = (y < L && rr <= y) ? Sc[(size_t)y * L + rr] : SR::zero();
-  }
-  if (tid < T) {
-    int b = tid, y = JT + b, rb = (I + 1) * T;
-    sm.Sb[b] = (t >= 1 && y < L && rb <= y) ? Sc[(size_t)y * L + rb] : SR::zero();
-    int yc = JT + b - 1;
-    sm.Fbot[b] = (t >= 1 && rb < L && yc < L && rb <= yc) ? F[(size_t)rb * L + yc] : SR::zero();
-    int ja = IT + b + 1, yl = JT - 1;
-    sm.Fleft[b] = (t >= 1 && ja < L && yl >= 0 && ja <= yl) ? F[(size_t)ja * L + yl] : SR::zero();
   }
   if (t == 0) {
     for (int idx = tid; idx < T * T; idx += T * T) {
