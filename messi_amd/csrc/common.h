@@ -92,6 +92,7 @@ struct InsideWork {
   double *zout = nullptr;         // where log Z (absolute, baseline-relative) is also written (device)
   DevBuf Lr, Lc, Sr, Sc, F;       // inside tables (row- and column-major copies)
   DevBuf acc;                     // bulk split-point accumulator of the tiled inside
+  DevBuf Lp, LpE, Sp, SpE;        // scaled path: product-ready operand copies (see inside.hip)
   DevBuf oSr, oSc, oFr, oL;       // outside tables
   DevBuf oWr, oWc, oaL, oaS;      // outside: parent weight W (row/column-major), accumulators
   DevBuf orng;                    // outside launch tables

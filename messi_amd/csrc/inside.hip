@@ -52,9 +52,17 @@ template <class SR> static Rules<SR> make_rules(double B) {
 // Instances: one inside problem each (own length and tables); a launch serves all.
 // ---------------------------------------------------------------------------
 template <class V> struct InstDesc {
-  int L, NT;
+  int L, NT, T;  // length, tiles per side, tile edge
   int KB;  // largest tile offset K - I whose L tile (I, K) can hold a finite value
   V *Lr, *Lc, *Sr, *Sc, *F, *acc, *ptv;
+  // acc is tile-major: tile (I, J) holds T*T cells at (I*NT + J)*T*T, row-major.
+  // Scaled path only, tile-major too: L as a product's A operand, each row of
+  // tile (I, K) aligned to its largest exponent (Lp block (I, K) row-major, LpE
+  // at (I*NT + K)*T + row), and S as the B operand, each column segment of rows
+  // KT+1 .. KT+T of tile column J aligned to its largest exponent (Sp block
+  // (K, J) column-major, SpE at (K*NT + J)*T + column).
+  float *Lp, *Sp;
+  int *LpE, *SpE;
   const double *u, *paired;
   double *lam, *C, *uhat, *z;
   const ExpandDesc *exp;  // pair term from the pair root entries when non-null
@@ -139,7 +147,7 @@ __global__ void inside_init_kernel(const InstDesc<typename SR::V> *__restrict__ 
   int j = blockIdx.y;
   if (y >= L || j >= L) return;
   size_t c = (size_t)j * L + y;
-  d.acc[c] = SR::zero();
+  d.acc[((size_t)(j / d.T) * d.NT + y / d.T) * d.T * d.T + (j % d.T) * d.T + y % d.T] = SR::zero();
   if (y == j) {
     double ub = d.u[j] * B - d.lam[j];
     typename SR::V sv = SR::from_log(ub + lp1 * B), lv = SR::from_log(ub + lp5 * B);
@@ -372,6 +380,106 @@ __device__ __forceinline__ void tile_products(const Operands<SR> &o0, bool do0, 
   __syncthreads();
 }
 
+// Scaled-path operands of one product read from the product-ready copies: this
+// thread's (r, c) = (tid / T, tid % T) element of each, as aligned float mantissas,
+// and the exponent of its row of A and of its column r of B.
+struct PackedOps {
+  float a, b;
+  int ea, eb;
+};
+
+template <int T>
+__device__ __forceinline__ PackedOps fetch_packed(const InstDesc<XFv> &d, int I, int K, int J) {
+  const int tid = threadIdx.x, r = tid / T, c = tid % T, L = d.L;
+  const int ar = I * T + r, ac = K * T + c, bcol = J * T + r, brow = K * T + 1 + c;
+  PackedOps o;
+  o.a = (ar < L && ac < L) ? d.Lp[((size_t)I * d.NT + K) * (T * T) + tid] : 0.0f;
+  o.b = (bcol < L && brow < L) ? d.Sp[((size_t)K * d.NT + J) * (T * T) + tid] : 0.0f;
+  o.ea = ar < L ? d.LpE[((size_t)I * d.NT + K) * T + r] : XF::EZ;
+  o.eb = bcol < L ? d.SpE[((size_t)K * d.NT + J) * T + r] : XF::EZ;
+  return o;
+}
+
+// Exact sum of one cell of product (I, K) x (K, J) from the tables themselves, for
+// the rare cell whose exponent-aligned sum lost its significant terms.
+template <int T>
+__device__ __forceinline__ XFv exact_cell_global(const InstDesc<XFv> &d, int I, int K, int J, int r, int c) {
+  const int L = d.L, row = I * T + r, col = J * T + c;
+  auto av = [&](int q) {
+    const int h = K * T + q;
+    return (row < L && h < L && row <= h) ? d.Lr[(size_t)row * L + h] : XF::zero();
+  };
+  auto bv = [&](int q) {
+    const int s = K * T + 1 + q;
+    return (col < L && s < L && s <= col) ? d.Sc[(size_t)col * L + s] : XF::zero();
+  };
+  int km = XF::kmin();
+#pragma unroll 1
+  for (int q = 0; q < T; ++q) km = max(km, XF::key(av(q), bv(q)));
+  float sum = 0.0f;
+#pragma unroll 1
+  for (int q = 0; q < T; ++q) sum += XF::contrib(av(q), bv(q), km);
+  return XF::finish(km, sum);
+}
+
+// tile_products for the scaled inside path from product-ready operands: the
+// same MFMA and fold, with no exponent search and half the operand bytes.
+template <int T>
+__device__ __forceinline__ void tile_products_packed(const InstDesc<XFv> &d, int I, int J, const PackedOps &o0,
+                                                     int K0, bool do0, const PackedOps &o1, int K1, bool do1,
+                                                     ProdLDS<XF, T> *pp, XFv &acc0, XFv &acc1) {
+  const int tid = threadIdx.x, r = tid / T, c = tid % T;
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const PackedOps &o = k ? o1 : o0;
+    if (!(k ? do1 : do0)) continue;
+    pp[k].Ap[r][c] = o.a;
+    pp[k].Bp[c][r] = o.b;
+    if (c == 0) {
+      pp[k].rA[r] = o.ea;
+      pp[k].cB[r] = o.eb;
+    }
+  }
+  __syncthreads();
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const int wave = tid >> 6, lane = tid & 63;
+  constexpr int NB = T / 16, NB2 = NB * NB;
+  const int k = wave / NB2, blk = wave % NB2;
+  const bool mf = wave < 2 * NB2 && (k == 0 ? do0 : do1);
+  const int bi = blk / NB, bj = blk % NB;
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+  if (mf) {
+#pragma unroll
+    for (int q = 0; q < T / 4; ++q) {
+      float av = pp[k].Ap[bi * 16 + (lane & 15)][4 * q + (lane >> 4)];
+      float bv = pp[k].Bp[4 * q + (lane >> 4)][bj * 16 + (lane & 15)];
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+    }
+  }
+  if constexpr (NB > 1) __syncthreads();  // R shares storage with Ap
+  if (mf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pp[k].R[bi * 16 + 4 * (lane >> 4) + q][bj * 16 + (lane & 15)] = acc[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k2 = 0; k2 < 2; ++k2) {
+    if (!(k2 ? do1 : do0)) continue;
+    const int ra = pp[k2].rA[r], cb = pp[k2].cB[c];
+    XFv contrib = XF::zero();
+    if (ra != XF::EZ && cb != XF::EZ) {
+      const float cv = pp[k2].R[r][c];
+      contrib = cv >= 0x1p-100f ? XF::norm(cv, ra + cb) : exact_cell_global<T>(d, I, k2 ? K1 : K0, J, r, c);
+    }
+    XFv &a = k2 ? acc1 : acc0;
+    a = XF::add(a, contrib);
+  }
+  __syncthreads();
+}
+
+// Bulk items per bulk workgroup (their loads overlap).
+constexpr int BULK_PER = 2;
+
 // Bulk items of launch t: tiles at distance d = t+1 .. dmax, NT - d of them per
 // distance; C(d) counts the items of distances t+1 .. d.
 __device__ __forceinline__ int bulk_count(int NT, int t, int d) {
@@ -386,12 +494,21 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   typedef typename SR::V V;
   __shared__ StepLDS<SR, T> sm;
   const int tid = threadIdx.x;
-  const int bx = xcd_chunk_id(blockIdx.x, gridDim.x);
-  // ---- instance of this workgroup: prefix[m] <= bx < prefix[m+1]
-  int lo = 0, hi = n;
+  // ---- work item.  The first workgroups (in dispatch order, so they start
+  // first, round-robin over the XCDs) finalise the tiles of every instance:
+  // prefix[0..n] counts them per instance; a few padding workgroups round their
+  // number up to a multiple of 8.  The rest are bulk workgroups, prefix[n+1 ..
+  // 2n+1] per instance, renumbered so that each XCD takes a contiguous run.
+  const int *fpre = prefix, *bpre = prefix + (n + 1);
+  const int nfin_all = fpre[n], fpad = (nfin_all + 7) & ~7;
+  const bool fin_wg = (int)blockIdx.x < fpad;
+  if (fin_wg && (int)blockIdx.x >= nfin_all) return;  // padding
+  const int bx = fin_wg ? (int)blockIdx.x : xcd_chunk_id((int)blockIdx.x - fpad, (int)gridDim.x - fpad);
+  const int *pre = fin_wg ? fpre : bpre;
+  int lo = 0, hi = n;  // instance: pre[lo] <= bx < pre[lo+1]
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
-    if (prefix[mid] <= bx) lo = mid;
+    if (pre[mid] <= bx) lo = mid;
     else hi = mid;
   }
   const InstDesc<V> ins = insts[lo];
@@ -403,25 +520,65 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   V *__restrict__ F = ins.F;
   V *__restrict__ acc = ins.acc;
   const V *__restrict__ ptv = ins.ptv;
-  // ---- decode the work item
-  int item = bx - prefix[lo], I, J;
-  bool fin;
-  if (item < NT - t) {
-    fin = true;
-    I = item;
-    J = item + t;
-  } else {
-    fin = false;
-    const int idx = item - (NT - t);
-    int lo = t + 1, hi = NT - 1;  // smallest distance d with bulk_count(d) > idx
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (bulk_count(NT, t, mid) > idx) hi = mid;
-      else lo = mid + 1;
+  // ---- decode the work item: workgroups NT-t .. are bulk workgroups, each
+  // taking BULK_PER consecutive bulk items, with every item's loads issued
+  // before the first product is computed.
+  const int item = bx - pre[lo];
+  if (!fin_wg) {
+    int dmax = min(2 * t - 2, NT - 1);
+    if (t - 1 > ins.KB) dmax = min(dmax, t - 1 + ins.KB);
+    const int nbulk = dmax >= t + 1 ? bulk_count(NT, t, dmax) : 0;
+    const int first = BULK_PER * item;
+    int Ib[BULK_PER], Jb[BULK_PER];
+    bool has[BULK_PER], d1[BULK_PER], d2[BULK_PER];
+    V av[BULK_PER];
+    Operands<SR> ob1[BULK_PER], ob2[BULK_PER];
+    PackedOps qb1[BULK_PER], qb2[BULK_PER];
+    const int a = tid / T, b = tid % T;
+#pragma unroll
+    for (int u = 0; u < BULK_PER; ++u) {
+      const int idx = first + u;
+      has[u] = idx < nbulk;
+      int dl = t + 1, dh = NT - 1;  // smallest distance d with bulk_count(d) > idx
+      while (dl < dh) {
+        const int mid = (dl + dh) >> 1;
+        if (bulk_count(NT, t, mid) > idx) dh = mid;
+        else dl = mid + 1;
+      }
+      Ib[u] = idx - bulk_count(NT, t, dl - 1);
+      Jb[u] = Ib[u] + dl;
+      const int k1 = Ib[u] + t - 1, k2 = Jb[u] - (t - 1);
+      d1[u] = has[u] && k1 - Ib[u] <= ins.KB;
+      d2[u] = has[u] && k2 != k1 && k2 - Ib[u] <= ins.KB;
+      const int j = Ib[u] * T + a, y = Jb[u] * T + b;
+      av[u] = (has[u] && j < L && y < L) ? acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid] : SR::zero();
+      if constexpr (SR::scaled) {
+        if (d1[u]) qb1[u] = fetch_packed<T>(ins, Ib[u], k1, Jb[u]);
+        if (d2[u]) qb2[u] = fetch_packed<T>(ins, Ib[u], k2, Jb[u]);
+      } else {
+        if (d1[u]) ob1[u] = fetch_operands<SR, T>(inside_src<SR, T>(Ib[u], k1, Jb[u], Lr, Sc), L);
+        if (d2[u]) ob2[u] = fetch_operands<SR, T>(inside_src<SR, T>(Ib[u], k2, Jb[u], Lr, Sc), L);
+      }
     }
-    I = idx - bulk_count(NT, t, lo - 1);
-    J = I + lo;
+#pragma unroll
+    for (int u = 0; u < BULK_PER; ++u) {
+      if (!has[u]) continue;
+      const int k1 = Ib[u] + t - 1, k2 = Jb[u] - (t - 1);
+      if (d1[u] || d2[u]) {
+        V acc2 = SR::zero();
+        if constexpr (SR::scaled)
+          tile_products_packed<T>(ins, Ib[u], Jb[u], qb1[u], k1, d1[u], qb2[u], k2, d2[u], sm.u.p, av[u], acc2);
+        else
+          tile_products<SR, T>(ob1[u], d1[u], ob2[u], d2[u], sm.u.p, av[u], acc2);
+        av[u] = SR::add(av[u], acc2);
+      }
+      const int j = Ib[u] * T + a, y = Jb[u] * T + b;
+      if (j < L && y < L) acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid] = av[u];
+    }
+    return;
   }
+  const bool fin = true;
+  const int I = item, J = item + t;
   const int IT = I * T, JT = J * T;
   // ---- split-point products that became ready with tile-diagonal t-1: every
   // global load (accumulator and both products' operands) is issued up front.
@@ -432,12 +589,18 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   V accv = SR::zero();
   {
     const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
-    if ((t >= 2 || !fin) && j < L && y < L) accv = acc[(size_t)j * L + y];
+    if ((t >= 2 || !fin) && j < L && y < L) accv = acc[((size_t)I * NT + J) * (T * T) + tid];
   }
-  const OpSrc<SR> s1 = inside_src<SR, T>(I, K1, J, Lr, Sc), s2 = inside_src<SR, T>(I, K2, J, Lr, Sc);
   Operands<SR> o1, o2;
-  if (do1) o1 = fetch_operands<SR, T>(s1, L);
-  if (do2) o2 = fetch_operands<SR, T>(s2, L);
+  PackedOps q1, q2;
+  if constexpr (SR::scaled) {
+    if (do1) q1 = fetch_packed<T>(ins, I, K1, J);
+    if (do2) q2 = fetch_packed<T>(ins, I, K2, J);
+  } else {
+    const OpSrc<SR> s1 = inside_src<SR, T>(I, K1, J, Lr, Sc), s2 = inside_src<SR, T>(I, K2, J, Lr, Sc);
+    if (do1) o1 = fetch_operands<SR, T>(s1, L);
+    if (do2) o2 = fetch_operands<SR, T>(s2, L);
+  }
   // ---- finalising workgroup: stage the tile's inputs now, while the product
   // operands are in flight (these LDS arrays are not shared with the products)
   if (fin) {
@@ -464,7 +627,8 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   }
   if (do1 || do2) {
     V acc2 = SR::zero();
-    tile_products<SR, T>(o1, do1, o2, do2, sm.u.p, accv, acc2);
+    if constexpr (SR::scaled) tile_products_packed<T>(ins, I, J, q1, K1, do1, q2, K2, do2, sm.u.p, accv, acc2);
+    else tile_products<SR, T>(o1, do1, o2, do2, sm.u.p, accv, acc2);
     accv = SR::add(accv, acc2);
   }
   if (!fin) {
@@ -529,29 +693,20 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     // F[j+1, y-1] for the pair term: inside this tile, or on its bottom / left border.
     const bool fin_in = a + 1 < T && b >= 1;
     for (int st = 0; st <= 2 * T - 2; ++st) {
-      const bool act = mine && st >= 1 && st <= own, fin = mine && st == own;
-      V l1 = SR::zero(), s1 = SR::zero(), l2 = SR::zero(), s2 = SR::zero(), f = SR::zero(), pt = SR::zero();
-      if (act) {
-        const int ap = b + T - st;  // column producer (ap, b), finalised at step st-1
-        if (ap > a && ap < T) {
-          l1 = sm.Ld[a][ap - 1];
-          s1 = sm.u.w.cS[ap][b];
-        }
-        const int bp = st - T + a;  // row producer (a, bp), finalised at step st-1
-        if (bp >= 0 && bp < b) {
-          l2 = sm.u.w.cL[a][bp];
-          s2 = sm.Sd[bp + 1][b];
-        }
-      }
-      if (fin) {
-        pt = sm.cP[a][b];
-        f = fin_in ? sm.u.w.cF[a + 1][b - 1] : (a + 1 == T ? sm.Fbot[b] : sm.Fleft[a]);
+      const bool act = mine && st >= 1 && st <= own;
+      // column producer (ap, b) and row producer (a, bp), finalised at step st-1;
+      // both terms are read at clamped (always valid) positions and masked, so
+      // the step has no divergent branch but the finalisation
+      const int ap = b + T - st, bp = st - T + a;
+      const int apc = min(max(ap, 1), T - 1), bpc = min(max(bp, 0), T - 2);
+      const V t1 = SR::mul_raw(sm.Ld[a][apc - 1], sm.u.w.cS[apc][b]);
+      const V t2 = SR::mul_raw(sm.u.w.cL[a][bpc], sm.Sd[bpc + 1][b]);
+      SR::acc_add2(ac, SR::keep(t1, act && ap > a && ap < T), SR::keep(t2, act && bp >= 0 && bp < b));
+      if (mine && st == own) {
+        V f = fin_in ? sm.u.w.cF[a + 1][b - 1] : (a + 1 == T ? sm.Fbot[b] : sm.Fleft[a]);
         if (j + 1 > y - 1) f = SR::zero();
-      }
-      if (act) SR::acc_add2(ac, SR::mul_raw(l1, s1), SR::mul_raw(l2, s2));
-      if (fin) {
         V vs, vf, vl;
-        SR::finalize(ac, SR::mul_raw(pt, f), rk, vs, vf, vl);
+        SR::finalize(ac, SR::mul_raw(sm.cP[a][b], f), rk, vs, vf, vl);
         sm.u.w.cS[a][b] = vs;
         sm.u.w.cF[a][b] = vf;
         sm.u.w.cL[a][b] = vl;
@@ -575,6 +730,35 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
       size_t cc = (size_t)y * L + j;
       Sc[cc] = sm.u.w.cS[a][b];
       Lc[cc] = sm.u.w.cL[a][b];
+    }
+  }
+  if constexpr (SR::scaled) {
+    // Product-ready copies of the finished tile.  A operand: each row segment of
+    // L aligned to its largest exponent (the T lanes of a row reduce by shuffles).
+    {
+      const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
+      const bool ok = j < L && y < L;
+      const XFv v = sm.u.w.cL[a][b];
+      int e = ok && v.m != 0.0f ? v.e : XF::EZ;
+#pragma unroll
+      for (int off = T / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+      if (ok) {
+        ins.Lp[((size_t)I * NT + J) * (T * T) + tid] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+        if (b == 0) ins.LpE[((size_t)I * NT + J) * T + a] = e;
+      }
+    }
+    // B operand: the column segment of S over rows IT+1 .. IT+T (the last one is
+    // row (I+1)T of the tile below, staged in Sb), aligned to its largest exponent.
+    {
+      const int c = tid % T, b = tid / T, row = IT + 1 + c, y = JT + b;
+      XFv v = XF::zero();
+      if (row < L && y < L && row <= y) v = c + 1 < T ? sm.u.w.cS[c + 1][b] : sm.Sb[b];
+      int e = v.m != 0.0f ? v.e : XF::EZ;
+#pragma unroll
+      for (int off = T / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+      if (row < L && y < L)
+        ins.Sp[((size_t)I * NT + J) * (T * T) + tid] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+      if (c == 0 && y < L) ins.SpE[((size_t)I * NT + J) * T + b] = e;
     }
   }
 }
@@ -915,8 +1099,17 @@ void inside_alloc(InsideWork &w, int L, bool outside) {
   w.Sr.ensure(n);
   w.Sc.ensure(n);
   w.F.ensure(n);
-  w.acc.ensure(n);
+  // tile-major buffers: room for the padded square at either tile edge
+  const size_t n16 = (size_t)(L + 15) / 16, n32 = (size_t)(L + 31) / 32;
+  const size_t cells = std::max(n16 * n16 * 256, n32 * n32 * 1024), rows = std::max(n16 * n16 * 16, n32 * n32 * 32);
+  w.acc.ensure(cells * sizeof(typename SR::V));
   w.ptv.ensure(n);
+  if (SR::scaled) {
+    w.Lp.ensure(sizeof(float) * cells);
+    w.Sp.ensure(sizeof(float) * cells);
+    w.LpE.ensure(sizeof(int) * rows);
+    w.SpE.ensure(sizeof(int) * rows);
+  }
   w.lam.ensure(sizeof(double) * L);
   w.C.ensure(sizeof(double) * (L + 1));
   w.uhat.ensure(sizeof(double) * L);
@@ -942,6 +1135,7 @@ static InstDesc<typename SR::V> inst_of(InsideWork &w) {
   InstDesc<V> d;
   d.L = w.L;
   d.NT = 0;
+  d.T = 16;
   d.KB = 0;
   d.Lr = w.Lr.as<V>();
   d.Lc = w.Lc.as<V>();
@@ -950,6 +1144,10 @@ static InstDesc<typename SR::V> inst_of(InsideWork &w) {
   d.F = w.F.as<V>();
   d.acc = w.acc.as<V>();
   d.ptv = w.ptv.as<V>();
+  d.Lp = SR::scaled ? w.Lp.as<float>() : nullptr;
+  d.Sp = SR::scaled ? w.Sp.as<float>() : nullptr;
+  d.LpE = SR::scaled ? w.LpE.as<int>() : nullptr;
+  d.SpE = SR::scaled ? w.SpE.as<int>() : nullptr;
   d.u = w.u.as<double>();
   d.paired = w.paired.as<double>();
   d.lam = w.lam.as<double>();
@@ -997,6 +1195,7 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
   for (int m = 0; m < n; ++m) {
     d[m] = inst_of<SR>(*ws[m]);
     d[m].NT = (d[m].L + TE - 1) / TE;
+    d[m].T = TE;
     // tile (I, K) holds an L cell with h - j <= band iff (K - I - 1) T + 1 <= band
     const int band = ws[m]->band;
     d[m].KB = (band < 0 || band >= d[m].L) ? d[m].NT : (band == 0 ? 0 : (band - 1) / TE + 1);
@@ -1006,23 +1205,30 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
   // Work items of launch t: the tiles (I, I+t) and the ready split-point products
   // (I, K) x (K, J) with K - I = t - 1 or J - K = t - 1; a bulk tile (I, I+dist)
   // is scheduled only when one of its two products lies within the band.
-  std::vector<int> prefix((size_t)NTmax * (n + 1));
+  // per launch: finalising workgroups per instance (prefix, n+1 entries), then
+  // bulk workgroups per instance (prefix, n+1 entries)
+  const size_t stride = 2 * (size_t)(n + 1);
+  std::vector<int> prefix((size_t)NTmax * stride);
   std::vector<int> total(NTmax);
   for (int t = 0; t < NTmax; ++t) {
-    int acc = 0;
+    int fa = 0, ba = 0;
+    int *fp = prefix.data() + (size_t)t * stride, *bp = fp + (n + 1);
     for (int m = 0; m < n; ++m) {
-      prefix[(size_t)t * (n + 1) + m] = acc;
+      fp[m] = fa;
+      bp[m] = ba;
       const int NT = d[m].NT, KB = d[m].KB;
       if (t < NT) {
-        int items = NT - t;
+        int nbulk = 0;
         int dmax = std::min(2 * t - 2, NT - 1);
         if (t - 1 > KB) dmax = std::min(dmax, t - 1 + KB);
-        for (int dist = t + 1; dist <= dmax; ++dist) items += NT - dist;
-        acc += items;
+        for (int dist = t + 1; dist <= dmax; ++dist) nbulk += NT - dist;
+        fa += NT - t;
+        ba += (nbulk + BULK_PER - 1) / BULK_PER;
       }
     }
-    prefix[(size_t)t * (n + 1) + n] = acc;
-    total[t] = acc;
+    fp[n] = fa;
+    bp[n] = ba;
+    total[t] = ((fa + 7) & ~7) + ba;
   }
   upload(ib.inst, d, s);
   upload(ib.prefix, prefix, s);
@@ -1036,7 +1242,7 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
   hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, di, B, lp[1],
                      lp[5]);
   for (int t = 0; t < NTmax; ++t) {
-    const int *pf = ib.prefix.as<int>() + (size_t)t * (n + 1);
+    const int *pf = ib.prefix.as<int>() + (size_t)t * stride;
     if (TE == 32)
       hipLaunchKernelGGL((inside_step_kernel<SR, 32>), dim3(total[t]), dim3(32 * 32), 0, s, di, pf, n, t, r);
     else

@@ -42,6 +42,7 @@ struct LogSR {
   }
   static __device__ __forceinline__ V mulc(V a, C c) { return a == -HUGE_VAL ? a : a + c; }
   static __device__ __forceinline__ V mul_raw(V a, V b) { return a + b; }
+  static __device__ __forceinline__ V keep(V t, bool c) { return c ? t : -HUGE_VAL; }
   // natural-log value of a cell (for export); baseline added by the caller
   static __device__ __forceinline__ double to_log(V a) { return a; }
   static __device__ __forceinline__ V from_log(double x) { return x; }
@@ -163,6 +164,7 @@ struct XF {
   static __device__ __forceinline__ Acc acc(V v) { return v; }
   static __device__ __forceinline__ V acc_value(const Acc &a) { return norm(a.m, a.e); }
   static __device__ __forceinline__ V mul_raw(V a, V b) { return V{a.m * b.m, a.e + b.e}; }
+  static __device__ __forceinline__ V keep(V t, bool c) { return c ? t : V{0.0f, EZ}; }
   static __device__ __forceinline__ void acc_add(Acc &a, V t) {
     const int ne = max(a.e, t.e);
     a.m = ldexpf(a.m, max(a.e - ne, -200)) + ldexpf(t.m, max(t.e - ne, -200));
