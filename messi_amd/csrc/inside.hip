@@ -913,14 +913,21 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   V fS = SR::zero(), fF = SR::zero(), fL = SR::zero();
   for (int st = 0; st <= 2 * T - 2; ++st) {
     const bool act = mine && st >= 1 && st <= own, fn = mine && st == own;
-    // row producer (a, bs) and column producer (as, b), finalised at step st-1;
-    // read at clamped positions and masked (no divergent branch)
-    const int bs = T - st + a, as = st - T + b;
-    const int bsc = min(max(bs, 0), T - 1), asc = min(max(as, 0), T - 1);
-    const V t1 = SR::mul_raw(sm.u.w.W[a][bsc], sm.u.w.iSd[b][bsc]);
-    const V t2 = SR::mul_raw(sm.u.w.iLd[asc][a], sm.u.w.W[asc][b]);
-    SR::acc_add2(acL, SR::keep(t1, act && bs > b && bs < T), SR::zero());
-    SR::acc_add2(acS, SR::keep(t2, act && as >= 0 && as < a), SR::zero());
+    V w1 = SR::zero(), s1 = SR::zero(), l2 = SR::zero(), w2 = SR::zero();
+    if (act) {
+      const int bs = T - st + a;  // row producer (a, bs), finalised at step st-1
+      if (bs > b && bs < T) {
+        w1 = sm.u.w.W[a][bs];
+        s1 = sm.u.w.iSd[b][bs];
+      }
+      const int as = st - T + b;  // column producer (as, b), finalised at step st-1
+      if (as >= 0 && as < a) {
+        l2 = sm.u.w.iLd[as][a];
+        w2 = sm.u.w.W[as][b];
+      }
+      SR::acc_add2(acL, SR::mul_raw(w1, s1), SR::zero());
+      SR::acc_add2(acS, SR::mul_raw(l2, w2), SR::zero());
+    }
     if (fn) {
       fL = lkeep ? SR::acc_value(acL) : SR::zero();
       fS = SR::acc_value(acS);
