@@ -830,10 +830,12 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   V *__restrict__ Wc = static_cast<V *>(od.Wc);
   V *__restrict__ accL = static_cast<V *>(od.accL);
   V *__restrict__ accS = static_cast<V *>(od.accS);
-  // ---- decode: u+1 finalising tiles first, then the bulk tiles by distance
+  // ---- decode: u+1 finalising tiles first, then the bulk tiles by distance,
+  // in plain dispatch order (round-robin over the XCDs: the bulk tiles of one
+  // distance read disjoint operands, and grouping them per XCD measured slower)
   int I, d, m;
   bool fin;
-  const int item = xcd_chunk_id(blockIdx.x, gridDim.x);
+  const int item = (int)blockIdx.x;
   if (item <= u) {
     fin = true;
     I = item;
