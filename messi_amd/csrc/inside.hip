@@ -690,9 +690,19 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     const int own = b + T - 1 - a;
     // split point h = IT+T-1: S[(I+1)T, y] lies in the tile below.
     if (mine) SR::acc_add2(ac, SR::mul_raw(sm.Ld[a][T - 1], sm.Sb[b]), SR::zero());
-    // F[j+1, y-1] for the pair term: inside this tile, or on its bottom / left border.
-    const bool fin_in = a + 1 < T && b >= 1;
-    for (int st = 0; st <= 2 * T - 2; ++st) {
+    // F[j+1, y-1] for the pair term: inside this tile, or on the tile's bottom /
+    // left border.  The inner cell (a+1, b-1) is finalised two steps before this
+    // cell (its step is own - 2, and own >= 2 for it): every step reads it at a
+    // fixed, always valid position for use in the next step, so the read overlaps
+    // a whole step instead of delaying the finalisation.
+    const bool fin_in = a + 1 < T && b >= 1, fzero = j + 1 > y - 1;
+    const V fborder = a + 1 == T ? sm.Fbot[b] : sm.Fleft[a];
+    const V pterm = sm.cP[a][b];
+    const int fa = min(a + 1, T - 1), fb = max(b - 1, 0);
+    V fin_next = SR::zero();
+    auto step = [&](int st) {
+      const V fin_f = fin_next;
+      fin_next = sm.u.w.cF[fa][fb];
       const bool act = mine && st >= 1 && st <= own;
       // column producer (ap, b) and row producer (a, bp), finalised at step st-1;
       // both terms are read at clamped (always valid) positions and masked, so
@@ -703,15 +713,23 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
       const V t2 = SR::mul_raw(sm.u.w.cL[a][bpc], sm.Sd[bpc + 1][b]);
       SR::acc_add2(ac, SR::keep(t1, act && ap > a && ap < T), SR::keep(t2, act && bp >= 0 && bp < b));
       if (mine && st == own) {
-        V f = fin_in ? sm.u.w.cF[a + 1][b - 1] : (a + 1 == T ? sm.Fbot[b] : sm.Fleft[a]);
-        if (j + 1 > y - 1) f = SR::zero();
+        V f = fin_in ? fin_f : fborder;
+        if (fzero) f = SR::zero();
         V vs, vf, vl;
-        SR::finalize(ac, SR::mul_raw(sm.cP[a][b], f), rk, vs, vf, vl);
+        SR::finalize(ac, SR::mul_raw(pterm, f), rk, vs, vf, vl);
         sm.u.w.cS[a][b] = vs;
         sm.u.w.cF[a][b] = vf;
         sm.u.w.cL[a][b] = vl;
       }
       __syncthreads();
+    };
+    // The FP64 step (exponentials and logarithms) stays a rolled loop: unrolled,
+    // the kernel outgrows the instruction cache.
+    if constexpr (SR::scaled) {
+      for (int st = 0; st <= 2 * T - 2; ++st) step(st);
+    } else {
+#pragma unroll 1
+      for (int st = 0; st <= 2 * T - 2; ++st) step(st);
     }
   }
   // ---- write the tile back (row-major copies with b fastest, column-major with a fastest)
@@ -913,7 +931,18 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   const bool lkeep = out_row_ok(d, T, od.lband);
   const int own = (T - 1 - b) + a;
   V fS = SR::zero(), fF = SR::zero(), fL = SR::zero();
-  for (int st = 0; st <= 2 * T - 2; ++st) {
+  // Q[j-1, y+1] for the pair term: inside this tile, or on the row above / the
+  // column right of it.  The inner cell (a-1, b+1) is finalised two steps before
+  // this cell (own - 2; own >= 2 for it): every step reads it at a fixed, always
+  // valid position for use in the next step, off the finalisation's path.
+  const bool q_in = a >= 1 && b + 1 < T;
+  const V qborder = a == 0 ? sm.Qtop[b] : sm.Qright[a];
+  const V pterm = sm.u.w.cP[a][b];
+  const int qa = max(a - 1, 0), qb = min(b + 1, T - 1);
+  V q_next = SR::zero();
+  auto step = [&](int st) {
+    const V q_f = q_next;
+    q_next = sm.u.w.Q[qa][qb];
     const bool act = mine && st >= 1 && st <= own, fn = mine && st == own;
     V w1 = SR::zero(), s1 = SR::zero(), l2 = SR::zero(), w2 = SR::zero();
     if (act) {
@@ -933,13 +962,19 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
     if (fn) {
       fL = lkeep ? SR::acc_value(acL) : SR::zero();
       fS = SR::acc_value(acS);
-      V q = a >= 1 && b + 1 < T ? sm.u.w.Q[a - 1][b + 1] : (a == 0 ? sm.Qtop[b] : sm.Qright[a]);
-      fF = SR::mul(sm.u.w.cP[a][b], q);
+      fF = SR::mul(pterm, q_in ? q_f : qborder);
       if (j == 0 && y == L - 1) fS = SR::from_log(0.0);  // the start symbol
       sm.u.w.W[a][b] = SR::add(SR::mulc(fS, r.p[0]), SR::mulc(fF, r.p[4]));
       sm.u.w.Q[a][b] = SR::add(SR::add(SR::mulc(fS, r.p[2]), SR::mulc(fF, r.p[3])), SR::mulc(fL, r.p[6]));
     }
     __syncthreads();
+  };
+  // as in the inside step: the FP64 step stays a rolled loop
+  if constexpr (SR::scaled) {
+    for (int st = 0; st <= 2 * T - 2; ++st) step(st);
+  } else {
+#pragma unroll 1
+    for (int st = 0; st <= 2 * T - 2; ++st) step(st);
   }
   // ---- write back: row-major from registers, column-major through LDS
   if (mine) {

@@ -186,8 +186,10 @@ struct XF {
     // pf is a raw product; bif an unnormalised sum.
     const int ne = max(bif.e, pf.e);
     const float xb = ldexpf(bif.m, max(bif.e - ne, -200)), xp = ldexpf(pf.m, max(pf.e - ne, -200));
-    S = norm(r.p0 * xb + r.p2 * xp, ne);
-    F = norm(r.p4 * xb + r.p3 * xp, ne);
+    // explicit fused forms: the rounding does not depend on how the compiler
+    // contracts the sums at each inlined call site
+    S = norm(fmaf(r.p0, xb, r.p2 * xp), ne);
+    F = norm(fmaf(r.p4, xb, r.p3 * xp), ne);
     Lv = norm(r.p6 * pf.m, pf.e);
   }
 };
