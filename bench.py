@@ -474,6 +474,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--groups", type=int, default=0,
+                    help="member groups of the pipelined batch evaluation (1..4, 0 = automatic)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--instances", action="store_true")
     ap.add_argument("--mcmc", action="store_true")
@@ -532,6 +534,7 @@ def main():
     comps = [_components(p) for p in ps]
     sites, pairs = [c[0] for c in comps], [c[1] for c in comps]
     bB = lib.Batch([dd] * args.batch)
+    bB.set_groups(args.groups)
     b1 = lib.Batch([dd])
     res = {}
     for prec in args.precs:
@@ -556,7 +559,8 @@ def main():
         "config": {"workload": f"config2: 50 sequences x 1500 nt, K=1, full-span pairs and inside; "
                                f"{args.batch} parameter vectors per step (one per chain)",
                    "L": L, "nseq": ds.numseqs, "batch": args.batch, "sum_U": stats["sum_U"],
-                   "root_entries": stats["root_entries"]},
+                   "root_entries": stats["root_entries"],
+                   "member_groups": args.groups or "automatic"},
         "variants": res,
     }
     if lead == "fp32":

@@ -229,6 +229,14 @@ int messi_batch_eval_loaded(messi_batch *b, double B, int prec, double *logZ);
  * sequence.  Set by the autotuner (messi_amd/autotune.py).  No counterpart in
  * the reference. */
 int messi_batch_set_tiles(messi_batch *b, const int *tiles);
+/* Member groups of an evaluation (1 .. 4; 0, the default, picks 2 groups for 4
+ * or more members, else 1).  Each group runs its pair phase and then its inside
+ * on a stream of its own, the pair phase of a group after that of the group
+ * before, so the bandwidth-bound pruning of one group overlaps the
+ * latency-bound inside launches of the others.  Results do not depend on the
+ * grouping.  With profiling on, an evaluation runs as one group.  No
+ * counterpart in the reference. */
+int messi_batch_set_groups(messi_batch *b, int groups);
 
 /* Learned MCMC proposal network: din inputs -> tanh(dh) -> tanh(dh) -> 2*din
  * outputs (mean shift mu, then log-scale ell, of a diagonal Gaussian proposal

@@ -63,6 +63,29 @@ struct Events {
   }
 };
 
+// Ordering events of a pipelined batch evaluation: per member group the end
+// of its pair phase and its end.
+struct PipeEvents {
+  static constexpr int N = messi_ctx::NPIPE + 1;
+  hipEvent_t pair[N] = {}, done[N] = {};
+  bool made = false;
+  void make() {
+    if (made) return;
+    for (int g = 0; g < N; ++g) {
+      MESSI_HIP(hipEventCreateWithFlags(&pair[g], hipEventDisableTiming));
+      MESSI_HIP(hipEventCreateWithFlags(&done[g], hipEventDisableTiming));
+    }
+    made = true;
+  }
+  ~PipeEvents() {
+    if (!made) return;
+    for (int g = 0; g < N; ++g) {
+      (void)hipEventDestroy(pair[g]);
+      (void)hipEventDestroy(done[g]);
+    }
+  }
+};
+
 }  // namespace
 
 struct messi_dataset {
@@ -126,6 +149,7 @@ struct messi_batch {
   const uint16_t *pair_leaf = nullptr;
   DevBuf site_nodes, pair_nodes, site_items, pair_items;
   std::vector<int> site_lv, pair_lv;
+  std::vector<int> pair_lvm;  // first pair item of member m at level h: [(h - 1) * (n + 1) + m]
   std::vector<int64_t> node_base, site_root, pair_root;
   int64_t nn_total = 0, site_entries = 0, pair_entries = 0, site_root_total = 0, pair_root_total = 0;
   int Lmax = 0;
@@ -135,6 +159,9 @@ struct messi_batch {
   std::vector<ExpandDesc> hexp_site, hexp_pair;  // what dexp_site / dexp_pair hold
   std::vector<std::unique_ptr<InsideWork>> w64, w32;
   InsideBatch ib, ib32;     // launch tables of the tile-16 and tile-32 groups
+  int groups = 0;           // member groups of a pipelined evaluation (0: automatic)
+  InsideBatch pib[messi_ctx::NPIPE + 1], pib32[messi_ctx::NPIPE + 1];  // their launch tables
+  PipeEvents pev;
   std::vector<int> tiles;   // per-member inside tile edge (0: the context's)
   DevBuf zall;              // gathered log Z of all members (2 per member)
   Events ev;
@@ -487,6 +514,7 @@ void build_batch(messi_batch *b) {
   std::vector<int2> si, pi;
   b->site_lv.assign(b->Hmax + 1, 0);
   b->pair_lv.assign(b->Hmax + 1, 0);
+  b->pair_lvm.assign((size_t)b->Hmax * (n + 1), 0);
   for (int m = 0; m < n; ++m) {
     messi_dataset *d = b->mem[m];
     const int di = distinct[d];
@@ -514,11 +542,13 @@ void build_batch(messi_batch *b) {
     b->site_lv[h - 1] = (int)si.size();
     b->pair_lv[h - 1] = (int)pi.size();
     for (int m = 0; m < n; ++m) {
+      b->pair_lvm[(size_t)(h - 1) * (n + 1) + m] = (int)pi.size();
       messi_dataset *d = b->mem[m];
       if (h > d->H) continue;
       for (int2 it : d->site_items[h - 1]) si.push_back(make_int2(it.x + (int)b->node_base[m], it.y));
       for (int2 it : d->pair_items[h - 1]) pi.push_back(make_int2(it.x + (int)b->node_base[m], it.y));
     }
+    b->pair_lvm[(size_t)(h - 1) * (n + 1) + n] = (int)pi.size();
   }
   b->site_lv[b->Hmax] = (int)si.size();
   b->pair_lv[b->Hmax] = (int)pi.size();
@@ -644,10 +674,28 @@ void run_site(messi_batch *b, int prec) {
   MESSI_HIP(hipGetLastError());
 }
 
+// The level launches of pair component k for members m0 .. m1-1 on stream s
+// (the items of a level are ordered by member).
+template <typename T>
+void pair_levels(messi_batch *b, hipStream_t s, int k, int m0, int m1) {
+  const int n = b->n;
+  for (int h = 1; h <= b->Hmax; ++h) {
+    const int lo = b->pair_lvm[(size_t)(h - 1) * (n + 1) + m0], hi = b->pair_lvm[(size_t)(h - 1) * (n + 1) + m1];
+    launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
+                                  b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), b->pair_expo.as<int>(),
+                                  b->c16.P.as<T>() + (size_t)k * b->nn_total * 256,
+                                  b->c16.pi.as<double>() + (size_t)k * b->n * 16,
+                                  b->c16.logw.as<double>() + (size_t)k * b->n, k == 0, b->pair_acc.as<double>(),
+                                  b->pair_comp.as<double>());
+  }
+}
+
 template <typename T>
 // expand: write each member's L x L pair matrix (the evaluation path instead
-// builds the inside pair term straight from the root entries).
-void run_pair(messi_batch *b, int prec, double *h_percomp, bool expand) {
+// builds the inside pair term straight from the root entries).  pair_only:
+// only size the buffers and upload the expansion descriptors (the launches are
+// made per member group by the pipelined evaluation).
+void run_pair(messi_batch *b, int prec, double *h_percomp, bool expand, bool pair_only = false) {
   hipStream_t s = b->ctx->stream;
   require_loaded(b->c16, prec, "pair");
   const int K = b->c16.K;
@@ -659,16 +707,9 @@ void run_pair(messi_batch *b, int prec, double *h_percomp, bool expand) {
     for (int m = 0; m < b->n; ++m)
       member_work(b, m, prec).paired.ensure(sizeof(double) * b->mem[m]->L * b->mem[m]->L);
   const ExpandDesc *dexp = upload_expand(b, prec, true);
+  if (pair_only) return;
   for (int k = 0; k < K; ++k) {
-    for (int h = 1; h <= b->Hmax; ++h) {
-      int lo = b->pair_lv[h - 1], hi = b->pair_lv[h];
-      launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
-                                    b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), b->pair_expo.as<int>(),
-                                    b->c16.P.as<T>() + (size_t)k * b->nn_total * 256,
-                                    b->c16.pi.as<double>() + (size_t)k * b->n * 16,
-                                    b->c16.logw.as<double>() + (size_t)k * b->n, k == 0, b->pair_acc.as<double>(),
-                                    b->pair_comp.as<double>());
-    }
+    pair_levels<T>(b, s, k, 0, b->n);
     if (h_percomp) {
       MESSI_REQUIRE(b->n == 1, "per-component output needs a single member");
       const int L = b->mem[0]->L;
@@ -738,21 +779,49 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
     inside_alloc<T>(*ws[m], b->mem[m]->L, keep_outside);
   }
   if (ctx->profiling) b->ev.make();
+  // Inside launch sequences of members m0 .. m1-1 on stream st, one per tile edge.
+  auto insides = [&](hipStream_t st, int m0, int m1, InsideBatch &i16, InsideBatch &i32) {
+    for (int TE : {16, 32}) {
+      std::vector<InsideWork *> g;
+      for (int m = m0; m < m1; ++m)
+        if (ws[m]->tile == TE) {
+          ws[m]->exp = b->dexp_pair.as<ExpandDesc>() + m;
+          g.push_back(ws[m]);
+        }
+      if (!g.empty())
+        inside_batch_run<T>(st, g.data(), (int)g.size(), TE == 32 ? i32 : i16, B, b->pair_acc.as<double>());
+    }
+  };
+  // Member groups: with G > 1 each group runs its pair phase and then its
+  // inside on a stream of its own, the pair phase of group g after that of
+  // group g-1, so the (bandwidth-bound) pruning of one group overlaps the
+  // (latency-bound) inside launches of the groups before it.  Profiling runs
+  // the phases one after another on the context's stream.
+  int G = b->groups > 0 ? b->groups : (b->n >= 4 ? 2 : 1);
+  G = std::min(std::min(G, messi_ctx::NPIPE + 1), b->n);
+  if (ctx->profiling) G = 1;
   record(b, 0);
   run_site<T>(b, prec);
   record(b, 1);
-  run_pair<T>(b, prec, nullptr, false);
-  record(b, 2);
-  // one launch sequence per tile edge, each over its members
-  for (int TE : {16, 32}) {
-    std::vector<InsideWork *> g;
-    for (int m = 0; m < b->n; ++m)
-      if (ws[m]->tile == TE) {
-        ws[m]->exp = b->dexp_pair.as<ExpandDesc>() + m;
-        g.push_back(ws[m]);
-      }
-    if (!g.empty())
-      inside_batch_run<T>(s, g.data(), (int)g.size(), TE == 32 ? b->ib32 : b->ib, B, b->pair_acc.as<double>());
+  if (G == 1) {
+    run_pair<T>(b, prec, nullptr, false);
+    record(b, 2);
+    insides(s, 0, b->n, b->ib, b->ib32);
+  } else {
+    run_pair<T>(b, prec, nullptr, false, true);
+    b->pev.make();
+    // group 0 runs on the context's stream, group g >= 1 on pipe[g - 1]: at
+    // most 4 streams, one per hardware queue
+    for (int g = 0; g < G; ++g) {
+      const int m0 = g * b->n / G, m1 = (g + 1) * b->n / G;
+      hipStream_t sg = g == 0 ? s : ctx->pipe[g - 1];
+      if (g > 0) MESSI_HIP(hipStreamWaitEvent(sg, b->pev.pair[g - 1], 0));
+      for (int k = 0; k < b->c16.K; ++k) pair_levels<T>(b, sg, k, m0, m1);
+      MESSI_HIP(hipEventRecord(b->pev.pair[g], sg));
+      insides(sg, m0, m1, b->pib[g], b->pib32[g]);
+      if (g > 0) MESSI_HIP(hipEventRecord(b->pev.done[g], sg));
+    }
+    for (int g = 1; g < G; ++g) MESSI_HIP(hipStreamWaitEvent(s, b->pev.done[g], 0));
   }
   for (int m = 0; m < b->n; ++m) {
     ws[m]->exp = nullptr;
@@ -827,6 +896,7 @@ int messi_ctx_create(int device, messi_ctx **out) {
     MESSI_HIP(hipSetDevice(device));
     ctx->device = device;
     MESSI_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (auto &p : ctx->pipe) MESSI_HIP(hipStreamCreateWithFlags(&p, hipStreamNonBlocking));
   });
   *out = ctx;
   return rc;
@@ -836,11 +906,15 @@ void messi_ctx_destroy(messi_ctx *ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto p : ctx->pipe)
+    if (p) (void)hipStreamSynchronize(p);
   ctx->w64.~InsideWork();
   new (&ctx->w64) InsideWork();
   ctx->w32.~InsideWork();
   new (&ctx->w32) InsideWork();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  for (auto p : ctx->pipe)
+    if (p) (void)hipStreamDestroy(p);
   delete ctx;
 }
 
@@ -1354,6 +1428,8 @@ void messi_batch_destroy(messi_batch *b) {
   if (!b) return;
   (void)hipSetDevice(b->ctx->device);
   (void)hipStreamSynchronize(b->ctx->stream);
+  for (auto p : b->ctx->pipe)
+    if (p) (void)hipStreamSynchronize(p);
   delete b;
 }
 
@@ -1414,6 +1490,14 @@ int messi_batch_set_tiles(messi_batch *b, const int *tiles) {
     for (int m = 0; m < b->n; ++m)
       MESSI_REQUIRE(tiles[m] == 0 || tiles[m] == 16 || tiles[m] == 32, "tile must be 0, 16 or 32");
     b->tiles.assign(tiles, tiles + b->n);
+  });
+}
+
+int messi_batch_set_groups(messi_batch *b, int groups) {
+  if (!b) return 2;
+  return guard(b->ctx, [&] {
+    MESSI_REQUIRE(groups >= 0 && groups <= messi_ctx::NPIPE + 1, "groups must be 0 (automatic) .. 4");
+    b->groups = groups;
   });
 }
 

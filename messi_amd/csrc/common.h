@@ -81,6 +81,7 @@ inline void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
 struct InsideBatch {
   DevBuf inst;    // instance descriptors
   DevBuf prefix;  // per launch t: first workgroup of each instance (n+1 ints)
+  std::vector<unsigned char> h_inst, h_prefix;  // what inst and prefix hold (uploads only on change)
 };
 
 // Scratch of the inside/outside/bpp phase, per precision.
@@ -129,6 +130,8 @@ struct SampleWork {
 struct messi_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  static constexpr int NPIPE = 3;
+  hipStream_t pipe[NPIPE] = {};  // streams of member groups 1.. of a pipelined batch evaluation
   std::string err;
   bool profiling = false;
   int tile = 16;  // tile edge of the inside schedule

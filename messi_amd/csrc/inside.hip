@@ -23,6 +23,7 @@
 // in a workgroup of its own.  A launch therefore carries the cell wavefronts
 // of one tile-diagonal plus the split-point products that just became ready.
 #include <algorithm>
+#include <cstring>
 
 #include "common.h"
 #include "prune.h"
@@ -1267,8 +1268,15 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
     bp[n] = ba;
     total[t] = ((fa + 7) & ~7) + ba;
   }
-  upload(ib.inst, d, s);
-  upload(ib.prefix, prefix, s);
+  // the tables change only when a member's buffers move or its shape changes
+  auto refresh = [&](DevBuf &dev, std::vector<unsigned char> &host, const void *p, size_t bytes) {
+    if (host.size() == bytes && (bytes == 0 || std::memcmp(host.data(), p, bytes) == 0)) return;
+    host.assign((const unsigned char *)p, (const unsigned char *)p + bytes);
+    dev.ensure(bytes);
+    if (bytes) MESSI_HIP(hipMemcpyAsync(dev.p, host.data(), bytes, hipMemcpyHostToDevice, s));
+  };
+  refresh(ib.inst, ib.h_inst, d.data(), d.size() * sizeof(InstDesc<V>));
+  refresh(ib.prefix, ib.h_prefix, prefix.data(), prefix.size() * sizeof(int));
   const InstDesc<V> *di = ib.inst.as<InstDesc<V>>();
   double lp[7];
   kh99_logprobs(lp);

@@ -81,6 +81,7 @@ SIGNATURES = {
     "messi_batch_load": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
     "messi_batch_eval_loaded": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp]),
     "messi_batch_set_tiles": (ctypes.c_int, [_vp, _ip]),
+    "messi_batch_set_groups": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_mlp_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "messi_mlp_destroy": (None, [_vp]),
     "messi_mlp_num_params": (ctypes.c_int, [_vp, _i64]),
@@ -589,6 +590,10 @@ class Batch:
         if t.shape != (len(self.members),):
             raise ValueError(f"expected {len(self.members)} tiles, got shape {t.shape}")
         self.ctx.check(self.ctx.lib.messi_batch_set_tiles(self._h, ptr(t, _ip)))
+
+    def set_groups(self, groups: int) -> None:
+        """Member groups of a pipelined evaluation (1..4; 0 = automatic)."""
+        self.ctx.check(self.ctx.lib.messi_batch_set_groups(self._h, int(groups)))
 
     def eval_loaded(self, B: float = 1.0, prec=FP64) -> np.ndarray:
         """log Z of every member from the components of the last load."""
