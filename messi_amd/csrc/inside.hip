@@ -727,6 +727,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     // The FP64 step (exponentials and logarithms) stays a rolled loop: unrolled,
     // the kernel outgrows the instruction cache.
     if constexpr (SR::scaled) {
+#pragma unroll
       for (int st = 0; st <= 2 * T - 2; ++st) step(st);
     } else {
 #pragma unroll 1
