@@ -444,7 +444,8 @@ def main_mcmc(args):
     res = mcmc.run_experiment(post, th0, net, rng, burn=args.burn, steps=args.mcmc_steps,
                               long_steps=args.long_steps, train_steps=args.train_steps, log=log,
                               mode_logp=lambda th: post(th, prec="fp64"), ind_prob=args.ind_prob,
-                              mix_components=args.mix_components, mix_flatten=args.mix_flatten)
+                              mix_components=args.mix_components, mix_flatten=args.mix_flatten,
+                              mix_inflate=args.mix_inflate)
     tr = res.pop("traces")
     if args.save_traces:
         np.savez_compressed(args.save_traces, learned=tr["learned"].samples, learned_kind=tr["learned"].kind,
@@ -496,6 +497,7 @@ def main():
     ap.add_argument("--save-traces", default="", help="write the chains to this .npz (MCMC mode)")
     ap.add_argument("--mix-components", type=int, default=6)
     ap.add_argument("--mix-flatten", type=float, default=0.5)
+    ap.add_argument("--mix-inflate", type=float, default=1.3)
     ap.add_argument("--long-steps", type=int, default=30000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--prec", default="fp32")

@@ -693,17 +693,19 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     if (mine) SR::acc_add2(ac, SR::mul_raw(sm.Ld[a][T - 1], sm.Sb[b]), SR::zero());
     // F[j+1, y-1] for the pair term: inside this tile, or on the tile's bottom /
     // left border.  The inner cell (a+1, b-1) is finalised two steps before this
-    // cell (its step is own - 2, and own >= 2 for it): every step reads it at a
-    // fixed, always valid position for use in the next step, so the read overlaps
-    // a whole step instead of delaying the finalisation.
-    const bool fin_in = a + 1 < T && b >= 1, fzero = j + 1 > y - 1;
-    const V fborder = a + 1 == T ? sm.Fbot[b] : sm.Fleft[a];
-    const V pterm = sm.cP[a][b];
-    const int fa = min(a + 1, T - 1), fb = max(b - 1, 0);
-    V fin_next = SR::zero();
+    // cell (its step is own - 2, and own >= 2 for it).  Every step reads the
+    // lane's F cell (inner or border, a fixed LDS position) for use in the next
+    // step, so the read overlaps a whole step instead of delaying the
+    // finalisation; with no inner pair (j + 1 > y - 1) the pair term is zero.
+    const bool fin_in = a + 1 < T && b >= 1;
+    const V *fsrc = fin_in ? &sm.u.w.cF[a + 1][b - 1] : (a + 1 == T ? &sm.Fbot[b] : &sm.Fleft[a]);
+    const V pterm = j + 1 > y - 1 ? SR::zero() : sm.cP[a][b];
+    // primed before step 0: a border cell may finalise at step 0 (own = 0) or 1;
+    // an inner cell's first reads are stale but unused (own >= 2)
+    V fin_next = *fsrc;
     auto step = [&](int st) {
       const V fin_f = fin_next;
-      fin_next = sm.u.w.cF[fa][fb];
+      fin_next = *fsrc;
       const bool act = mine && st >= 1 && st <= own;
       // column producer (ap, b) and row producer (a, bp), finalised at step st-1;
       // both terms are read at clamped (always valid) positions and masked, so
@@ -714,10 +716,8 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
       const V t2 = SR::mul_raw(sm.u.w.cL[a][bpc], sm.Sd[bpc + 1][b]);
       SR::acc_add2(ac, SR::keep(t1, act && ap > a && ap < T), SR::keep(t2, act && bp >= 0 && bp < b));
       if (mine && st == own) {
-        V f = fin_in ? fin_f : fborder;
-        if (fzero) f = SR::zero();
         V vs, vf, vl;
-        SR::finalize(ac, SR::mul_raw(pterm, f), rk, vs, vf, vl);
+        SR::finalize(ac, SR::mul_raw(pterm, fin_f), rk, vs, vf, vl);
         sm.u.w.cS[a][b] = vs;
         sm.u.w.cF[a][b] = vf;
         sm.u.w.cL[a][b] = vl;
@@ -935,16 +935,16 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   V fS = SR::zero(), fF = SR::zero(), fL = SR::zero();
   // Q[j-1, y+1] for the pair term: inside this tile, or on the row above / the
   // column right of it.  The inner cell (a-1, b+1) is finalised two steps before
-  // this cell (own - 2; own >= 2 for it): every step reads it at a fixed, always
-  // valid position for use in the next step, off the finalisation's path.
+  // this cell (own - 2; own >= 2 for it): every step reads the lane's Q cell
+  // (inner or border, a fixed LDS position) for use in the next step, off the
+  // finalisation's path.
   const bool q_in = a >= 1 && b + 1 < T;
-  const V qborder = a == 0 ? sm.Qtop[b] : sm.Qright[a];
+  const V *qsrc = q_in ? &sm.u.w.Q[a - 1][b + 1] : (a == 0 ? &sm.Qtop[b] : &sm.Qright[a]);
   const V pterm = sm.u.w.cP[a][b];
-  const int qa = max(a - 1, 0), qb = min(b + 1, T - 1);
-  V q_next = SR::zero();
+  V q_next = *qsrc;  // primed: a border cell may finalise at step 0 or 1 (see the inside step)
   auto step = [&](int st) {
     const V q_f = q_next;
-    q_next = sm.u.w.Q[qa][qb];
+    q_next = *qsrc;
     const bool act = mine && st >= 1 && st <= own, fn = mine && st == own;
     V w1 = SR::zero(), s1 = SR::zero(), l2 = SR::zero(), w2 = SR::zero();
     if (act) {
@@ -964,7 +964,7 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
     if (fn) {
       fL = lkeep ? SR::acc_value(acL) : SR::zero();
       fS = SR::acc_value(acS);
-      fF = SR::mul(pterm, q_in ? q_f : qborder);
+      fF = SR::mul(pterm, q_f);
       if (j == 0 && y == L - 1) fS = SR::from_log(0.0);  // the start symbol
       sm.u.w.W[a][b] = SR::add(SR::mulc(fS, r.p[0]), SR::mulc(fF, r.p[4]));
       sm.u.w.Q[a][b] = SR::add(SR::add(SR::mulc(fS, r.p[2]), SR::mulc(fF, r.p[3])), SR::mulc(fL, r.p[6]));

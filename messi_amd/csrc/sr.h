@@ -176,16 +176,18 @@ struct XF {
   static __device__ __forceinline__ FinRules fin_rules(float, float p0, float p2, float p3, float p4, float p6) {
     return FinRules{p0, p2, p3, p4, p6};
   }
+  // In acc_add2 and finalize the exponent differences are <= 0 and the mantissas
+  // are far below 2^49, so a difference below -200 already scales to zero:
+  // ldexp takes it unclamped (any int32) with the same result.
   static __device__ __forceinline__ void acc_add2(Acc &a, V t1, V t2) {
     const int ne = max(a.e, max(t1.e, t2.e));
-    a.m = ldexpf(a.m, max(a.e - ne, -200)) + ldexpf(t1.m, max(t1.e - ne, -200)) +
-          ldexpf(t2.m, max(t2.e - ne, -200));
+    a.m = ldexpf(a.m, a.e - ne) + ldexpf(t1.m, t1.e - ne) + ldexpf(t2.m, t2.e - ne);
     a.e = ne;
   }
   static __device__ __forceinline__ void finalize(const Acc &bif, V pf, const FinRules &r, V &S, V &F, V &Lv) {
     // pf is a raw product; bif an unnormalised sum.
     const int ne = max(bif.e, pf.e);
-    const float xb = ldexpf(bif.m, max(bif.e - ne, -200)), xp = ldexpf(pf.m, max(pf.e - ne, -200));
+    const float xb = ldexpf(bif.m, bif.e - ne), xp = ldexpf(pf.m, pf.e - ne);
     // explicit fused forms: the rounding does not depend on how the compiler
     // contracts the sums at each inlined call site
     S = norm(fmaf(r.p0, xb, r.p2 * xp), ne);
