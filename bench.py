@@ -492,7 +492,7 @@ def main():
     ap.add_argument("--genome-spans", type=int, nargs="+", default=[1000000, 500])
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--burn", type=int, default=2000)
-    ap.add_argument("--mcmc-steps", type=int, default=2000)
+    ap.add_argument("--mcmc-steps", type=int, default=3000)
     ap.add_argument("--ind-prob", type=float, default=0.25)
     ap.add_argument("--save-traces", default="", help="write the chains to this .npz (MCMC mode)")
     ap.add_argument("--mix-components", type=int, default=6)
