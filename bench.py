@@ -395,6 +395,19 @@ def main_structure(args):
             ll, parts = dd.structure_loglik(site, pair, paired, prec)
         ms = (time.perf_counter() - t0) * 1e3 / args.steps
         res[prec] = {"ms_per_eval": ms, "evals_per_s": 1e3 / ms, "loglik": ll, "parts": parts}
+    # state sampling given the structure (samplestates), 4 site categories: one step =
+    # per-category site pruning, per-component pair pruning of the structure's pairs,
+    # one draw per unpaired column and per pair
+    p4 = model.getparams(a.params, ds.tree.branchlengths, siteCats=4)
+    site4, pair4 = _components(p4)
+    for prec in ("fp64", "fp32"):
+        for _ in range(args.warmup):
+            dd.sample_states(site4, pair4, paired, 4, seed=0, prec=prec)
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            dd.sample_states(site4, pair4, paired, 4, seed=k, prec=prec)
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        res[prec]["states_4_categories"] = {"ms_per_draw": ms, "draws_per_s": 1e3 / ms}
     out = {"metric": "fixed-structure log-likelihood evals/s at 1500nt x 50seq",
            "value": res["fp32"]["evals_per_s"], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res["fp32"]["ms_per_eval"], "higher_is_better": True,
@@ -419,6 +432,18 @@ def main_structure(args):
         out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "evals/s", "cores": 1, "kind": "port",
                                "sample": f"one full oracle evaluation (all sites, the {len(x)} structure pairs)",
                                "loglik": ll}
+        # the oracle's state draw with 4 categories: per-component site and pair
+        # log-likelihoods, then the draws
+        t0 = time.perf_counter()
+        sitec = np.stack([oracle.site_loglik(ds, [c]) for c in model.site_components(p4)])
+        pc4 = model.pair_components(p4)
+        pairc = np.full((len(pc4), ds.numcols), -np.inf)
+        for k, (P, pi, _) in enumerate(pc4):
+            Mk = oracle.pair_loglik(ds, [(P, pi, 0.0)], mask=mask)
+            pairc[k, x] = Mk[x, paired[x] - 1]
+        oracle.sample_states(sitec, pairc, [c[2] for c in pc4], paired, 4, 1.0, seed=0)
+        out["cpu_baseline_states"] = {"value": 1.0 / (time.perf_counter() - t0), "unit": "draws/s", "cores": 1,
+                                      "kind": "port", "sample": "one full state draw, 4 site categories"}
     print(json.dumps(out), flush=True)
 
 

@@ -167,6 +167,25 @@ int messi_sample_posterior(messi_dataset *ds, int K4, const double *P4, const do
 int messi_structure_loglik(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
                            int K16, const double *P16, const double *pi16, const double *logw16,
                            const int *paired, int prec, double *loglik, double *parts);
+/* State sampling given a structure: samplestates (src/FARCE.jl:2534-2538), i.e.
+ * the sampling branch of computeuncachedlikelihood (src/FARCE.jl:1336-1486).  The
+ * site components are the cats rate categories (K4 == cats); the pair components
+ * are ordered category 1, category 2, then K16 / cats^2 lambda classes, as
+ * felsensteincuda orders them.  Every unpaired column draws a category with
+ * probability proportional to (w_c L_c / sum)^B; the opener of each pair draws the
+ * category pair from the pair conditionals (summed over lambda classes),
+ * normalised and raised to B.  Outputs (L each): states (1-based categories),
+ * unpaired (the column's mixture log-likelihood, 0 for paired columns), pairedll
+ * (the pair's mixture log-likelihood at its opener, -inf elsewhere).  Optional:
+ * siteconds (K4 x L: log w_c + log L_c) and paircomps (K16 x L: each pair
+ * component's log-likelihood at the openers, -inf elsewhere).  The reference's
+ * rng becomes seed: the draw of column x uses a uniform keyed by (seed, x), as
+ * messi_sample_structures keys its intervals.  Every pair must be in the
+ * dataset's pair list (distance bound and mask); 1 <= cats <= 8. */
+int messi_sample_states(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4, int K16,
+                        const double *P16, const double *pi16, const double *logw16, int cats, const int *paired,
+                        double B, uint64_t seed, int prec, int *states, double *unpaired, double *pairedll,
+                        double *siteconds, double *paircomps);
 
 /* ---- Input formats (host only, no device context) ----
  * Alignment handle from a FASTA file and the first line of a Newick file

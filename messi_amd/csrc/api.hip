@@ -102,6 +102,7 @@ struct messi_dataset {
   DevBuf d_site_idx, d_site_leaf, d_pair_idx, d_pair_leaf;
   DevBuf d_pid0, d_T0, d_mask;
   DevBuf d_partner, d_sums;     // fixed-structure likelihood: the structure and its sums
+  DevBuf d_sitec, d_pairc, d_sout, d_miss;  // state sampling: conditionals, outputs, missing pairs
   bool has_mask = false;
   messi_batch *self = nullptr;  // batch of one, created on first use
   ~messi_dataset();
@@ -649,6 +650,19 @@ const ExpandDesc *upload_expand(messi_batch *b, int prec, bool pairs) {
     h = e;
   }
   return d.as<ExpandDesc>();
+}
+
+// The site level launches of component k alone: with first set, the root entries
+// hold log w_k + the component's log-likelihood.
+template <typename T>
+void site_levels(messi_batch *b, hipStream_t s, int k) {
+  for (int h = 1; h <= b->Hmax; ++h) {
+    const int lo = b->site_lv[h - 1], hi = b->site_lv[h];
+    launch_site_level<T>(s, b->site_items.as<int2>() + lo, hi - lo, b->site_nodes.as<NodeDesc>(), b->site_idx,
+                         b->site_leaf, b->site_part.as<T>(), b->site_expo.as<int>(),
+                         b->c4.P.as<T>() + (size_t)k * b->nn_total * 16, b->c4.pi.as<double>() + (size_t)k * b->n * 4,
+                         b->c4.logw.as<double>() + (size_t)k * b->n, 1, b->site_acc.as<double>());
+  }
 }
 
 template <typename T>
@@ -1323,6 +1337,76 @@ int messi_structure_loglik(messi_dataset *ds, int K4, const double *P4, const do
       parts[0] = h[0];
       parts[1] = h[1];
     }
+  });
+}
+
+int messi_sample_states(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4, int K16,
+                        const double *P16, const double *pi16, const double *logw16, int cats, const int *paired,
+                        double B, uint64_t seed, int prec, int *states, double *unpaired, double *pairedll,
+                        double *siteconds, double *paircomps) {
+  if (!ds) return 2;
+  messi_ctx *ctx = ds->ctx;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(P4 && pi4 && logw4 && P16 && pi16 && logw16 && paired && states && unpaired && pairedll,
+                  "null argument");
+    check_prec(prec);
+    MESSI_REQUIRE(cats >= 1 && cats <= 8, "cats (site categories) must be 1 .. 8");
+    MESSI_REQUIRE(K4 == cats, "K4 must equal the number of site categories");
+    MESSI_REQUIRE(K16 >= cats * cats && K16 % (cats * cats) == 0,
+                  "K16 must be cats^2 times the number of lambda classes");
+    MESSI_REQUIRE(B > 0.0 && std::isfinite(B), "B must be positive");
+    const int L = ds->L;
+    for (int x = 0; x < L; ++x) {
+      const int p = paired[x];
+      MESSI_REQUIRE(p >= 0 && p <= L && p != x + 1, "paired must hold 1-based partners (0 unpaired)");
+      MESSI_REQUIRE(p == 0 || paired[p - 1] == x + 1, "paired is not symmetric");
+    }
+    MESSI_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    messi_batch *b = self_batch(ds);
+    ds->d_partner.ensure(sizeof(int) * L);
+    ds->d_sitec.ensure(sizeof(double) * (size_t)K4 * L);
+    ds->d_pairc.ensure(sizeof(double) * (size_t)K16 * L);
+    ds->d_sout.ensure((sizeof(int) + 2 * sizeof(double)) * (size_t)L);
+    ds->d_miss.ensure(sizeof(int));
+    MESSI_HIP(hipMemcpyAsync(ds->d_partner.p, paired, sizeof(int) * L, hipMemcpyHostToDevice, s));
+    MESSI_HIP(hipMemsetAsync(ds->d_miss.p, 0, sizeof(int), s));
+    double *sitec = ds->d_sitec.as<double>(), *pairc = ds->d_pairc.as<double>();
+    double *d_unp = ds->d_sout.as<double>(), *d_pll = d_unp + L;
+    int *d_states = reinterpret_cast<int *>(d_pll + L);
+    auto go = [&](auto tag) {
+      using T = decltype(tag);
+      load_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+      run_site<T>(b, prec);
+      const ExpandDesc *dsite = upload_expand(b, prec, false);
+      for (int k = 0; k < K4; ++k) {
+        site_levels<T>(b, s, k);
+        launch_site_gather(s, L, dsite, b->site_acc.as<double>(), sitec + (size_t)k * L);
+      }
+      run_pair<T>(b, prec, nullptr, false, true);
+      const ExpandDesc *dpair = upload_expand(b, prec, true);
+      for (int k = 0; k < K16; ++k) {
+        pair_levels<T>(b, s, k, 0, 1);
+        launch_pair_gather(s, L, dpair, b->pair_comp.as<double>(), ds->d_partner.as<int>(), pairc + (size_t)k * L,
+                           ds->d_miss.as<int>());
+      }
+      MESSI_HIP(hipGetLastError());
+    };
+    if (prec == MESSI_FP32) go(float{});
+    else go(double{});
+    launch_state_sample(s, L, cats, K16 / (cats * cats), sitec, pairc, b->c16.logw.as<double>(),
+                        ds->d_partner.as<int>(), B, seed, d_states, d_unp, d_pll);
+    int miss = 0;
+    MESSI_HIP(hipMemcpyAsync(&miss, ds->d_miss.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    MESSI_HIP(hipMemcpyAsync(states, d_states, sizeof(int) * L, hipMemcpyDeviceToHost, s));
+    MESSI_HIP(hipMemcpyAsync(unpaired, d_unp, sizeof(double) * L, hipMemcpyDeviceToHost, s));
+    MESSI_HIP(hipMemcpyAsync(pairedll, d_pll, sizeof(double) * L, hipMemcpyDeviceToHost, s));
+    if (siteconds)
+      MESSI_HIP(hipMemcpyAsync(siteconds, sitec, sizeof(double) * (size_t)K4 * L, hipMemcpyDeviceToHost, s));
+    if (paircomps)
+      MESSI_HIP(hipMemcpyAsync(paircomps, pairc, sizeof(double) * (size_t)K16 * L, hipMemcpyDeviceToHost, s));
+    MESSI_HIP(hipStreamSynchronize(s));
+    MESSI_REQUIRE(miss == 0, "the dataset's pair list lacks a pair of the structure (distance bound or mask)");
   });
 }
 

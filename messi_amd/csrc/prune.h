@@ -49,6 +49,21 @@ void launch_pair_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, con
 // Fixed-structure sums of one member (descriptor d, pair root values acc):
 // out[0] unpaired site log-likelihoods, out[1] pair log-likelihoods, out[2] missing pairs.
 void launch_structure_sum(hipStream_t s, const ExpandDesc *d, const double *acc, const int *partner, double *out);
+// State sampling given a structure (one member, descriptors ds / dp of its site and
+// pair root entries, partner 1-based):
+// out[x] = site root value of column x (one site component's pass);
+void launch_site_gather(hipStream_t s, int L, const ExpandDesc *ds, const double *acc, double *out);
+// out[x] = pair root value of the pair opened at x (-inf elsewhere); pairs the
+// member's pair list lacks are counted in *missing (not cleared here).
+void launch_pair_gather(hipStream_t s, int L, const ExpandDesc *dp, const double *acc, const int *partner,
+                        double *out, int *missing);
+// One categorical draw per unpaired column and per base pair (keyed uniforms, as
+// the structure sampler): sitec (K4 x L) the site conditionals log w_c + ll_c,
+// pairc (K16 x L) the pair component log-likelihoods at the openers, components
+// ordered (category 1, category 2, M lambda classes), logw16 their log weights.
+void launch_state_sample(hipStream_t s, int L, int cats, int M, const double *sitec, const double *pairc,
+                         const double *logw16, const int *partner, double B, uint64_t seed, int *states,
+                         double *unpaired, double *pairedll);
 template <typename T> bool mfma_probe(hipStream_t s);
 bool mfma_bf16_probe(hipStream_t s);
 

@@ -338,6 +338,40 @@ void launch_structure_sum(hipStream_t s, const ExpandDesc *d, const double *acc,
   hipLaunchKernelGGL(structure_sum_kernel, dim3(1), dim3(1024), 0, s, d, acc, partner, out);
 }
 
+__global__ void site_gather_kernel(const ExpandDesc *__restrict__ dd, const double *__restrict__ acc,
+                                   double *__restrict__ out) {
+  const ExpandDesc d = *dd;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x < d.L) out[x] = acc[d.rootoff + d.pid0[x]];
+}
+
+__global__ void pair_gather_kernel(const ExpandDesc *__restrict__ dd, const double *__restrict__ acc,
+                                   const int *__restrict__ partner, double *__restrict__ out, int *missing) {
+  const ExpandDesc d = *dd;
+  const int L = d.L, x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= L) return;
+  const int p = partner[x];
+  double v = -HUGE_VAL;
+  if (p - 1 > x) {
+    const int y = p - 1;
+    int k = -1;
+    if (y - x <= d.maxbp && (!d.mask || d.mask[(size_t)x + (size_t)L * y]))
+      k = d.T0[(size_t)d.pid0[x] * d.n0 + d.pid0[y]];
+    if (k >= 0) v = acc[d.rootoff + k];
+    else atomicAdd(missing, 1);
+  }
+  out[x] = v;
+}
+
+void launch_site_gather(hipStream_t s, int L, const ExpandDesc *ds, const double *acc, double *out) {
+  hipLaunchKernelGGL(site_gather_kernel, dim3((L + 255) / 256), dim3(256), 0, s, ds, acc, out);
+}
+
+void launch_pair_gather(hipStream_t s, int L, const ExpandDesc *dp, const double *acc, const int *partner,
+                        double *out, int *missing) {
+  hipLaunchKernelGGL(pair_gather_kernel, dim3((L + 255) / 256), dim3(256), 0, s, dp, acc, partner, out, missing);
+}
+
 template <typename T, int NB>
 void launch_pair_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes, const int *idx,
                        const uint16_t *leafc, T *part, int *expo, const T *P, const double *pi16,

@@ -169,7 +169,93 @@ __global__ __launch_bounds__(64) void sample_kernel(SampleArgs<SR> a) {
   if (lane == 0) a.prior[blockIdx.x] = prior;
 }
 
+// log(e^a + e^b) with the reference's -inf guards (src/CommonUtils.jl:145-155).
+__device__ __forceinline__ double lse_states(double a, double b) {
+  if (a == -HUGE_VAL) return b;
+  if (b == -HUGE_VAL) return a;
+  return a < b ? b + log1p(exp(a - b)) : a + log1p(exp(b - a));
+}
+
+// CommonUtils.sample (src/CommonUtils.jl:129-142) over w[0..n-1]: the first index
+// whose running sum of w/total reaches u, else the last.
+__device__ __forceinline__ int draw_index(const double *w, int n, double u) {
+  double tot = 0.0;
+  for (int i = 0; i < n; ++i) tot += w[i];
+  double cum = 0.0;
+  for (int i = 0; i < n; ++i) {
+    cum += w[i] / tot;
+    if (cum >= u) return i;
+  }
+  return n - 1;
+}
+
+constexpr int STATES_MAX_CATS = 8, STATES_MAX_COND = STATES_MAX_CATS * STATES_MAX_CATS;
+
+// State sampling given a structure: the sampling part of computeuncachedlikelihood
+// (src/FARCE.jl:1398-1486, called by samplestates, 2534-2538).  One thread per
+// column.  An unpaired column draws its rate category from exp(sitec - lse)^B; the
+// opener x of a pair (x, y) draws the category pair (c1, c2) from the pair
+// conditionals (the components of (c1, c2) summed over the M lambda classes),
+// normalised and raised to B, and writes both states.  The uniform of column x is
+// keyed by (seed, 0, 3, x, x), as the structure sampler keys its intervals.
+__global__ void state_sample_kernel(int L, int cats, int M, const double *__restrict__ sitec,
+                                    const double *__restrict__ pairc, const double *__restrict__ logw16,
+                                    const int *__restrict__ partner, double B, uint64_t seed,
+                                    int *__restrict__ states, double *__restrict__ unpaired,
+                                    double *__restrict__ pairedll) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= L) return;
+  const int p = partner[x];
+  double w[STATES_MAX_COND];
+  if (p == 0) {
+    double tot = -HUGE_VAL;
+    for (int c = 0; c < cats; ++c) tot = c == 0 ? sitec[x] : lse_states(tot, sitec[(size_t)c * L + x]);
+    for (int c = 0; c < cats; ++c) {
+      const double pr = exp(sitec[(size_t)c * L + x] - tot);
+      w[c] = B == 1.0 ? pr : pow(pr, B);
+    }
+    states[x] = draw_index(w, cats, keyed_uniform(seed, 0, 3, x, x)) + 1;
+    unpaired[x] = tot;
+    pairedll[x] = -HUGE_VAL;
+  } else if (p - 1 > x) {
+    const int nc = cats * cats;
+    double tot = -HUGE_VAL;
+    for (int i = 0; i < nc; ++i) {
+      double ci = -HUGE_VAL;
+      for (int m = 0; m < M; ++m) {
+        const int k = i * M + m;
+        const double v = logw16[k] + pairc[(size_t)k * L + x];
+        tot = lse_states(tot, v);
+        ci = lse_states(ci, v);
+      }
+      w[i] = ci;
+    }
+    for (int i = 0; i < nc; ++i) {
+      const double pr = exp(w[i] - tot);
+      w[i] = B == 1.0 ? pr : pow(pr, B);
+    }
+    const int r = draw_index(w, nc, keyed_uniform(seed, 0, 3, x, x));
+    states[x] = r / cats + 1;
+    states[p - 1] = r % cats + 1;
+    unpaired[x] = 0.0;
+    pairedll[x] = tot;
+  } else {
+    unpaired[x] = 0.0;  // the closing column of a pair: its state comes from the opener
+    pairedll[x] = -HUGE_VAL;
+  }
+}
+
 }  // namespace
+
+void launch_state_sample(hipStream_t s, int L, int cats, int M, const double *sitec, const double *pairc,
+                         const double *logw16, const int *partner, double B, uint64_t seed, int *states,
+                         double *unpaired, double *pairedll) {
+  MESSI_REQUIRE(cats >= 1 && cats <= STATES_MAX_CATS, "state sampling supports 1 .. 8 site categories");
+  MESSI_REQUIRE(M >= 1, "need at least one lambda class");
+  hipLaunchKernelGGL(state_sample_kernel, dim3((L + 255) / 256), dim3(256), 0, s, L, cats, M, sitec, pairc, logw16,
+                     partner, B, seed, states, unpaired, pairedll);
+  MESSI_HIP(hipGetLastError());
+}
 
 template <typename T>
 void sample_run(hipStream_t s, InsideWork &w, int L, int n, uint64_t seed, int first, double B, int *d_out,

@@ -257,6 +257,21 @@ def felsensteincudapaired(dataset, paired, params: ModelParameters, museparams=N
     return pairsum
 
 
+def samplestates(seed: int, dataset, params: ModelParameters, paired, B: float = 1.0, prec=lib.FP64):
+    """samplestates (src/FARCE.jl:2534-2538): draws the site category of every
+    column given the structure paired (1-based partners), one category per
+    unpaired column and one category pair per base pair, from the conditionals of
+    computeuncachedlikelihood (src/FARCE.jl:1336-1486) raised to B; sets
+    params.states and returns (unpaired log-likelihoods, paired log-likelihoods at
+    the openers, states).  The reference's rng becomes a seed (the draw of column
+    x is keyed by (seed, x)); its museconditionals output is not produced."""
+    dd = structure_dataset(dataset, paired)
+    states, unp, pll = dd.sample_states(_stack(site_components(params)), _stack(pair_components(params)),
+                                        paired, params.siteCats, seed=seed, B=B, prec=prec)
+    params.states = states
+    return unp, pll, states
+
+
 def computemodellikelihood(dataset, params: ModelParameters, paired, samplebranchlengths: bool = False,
                            useprior: bool = True, fixLambdaGT: bool = False, prec=lib.FP64) -> float:
     """Fixed-structure log posterior: prior + unpaired site log-likelihoods + pair

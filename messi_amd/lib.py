@@ -65,6 +65,9 @@ SIGNATURES = {
     "messi_parse_dbn": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _ip, _ip]),
     "messi_structure_loglik": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _ip,
                                               ctypes.c_int, _dp, _dp]),
+    "messi_sample_states": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
+                                           ctypes.c_int, _ip, ctypes.c_double, ctypes.c_uint64, ctypes.c_int,
+                                           _ip, _dp, _dp, _dp, _dp]),
     "messi_sample_structures": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_uint64, ctypes.c_double, _ip, _dp]),
     "messi_sample_posterior": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
@@ -350,6 +353,35 @@ class DeviceDataset:
             self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c), ptr(pi16), ptr(lw16),
             pa.ctypes.data_as(_ip), prec_code(prec), ctypes.byref(ll), ptr(parts)))
         return ll.value, (float(parts[0]), float(parts[1]))
+
+    def sample_states(self, site, pair, paired, cats: int, seed: int = 0, B: float = 1.0, prec=FP64,
+                      conditionals: bool = False):
+        """Site-category states given the structure paired (1-based partners):
+        (states 1-based, unpaired log-likelihoods, paired log-likelihoods at the
+        openers), plus (site conditionals K4 x L, pair components K16 x L) when
+        conditionals is set (messi_sample_states)."""
+        P4, pi4, lw4 = site
+        P16, pi16, lw16 = pair
+        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
+        K4, K16 = P4c.shape[0], P16c.shape[0]
+        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
+        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
+        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
+        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        pa = np.ascontiguousarray(paired, dtype=np.intc)
+        if pa.shape != (self.L,):
+            raise ValueError(f"paired must have length {self.L}")
+        st = np.zeros(self.L, dtype=np.intc)
+        unp = np.zeros(self.L)
+        pll = np.zeros(self.L)
+        sc = np.zeros((K4, self.L)) if conditionals else None
+        pc = np.zeros((K16, self.L)) if conditionals else None
+        self.ctx.check(self.ctx.lib.messi_sample_states(
+            self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c), ptr(pi16), ptr(lw16), int(cats),
+            pa.ctypes.data_as(_ip), float(B), int(seed), prec_code(prec), st.ctypes.data_as(_ip), ptr(unp),
+            ptr(pll), ptr(sc), ptr(pc)))
+        out = (st.astype(np.int64), unp, pll)
+        return out + (sc, pc) if conditionals else out
 
     def sample_posterior(self, site, pair, nsamples: int, seed: int = 0, B: float = 1.0, prec=FP64):
         """log Z, then nsamples structures (nsamples x L, 1-based partners) drawn from

@@ -18,7 +18,7 @@ definition:
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -171,6 +171,7 @@ class ModelParameters:
     lambdaGammaShapeGC: float = 1.0
     lambdaGammaScaleGC: float = 1.0
     siteGammaShape: float = 1.0
+    states: Optional[np.ndarray] = None  # per-column site categories (1-based), set by samplestates
 
     @property
     def siteCats(self) -> int:

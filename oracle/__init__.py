@@ -56,6 +56,8 @@ def lib():
         L.oracle_sample_structure.restype = ctypes.c_double
         L.oracle_kh99_prior.argtypes = [ctypes.c_int, _ip]
         L.oracle_kh99_prior.restype = ctypes.c_double
+        L.oracle_sample_states.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip,
+                                           ctypes.c_double, ctypes.c_uint64, _ip, _dp, _dp]
         _lib = L
     return _lib
 
@@ -191,3 +193,22 @@ def kh99_prior(partner) -> float:
 
 def sample_uniform(seed: int, sample: int, sym: int, x: int, y: int) -> float:
     return float(lib().oracle_sample_uniform(int(seed), sample, sym, x, y))
+
+
+def sample_states(sitec, pairc, logw16, partner, cats: int, B: float = 1.0, seed: int = 0):
+    """State sampling given a structure (oracle_sample_states): sitec (K4 x L),
+    pairc (K16 x L), logw16 (K16), partner 1-based.  Returns (states 1-based,
+    unpaired log-likelihoods, paired log-likelihoods at the openers)."""
+    sitec = np.ascontiguousarray(sitec, dtype=np.float64)
+    pairc = np.ascontiguousarray(pairc, dtype=np.float64)
+    lw = np.ascontiguousarray(logw16, dtype=np.float64)
+    part = np.ascontiguousarray(partner, dtype=np.int32)
+    L = len(part)
+    K16 = pairc.shape[0]
+    assert sitec.shape == (cats, L) and pairc.shape == (K16, L) and K16 % (cats * cats) == 0
+    st = np.zeros(L, dtype=np.int32)
+    unp = np.zeros(L)
+    pll = np.zeros(L)
+    lib().oracle_sample_states(L, cats, K16 // (cats * cats), _p(sitec), _p(pairc), _p(lw), _p(part, _ip),
+                               float(B), int(seed), _p(st, _ip), _p(unp), _p(pll))
+    return st.astype(np.int64), unp, pll

@@ -22,6 +22,9 @@
  *   oracle_sample_structure -- samplestructure / samplestructurehelper
  *                           (src/InsideOutsideAlgorithm.jl:570-629), keyed uniforms
  *   oracle_kh99_prior    -- calculateKH99prior (src/InsideOutsideAlgorithm.jl:86-144)
+ *   oracle_sample_states -- the state sampling of computeuncachedlikelihood
+ *                           (src/FARCE.jl:1398-1486) called by samplestates
+ *                           (src/FARCE.jl:2534-2538), keyed uniforms
  *   lse                  -- CommonUtils.logsumexp (src/CommonUtils.jl:145-155)
  *
  * Conventions: trees are flat pre-order arrays with the root at index 0,
@@ -539,4 +542,74 @@ double oracle_kh99_prior(int L, const int *partner) {
   }
   free(st);
   return ll;
+}
+
+/* CommonUtils.sample (src/CommonUtils.jl:129-142): the first index whose running
+ * sum of w/total reaches u, else the last. */
+static int draw_index(const double *w, int n, double u) {
+  double tot = 0.0, cum = 0.0;
+  for (int i = 0; i < n; ++i) tot += w[i];
+  for (int i = 0; i < n; ++i) {
+    cum += w[i] / tot;
+    if (cum >= u) return i;
+  }
+  return n - 1;
+}
+
+/* State sampling given a structure, src/FARCE.jl:1398-1486.  sitec (K4 x L,
+ * row k = site category k) holds log w_k + the column's log-likelihood under
+ * category k (unpairedconditionals); pairc (K16 x L) the log-likelihood of the
+ * pair opened at column x under pair component k, components ordered (c1, c2,
+ * then M lambda classes) with log weights logw16 (fullconditionals = logw16 +
+ * pairc).  An unpaired column x draws c with probability proportional to
+ * exp(sitec - lse)^B; the opener x of (x, y) draws (c1, c2) from the lambda-summed
+ * pair conditionals, normalised and raised to B, and sets states[x] = c1+1,
+ * states[y] = c2+1.  unpaired[x] is the lse over categories (0 for paired
+ * columns), pairedll[x] the lse over all pair components at openers (-inf
+ * elsewhere).  The uniform of column x is oracle_sample_uniform(seed, 0, 3, x, x)
+ * (the reference draws from its rng in column order). */
+void oracle_sample_states(int L, int cats, int M, const double *sitec, const double *pairc, const double *logw16,
+                          const int *partner, double B, uint64_t seed, int *states, double *unpaired,
+                          double *pairedll) {
+  const int nc = cats * cats;
+  double *w = (double *)malloc(sizeof(double) * (size_t)(nc > cats ? nc : cats));
+  for (int x = 0; x < L; ++x) {
+    const int p = partner[x];
+    if (p == 0) {
+      double tot = -INFINITY;
+      for (int c = 0; c < cats; ++c) tot = c == 0 ? sitec[x] : lse(tot, sitec[(size_t)c * L + x]);
+      for (int c = 0; c < cats; ++c) {
+        const double pr = exp(sitec[(size_t)c * L + x] - tot);
+        w[c] = B == 1.0 ? pr : pow(pr, B);
+      }
+      states[x] = draw_index(w, cats, oracle_sample_uniform(seed, 0, 3, x, x)) + 1;
+      unpaired[x] = tot;
+      pairedll[x] = -INFINITY;
+    } else if (p - 1 > x) {
+      double tot = -INFINITY;
+      for (int i = 0; i < nc; ++i) {
+        double ci = -INFINITY;
+        for (int m = 0; m < M; ++m) {
+          const int k = i * M + m;
+          const double v = logw16[k] + pairc[(size_t)k * L + x];
+          tot = lse(tot, v);
+          ci = lse(ci, v);
+        }
+        w[i] = ci;
+      }
+      for (int i = 0; i < nc; ++i) {
+        const double pr = exp(w[i] - tot);
+        w[i] = B == 1.0 ? pr : pow(pr, B);
+      }
+      const int r = draw_index(w, nc, oracle_sample_uniform(seed, 0, 3, x, x));
+      states[x] = r / cats + 1;
+      states[p - 1] = r % cats + 1;
+      unpaired[x] = 0.0;
+      pairedll[x] = tot;
+    } else {
+      unpaired[x] = 0.0;
+      pairedll[x] = -INFINITY;
+    }
+  }
+  free(w);
 }
