@@ -1434,7 +1434,7 @@ int messi_sample_structures(messi_ctx *ctx, int L, const double *in3, const doub
       inside_prepare<T>(s, w, L, 1.0);
       MESSI_HIP(hipMemcpyAsync(w.out3.p, in3, sizeof(double) * 3 * LL, hipMemcpyHostToDevice, s));
       inside_import<T>(s, w, L, w.out3.as<double>());
-      sample_run<T>(s, w, L, nsamples, seed, 0, B, sw.out.as<int>(), sw.prior.as<double>());
+      sample_run<T>(s, w, sw, L, nsamples, seed, 0, B, sw.out.as<int>(), sw.prior.as<double>());
     };
     if (prec == MESSI_FP32) go(float{});
     else go(double{});
@@ -1470,7 +1470,7 @@ int messi_sample_posterior(messi_dataset *ds, int K4, const double *P4, const do
       InsideWork &w = member_work(b, 0, sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64);
       load_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
       eval_impl<T>(b, 1.0, true, logZ);
-      sample_run<T>(s, w, L, nsamples, seed, 0, B, sw.out.as<int>(), sw.prior.as<double>());
+      sample_run<T>(s, w, sw, L, nsamples, seed, 0, B, sw.out.as<int>(), sw.prior.as<double>());
     };
     if (prec == MESSI_FP32) go(float{});
     else go(double{});

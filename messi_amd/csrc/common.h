@@ -123,6 +123,7 @@ struct MeaWork {
 // Outputs of structure sampling.
 struct SampleWork {
   DevBuf out, prior;
+  DevBuf logs;  // FP32 tables: L, S, F and the pair term as natural-log doubles (4 L^2)
 };
 
 }  // namespace messi

@@ -95,8 +95,8 @@ void mea_run_traceback(hipStream_t s, MeaWork &w, int L, int *d_paired);
 // inside tables: d_out (n x L) receives 1-based partners, d_prior (n) the KH99 log priors.
 constexpr int SAMPLE_MAX_L = 16383;
 template <typename T>
-void sample_run(hipStream_t s, InsideWork &w, int L, int n, uint64_t seed, int first, double B, int *d_out,
-                double *d_prior);
+void sample_run(hipStream_t s, InsideWork &w, SampleWork &sw, int L, int n, uint64_t seed, int first, double B,
+                int *d_out, double *d_prior);
 // E (as double) and S of the last mea_run, column-major, zero on and below the diagonal.
 void mea_export(hipStream_t s, MeaWork &w, int L, double *d_E, int *d_S);
 

@@ -257,24 +257,27 @@ void launch_state_sample(hipStream_t s, int L, int cats, int M, const double *si
   MESSI_HIP(hipGetLastError());
 }
 
-template <typename T>
-void sample_run(hipStream_t s, InsideWork &w, int L, int n, uint64_t seed, int first, double B, int *d_out,
-                double *d_prior) {
-  typedef typename SRof<T>::type SR;
-  typedef typename SR::V V;
-  MESSI_REQUIRE(L >= 1 && L <= SAMPLE_MAX_L, "structure sampling supports 1 <= L <= 16383");
-  MESSI_REQUIRE(n >= 0, "negative sample count");
-  if (n == 0) return;
+// Natural-log doubles of n scaled FP32 cells (the conversion the FP32 sampler
+// would otherwise repeat for every candidate of every interval).
+__global__ void xf_to_log_kernel(const XFv *__restrict__ src, double *__restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = XF::to_log(src[i]);
+}
+
+template <class SR>
+void sample_launch(hipStream_t s, const typename SR::V *Lr, const typename SR::V *Sc, const typename SR::V *F,
+                   const typename SR::V *ptv, int L, int n, uint64_t seed, int first, double B, int *d_out,
+                   double *d_prior) {
   SampleArgs<SR> a;
   a.L = L;
   a.n = n;
   a.first = first;
   a.seed = seed;
   a.B = B;
-  a.Lr = w.Lr.as<V>();
-  a.Sc = w.Sc.as<V>();
-  a.F = w.F.as<V>();
-  a.ptv = w.ptv.as<V>();
+  a.Lr = Lr;
+  a.Sc = Sc;
+  a.F = F;
+  a.ptv = ptv;
   kh99_logprobs(a.lp);
   a.out = d_out;
   a.prior = d_prior;
@@ -283,7 +286,34 @@ void sample_run(hipStream_t s, InsideWork &w, int L, int n, uint64_t seed, int f
   MESSI_HIP(hipGetLastError());
 }
 
-template void sample_run<double>(hipStream_t, InsideWork &, int, int, uint64_t, int, double, int *, double *);
-template void sample_run<float>(hipStream_t, InsideWork &, int, int, uint64_t, int, double, int *, double *);
+// The FP32 tables are converted to natural-log doubles once (XF::to_log, the
+// conversion the FP32 kernel applied per candidate, so the draws are unchanged)
+// and sampled by the FP64 kernel.
+template <typename T>
+void sample_run(hipStream_t s, InsideWork &w, SampleWork &sw, int L, int n, uint64_t seed, int first, double B,
+                int *d_out, double *d_prior) {
+  MESSI_REQUIRE(L >= 1 && L <= SAMPLE_MAX_L, "structure sampling supports 1 <= L <= 16383");
+  MESSI_REQUIRE(n >= 0, "negative sample count");
+  if (n == 0) return;
+  if constexpr (sizeof(T) == 8) {
+    sample_launch<LogSR>(s, w.Lr.as<double>(), w.Sc.as<double>(), w.F.as<double>(), w.ptv.as<double>(), L, n, seed,
+                         first, B, d_out, d_prior);
+  } else {
+    const size_t LL = (size_t)L * L;
+    sw.logs.ensure(sizeof(double) * 4 * LL);
+    double *lg = sw.logs.as<double>();
+    const XFv *src[4] = {w.Lr.as<XFv>(), w.Sc.as<XFv>(), w.F.as<XFv>(), w.ptv.as<XFv>()};
+    const int blocks = (int)std::min<size_t>((LL + 255) / 256, 8192);
+    for (int k = 0; k < 4; ++k)
+      hipLaunchKernelGGL(xf_to_log_kernel, dim3(blocks), dim3(256), 0, s, src[k], lg + k * LL, LL);
+    MESSI_HIP(hipGetLastError());
+    sample_launch<LogSR>(s, lg, lg + LL, lg + 2 * LL, lg + 3 * LL, L, n, seed, first, B, d_out, d_prior);
+  }
+}
+
+template void sample_run<double>(hipStream_t, InsideWork &, SampleWork &, int, int, uint64_t, int, double, int *,
+                                 double *);
+template void sample_run<float>(hipStream_t, InsideWork &, SampleWork &, int, int, uint64_t, int, double, int *,
+                                double *);
 
 }  // namespace messi
