@@ -100,8 +100,12 @@ __global__ __launch_bounds__(T * T) void mea_step_kernel(MeaDesc<V> d, int t) {
   __shared__ MeaLDS<V, T> sm;
   const int tid = threadIdx.x;
   const int L = d.L, NT = d.NT;
-  // ---- decode the work item
-  const int item = xcd_chunk_id(blockIdx.x, gridDim.x);
+  // ---- decode the work item: the NT - t finalising tiles are the first
+  // workgroups in dispatch order (so the launch's critical path starts first);
+  // the bulk items after them are renumbered so that each XCD takes a contiguous run
+  const int nfin = NT - t;
+  const int item = (int)blockIdx.x < nfin ? (int)blockIdx.x
+                                           : nfin + xcd_chunk_id((int)blockIdx.x - nfin, (int)gridDim.x - nfin);
   int I, J;
   bool fin;
   if (item < NT - t) {
