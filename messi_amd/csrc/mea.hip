@@ -234,36 +234,48 @@ __global__ __launch_bounds__(T * T) void mea_step_kernel(MeaDesc<V> d, int t) {
     const int own = b + T - 1 - a;
     // split point k = IT+T-1: E[(I+1)T, j] lies in the tile below
     if (mine && a <= T - 2) fold(sm.Ud[a][T - 1] + sm.Ebot[b], IT + T - 1, bv, bk);
-    for (int st = 0; st <= 2 * T - 2; ++st) {
-      if (mine && st <= own) {
-        if (st >= 1) {
-          const int ap = b + T - st;  // column producer (ap, b), final since step st-1
-          if (ap >= a + 2 && ap <= T - 1) fold(sm.Ud[a][ap - 1] + sm.cE[ap][b], IT + ap - 1, bv, bk);
-          const int bp = st - T + a;  // row producer (a, bp), final since step st-1
-          if (bp >= 0 && bp < b) fold(sm.cU[a][bp] + sm.Ed[bp + 1][b], JT + bp, bv, bk);
+    // Per-cell constants in registers.  The cell below (a+1, b) is the column
+    // producer of this cell's own step, so its E comes with that step's reads;
+    // the diagonal neighbour (a+1, b-1) is final two steps earlier (own >= 2 for
+    // it) and is read one step ahead from a fixed LDS position (its border value
+    // for the last row / first column; primed before step 0).
+    const V sgv = sm.sg[a], pv = sm.cP[a][b], ebot = sm.Ebot[b];
+    const V *einsrc = (a + 1 < T && b >= 1) ? &sm.cE[a + 1][b - 1] : (a + 1 == T ? &sm.Ebot2[b] : &sm.Eleft[a]);
+    const bool ezero = i + 1 > j - 1;
+    V ein_next = *einsrc;
+    auto step = [&](int st) {
+      const V ein_f = ein_next;
+      ein_next = *einsrc;
+      const int ap = b + T - st;  // column producer (ap, b), final since step st-1
+      const int bp = st - T + a;  // row producer (a, bp), final since step st-1
+      const int apc = min(max(ap, 1), T - 1), bpc = min(max(bp, 0), T - 2);
+      const V ecol = sm.cE[apc][b];
+      const V tc = sm.Ud[a][apc - 1] + ecol, tr = sm.cU[a][bpc] + sm.Ed[bpc + 1][b];
+      if (mine && st >= 1 && st <= own) {
+        if (ap >= a + 2 && ap <= T - 1) fold(tc, IT + ap - 1, bv, bk);
+        if (bp >= 0 && bp < b) fold(tr, JT + bp, bv, bk);
+      }
+      if (mine && st == own) {
+        const V eb = a + 1 < T ? ecol : ebot;  // at this step apc = a + 1
+        const V e1 = sgv + eb;
+        const V u = pair_term(d.alpha, pv, ezero ? V(0) : ein_f);
+        V val = e1;
+        sres = -1;
+        if (u > val) {
+          val = u;
+          sres = -2;
         }
-        if (st == own) {
-          const V eb = a + 1 < T ? sm.cE[a + 1][b] : sm.Ebot[b];
-          V ein = (a + 1 < T && b >= 1) ? sm.cE[a + 1][b - 1] : (a + 1 == T ? sm.Ebot2[b] : sm.Eleft[a]);
-          if (i + 1 > j - 1) ein = V(0);
-          const V e1 = sm.sg[a] + eb;
-          const V u = pair_term(d.alpha, sm.cP[a][b], ein);
-          V val = e1;
-          sres = -1;
-          if (u > val) {
-            val = u;
-            sres = -2;
-          }
-          if (bv > val) {
-            val = bv;
-            sres = bk + 1;
-          }
-          sm.cE[a][b] = val;
-          sm.cU[a][b] = u;
+        if (bv > val) {
+          val = bv;
+          sres = bk + 1;
         }
+        sm.cE[a][b] = val;
+        sm.cU[a][b] = u;
       }
       __syncthreads();
-    }
+    };
+#pragma unroll
+    for (int st = 0; st <= 2 * T - 2; ++st) step(st);
   }
   // ---- write the tile back
   if (mine) d.S[(size_t)i * L + j] = sres;
