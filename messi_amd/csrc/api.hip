@@ -652,19 +652,6 @@ const ExpandDesc *upload_expand(messi_batch *b, int prec, bool pairs) {
   return d.as<ExpandDesc>();
 }
 
-// The site level launches of component k alone: with first set, the root entries
-// hold log w_k + the component's log-likelihood.
-template <typename T>
-void site_levels(messi_batch *b, hipStream_t s, int k) {
-  for (int h = 1; h <= b->Hmax; ++h) {
-    const int lo = b->site_lv[h - 1], hi = b->site_lv[h];
-    launch_site_level<T>(s, b->site_items.as<int2>() + lo, hi - lo, b->site_nodes.as<NodeDesc>(), b->site_idx,
-                         b->site_leaf, b->site_part.as<T>(), b->site_expo.as<int>(),
-                         b->c4.P.as<T>() + (size_t)k * b->nn_total * 16, b->c4.pi.as<double>() + (size_t)k * b->n * 4,
-                         b->c4.logw.as<double>() + (size_t)k * b->n, 1, b->site_acc.as<double>());
-  }
-}
-
 template <typename T>
 void run_site(messi_batch *b, int prec) {
   hipStream_t s = b->ctx->stream;
@@ -1374,22 +1361,54 @@ int messi_sample_states(messi_dataset *ds, int K4, const double *P4, const doubl
     double *sitec = ds->d_sitec.as<double>(), *pairc = ds->d_pairc.as<double>();
     double *d_unp = ds->d_sout.as<double>(), *d_pll = d_unp + L;
     int *d_states = reinterpret_cast<int *>(d_pll + L);
+    // Every component in the same launches (grid row = component), each with its
+    // own partials and root entries: one launch per tree level and kind.
     auto go = [&](auto tag) {
       using T = decltype(tag);
       load_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
-      run_site<T>(b, prec);
+      const int64_t se = std::max<int64_t>(1, b->site_entries), sr = std::max<int64_t>(1, b->site_root_total);
+      const int64_t pe = std::max<int64_t>(1, b->pair_entries), pr = std::max<int64_t>(1, b->pair_root_total);
+      b->site_part.ensure(sizeof(T) * 4 * se * K4);
+      b->site_expo.ensure(sizeof(int) * se * K4);
+      b->site_acc.ensure(sizeof(double) * sr * K4);
+      b->pair_part.ensure(sizeof(T) * 16 * pe * K16);
+      b->pair_expo.ensure(sizeof(int) * pe * K16);
+      b->pair_acc.ensure(sizeof(double) * pr * K16);
+      b->pair_comp.ensure(sizeof(double) * pr * K16);
       const ExpandDesc *dsite = upload_expand(b, prec, false);
-      for (int k = 0; k < K4; ++k) {
-        site_levels<T>(b, s, k);
-        launch_site_gather(s, L, dsite, b->site_acc.as<double>(), sitec + (size_t)k * L);
+      CompBatch cs;
+      cs.K = K4;
+      cs.part = 4 * se;
+      cs.expo = se;
+      cs.root = sr;
+      cs.P = b->nn_total * 16;
+      cs.pi = (int64_t)b->n * 4;
+      cs.logw = b->n;
+      for (int h = 1; h <= b->Hmax; ++h) {
+        const int lo = b->site_lv[h - 1], hi = b->site_lv[h];
+        launch_site_level<T>(s, b->site_items.as<int2>() + lo, hi - lo, b->site_nodes.as<NodeDesc>(), b->site_idx,
+                             b->site_leaf, b->site_part.as<T>(), b->site_expo.as<int>(), b->c4.P.as<T>(),
+                             b->c4.pi.as<double>(), b->c4.logw.as<double>(), 1, b->site_acc.as<double>(), cs);
       }
-      run_pair<T>(b, prec, nullptr, false, true);
+      launch_site_gather(s, L, K4, sr, dsite, b->site_acc.as<double>(), sitec);
       const ExpandDesc *dpair = upload_expand(b, prec, true);
-      for (int k = 0; k < K16; ++k) {
-        pair_levels<T>(b, s, k, 0, 1);
-        launch_pair_gather(s, L, dpair, b->pair_comp.as<double>(), ds->d_partner.as<int>(), pairc + (size_t)k * L,
-                           ds->d_miss.as<int>());
+      CompBatch cp;
+      cp.K = K16;
+      cp.part = 16 * pe;
+      cp.expo = pe;
+      cp.root = pr;
+      cp.P = b->nn_total * 256;
+      cp.pi = (int64_t)b->n * 16;
+      cp.logw = b->n;
+      for (int h = 1; h <= b->Hmax; ++h) {
+        const int lo = b->pair_lv[h - 1], hi = b->pair_lv[h];
+        launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
+                                      b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), b->pair_expo.as<int>(),
+                                      b->c16.P.as<T>(), b->c16.pi.as<double>(), b->c16.logw.as<double>(), 1,
+                                      b->pair_acc.as<double>(), b->pair_comp.as<double>(), cp);
       }
+      launch_pair_gather(s, L, K16, pr, dpair, b->pair_comp.as<double>(), ds->d_partner.as<int>(), pairc,
+                         ds->d_miss.as<int>());
       MESSI_HIP(hipGetLastError());
     };
     if (prec == MESSI_FP32) go(float{});

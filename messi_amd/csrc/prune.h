@@ -21,17 +21,26 @@ struct NodeDesc {
   int root;                 // this node is its tree's root
 };
 
+// Several mixture components in one launch (grid row c = component c): each
+// component's pointers advance by these strides (in elements of their arrays);
+// the default is one component.
+struct CompBatch {
+  int K = 1;
+  int64_t part = 0, expo = 0, root = 0, P = 0, pi = 0, logw = 0;
+};
+
 // P / pi / logw point at one mixture component for all members: the matrix of global
 // node g is P + 256 g (16 g for 4 states), member m uses pi + 16 m (4 m) and logw[m].
 template <typename T, int NB>
 void launch_pair_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes,
                        const int *idx, const uint16_t *leafc, T *part, int *expo, const T *P,
                        const double *pi16, const double *logw, int first, double *rootacc,
-                       double *rootcomp);
+                       double *rootcomp, const CompBatch &cb = CompBatch());
 template <typename T>
 void launch_site_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes,
                        const int *idx, const uint8_t *leafc, T *part, int *expo, const T *P4,
-                       const double *pi4, const double *logw, int first, double *rootacc);
+                       const double *pi4, const double *logw, int first, double *rootacc,
+                       const CompBatch &cb = CompBatch());
 
 // Expansion of root entries to each member's outputs.
 struct ExpandDesc {
@@ -52,11 +61,13 @@ void launch_structure_sum(hipStream_t s, const ExpandDesc *d, const double *acc,
 // State sampling given a structure (one member, descriptors ds / dp of its site and
 // pair root entries, partner 1-based):
 // out[x] = site root value of column x (one site component's pass);
-void launch_site_gather(hipStream_t s, int L, const ExpandDesc *ds, const double *acc, double *out);
+// (K components: acc + c * root_stride into out + c * L.)
+void launch_site_gather(hipStream_t s, int L, int K, int64_t root_stride, const ExpandDesc *ds, const double *acc,
+                        double *out);
 // out[x] = pair root value of the pair opened at x (-inf elsewhere); pairs the
 // member's pair list lacks are counted in *missing (not cleared here).
-void launch_pair_gather(hipStream_t s, int L, const ExpandDesc *dp, const double *acc, const int *partner,
-                        double *out, int *missing);
+void launch_pair_gather(hipStream_t s, int L, int K, int64_t root_stride, const ExpandDesc *dp, const double *acc,
+                        const int *partner, double *out, int *missing);
 // One categorical draw per unpaired column and per base pair (keyed uniforms, as
 // the structure sampler): sitec (K4 x L) the site conditionals log w_c + ll_c,
 // pairc (K16 x L) the pair component log-likelihoods at the openers, components

@@ -94,9 +94,19 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
     const int *__restrict__ idx, const uint16_t *__restrict__ leafc, T *__restrict__ part,
     int *__restrict__ expo, const T *__restrict__ P, const double *__restrict__ pi16,
     const double *__restrict__ logws, int first, double *__restrict__ rootacc,
-    double *__restrict__ rootcomp) {
+    double *__restrict__ rootcomp, CompBatch cb) {
   typedef typename Mfma<T>::v4 v4;
   const int lane = threadIdx.x & 63;
+  {
+    const int64_t c = blockIdx.y;  // component of this grid row
+    part += c * cb.part;
+    expo += c * cb.expo;
+    P += c * cb.P;
+    pi16 += c * cb.pi;
+    logws += c * cb.logw;
+    rootacc += c * cb.root;
+    rootcomp += c * cb.root;
+  }
   const int wave = (int)((xcd_chunk_id(blockIdx.x, gridDim.x) * (unsigned)blockDim.x + threadIdx.x) >> 6);
   if (wave >= nitems) return;  // uniform per wavefront
   const int2 it = items[wave];
@@ -206,8 +216,17 @@ __global__ __launch_bounds__(256) void site_level_kernel(
     const int2 *__restrict__ items, int nitems, const NodeDesc *__restrict__ nodes,
     const int *__restrict__ idx, const uint8_t *__restrict__ leafc, T *__restrict__ part,
     int *__restrict__ expo, const T *__restrict__ P4, const double *__restrict__ pi4,
-    const double *__restrict__ logws, int first, double *__restrict__ rootacc) {
+    const double *__restrict__ logws, int first, double *__restrict__ rootacc, CompBatch cb) {
   if ((int)blockIdx.x >= nitems) return;
+  {
+    const int64_t c = blockIdx.y;  // component of this grid row
+    part += c * cb.part;
+    expo += c * cb.expo;
+    P4 += c * cb.P;
+    pi4 += c * cb.pi;
+    logws += c * cb.logw;
+    rootacc += c * cb.root;
+  }
   const int2 it = items[blockIdx.x];
   const NodeDesc nd = nodes[it.x];
   const int64_t e = (int64_t)it.y + threadIdx.x;
@@ -338,17 +357,21 @@ void launch_structure_sum(hipStream_t s, const ExpandDesc *d, const double *acc,
   hipLaunchKernelGGL(structure_sum_kernel, dim3(1), dim3(1024), 0, s, d, acc, partner, out);
 }
 
-__global__ void site_gather_kernel(const ExpandDesc *__restrict__ dd, const double *__restrict__ acc,
+__global__ void site_gather_kernel(const ExpandDesc *__restrict__ dd, int64_t rstride, const double *__restrict__ acc,
                                    double *__restrict__ out) {
   const ExpandDesc d = *dd;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  acc += blockIdx.y * rstride;
+  out += (size_t)blockIdx.y * d.L;
   if (x < d.L) out[x] = acc[d.rootoff + d.pid0[x]];
 }
 
-__global__ void pair_gather_kernel(const ExpandDesc *__restrict__ dd, const double *__restrict__ acc,
+__global__ void pair_gather_kernel(const ExpandDesc *__restrict__ dd, int64_t rstride, const double *__restrict__ acc,
                                    const int *__restrict__ partner, double *__restrict__ out, int *missing) {
   const ExpandDesc d = *dd;
   const int L = d.L, x = blockIdx.x * blockDim.x + threadIdx.x;
+  acc += blockIdx.y * rstride;
+  out += (size_t)blockIdx.y * L;
   if (x >= L) return;
   const int p = partner[x];
   double v = -HUGE_VAL;
@@ -363,32 +386,34 @@ __global__ void pair_gather_kernel(const ExpandDesc *__restrict__ dd, const doub
   out[x] = v;
 }
 
-void launch_site_gather(hipStream_t s, int L, const ExpandDesc *ds, const double *acc, double *out) {
-  hipLaunchKernelGGL(site_gather_kernel, dim3((L + 255) / 256), dim3(256), 0, s, ds, acc, out);
+void launch_site_gather(hipStream_t s, int L, int K, int64_t root_stride, const ExpandDesc *ds, const double *acc,
+                        double *out) {
+  hipLaunchKernelGGL(site_gather_kernel, dim3((L + 255) / 256, K), dim3(256), 0, s, ds, root_stride, acc, out);
 }
 
-void launch_pair_gather(hipStream_t s, int L, const ExpandDesc *dp, const double *acc, const int *partner,
-                        double *out, int *missing) {
-  hipLaunchKernelGGL(pair_gather_kernel, dim3((L + 255) / 256), dim3(256), 0, s, dp, acc, partner, out, missing);
+void launch_pair_gather(hipStream_t s, int L, int K, int64_t root_stride, const ExpandDesc *dp, const double *acc,
+                        const int *partner, double *out, int *missing) {
+  hipLaunchKernelGGL(pair_gather_kernel, dim3((L + 255) / 256, K), dim3(256), 0, s, dp, root_stride, acc, partner,
+                     out, missing);
 }
 
 template <typename T, int NB>
 void launch_pair_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes, const int *idx,
                        const uint16_t *leafc, T *part, int *expo, const T *P, const double *pi16,
-                       const double *logw, int first, double *rootacc, double *rootcomp) {
+                       const double *logw, int first, double *rootacc, double *rootcomp, const CompBatch &cb) {
   if (nitems <= 0) return;
   int blocks = (nitems + 3) / 4;
-  hipLaunchKernelGGL((pair_level_kernel<T, NB>), dim3(blocks), dim3(256), 0, s, items, nitems, nodes, idx, leafc,
-                     part, expo, P, pi16, logw, first, rootacc, rootcomp);
+  hipLaunchKernelGGL((pair_level_kernel<T, NB>), dim3(blocks, cb.K), dim3(256), 0, s, items, nitems, nodes, idx,
+                     leafc, part, expo, P, pi16, logw, first, rootacc, rootcomp, cb);
 }
 
 template <typename T>
 void launch_site_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes, const int *idx,
                        const uint8_t *leafc, T *part, int *expo, const T *P4, const double *pi4,
-                       const double *logw, int first, double *rootacc) {
+                       const double *logw, int first, double *rootacc, const CompBatch &cb) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL((site_level_kernel<T>), dim3(nitems), dim3(256), 0, s, items, nitems, nodes, idx, leafc, part,
-                     expo, P4, pi4, logw, first, rootacc);
+  hipLaunchKernelGGL((site_level_kernel<T>), dim3(nitems, cb.K), dim3(256), 0, s, items, nitems, nodes, idx, leafc,
+                     part, expo, P4, pi4, logw, first, rootacc, cb);
 }
 
 void launch_site_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, const double *acc) {
@@ -402,16 +427,16 @@ void launch_pair_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, con
 
 template void launch_pair_level<double, PAIR_NB>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
                                                  const uint16_t *, double *, int *, const double *, const double *,
-                                                 const double *, int, double *, double *);
+                                                 const double *, int, double *, double *, const CompBatch &);
 template void launch_pair_level<float, PAIR_NB>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
                                                 const uint16_t *, float *, int *, const float *, const double *,
-                                                const double *, int, double *, double *);
+                                                const double *, int, double *, double *, const CompBatch &);
 template void launch_site_level<double>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
                                         const uint8_t *, double *, int *, const double *, const double *,
-                                        const double *, int, double *);
+                                        const double *, int, double *, const CompBatch &);
 template void launch_site_level<float>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
                                        const uint8_t *, float *, int *, const float *, const double *,
-                                       const double *, int, double *);
+                                       const double *, int, double *, const CompBatch &);
 
 // ---------------------------------------------------------------------------
 // Self-test of the MFMA operand layouts the pair kernel relies on.
