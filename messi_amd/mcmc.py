@@ -676,7 +676,7 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
                    mix_nu: float = 7.0, mix_flatten: float = 0.5) -> dict:
     """Learned proposal against the adaptive random walk, each with a burn-in of
     ``burn`` steps from the same start, then ``steps`` production steps each and
-    a long random-walk chain as the judge of the posterior means.
+    a long random-walk chain as the yardstick for the posterior means.
 
     Learned method: adaptive random walk for burn/2 steps; then for each
     denoising noise level the network is trained on every burn-in sample so far
