@@ -487,6 +487,25 @@ __device__ __forceinline__ int bulk_count(int NT, int t, int d) {
   return (d - t) * NT - (d * (d + 1) - t * (t + 1)) / 2;
 }
 
+// Cell of a T x T off-diagonal tile that thread k takes in the in-tile wavefront:
+// the cells in decreasing order of their finalisation step own = b + T-1 - a
+// (ties by row).  Each wavefront then holds cells of neighbouring steps: it has
+// nothing left to do after its first cell's step, and the cells finalised in one
+// step sit in one or two wavefronts.
+template <int T> __device__ __forceinline__ void cell_by_step(int k, int &a, int &b) {
+  a = 0;
+  b = T - 1;
+  for (int o = 2 * T - 2; o >= 0; --o) {
+    const int alo = max(0, T - 1 - o), ahi = min(T - 1, 2 * T - 2 - o), n = ahi - alo + 1;
+    if (k < n) {
+      a = alo + k;
+      b = o - (T - 1) + a;
+      return;
+    }
+    k -= n;
+  }
+}
+
 // One launch of the inside schedule (see the file header).
 template <class SR, int T>
 __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
@@ -495,6 +514,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   typedef typename SR::V V;
   __shared__ StepLDS<SR, T> sm;
   const int tid = threadIdx.x;
+  constexpr bool by_step = !SR::scaled;  // in-tile wavefront cells in step order (see below)
   // ---- work item.  The first workgroups (in dispatch order, so they start
   // first, round-robin over the XCDs) finalise the tiles of every instance:
   // prefix[0..n] counts them per instance; a few padding workgroups round their
@@ -646,16 +666,25 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
       sm.u.w.cL[a][b] = d ? Lr[(size_t)j * L + j] : SR::zero();
       sm.u.w.cF[a][b] = SR::zero();
     }
+  } else if (by_step) {
+    sm.u.w.cF[tid / T][tid % T] = accv;  // to the thread that takes the cell below
   }
   __syncthreads();
   // One thread per cell.  A cell's split-point sum is accumulated online: at
   // step s it adds the (at most two) terms whose newest operand was finalised at
-  // step s-1, and it finalises itself at its own step.
-  const int a = tid / T, b = tid % T;
+  // step s-1, and it finalises itself at its own step.  In the FP64 path
+  // off-diagonal tiles give the cells to the threads in step order
+  // (cell_by_step): a wavefront's cells then finish together and the costly
+  // finalisations of one step fall in one or two wavefronts; each thread reads its
+  // accumulated products back from the cell's F slot, which only the cell's own
+  // finalisation overwrites.  (In the FP32 path the extra index work costs more
+  // than it saves; its cells stay in row order.)
+  int a = tid / T, b = tid % T;
+  if (by_step && t > 0) cell_by_step<T>(tid, a, b);
   const int j = IT + a, y = JT + b;
   const bool mine = j < L && y < L && (t > 0 || a < b);
   const typename SR::FinRules rk = SR::fin_rules(r.p[6], r.lin[0], r.lin[2], r.lin[3], r.lin[4], r.lin[6]);
-  typename SR::Acc ac = SR::acc(accv);
+  typename SR::Acc ac = SR::acc(by_step && t > 0 ? sm.u.w.cF[a][b] : accv);
   if (t == 0) {
     const int own = b - a;
     for (int st = 1; st < T; ++st) {
@@ -703,7 +732,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     // primed before step 0: a border cell may finalise at step 0 (own = 0) or 1;
     // an inner cell's first reads are stale but unused (own >= 2)
     V fin_next = *fsrc;
-    auto step = [&](int st) {
+    auto work = [&](int st) {
       const V fin_f = fin_next;
       fin_next = *fsrc;
       const bool act = mine && st >= 1 && st <= own;
@@ -722,6 +751,21 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
         sm.u.w.cF[a][b] = vf;
         sm.u.w.cL[a][b] = vl;
       }
+    };
+    // the last step with work for this wavefront (a scalar, so the skip below is
+    // a uniform branch and every step has one barrier): that of its first cell in
+    // step order, or of the last cell of its first row in row order
+    int wlast;
+    if constexpr (by_step) {
+      int wa, wb;
+      cell_by_step<T>(tid & ~63, wa, wb);
+      wlast = wb + T - 1 - wa;
+    } else {
+      wlast = 2 * T - 2 - (tid & ~63) / T;
+    }
+    const int wave_last = __builtin_amdgcn_readfirstlane(wlast);
+    auto step = [&](int st) {
+      if (st <= wave_last) work(st);
       __syncthreads();
     };
     // The FP64 step (exponentials and logarithms) stays a rolled loop: unrolled,
