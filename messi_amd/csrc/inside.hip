@@ -506,6 +506,21 @@ template <int T> __device__ __forceinline__ void cell_by_step(int k, int &a, int
   }
 }
 
+// The outside's cell order: its cells finalise at step (T-1 - b) + a.
+template <int T> __device__ __forceinline__ void out_cell_by_step(int k, int &a, int &b) {
+  a = T - 1;
+  b = 0;
+  for (int o = 2 * T - 2; o >= 0; --o) {
+    const int alo = max(0, o - (T - 1)), ahi = min(T - 1, o), n = ahi - alo + 1;
+    if (k < n) {
+      a = alo + k;
+      b = a - o + (T - 1);
+      return;
+    }
+    k -= n;
+  }
+}
+
 // One launch of the inside schedule (see the file header).
 template <class SR, int T>
 __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
@@ -943,17 +958,20 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
     return;
   }
   // ---- finalise tile (I, J)
-  const int a = a0, b = b0, j = j0, y = y0;
-  typename SR::Acc acL = SR::acc(vL);
-  typename SR::Acc acS = SR::acc(vS);
+  // In the FP64 path the cells are given to the threads in order of their
+  // finalisation step (as in the inside step); the products' sums reach the
+  // cell's new thread through its W and Q slots, which only the cell's own
+  // finalisation overwrites (W and Q of a cell are read only once it is final).
+  constexpr bool by_step = !SR::scaled;
   {
+    const int a = a0, b = b0, j = j0, y = y0;
     const int rr = JT + a + 1, cc = JT + b;  // iSd[a][b] = iS[JT+a+1, JT+b]
     sm.u.w.iSd[a][b] = (rr < L && cc < L && rr <= cc) ? iSr[(size_t)rr * L + cc] : SR::zero();
     const int lr = IT + a, lc = IT + b - 1;  // iLd[a][b] = iL[IT+a, IT+b-1]
     sm.u.w.iLd[a][b] = (lc >= 0 && lr < L && lc < L && lr <= lc) ? iLc[(size_t)lc * L + lr] : SR::zero();
     sm.u.w.cP[a][b] = (j >= 1 && j <= y && y + 1 < L) ? ptv[(size_t)(j - 1) * L + (y + 1)] : SR::zero();
-    sm.u.w.W[a][b] = SR::zero();
-    sm.u.w.Q[a][b] = SR::zero();
+    sm.u.w.W[a][b] = by_step ? vL : SR::zero();
+    sm.u.w.Q[a][b] = by_step ? vS : SR::zero();
   }
   if (tid < T) {
     // Q on the row above the tile (IT-1, JT+b+1) and the column right of it (IT+b-1, JT+T)
@@ -973,6 +991,11 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
     sm.Qright[bq] = qr;
   }
   __syncthreads();
+  int a = a0, b = b0;
+  if constexpr (by_step) out_cell_by_step<T>(tid, a, b);
+  const int j = IT + a, y = JT + b;
+  typename SR::Acc acL = SR::acc(by_step ? sm.u.w.W[a][b] : vL);
+  typename SR::Acc acS = SR::acc(by_step ? sm.u.w.Q[a][b] : vS);
   const bool mine = j < L && y < L && j <= y;
   const bool lkeep = out_row_ok(d, T, od.lband);
   const int own = (T - 1 - b) + a;
@@ -986,7 +1009,17 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   const V *qsrc = q_in ? &sm.u.w.Q[a - 1][b + 1] : (a == 0 ? &sm.Qtop[b] : &sm.Qright[a]);
   const V pterm = sm.u.w.cP[a][b];
   V q_next = *qsrc;  // primed: a border cell may finalise at step 0 or 1 (see the inside step)
-  auto step = [&](int st) {
+  // the last step with work for this wavefront (scalar: a uniform skip, one barrier a step)
+  int wlast;
+  if constexpr (by_step) {
+    int wa, wb;
+    out_cell_by_step<T>(tid & ~63, wa, wb);
+    wlast = (T - 1 - wb) + wa;
+  } else {
+    wlast = 2 * T - 2;
+  }
+  const int wave_last = __builtin_amdgcn_readfirstlane(wlast);
+  auto work = [&](int st) {
     const V q_f = q_next;
     q_next = *qsrc;
     const bool act = mine && st >= 1 && st <= own, fn = mine && st == own;
@@ -1013,6 +1046,9 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
       sm.u.w.W[a][b] = SR::add(SR::mulc(fS, r.p[0]), SR::mulc(fF, r.p[4]));
       sm.u.w.Q[a][b] = SR::add(SR::add(SR::mulc(fS, r.p[2]), SR::mulc(fF, r.p[3])), SR::mulc(fL, r.p[6]));
     }
+  };
+  auto step = [&](int st) {
+    if (st <= wave_last) work(st);
     __syncthreads();
   };
   // as in the inside step: the FP64 step stays a rolled loop
