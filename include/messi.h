@@ -47,6 +47,13 @@ int messi_set_profiling(messi_ctx *ctx, int on);
 int messi_set_tile(messi_ctx *ctx, int tile);
 /* Checks the MFMA operand layouts the kernels rely on against a CPU product. */
 int messi_selftest(messi_ctx *ctx);
+/* Page-locked host memory on the context's device (hipHostMalloc), for callers
+ * that receive large results repeatedly (an L x L posterior matrix is copied into
+ * it at the link's full rate instead of through pageable staging); freed by
+ * messi_host_free, which does not use the context (it may be destroyed
+ * already).  No counterpart in the reference. */
+int messi_host_alloc(messi_ctx *ctx, size_t bytes, void **out);
+int messi_host_free(messi_ctx *ctx, void *p);
 
 /* Dataset handle: one-hot alignment, rooted binary tree, base-pair distance
  * bound and optional pair mask (mask[x + ncol*y] != 0 allows the pair, NULL

@@ -921,6 +921,22 @@ void messi_ctx_destroy(messi_ctx *ctx) {
 
 const char *messi_last_error(const messi_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int messi_host_alloc(messi_ctx *ctx, size_t bytes, void **out) {
+  if (!ctx || !out) return 2;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(bytes > 0, "empty host allocation");
+    MESSI_HIP(hipSetDevice(ctx->device));
+    MESSI_HIP(hipHostMalloc(out, bytes, hipHostMallocDefault));
+  });
+}
+
+// Does not touch the context (it may be destroyed already when a caller frees).
+int messi_host_free(messi_ctx *ctx, void *p) {
+  (void)ctx;
+  return p && hipHostFree(p) != hipSuccess ? 1 : 0;
+}
+
 int messi_set_profiling(messi_ctx *ctx, int on) {
   if (!ctx) return 2;
   ctx->profiling = on != 0;

@@ -85,6 +85,8 @@ SIGNATURES = {
     "messi_batch_eval_loaded": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp]),
     "messi_batch_set_tiles": (ctypes.c_int, [_vp, _ip]),
     "messi_batch_set_groups": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "messi_host_alloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "messi_host_free": (ctypes.c_int, [_vp, _vp]),
     "messi_mlp_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "messi_mlp_destroy": (None, [_vp]),
     "messi_mlp_num_params": (ctypes.c_int, [_vp, _i64]),
@@ -133,6 +135,21 @@ def load() -> ctypes.CDLL:
                 f.argtypes = args
             _lib = lib
     return _lib
+
+
+def pinned_empty(shape, dtype=np.float64, order: str = "C", ctx: Optional["Context"] = None) -> np.ndarray:
+    """Uninitialised array in page-locked host memory (messi_host_alloc), freed
+    when the array and its views are gone.  Device-to-host copies into it run at
+    the link's full rate."""
+    ctx = ctx or context()
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    p = ctypes.c_void_p()
+    ctx.check(ctx.lib.messi_host_alloc(ctx.handle, max(n, 1), ctypes.byref(p)))
+    buf = (ctypes.c_char * max(n, 1)).from_address(p.value)
+    fin = weakref.finalize(buf, ctx.lib.messi_host_free, None, ctypes.c_void_p(p.value))
+    fin.atexit = False  # at exit the process's memory goes with it
+    return np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape, order=order)
 
 
 def ptr(a: Optional[np.ndarray], t=_dp):
@@ -298,7 +315,10 @@ class DeviceDataset:
                                                ctypes.byref(z)))
         return z.value
 
-    def posterior(self, site, pair, prec=FP64, want_pairs: bool = True):
+    def posterior(self, site, pair, prec=FP64, want_pairs: bool = True, out: Optional[np.ndarray] = None):
+        """log Z, unpaired posteriors (L) and the L x L pair posteriors (Julia
+        layout).  out: an L x L float64 Fortran-ordered array to receive the pair
+        posteriors (e.g. from pinned_empty, reused across calls)."""
         P4, pi4, lw4 = site
         P16, pi16, lw16 = pair
         P4c, P16c = colmajor_P(P4), colmajor_P(P16)
@@ -309,7 +329,12 @@ class DeviceDataset:
         lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
         z = ctypes.c_double()
         unp = np.empty(self.L)
-        pp = np.empty((self.L, self.L), order="F") if want_pairs else None
+        if out is not None:
+            if out.shape != (self.L, self.L) or out.dtype != np.float64 or not out.flags.f_contiguous:
+                raise ValueError(f"out must be a Fortran-ordered float64 array of shape {(self.L, self.L)}")
+            pp = out
+        else:
+            pp = np.empty((self.L, self.L), order="F") if want_pairs else None
         self.ctx.check(self.ctx.lib.messi_posterior(self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c),
                                                     ptr(pi16), ptr(lw16), prec_code(prec), ctypes.byref(z),
                                                     ptr(unp), ptr(pp)))

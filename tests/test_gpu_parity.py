@@ -396,3 +396,22 @@ def test_batch_member_groups_are_bit_identical(ctx):
             b.set_groups(5)
     finally:
         b.close()
+
+
+def test_posterior_into_pinned_buffer(ctx, c1):
+    """messi_posterior writes into a caller's page-locked buffer (messi_host_alloc)
+    exactly what it returns in a fresh array; the buffer can be reused."""
+    _say("posterior into a pinned buffer")
+    dd = hotpath.device_dataset(c1["ds"])
+    site, pair = _stack_params(c1["p"])
+    L = c1["ds"].numcols
+    buf = lib.pinned_empty((L, L), order="F")
+    assert buf.shape == (L, L) and buf.flags.f_contiguous and buf.flags.writeable
+    for prec in ("fp64", "fp32", "fp64"):
+        z, unp, pp = dd.posterior(site, pair, prec)
+        z2, unp2, pp2 = dd.posterior(site, pair, prec, out=buf)
+        assert pp2 is buf
+        assert z == z2 and np.array_equal(unp, unp2) and np.array_equal(pp, buf)
+    with pytest.raises(ValueError):
+        dd.posterior(site, pair, "fp64", out=np.zeros((L, L)))   # C order
+    del buf
