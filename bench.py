@@ -515,6 +515,7 @@ def main():
     ap.add_argument("--samples", type=int, default=1024)
     ap.add_argument("--mea-lengths", type=int, nargs="+", default=[1500, 9000])
     ap.add_argument("--tile", type=int, default=0, help="inside tile edge 16 or 32 (default: library's)")
+    ap.add_argument("--window", type=int, default=0, help="inside bulk window 1, 2 or 4 (default: library's)")
     ap.add_argument("--precs", nargs="+", default=["fp64", "fp32"], help="precisions to time (profiling runs)")
     ap.add_argument("--no-posterior", action="store_true", help="skip the posterior error check (profiling runs)")
     ap.add_argument("--genome-spans", type=int, nargs="+", default=[1000000, 500])
@@ -538,6 +539,9 @@ def main():
     if args.tile:
         from messi_amd import lib
         lib.context().set_tile(args.tile)
+    if args.window:
+        from messi_amd import lib
+        lib.context().set_bulk_window(args.window)
     if args.instances:
         return main_instances(args)
     if args.mcmc:

@@ -45,6 +45,11 @@ const char *messi_last_error(const messi_ctx *ctx);
 int messi_set_profiling(messi_ctx *ctx, int on);
 /* Tile edge (16 or 32) of the inside schedule used by later calls on this context. */
 int messi_set_tile(messi_ctx *ctx, int tile);
+/* Bulk window (1, 2 or 4 launches) of the inside schedule used by later calls on
+ * this context: each bulk accumulator tile is read and written once per window,
+ * adding the split-point products of every launch in it.  Results do not depend
+ * on it.  Default 4.  No counterpart in the reference, whose inside has no tiles. */
+int messi_set_bulk_window(messi_ctx *ctx, int window);
 /* Checks the MFMA operand layouts the kernels rely on against a CPU product. */
 int messi_selftest(messi_ctx *ctx);
 /* Page-locked host memory on the context's device (hipHostMalloc), for callers

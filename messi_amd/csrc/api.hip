@@ -773,6 +773,7 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
   for (int m = 0; m < b->n; ++m) {
     ws[m] = &member_work(b, m, prec);
     ws[m]->tile = (m < (int)b->tiles.size() && b->tiles[m]) ? b->tiles[m] : ctx->tile;
+    ws[m]->window = ctx->window;
     ws[m]->zout = b->zall.as<double>() + 2 * m;
     // pair scores beyond the distance bound are -inf (pair_expand writes them so)
     const int D = b->mem[m]->maxbp, Lm = b->mem[m]->L;
@@ -953,6 +954,16 @@ int messi_set_tile(messi_ctx *ctx, int tile) {
   });
 }
 
+int messi_set_bulk_window(messi_ctx *ctx, int window) {
+  if (!ctx) return 2;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(window == 1 || window == 2 || window == 4, "bulk window must be 1, 2 or 4");
+    ctx->window = window;
+    ctx->w64.window = window;
+    ctx->w32.window = window;
+  });
+}
+
 int messi_selftest(messi_ctx *ctx) {
   if (!ctx) return 2;
   return guard(ctx, [&] {
@@ -1076,6 +1087,7 @@ int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, d
     hipStream_t s = ctx->stream;
     InsideWork &w = work(ctx, prec);
     w.tile = ctx->tile;
+    w.window = ctx->window;
     w.band = pairing_band(L, paired);
     const size_t LL = (size_t)L * L;
     auto go = [&](auto tag) {

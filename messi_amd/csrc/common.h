@@ -88,6 +88,7 @@ struct InsideBatch {
 struct InsideWork {
   int L = 0;
   int tile = 16;                  // tile edge of the inside schedule (16 or 32)
+  int window = 4;                 // bulk window of the inside schedule (1, 2 or 4 launches)
   int band = -1;                  // paired[j,y] is -inf for y - j > band (-1: no such bound)
   const void *exp = nullptr;      // pair-root expansion descriptor of this member (device), or null
   double *zout = nullptr;         // where log Z (absolute, baseline-relative) is also written (device)
@@ -136,6 +137,7 @@ struct messi_ctx {
   std::string err;
   bool profiling = false;
   int tile = 16;  // tile edge of the inside schedule
+  int window = 4;  // bulk window of the inside schedule
   messi::InsideWork w64, w32;
   messi::MeaWork mea;
   messi::SampleWork samp;

@@ -212,6 +212,23 @@ template <class SR, int T> struct ProdLDS<SR, T, true> {
   int rA[T], cB[T];
 };
 
+// Staging of one product-ready (scaled) tile product: aligned mantissas, the
+// row / column exponents, and the MFMA result over A's mantissas.
+template <int T> struct PackLDS {
+  union {
+    float Ap[T][T + 1];
+    float R[T][T + 1];
+  };
+  float Bp[T][T + 1];
+  int rA[T], cB[T];
+};
+template <class SR, int T, bool S = SR::scaled> struct PackArea {
+  char unused[4];
+};
+template <class SR, int T> struct PackArea<SR, T, true> {
+  PackLDS<T> q[4];  // the four products of one bulk call
+};
+
 template <class SR, int T> struct StepLDS {
   typedef typename SR::V V;
   V Ld[T][T + 1];   // L of the diagonal tile (I, I)
@@ -222,6 +239,7 @@ template <class SR, int T> struct StepLDS {
       V cS[T][T + 1], cF[T][T + 1], cL[T][T + 1];
     } w;                  // the tile being finalised
     ProdLDS<SR, T> p[2];  // the (at most two) split-point products of a launch
+    PackArea<SR, T> pk;   // scaled path: up to four products of a bulk window
   } u;
   V Sb[T], Fbot[T], Fleft[T];
 };
@@ -478,13 +496,19 @@ __device__ __forceinline__ void tile_products_packed(const InstDesc<XFv> &d, int
   __syncthreads();
 }
 
-// Bulk items per bulk workgroup (their loads overlap).
+// Bulk items per bulk workgroup (their loads overlap): two with a window of one
+// launch, one with a wider window.
 constexpr int BULK_PER = 2;
+__host__ __device__ constexpr int bulk_per(int W) { return W == 1 ? BULK_PER : 1; }
 
-// Bulk items of launch t: tiles at distance d = t+1 .. dmax, NT - d of them per
-// distance; C(d) counts the items of distances t+1 .. d.
-__device__ __forceinline__ int bulk_count(int NT, int t, int d) {
-  return (d - t) * NT - (d * (d + 1) - t * (t + 1)) / 2;
+// Bulk window W: a bulk tile at distance d is visited in the launches t = d-1,
+// d-1-W, d-1-2W, ... (those with 2t - 2 >= d), each visit adding the products
+// that became ready since the previous one, so its accumulator is read and
+// written once per W launches.  Bulk items of launch t: tiles at distances
+// t+1, t+1+W, ..., NT - d of them per distance d; bulk_items(k) counts the items
+// of the first k distances.
+__host__ __device__ __forceinline__ int bulk_items(int NT, int t, int W, int k) {
+  return k * (NT - t - 1) - W * (k * (k - 1) / 2);
 }
 
 // Cell of a T x T off-diagonal tile that thread k takes in the in-tile wavefront:
@@ -521,8 +545,69 @@ template <int T> __device__ __forceinline__ void out_cell_by_step(int k, int &a,
   }
 }
 
+// Up to four product-ready products k = 0..3 (pairs (0, 1) and (2, 3) of two
+// launches) folded into acc, one MFMA chain set per product on its own
+// wavefronts.  The fold repeats, per pair, the additions of tile_products_packed
+// followed by the caller's acc = acc + acc2, so the result has the same bits.
+template <int T>
+__device__ __forceinline__ void tile_products_packed4(const InstDesc<XFv> &d, int I, int J, const PackedOps *o,
+                                                      const int *K, const bool *dok, PackLDS<T> *pp, XFv &acc) {
+  const int tid = threadIdx.x, r = tid / T, c = tid % T;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (!dok[k]) continue;
+    pp[k].Ap[r][c] = o[k].a;
+    pp[k].Bp[c][r] = o[k].b;
+    if (c == 0) {
+      pp[k].rA[r] = o[k].ea;
+      pp[k].cB[r] = o[k].eb;
+    }
+  }
+  __syncthreads();
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const int wave = tid >> 6, lane = tid & 63;
+  constexpr int NB = T / 16, NB2 = NB * NB;
+  static_assert(T * T / 64 >= 4 * NB2, "one wavefront per 16 x 16 block of four products");
+  const int k = wave / NB2, blk = wave % NB2;
+  const bool mf = k < 4 && dok[k < 4 ? k : 0];
+  const int bi = blk / NB, bj = blk % NB;
+  v4f m = {0.f, 0.f, 0.f, 0.f};
+  if (mf) {
+#pragma unroll
+    for (int q = 0; q < T / 4; ++q) {
+      float av = pp[k].Ap[bi * 16 + (lane & 15)][4 * q + (lane >> 4)];
+      float bv = pp[k].Bp[4 * q + (lane >> 4)][bj * 16 + (lane & 15)];
+      m = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, m, 0, 0, 0);
+    }
+  }
+  if constexpr (NB > 1) __syncthreads();  // R shares storage with Ap
+  if (mf) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pp[k].R[bi * 16 + 4 * (lane >> 4) + q][bj * 16 + (lane & 15)] = m[q];
+  }
+  __syncthreads();
+  auto contrib = [&](int k2) {
+    const int ra = pp[k2].rA[r], cb = pp[k2].cB[c];
+    XFv v = XF::zero();
+    if (ra != XF::EZ && cb != XF::EZ) {
+      const float cv = pp[k2].R[r][c];
+      v = cv >= 0x1p-100f ? XF::norm(cv, ra + cb) : exact_cell_global<T>(d, I, K[k2], J, r, c);
+    }
+    return v;
+  };
+#pragma unroll
+  for (int h = 0; h < 4; h += 2) {
+    if (!(dok[h] || dok[h + 1])) continue;
+    if (dok[h]) acc = XF::add(acc, contrib(h));
+    XFv acc2 = XF::zero();
+    if (dok[h + 1]) acc2 = XF::add(acc2, contrib(h + 1));
+    acc = XF::add(acc, acc2);
+  }
+  __syncthreads();
+}
+
 // One launch of the inside schedule (see the file header).
-template <class SR, int T>
+template <class SR, int T, int W>
 __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
                                                             const int *__restrict__ prefix, int n, int t,
                                                             Rules<SR> r) {
@@ -530,6 +615,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   __shared__ StepLDS<SR, T> sm;
   const int tid = threadIdx.x;
   constexpr bool by_step = !SR::scaled;  // in-tile wavefront cells in step order (see below)
+  constexpr int BP = bulk_per(W);
   // ---- work item.  The first workgroups (in dispatch order, so they start
   // first, round-robin over the XCDs) finalise the tiles of every instance:
   // prefix[0..n] counts them per instance; a few padding workgroups round their
@@ -557,59 +643,87 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   V *__restrict__ acc = ins.acc;
   const V *__restrict__ ptv = ins.ptv;
   // ---- decode the work item: workgroups NT-t .. are bulk workgroups, each
-  // taking BULK_PER consecutive bulk items, with every item's loads issued
+  // taking bulk_per(W) consecutive bulk items, with every item's loads issued
   // before the first product is computed.
   const int item = bx - pre[lo];
   if (!fin_wg) {
+    // The tiles (I, I+d) at the k-th distance d = t+1+kW gather the products of
+    // the launches s = max(t-W+1, smin(d)) .. t in increasing s, the two of each
+    // s in the order a window of one launch adds them.
     int dmax = min(2 * t - 2, NT - 1);
-    if (t - 1 > ins.KB) dmax = min(dmax, t - 1 + ins.KB);
-    const int nbulk = dmax >= t + 1 ? bulk_count(NT, t, dmax) : 0;
-    const int first = BULK_PER * item;
-    int Ib[BULK_PER], Jb[BULK_PER];
-    bool has[BULK_PER], d1[BULK_PER], d2[BULK_PER];
-    V av[BULK_PER];
-    Operands<SR> ob1[BULK_PER], ob2[BULK_PER];
-    PackedOps qb1[BULK_PER], qb2[BULK_PER];
+    if (t - W > ins.KB) dmax = min(dmax, t - 1 + ins.KB);
+    const int nd = dmax >= t + 1 ? (dmax - t - 1) / W + 1 : 0;
+    const int nbulk = bulk_items(NT, t, W, nd);
+    const int first = BP * item;
+    int Ib[BP], Jb[BP], smin[BP];
+    bool has[BP], d1[BP][W], d2[BP][W];
+    V av[BP];
+    Operands<SR> ob1[BP][W], ob2[BP][W];
+    PackedOps qb1[BP][W], qb2[BP][W];
     const int a = tid / T, b = tid % T;
 #pragma unroll
-    for (int u = 0; u < BULK_PER; ++u) {
+    for (int u = 0; u < BP; ++u) {
       const int idx = first + u;
       has[u] = idx < nbulk;
-      int dl = t + 1, dh = NT - 1;  // smallest distance d with bulk_count(d) > idx
-      while (dl < dh) {
-        const int mid = (dl + dh) >> 1;
-        if (bulk_count(NT, t, mid) > idx) dh = mid;
-        else dl = mid + 1;
+      int lo = 0, hi = nd - 1;  // the distance: bulk_items(lo) <= idx < bulk_items(lo + 1)
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bulk_items(NT, t, W, mid) <= idx) lo = mid;
+        else hi = mid - 1;
       }
-      Ib[u] = idx - bulk_count(NT, t, dl - 1);
-      Jb[u] = Ib[u] + dl;
-      const int k1 = Ib[u] + t - 1, k2 = Jb[u] - (t - 1);
-      d1[u] = has[u] && k1 - Ib[u] <= ins.KB;
-      d2[u] = has[u] && k2 != k1 && k2 - Ib[u] <= ins.KB;
+      const int dist = t + 1 + lo * W;
+      Ib[u] = idx - bulk_items(NT, t, W, lo);
+      Jb[u] = Ib[u] + dist;
+      smin[u] = max(t - W + 1, (dist + 3) / 2);  // a product of launch s needs dist <= 2s - 2
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
       av[u] = (has[u] && j < L && y < L) ? acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid] : SR::zero();
-      if constexpr (SR::scaled) {
-        if (d1[u]) qb1[u] = fetch_packed<T>(ins, Ib[u], k1, Jb[u]);
-        if (d2[u]) qb2[u] = fetch_packed<T>(ins, Ib[u], k2, Jb[u]);
-      } else {
-        if (d1[u]) ob1[u] = fetch_operands<SR, T>(inside_src<SR, T>(Ib[u], k1, Jb[u], Lr, Sc), L);
-        if (d2[u]) ob2[u] = fetch_operands<SR, T>(inside_src<SR, T>(Ib[u], k2, Jb[u], Lr, Sc), L);
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const int s = t - W + 1 + i, k1 = Ib[u] + s - 1, k2 = Jb[u] - (s - 1);
+        d1[u][i] = has[u] && s >= smin[u] && k1 - Ib[u] <= ins.KB;
+        d2[u][i] = has[u] && s >= smin[u] && k2 != k1 && k2 - Ib[u] <= ins.KB;
+        if constexpr (SR::scaled) {
+          if (d1[u][i]) qb1[u][i] = fetch_packed<T>(ins, Ib[u], k1, Jb[u]);
+          if (d2[u][i]) qb2[u][i] = fetch_packed<T>(ins, Ib[u], k2, Jb[u]);
+        } else {
+          if (d1[u][i]) ob1[u][i] = fetch_operands<SR, T>(inside_src<SR, T>(Ib[u], k1, Jb[u], Lr, Sc), L);
+          if (d2[u][i]) ob2[u][i] = fetch_operands<SR, T>(inside_src<SR, T>(Ib[u], k2, Jb[u], Lr, Sc), L);
+        }
       }
     }
 #pragma unroll
-    for (int u = 0; u < BULK_PER; ++u) {
+    for (int u = 0; u < BP; ++u) {
       if (!has[u]) continue;
-      const int k1 = Ib[u] + t - 1, k2 = Jb[u] - (t - 1);
-      if (d1[u] || d2[u]) {
-        V acc2 = SR::zero();
-        if constexpr (SR::scaled)
-          tile_products_packed<T>(ins, Ib[u], Jb[u], qb1[u], k1, d1[u], qb2[u], k2, d2[u], sm.u.p, av[u], acc2);
-        else
-          tile_products<SR, T>(ob1[u], d1[u], ob2[u], d2[u], sm.u.p, av[u], acc2);
-        av[u] = SR::add(av[u], acc2);
+      bool any = false;
+      if constexpr (SR::scaled && W >= 2) {
+        // two launches' products per call, four wavefronts on MFMA
+#pragma unroll
+        for (int i = 0; i < W; i += 2) {
+          const bool dk[4] = {d1[u][i], d2[u][i], d1[u][i + 1], d2[u][i + 1]};
+          if (!(dk[0] || dk[1] || dk[2] || dk[3])) continue;
+          any = true;
+          const int s = t - W + 1 + i;
+          const int Ks[4] = {Ib[u] + s - 1, Jb[u] - (s - 1), Ib[u] + s, Jb[u] - s};
+          const PackedOps ops[4] = {qb1[u][i], qb2[u][i], qb1[u][i + 1], qb2[u][i + 1]};
+          tile_products_packed4<T>(ins, Ib[u], Jb[u], ops, Ks, dk, sm.u.pk.q, av[u]);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+          if (!(d1[u][i] || d2[u][i])) continue;
+          any = true;
+          const int s = t - W + 1 + i, k1 = Ib[u] + s - 1, k2 = Jb[u] - (s - 1);
+          V acc2 = SR::zero();
+          if constexpr (SR::scaled)
+            tile_products_packed<T>(ins, Ib[u], Jb[u], qb1[u][i], k1, d1[u][i], qb2[u][i], k2, d2[u][i], sm.u.p,
+                                    av[u], acc2);
+          else
+            tile_products<SR, T>(ob1[u][i], d1[u][i], ob2[u][i], d2[u][i], sm.u.p, av[u], acc2);
+          av[u] = SR::add(av[u], acc2);
+        }
       }
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
-      if (j < L && y < L) acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid] = av[u];
+      if (any && j < L && y < L) acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid] = av[u];
     }
     return;
   }
@@ -1309,6 +1423,8 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
   typedef typename SR::V V;
   if (n <= 0) return;
   const int TE = ws[0]->tile == 32 ? 32 : 16;
+  const int W = ws[0]->window;
+  MESSI_REQUIRE(W == 1 || W == 2 || W == 4, "bulk window must be 1, 2 or 4");
   std::vector<InstDesc<V>> d(n);
   int Lmax = 0, NTmax = 0;
   for (int m = 0; m < n; ++m) {
@@ -1339,10 +1455,10 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
       if (t < NT) {
         int nbulk = 0;
         int dmax = std::min(2 * t - 2, NT - 1);
-        if (t - 1 > KB) dmax = std::min(dmax, t - 1 + KB);
-        for (int dist = t + 1; dist <= dmax; ++dist) nbulk += NT - dist;
+        if (t - W > KB) dmax = std::min(dmax, t - 1 + KB);
+        for (int dist = t + 1; dist <= dmax; dist += W) nbulk += NT - dist;
         fa += NT - t;
-        ba += (nbulk + BULK_PER - 1) / BULK_PER;
+        ba += (nbulk + bulk_per(W) - 1) / bulk_per(W);
       }
     }
     fp[n] = fa;
@@ -1369,10 +1485,18 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
                      lp[5]);
   for (int t = 0; t < NTmax; ++t) {
     const int *pf = ib.prefix.as<int>() + (size_t)t * stride;
-    if (TE == 32)
-      hipLaunchKernelGGL((inside_step_kernel<SR, 32>), dim3(total[t]), dim3(32 * 32), 0, s, di, pf, n, t, r);
-    else
-      hipLaunchKernelGGL((inside_step_kernel<SR, 16>), dim3(total[t]), dim3(16 * 16), 0, s, di, pf, n, t, r);
+    auto go = [&](auto kern, int threads) {
+      hipLaunchKernelGGL(kern, dim3(total[t]), dim3(threads), 0, s, di, pf, n, t, r);
+    };
+    if (TE == 32) {
+      if (W == 4) go(inside_step_kernel<SR, 32, 4>, 32 * 32);
+      else if (W == 2) go(inside_step_kernel<SR, 32, 2>, 32 * 32);
+      else go(inside_step_kernel<SR, 32, 1>, 32 * 32);
+    } else {
+      if (W == 4) go(inside_step_kernel<SR, 16, 4>, 16 * 16);
+      else if (W == 2) go(inside_step_kernel<SR, 16, 2>, 16 * 16);
+      else go(inside_step_kernel<SR, 16, 1>, 16 * 16);
+    }
   }
   hipLaunchKernelGGL(zlog_kernel<SR>, dim3((n + 63) / 64), dim3(64), 0, s, di, n);
   MESSI_HIP(hipGetLastError());

@@ -76,6 +76,7 @@ SIGNATURES = {
     "messi_posterior_consensus": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip]),
     "messi_set_tile": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "messi_set_bulk_window": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "messi_batch_destroy": (None, [_vp]),
     "messi_batch_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
@@ -199,6 +200,10 @@ class Context:
 
     def set_tile(self, tile: int) -> None:
         self.check(self.lib.messi_set_tile(self._h, int(tile)))
+
+    def set_bulk_window(self, window: int) -> None:
+        """Launches per bulk accumulator visit of the inside schedule (1, 2 or 4; default 4)."""
+        self.check(self.lib.messi_set_bulk_window(self._h, int(window)))
 
     def close(self) -> None:
         if self._h:
