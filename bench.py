@@ -601,13 +601,13 @@ def main():
         achieved = args.batch * 2 * T_in / (r["phase_ms"]["inside"] * 1e-3) / 1e12
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
                            "frac": achieved / PEAK["fp32"], "traffic": None,
-                           "kernel": f"inside_step_kernel<XF,16> (inside phase of a batch of {args.batch}, 2 flops "
+                           "kernel": f"inside_step_kernel<XF,16,4> (inside phase of a batch of {args.batch}, 2 flops "
                                      f"per bifurcation term, {int(T_in)} terms per evaluation)"}
         # HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE passes of this same workload
         # (copy-calibrated); only quoted for the shape the counters were taken on
         tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                           "inside_xf16_config2_batch16_traffic.json")
-        if args.batch == 16 and args.tile in (0, 16) and os.path.exists(tf):
+        if args.batch == 16 and args.tile in (0, 16) and args.window in (0, 4) and os.path.exists(tf):
             with open(tf) as f:
                 tj = json.load(f)
             out["roofline"]["traffic"] = tj["hbm_bytes_per_launch"]
