@@ -98,7 +98,7 @@ struct InsideWork {
   DevBuf oSr, oSc, oFr, oL;       // outside tables
   DevBuf oWr, oWc, oaL, oaS;      // outside: parent weight W (row/column-major), accumulators
   DevBuf orng;                    // outside launch tables
-  int orng_key[4] = {0, 0, 0, 0};   // (L, tile, band, lband) the launch tables were built for
+  int orng_key[5] = {0, 0, 0, 0, 0};  // (L, tile, band, lband, window) the launch tables were built for
   DevBuf ptv;                     // scaled pair term per cell
   DevBuf lam, C;                  // per-position baseline and its prefix sums (double)
   DevBuf uhat;                    // u - lambda (double)

@@ -1005,7 +1005,7 @@ template <class SR, int T> struct OutLDS {
   V Qtop[T], Qright[T];
 };
 
-template <class SR, int T>
+template <class SR, int T, int W>
 __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const OutRange *__restrict__ ranges,
                                                              int u, Rules<SR> r) {
   typedef typename SR::V V;
@@ -1050,26 +1050,55 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
     m = du + 1 - d;
   }
   const int J = I + d, IT = I * T, JT = J * T;
-  const bool dorow = out_row_ok(d, T, od.lband) && J + m <= NT - 1;
-  const bool docol = out_col_ok(m, T, od.band) && I - m >= 0;
-  // ---- accumulators and the ready products (all global loads issued first)
   const int a0 = tid / T, b0 = tid % T, j0 = IT + a0, y0 = JT + b0;
   const bool cell0 = j0 < L && y0 < L;
   V vL = SR::zero(), vS = SR::zero();
-  if (cell0 && (fin || dorow)) vL = accL[(size_t)j0 * L + y0];
-  if (cell0 && (fin || docol)) vS = accS[(size_t)j0 * L + y0];
-  const OpSrc<SR> srow{Wr, IT, (J + m) * T, true, iSr, JT + 1, (J + m) * T, true};
-  const OpSrc<SR> scol{iLc, IT - 1, (I - m) * T, false, Wc, JT, (I - m) * T, false};
-  Operands<SR> orow, ocol;
-  if (dorow) orow = fetch_operands<SR, T>(srow, L);
-  if (docol) ocol = fetch_operands<SR, T>(scol, L);
-  if (dorow || docol) tile_products<SR, T>(orow, dorow, ocol, docol, sm.u.p, vL, vS);
   if (!fin) {
+    // A bulk tile is visited once per W launches (the host table lists it only
+    // in the launches with (m - 2) % W == 0) and adds the products of the
+    // offsets m + W - 1 .. m, in that order: the order of the launches that made
+    // them ready.
+    unsigned rows = 0, cols = 0;  // bit i: offset m + W - 1 - i has a row / column product
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      const int mi = m + W - 1 - i;
+      if (out_row_ok(d, T, od.lband) && J + mi <= NT - 1) rows |= 1u << i;
+      if (out_col_ok(mi, T, od.band) && I - mi >= 0) cols |= 1u << i;
+    }
+    if (cell0 && rows) vL = accL[(size_t)j0 * L + y0];
+    if (cell0 && cols) vS = accS[(size_t)j0 * L + y0];
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      const bool dr = (rows >> i) & 1u, dc = (cols >> i) & 1u;
+      if (!(dr || dc)) continue;
+      const int mi = m + W - 1 - i;
+      const OpSrc<SR> srow{Wr, IT, (J + mi) * T, true, iSr, JT + 1, (J + mi) * T, true};
+      const OpSrc<SR> scol{iLc, IT - 1, (I - mi) * T, false, Wc, JT, (I - mi) * T, false};
+      Operands<SR> orow, ocol;
+      if (dr) orow = fetch_operands<SR, T>(srow, L);
+      if (dc) ocol = fetch_operands<SR, T>(scol, L);
+      tile_products<SR, T>(orow, dr, ocol, dc, sm.u.p, vL, vS);
+    }
     if (cell0) {
-      if (dorow) accL[(size_t)j0 * L + y0] = vL;
-      if (docol) accS[(size_t)j0 * L + y0] = vS;
+      if (rows) accL[(size_t)j0 * L + y0] = vL;
+      if (cols) accS[(size_t)j0 * L + y0] = vS;
     }
     return;
+  }
+  // ---- the finalising workgroup's own products (offset 1), loads issued first
+  {
+    const bool dorow = out_row_ok(d, T, od.lband) && J + 1 <= NT - 1;
+    const bool docol = out_col_ok(1, T, od.band) && I - 1 >= 0;
+    if (cell0) {
+      vL = accL[(size_t)j0 * L + y0];
+      vS = accS[(size_t)j0 * L + y0];
+    }
+    const OpSrc<SR> srow{Wr, IT, (J + 1) * T, true, iSr, JT + 1, (J + 1) * T, true};
+    const OpSrc<SR> scol{iLc, IT - 1, (I - 1) * T, false, Wc, JT, (I - 1) * T, false};
+    Operands<SR> orow, ocol;
+    if (dorow) orow = fetch_operands<SR, T>(srow, L);
+    if (docol) ocol = fetch_operands<SR, T>(scol, L);
+    if (dorow || docol) tile_products<SR, T>(orow, dorow, ocol, docol, sm.u.p, vL, vS);
   }
   // ---- finalise tile (I, J)
   // In the FP64 path the cells are given to the threads in order of their
@@ -1548,17 +1577,22 @@ void outside_run(hipStream_t s, InsideWork &w, int L, bool full_L) {
   const int lband = full_L ? -1 : band;
   // launch tables: the bulk tile ranges of every launch (host, cached per shape)
   std::vector<int> total(NT);
-  const int key[4] = {L, TE, band, lband};
-  const bool fresh = std::equal(key, key + 4, w.orng_key);
+  const int W = w.window;
+  MESSI_REQUIRE(W == 1 || W == 2 || W == 4, "bulk window must be 1, 2 or 4");
+  const int key[5] = {L, TE, band, lband, W};
+  const bool fresh = std::equal(key, key + 5, w.orng_key);
   std::vector<OutRange> rg(fresh ? 0 : (size_t)NT * NT);
   for (int u = 0; u < NT; ++u) {
     const int du = NT - 1 - u;
     int cum = 0;
     for (int d = 0; d < du; ++d) {
+      // offset m = du + 1 - d >= 2; a tile is visited when (m - 2) % W == 0, and
+      // its tiles with a ready product in the window are those of offset m
       const int m = du + 1 - d;
-      int rend = out_row_ok(d, TE, lband) ? u : 0;
+      const bool visit = (m - 2) % W == 0;
+      int rend = visit && out_row_ok(d, TE, lband) ? u : 0;
       int cbeg = rend, cend = rend;
-      if (out_col_ok(m, TE, band) && m <= NT - 1 - d) {
+      if (visit && out_col_ok(m, TE, band) && m <= NT - 1 - d) {
         cbeg = m;
         cend = NT - d;
         if (cbeg <= rend) {  // the two ranges meet: one range
@@ -1573,7 +1607,7 @@ void outside_run(hipStream_t s, InsideWork &w, int L, bool full_L) {
   }
   if (!fresh) {
     upload(w.orng, rg, s);
-    std::copy(key, key + 4, w.orng_key);
+    std::copy(key, key + 5, w.orng_key);
   }
   OutDesc od;
   od.L = L;
@@ -1595,10 +1629,16 @@ void outside_run(hipStream_t s, InsideWork &w, int L, bool full_L) {
                      w.oaS.as<V>());
   const OutRange *dr = w.orng.as<OutRange>();
   for (int u = 0; u < NT; ++u) {
-    if (TE == 32)
-      hipLaunchKernelGGL((outside_step_kernel<SR, 32>), dim3(total[u]), dim3(32 * 32), 0, s, od, dr, u, r);
-    else
-      hipLaunchKernelGGL((outside_step_kernel<SR, 16>), dim3(total[u]), dim3(16 * 16), 0, s, od, dr, u, r);
+    auto go = [&](auto kern, int threads) { hipLaunchKernelGGL(kern, dim3(total[u]), dim3(threads), 0, s, od, dr, u, r); };
+    if (TE == 32) {
+      if (W == 4) go(outside_step_kernel<SR, 32, 4>, 32 * 32);
+      else if (W == 2) go(outside_step_kernel<SR, 32, 2>, 32 * 32);
+      else go(outside_step_kernel<SR, 32, 1>, 32 * 32);
+    } else {
+      if (W == 4) go(outside_step_kernel<SR, 16, 4>, 16 * 16);
+      else if (W == 2) go(outside_step_kernel<SR, 16, 2>, 16 * 16);
+      else go(outside_step_kernel<SR, 16, 1>, 16 * 16);
+    }
   }
   MESSI_HIP(hipGetLastError());
 }
