@@ -81,9 +81,16 @@ template <typename V, int T> struct MeaLDS {
   } p[2];
 };
 
-// Bulk items of launch t: tiles at distance t+1 .. d, NT - d' of them for each d'.
-__device__ __forceinline__ int mea_bulk_count(int NT, int t, int d) {
-  return (d - t) * NT - (d * (d + 1) - t * (t + 1)) / 2;
+// Bulk window: a bulk tile at distance d is visited in the launches t = d-1,
+// d-1-W, ... (those with 2t - 2 >= d), each visit folding the products of the
+// launches since the previous one, so its accumulator is read and written once
+// per W launches.  The fold keeps the first maximum whatever the order, so the
+// window does not change any result.  Bulk items of launch t: the tiles at
+// distances t+1, t+1+W, ..., NT - d of them per distance d; mea_bulk_items(k)
+// counts the items of the first k distances.
+constexpr int MEA_W = 4;
+__host__ __device__ __forceinline__ int mea_bulk_items(int NT, int t, int k) {
+  return k * (NT - t - 1) - MEA_W * (k * (k - 1) / 2);
 }
 
 template <typename V>
@@ -115,33 +122,37 @@ __global__ __launch_bounds__(T * T) void mea_step_kernel(MeaDesc<V> d, int t) {
   } else {
     fin = false;
     const int idx = item - (NT - t);
-    int lo = t + 1, hi = min(2 * t - 2, NT - 1);  // smallest distance with count > idx
+    const int nd = (min(2 * t - 2, NT - 1) - t - 1) / MEA_W + 1;  // distances with bulk items (>= 1 here)
+    int lo = 0, hi = nd - 1;  // the distance: mea_bulk_items(lo) <= idx < mea_bulk_items(lo + 1)
     while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (mea_bulk_count(NT, t, mid) > idx) hi = mid;
-      else lo = mid + 1;
+      const int mid = (lo + hi + 1) >> 1;
+      if (mea_bulk_items(NT, t, mid) <= idx) lo = mid;
+      else hi = mid - 1;
     }
-    I = idx - mea_bulk_count(NT, t, lo - 1);
-    J = I + lo;
+    I = idx - mea_bulk_items(NT, t, lo);
+    J = I + t + 1 + lo * MEA_W;
   }
   const int IT = I * T, JT = J * T;
   const int a = tid / T, b = tid % T;  // this thread's cell
   const int i = IT + a, j = JT + b;
   const bool inside_L = i < L && j < L;
-  // ---- split-point products that became ready with launch t-1
-  const int K1 = I + t - 1, K2 = J - t + 1;
-  const bool do1 = t >= 2, do2 = t >= 2 && K2 != K1;
-  // The products of a tile at distance d run in launches ceil(d/2)+1 .. d: the
-  // accumulator holds earlier ones from launch ceil(d/2)+2 on (d >= 4 when finalising).
-  const int dist = J - I;
-  const bool prior = fin ? t >= 4 : t > (dist + 1) / 2 + 1;
+  // ---- split-point products that became ready with launches t-W+1 .. t (bulk)
+  // or t (finalising).  The products of a tile at distance d run in launches
+  // smin = ceil(d/2)+1 .. d: the accumulator holds earlier ones once a bulk visit
+  // came before (d >= 4 when finalising).
+  const int dist = J - I, smin = (dist + 3) / 2;
+  const bool prior = fin ? t >= 4 : t - MEA_W >= smin;
   V bv = neg_inf<V>();
   int bk = 0x7fffffff;
   if (prior && inside_L) {
     bv = d.accV[(size_t)i * L + j];
     bk = d.accK[(size_t)i * L + j];
   }
-  if (do1) {
+#pragma unroll 1
+  for (int st = fin ? t : max(t - MEA_W + 1, smin); st <= t; ++st) {
+    const int K1 = I + st - 1, K2 = J - st + 1;
+    const bool do1 = st >= 2, do2 = st >= 2 && K2 != K1;
+    if (!do1) continue;
     // coalesced loads: the fast index runs down a column
     const int r = tid % T, c = tid / T;
 #pragma unroll
@@ -170,6 +181,7 @@ __global__ __launch_bounds__(T * T) void mea_step_kernel(MeaDesc<V> d, int t) {
       }
       fold(pv, pk, bv, bk);
     }
+    __syncthreads();  // the next launch's operands reuse the staging
   }
   if (!fin) {
     if (inside_L) {
@@ -404,10 +416,9 @@ void mea_run(hipStream_t s, MeaWork &w, int L, const double *d_single, const dou
   d.accV = w.accV.as<V>();
   d.accK = w.accK.as<int>();
   const int NT = d.NT;
-  auto bulk_count = [&](int t, int dd) { return (dd - t) * NT - (dd * (dd + 1) - t * (t + 1)) / 2; };
   for (int t = 0; t < NT; ++t) {
     const int dmax = std::min(2 * t - 2, NT - 1);
-    const int nb = dmax >= t + 1 ? bulk_count(t, dmax) : 0;
+    const int nb = dmax >= t + 1 ? mea_bulk_items(NT, t, (dmax - t - 1) / MEA_W + 1) : 0;
     hipLaunchKernelGGL((mea_step_kernel<V, T>), dim3(NT - t + nb), dim3(T * T), 0, s, d, t);
   }
   MESSI_HIP(hipGetLastError());
