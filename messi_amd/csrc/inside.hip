@@ -650,8 +650,10 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     // The tiles (I, I+d) at the k-th distance d = t+1+kW gather the products of
     // the launches s = max(t-W+1, smin(d)) .. t in increasing s, the two of each
     // s in the order a window of one launch adds them.
+    // beyond distance t-1+KB only a product with s-1 <= KB could be in the band,
+    // and such a product is ready only for distances <= 2KB < t-1+KB
     int dmax = min(2 * t - 2, NT - 1);
-    if (t - W > ins.KB) dmax = min(dmax, t - 1 + ins.KB);
+    if (t - 1 > ins.KB) dmax = min(dmax, t - 1 + ins.KB);
     const int nd = dmax >= t + 1 ? (dmax - t - 1) / W + 1 : 0;
     const int nbulk = bulk_items(NT, t, W, nd);
     const int first = BP * item;
@@ -1484,7 +1486,7 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
       if (t < NT) {
         int nbulk = 0;
         int dmax = std::min(2 * t - 2, NT - 1);
-        if (t - W > KB) dmax = std::min(dmax, t - 1 + KB);
+        if (t - 1 > KB) dmax = std::min(dmax, t - 1 + KB);
         for (int dist = t + 1; dist <= dmax; dist += W) nbulk += NT - dist;
         fa += NT - t;
         ba += (nbulk + bulk_per(W) - 1) / bulk_per(W);
