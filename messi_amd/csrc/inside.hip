@@ -18,10 +18,12 @@
 // finalises every tile (I, I+t) with an in-LDS wavefront over its cells, from
 // (a) split points h inside tile I, (c) split points inside tile J and (b) the
 // bulk of split points in the tiles strictly between, accumulated earlier: the
-// tile product (I,K) x (K,J) runs in launch max(K-I, J-K) + 1, as soon as both
-// operand tiles are final, either inside the workgroup that finalises (I,J) or
-// in a workgroup of its own.  A launch therefore carries the cell wavefronts
-// of one tile-diagonal plus the split-point products that just became ready.
+// tile product (I,K) x (K,J) becomes ready in launch max(K-I, J-K) + 1, when
+// both operand tiles are final, and runs either in that launch inside the
+// workgroup that finalises (I,J) (K = I+1 and K = J-1) or in the tile's next
+// bulk visit (a workgroup of its own every W launches, see bulk_items).  A
+// launch therefore carries the cell wavefronts of one tile-diagonal plus the
+// split-point products that became ready since the tiles' previous visits.
 #include <algorithm>
 #include <cstring>
 

@@ -21,9 +21,10 @@
 // (I, I+t).  The k in a tile K strictly between I and J form the split-point
 // product U(I,K) x E(rows KT+1..KT+T, J); the product for K becomes ready when
 // both operand tiles are final, after launch max(K-I, J-K), and runs in launch
-// max(K-I, J-K)+1: in the finalising workgroup for K = I+1 and K = J-1, in bulk
-// workgroups (tiles at distance t+1 .. 2t-2, at most two products each) for the
-// rest, accumulated as (value, first k) in HBM.  The remaining k (inside tile I
+// max(K-I, J-K)+1: in the finalising workgroup for K = I+1 and K = J-1; the rest
+// in the tile's next bulk visit (a bulk workgroup per tile every MEA_W launches,
+// folding the products that became ready since its previous visit), accumulated
+// as (value, first k) in HBM.  The remaining k (inside tile I
 // and tile J) are pulled by an in-LDS wavefront over the tile's cells, one
 // column and one row term per cell per step.
 #include <algorithm>
