@@ -594,7 +594,8 @@ def main():
                                f"{args.batch} parameter vectors per step (one per chain)",
                    "L": L, "nseq": ds.numseqs, "batch": args.batch, "sum_U": stats["sum_U"],
                    "root_entries": stats["root_entries"],
-                   "member_groups": args.groups or "automatic"},
+                   "member_groups": args.groups or "automatic",
+                   "bulk_window": args.window or 4},
         "variants": res,
     }
     if lead == "fp32":
