@@ -100,12 +100,18 @@ def time_batch(b, sites, pairs, prec, steps, warmup):
     ms_host = (time.perf_counter() - t0) * 1e3 / steps
     ctx.set_profiling(True)
     ph = np.zeros(5)
+    steps_ms, steps_n = np.zeros(2), np.zeros(2)
     nprof = max(3, min(10, steps))
     for _ in range(nprof):
         b.eval_loaded(1.0, prec)
         ph += b.timings()
+        sm, sn = b.step_timings()
+        steps_ms += sm
+        steps_n += sn
     ph /= nprof
     ctx.set_profiling(False)
+    # HIP events around the inside step kernels alone: their mean launch duration
+    b.step_launch_us = float(steps_ms.sum() / steps_n.sum() * 1e3) if steps_n.sum() else None
     return ms, z, ph, ms_host
 
 
@@ -579,7 +585,8 @@ def main():
                                 "phase_ms": {"site": ph1[0], "pair": ph1[1], "inside": ph1[2]}},
                      f"batch{args.batch}": {"ms_per_step": msB, "evals_per_s": args.batch * 1e3 / msB,
                                             "evals_per_s_host_inputs": args.batch * 1e3 / msBh,
-                                            "phase_ms": {"site": phB[0], "pair": phB[1], "inside": phB[2]}},
+                                            "phase_ms": {"site": phB[0], "pair": phB[1], "inside": phB[2]},
+                                            "inside_step_launch_us": bB.step_launch_us},
                      "logZ_batch": zB.tolist()}
     T_in = (L ** 3 - L) / 6
     lead = "fp32" if "fp32" in res else args.precs[0]
@@ -599,11 +606,18 @@ def main():
         "variants": res,
     }
     if lead == "fp32":
-        achieved = args.batch * 2 * T_in / (r["phase_ms"]["inside"] * 1e-3) / 1e12
+        # algorithmic flops per launch (2 per bifurcation term, the batch's terms spread over
+        # the NT step launches) over the step kernel's mean launch duration (HIP events)
+        NT = -(-L // (args.tile or 16))
+        flops_launch = args.batch * 2 * T_in / NT
+        launch_us = r.get("inside_step_launch_us") or r["phase_ms"]["inside"] * 1e3 / NT
+        achieved = flops_launch / (launch_us * 1e-6) / 1e12
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
                            "frac": achieved / PEAK["fp32"], "traffic": None,
-                           "kernel": f"inside_step_kernel<XF,16,4> (inside phase of a batch of {args.batch}, 2 flops "
-                                     f"per bifurcation term, {int(T_in)} terms per evaluation)"}
+                           "kernel": f"inside_step_kernel<XF,16,4> (batch of {args.batch}: {flops_launch:.4g} flops "
+                                     f"per launch = 2 per bifurcation term, {int(T_in)} terms per evaluation, "
+                                     f"{NT} launches; mean launch {launch_us:.1f} us)",
+                           "inside_phase_tflops": args.batch * 2 * T_in / (r["phase_ms"]["inside"] * 1e-3) / 1e12}
         # HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE passes of this same workload
         # (copy-calibrated); only quoted for the shape the counters were taken on
         tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",

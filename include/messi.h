@@ -246,6 +246,12 @@ int messi_batch_eval(messi_batch *b, int K4, const double *P4, const double *pi4
                      int K16, const double *P16, const double *pi16, const double *logw16, double B,
                      int prec, double *logZ);
 int messi_batch_timings(const messi_batch *b, double *t);
+/* Device time of the inside step kernels alone in the last evaluation made with
+ * profiling on (messi_set_profiling): ms[0], launches[0] for the tile-16
+ * members, ms[1], launches[1] for the tile-32 ones (0 where none), measured with
+ * HIP events on the context's stream around the launches.  No counterpart in
+ * the reference. */
+int messi_batch_step_timings(messi_batch *b, double *ms, int *launches);
 /* messi_batch_eval in two steps: messi_batch_load copies the mixture
  * components (same arguments and layouts as messi_batch_eval) into device
  * memory, converted to the precision prec; messi_batch_eval_loaded evaluates

@@ -166,6 +166,8 @@ struct messi_batch {
   std::vector<int> tiles;   // per-member inside tile edge (0: the context's)
   DevBuf zall;              // gathered log Z of all members (2 per member)
   Events ev;
+  Events sev;                       // profiling: begin / end of the tile-16 and tile-32 step kernels
+  int steps_launches[2] = {0, 0};   // step kernel launches of the last profiled evaluation, per tile edge
   double t[5] = {0, 0, 0, 0, 0};
 };
 
@@ -780,7 +782,11 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
     ws[m]->band = D < Lm - 1 ? D : -1;
     inside_alloc<T>(*ws[m], b->mem[m]->L, keep_outside);
   }
-  if (ctx->profiling) b->ev.make();
+  if (ctx->profiling) {
+    b->ev.make();
+    b->sev.make();
+    b->steps_launches[0] = b->steps_launches[1] = 0;
+  }
   // Inside launch sequences of members m0 .. m1-1 on stream st, one per tile edge.
   auto insides = [&](hipStream_t st, int m0, int m1, InsideBatch &i16, InsideBatch &i32) {
     for (int TE : {16, 32}) {
@@ -790,8 +796,14 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
           ws[m]->exp = b->dexp_pair.as<ExpandDesc>() + m;
           g.push_back(ws[m]);
         }
-      if (!g.empty())
-        inside_batch_run<T>(st, g.data(), (int)g.size(), TE == 32 ? i32 : i16, B, b->pair_acc.as<double>());
+      if (g.empty()) continue;
+      // profiling (one stream): events around each tile edge's step kernels
+      const int k = TE == 32 ? 2 : 0;
+      const bool timed = ctx->profiling;
+      const int nl = inside_batch_run<T>(st, g.data(), (int)g.size(), TE == 32 ? i32 : i16, B,
+                                         b->pair_acc.as<double>(), timed ? b->sev.e[k] : nullptr,
+                                         timed ? b->sev.e[k + 1] : nullptr);
+      if (timed) b->steps_launches[k / 2] = nl;
     }
   };
   // Member groups: with G > 1 each group runs its pair phase and then its
@@ -1636,6 +1648,22 @@ int messi_batch_timings(const messi_batch *b, double *t) {
   if (!b || !t) return 2;
   for (int i = 0; i < 5; ++i) t[i] = b->t[i];
   return 0;
+}
+
+int messi_batch_step_timings(messi_batch *b, double *ms, int *launches) {
+  if (!b || !ms || !launches) return 2;
+  return guard(b->ctx, [&] {
+    MESSI_REQUIRE(b->sev.made, "no profiled evaluation on this batch");
+    for (int k = 0; k < 2; ++k) {
+      float v = 0.0f;
+      if (b->steps_launches[k] > 0) {
+        MESSI_HIP(hipEventSynchronize(b->sev.e[2 * k + 1]));
+        MESSI_HIP(hipEventElapsedTime(&v, b->sev.e[2 * k], b->sev.e[2 * k + 1]));
+      }
+      ms[k] = v;
+      launches[k] = b->steps_launches[k];
+    }
+  });
 }
 
 }  // extern "C"

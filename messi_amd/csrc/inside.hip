@@ -1450,11 +1450,11 @@ void inside_prepare(hipStream_t s, InsideWork &w, int L, double B) {
 // Inside pass (preparation included) for a batch of instances; each work's red[0]
 // receives its log Z.
 template <typename T>
-void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &ib, double B,
-                      const double *rootacc) {
+int inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &ib, double B,
+                     const double *rootacc, hipEvent_t steps_begin, hipEvent_t steps_end) {
   typedef typename SRof<T>::type SR;
   typedef typename SR::V V;
-  if (n <= 0) return;
+  if (n <= 0) return 0;
   const int TE = ws[0]->tile == 32 ? 32 : 16;
   const int W = ws[0]->window;
   MESSI_REQUIRE(W == 1 || W == 2 || W == 4, "bulk window must be 1, 2 or 4");
@@ -1516,6 +1516,7 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
   prepare_batch<SR>(s, di, n, Lmax, B, rootacc);
   hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, di, B, lp[1],
                      lp[5]);
+  if (steps_begin) MESSI_HIP(hipEventRecord(steps_begin, s));
   for (int t = 0; t < NTmax; ++t) {
     const int *pf = ib.prefix.as<int>() + (size_t)t * stride;
     auto go = [&](auto kern, int threads) {
@@ -1531,8 +1532,10 @@ void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &
       else go(inside_step_kernel<SR, 16, 1>, 16 * 16);
     }
   }
+  if (steps_end) MESSI_HIP(hipEventRecord(steps_end, s));
   hipLaunchKernelGGL(zlog_kernel<SR>, dim3((n + 63) / 64), dim3(64), 0, s, di, n);
   MESSI_HIP(hipGetLastError());
+  return NTmax;
 }
 
 template <typename T>
@@ -1688,8 +1691,8 @@ void bpp_run(hipStream_t s, InsideWork &w, int L, double *d_unpaired, double *d_
   template void inside_alloc<T>(InsideWork &, int, bool);                               \
   template void inside_prepare<T>(hipStream_t, InsideWork &, int, double);              \
   template void inside_run<T>(hipStream_t, InsideWork &, int, double);                  \
-  template void inside_batch_run<T>(hipStream_t, InsideWork *const *, int, InsideBatch &, double,    \
-                                    const double *);                                           \
+  template int inside_batch_run<T>(hipStream_t, InsideWork *const *, int, InsideBatch &, double,     \
+                                   const double *, hipEvent_t, hipEvent_t);                   \
   template double inside_logZ<T>(hipStream_t, InsideWork &, int);                       \
   template void inside_export<T>(hipStream_t, InsideWork &, int, double *);             \
   template void inside_import<T>(hipStream_t, InsideWork &, int, const double *);       \

@@ -83,10 +83,12 @@ template <typename T> void inside_prepare(hipStream_t s, InsideWork &w, int L, d
 template <typename T> void inside_run(hipStream_t s, InsideWork &w, int L, double B);
 // With rootacc the pair term of each member is gathered from its pair root
 // entries (its work's exp descriptor) instead of being read from its L x L pair
-// matrix.  All members take the tile edge of ws[0].
+// matrix.  All members take the tile edge of ws[0].  steps_begin / steps_end,
+// when given, are recorded around the step kernels; returns their launch count.
 template <typename T>
-void inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &ib, double B,
-                      const double *rootacc = nullptr);
+int inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &ib, double B,
+                     const double *rootacc = nullptr, hipEvent_t steps_begin = nullptr,
+                     hipEvent_t steps_end = nullptr);
 template <typename T> double inside_logZ(hipStream_t s, InsideWork &w, int L);
 template <typename T> void inside_export(hipStream_t s, InsideWork &w, int L, double *d_in3);
 template <typename T> void inside_import(hipStream_t s, InsideWork &w, int L, const double *d_in3);

@@ -82,6 +82,7 @@ SIGNATURES = {
     "messi_batch_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                                         ctypes.c_double, ctypes.c_int, _dp]),
     "messi_batch_timings": (ctypes.c_int, [_vp, _dp]),
+    "messi_batch_step_timings": (ctypes.c_int, [_vp, _dp, _ip]),
     "messi_batch_load": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
     "messi_batch_eval_loaded": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp]),
     "messi_batch_set_tiles": (ctypes.c_int, [_vp, _ip]),
@@ -701,6 +702,14 @@ class Batch:
         t = np.zeros(5)
         self.ctx.check(self.ctx.lib.messi_batch_timings(self._h, ptr(t)))
         return t
+
+    def step_timings(self):
+        """(ms, launches) of the inside step kernels of the last profiled
+        evaluation, per tile edge (16, 32)."""
+        ms = np.zeros(2)
+        n = np.zeros(2, dtype=np.intc)
+        self.ctx.check(self.ctx.lib.messi_batch_step_timings(self._h, ptr(ms), ptr(n, _ip)))
+        return ms, n
 
     def close(self) -> None:
         if self._h:

@@ -99,3 +99,28 @@ def test_window_rejects_bad_values(ctx):
     for w in (0, 3, 8, -1):
         with pytest.raises(lib.MessiError, match="window"):
             ctx.set_bulk_window(w)
+
+
+def test_step_timings(ctx):
+    """messi_batch_step_timings: the inside step kernels' launches per tile edge
+    and their device time, from a profiled evaluation only."""
+    als = [synth.generate(6, 200, 400), synth.generate(7, 300, 401)]
+    dss = [synth.dataset_of(a) for a in als]
+    stacked = [_stack(model.getparams(a.params, d.tree.branchlengths)) for a, d in zip(als, dss)]
+    b = lib.Batch([hotpath.device_dataset(d) for d in dss])
+    try:
+        with pytest.raises(lib.MessiError, match="profiled"):
+            b.step_timings()
+        b.set_tiles([16, 32])
+        ctx.set_profiling(True)
+        try:
+            _say("step timings")
+            b.eval([s for s, _ in stacked], [q for _, q in stacked], 1.0, "fp32")
+            ms, n = b.step_timings()
+        finally:
+            ctx.set_profiling(False)
+        assert list(n) == [-(-200 // 16), -(-300 // 32)]
+        assert ms[0] > 0 and ms[1] > 0
+        assert ms.sum() <= b.timings()[2]   # within the inside phase
+    finally:
+        b.close()
