@@ -140,26 +140,23 @@ __global__ void ptv_root_kernel(const InstDesc<typename SR::V> *__restrict__ ins
   d.ptv[(size_t)j * L + y] = SR::from_log(x);
 }
 
-// Zero the bulk accumulator; set the diagonal of S and L (F has no s rule).
+// Set the diagonal of S and L (F has no s rule).  The bulk accumulator needs no
+// clearing: it is read only after a bulk visit wrote it (had_visit).
 template <class SR>
 __global__ void inside_init_kernel(const InstDesc<typename SR::V> *__restrict__ insts, double B, double lp1,
                                    double lp5) {
   const InstDesc<typename SR::V> d = insts[blockIdx.z];
   const int L = d.L;
-  int y = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y;
-  if (y >= L || j >= L) return;
-  size_t c = (size_t)j * L + y;
-  d.acc[((size_t)(j / d.T) * d.NT + y / d.T) * d.T * d.T + (j % d.T) * d.T + y % d.T] = SR::zero();
-  if (y == j) {
-    double ub = d.u[j] * B - d.lam[j];
-    typename SR::V sv = SR::from_log(ub + lp1 * B), lv = SR::from_log(ub + lp5 * B);
-    d.Sr[c] = sv;
-    d.Sc[c] = sv;
-    d.Lr[c] = lv;
-    d.Lc[c] = lv;
-    d.F[c] = SR::zero();
-  }
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= L) return;
+  const size_t c = (size_t)j * L + j;
+  const double ub = d.u[j] * B - d.lam[j];
+  const typename SR::V sv = SR::from_log(ub + lp1 * B), lv = SR::from_log(ub + lp5 * B);
+  d.Sr[c] = sv;
+  d.Sc[c] = sv;
+  d.Lr[c] = lv;
+  d.Lc[c] = lv;
+  d.F[c] = SR::zero();
 }
 
 // log Z (natural, absolute) = S[0, L-1] plus the baseline of the whole span, per instance.
@@ -513,6 +510,18 @@ __host__ __device__ __forceinline__ int bulk_items(int NT, int t, int W, int k) 
   return k * (NT - t - 1) - W * (k * (k - 1) / 2);
 }
 
+// Whether a tile at distance d had a bulk visit before launch t (a bulk visit
+// in launch t, or its finalisation in launch t = d): the visits are t' = d-1,
+// d-1-W, ... >= (d+3)/2, each scheduled unless t'-1 > KB and d > t'-1+KB.
+// Every scheduled visit writes the tile's accumulator, so the accumulator is
+// read only when this holds and needs no clearing.
+__device__ __forceinline__ bool had_visit(int t, int d, int W, int KB, bool fin) {
+  const int smin = (d + 3) / 2, last = fin ? d - 1 : t - W;
+  if (last < smin) return false;
+  const int first = last - W * ((last - smin) / W);
+  return first - 1 <= KB || d <= last - 1 + KB;
+}
+
 // Cell of a T x T off-diagonal tile that thread k takes in the in-tile wavefront:
 // the cells in decreasing order of their finalisation step own = b + T-1 - a
 // (ties by row).  Each wavefront then holds cells of neighbouring steps: it has
@@ -680,7 +689,9 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
       Jb[u] = Ib[u] + dist;
       smin[u] = max(t - W + 1, (dist + 3) / 2);  // a product of launch s needs dist <= 2s - 2
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
-      av[u] = (has[u] && j < L && y < L) ? acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid] : SR::zero();
+      av[u] = (has[u] && had_visit(t, Jb[u] - Ib[u], W, ins.KB, false) && j < L && y < L)
+                  ? acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid]
+                  : SR::zero();
 #pragma unroll
       for (int i = 0; i < W; ++i) {
         const int s = t - W + 1 + i, k1 = Ib[u] + s - 1, k2 = Jb[u] - (s - 1);
@@ -698,14 +709,12 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
 #pragma unroll
     for (int u = 0; u < BP; ++u) {
       if (!has[u]) continue;
-      bool any = false;
       if constexpr (SR::scaled && W >= 2) {
         // two launches' products per call, four wavefronts on MFMA
 #pragma unroll
         for (int i = 0; i < W; i += 2) {
           const bool dk[4] = {d1[u][i], d2[u][i], d1[u][i + 1], d2[u][i + 1]};
           if (!(dk[0] || dk[1] || dk[2] || dk[3])) continue;
-          any = true;
           const int s = t - W + 1 + i;
           const int Ks[4] = {Ib[u] + s - 1, Jb[u] - (s - 1), Ib[u] + s, Jb[u] - s};
           const PackedOps ops[4] = {qb1[u][i], qb2[u][i], qb1[u][i + 1], qb2[u][i + 1]};
@@ -715,7 +724,6 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
 #pragma unroll
         for (int i = 0; i < W; ++i) {
           if (!(d1[u][i] || d2[u][i])) continue;
-          any = true;
           const int s = t - W + 1 + i, k1 = Ib[u] + s - 1, k2 = Jb[u] - (s - 1);
           V acc2 = SR::zero();
           if constexpr (SR::scaled)
@@ -727,7 +735,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
         }
       }
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
-      if (any && j < L && y < L) acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid] = av[u];
+      if (j < L && y < L) acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid] = av[u];  // every visit writes
     }
     return;
   }
@@ -743,7 +751,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   V accv = SR::zero();
   {
     const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
-    if ((t >= 2 || !fin) && j < L && y < L) accv = acc[((size_t)I * NT + J) * (T * T) + tid];
+    if (had_visit(t, t, W, ins.KB, true) && j < L && y < L) accv = acc[((size_t)I * NT + J) * (T * T) + tid];
   }
   Operands<SR> o1, o2;
   PackedOps q1, q2;
@@ -1514,7 +1522,7 @@ int inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &i
   for (int m = 0; m < n; ++m)
     MESSI_REQUIRE((rootacc != nullptr) == (ws[m]->exp != nullptr), "pair term source differs between members");
   prepare_batch<SR>(s, di, n, Lmax, B, rootacc);
-  hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, di, B, lp[1],
+  hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((Lmax + 255) / 256, 1, n), dim3(256), 0, s, di, B, lp[1],
                      lp[5]);
   if (steps_begin) MESSI_HIP(hipEventRecord(steps_begin, s));
   for (int t = 0; t < NTmax; ++t) {

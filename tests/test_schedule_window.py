@@ -29,9 +29,21 @@ def _decode(NT, t, W, nd, idx):
     return I, I + dist
 
 
+def _prior(t, d, W, KB, fin):
+    """The kernels' test for an earlier scheduled bulk visit of a tile at
+    distance d (visits t' = d-1, d-1-W, ... >= (d+3)//2; a visit is scheduled
+    unless t'-1 > KB and d > t'-1+KB)."""
+    smin = (d + 3) // 2
+    last = d - 1 if fin else t - W
+    if last < smin:
+        return False
+    first = last - W * ((last - smin) // W)
+    return first - 1 <= KB or d <= last - 1 + KB
+
+
 def _inside_schedule(NT, KB, W):
     bp = 2 if W == 1 else 1
-    done = {}
+    done, written = {}, set()
     for t in range(NT):
         dmax = min(2 * t - 2, NT - 1)
         if t - 1 > KB:
@@ -44,6 +56,8 @@ def _inside_schedule(NT, KB, W):
                 continue
             I, J = _decode(NT, t, W, nd, idx)
             assert 0 <= I < J < NT
+            assert _prior(t, J - I, W, KB, False) == ((I, J) in written)
+            written.add((I, J))  # every scheduled visit writes its accumulator tile
             smin = max(t - W + 1, (J - I + 3) // 2)
             for s in range(t - W + 1, t + 1):
                 k1, k2 = I + s - 1, J - (s - 1)
@@ -53,6 +67,7 @@ def _inside_schedule(NT, KB, W):
                     done.setdefault((I, J, k2), []).append((t, s))
         for I in range(NT - t):  # finalising workgroups: launch t's own products
             J = I + t
+            assert _prior(t, t, W, KB, True) == ((I, J) in written)
             k1, k2 = I + t - 1, J - (t - 1)
             if t >= 2 and k1 - I <= KB:
                 done.setdefault((I, J, k1), []).append((t, t))
