@@ -50,6 +50,11 @@ int messi_set_tile(messi_ctx *ctx, int tile);
  * adding the split-point products of every launch in it.  Results do not depend
  * on it.  Default 4.  No counterpart in the reference, whose inside has no tiles. */
 int messi_set_bulk_window(messi_ctx *ctx, int window);
+/* Whether later inside / outside passes on this context skip the split-point
+ * terms that a pairing band (pair scores -inf beyond a distance) makes -inf
+ * (default 1).  Results do not depend on it; 0 runs the full-span recursions,
+ * as the reference always does (src/InsideOutsideAlgorithm.jl:447-483,648-749). */
+int messi_set_band_skip(messi_ctx *ctx, int on);
 /* Checks the MFMA operand layouts the kernels rely on against a CPU product. */
 int messi_selftest(messi_ctx *ctx);
 /* Page-locked host memory on the context's device (hipHostMalloc), for callers

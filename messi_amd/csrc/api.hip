@@ -779,7 +779,7 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
     ws[m]->zout = b->zall.as<double>() + 2 * m;
     // pair scores beyond the distance bound are -inf (pair_expand writes them so)
     const int D = b->mem[m]->maxbp, Lm = b->mem[m]->L;
-    ws[m]->band = D < Lm - 1 ? D : -1;
+    ws[m]->band = (ctx->band_skip && D < Lm - 1) ? D : -1;
     inside_alloc<T>(*ws[m], b->mem[m]->L, keep_outside);
   }
   if (ctx->profiling) {
@@ -976,6 +976,12 @@ int messi_set_bulk_window(messi_ctx *ctx, int window) {
   });
 }
 
+int messi_set_band_skip(messi_ctx *ctx, int on) {
+  if (!ctx) return 2;
+  ctx->band_skip = on != 0;
+  return 0;
+}
+
 int messi_selftest(messi_ctx *ctx) {
   if (!ctx) return 2;
   return guard(ctx, [&] {
@@ -1100,7 +1106,7 @@ int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, d
     InsideWork &w = work(ctx, prec);
     w.tile = ctx->tile;
     w.window = ctx->window;
-    w.band = pairing_band(L, paired);
+    w.band = ctx->band_skip ? pairing_band(L, paired) : -1;
     const size_t LL = (size_t)L * L;
     auto go = [&](auto tag) {
       using T = decltype(tag);
@@ -1129,7 +1135,7 @@ int messi_outside(messi_ctx *ctx, int L, const double *in3, const double *u, con
     MESSI_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     InsideWork &w = work(ctx, prec);
-    w.band = pairing_band(L, paired);
+    w.band = ctx->band_skip ? pairing_band(L, paired) : -1;
     const size_t LL = (size_t)L * L;
     auto go = [&](auto tag) {
       using T = decltype(tag);

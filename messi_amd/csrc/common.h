@@ -138,6 +138,7 @@ struct messi_ctx {
   bool profiling = false;
   int tile = 16;  // tile edge of the inside schedule
   int window = 4;  // bulk window of the inside schedule
+  bool band_skip = true;  // skip the split-point terms a pairing band makes -inf
   messi::InsideWork w64, w32;
   messi::MeaWork mea;
   messi::SampleWork samp;

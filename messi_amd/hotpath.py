@@ -20,6 +20,9 @@ Function here                    Reference
 ``samplestructure``              src/InsideOutsideAlgorithm.jl:623-627
 ``samplestructures``             the same, many samples per device launch
 ``calculateKH99prior``           src/InsideOutsideAlgorithm.jl:86-144 (host, for structures given by the caller)
+``maskgapped``                   src/RNATools.jl:441-451 (host, in place; ``gapmask`` gives the same rule as a
+                                 dataset pair mask, so the device never prunes the masked pairs)
+``maskgapped_structure``         src/RNATools.jl:453-466
 ===============================  ==========================================================
 
 ``prec`` selects the plain FP64 path (default) or the scaled FP32 path.
@@ -43,14 +46,84 @@ class KH99:
              ("F", "dFd", 0.787640), ("F", "LS", 0.21236), ("L", "s", 0.894603), ("L", "dFd", 0.105397))
 
 
+# Device datasets built with a pair mask, kept per dataset (least recently used first out).
+MASKED_CACHE = 4
+
+
+def _cached(dataset, attr: str, key, make, limit: int):
+    """Dataset-attached cache of device datasets keyed by content: at most ``limit``
+    entries, the least recently used one closed when a new one arrives."""
+    cache = dataset.__dict__.setdefault(attr, {})
+    if key in cache:
+        cache[key] = cache.pop(key)   # most recently used last
+        return cache[key]
+    while len(cache) >= limit:
+        old = cache.pop(next(iter(cache)))
+        old.close()
+    cache[key] = make()
+    return cache[key]
+
+
+def mask_key(mask: np.ndarray):
+    """Content key of a pair mask: its shape and the packed bits of mask > 0
+    (column-major, as the library reads it)."""
+    m = np.asfortranarray(np.asarray(mask) > 0)
+    return m.shape, np.packbits(m.reshape(-1, order="F")).tobytes()
+
+
 def device_dataset(dataset, maxbasepairdistance: int = 1000000, mask: Optional[np.ndarray] = None
                    ) -> "lib.DeviceDataset":
-    """Device-resident copy of a Dataset for one distance bound (cached on the dataset)."""
-    cache = dataset.__dict__.setdefault("_device", {})
-    key = (int(min(maxbasepairdistance, 2**31 - 1)), None if mask is None else id(mask))
-    if key not in cache:
-        cache[key] = lib.DeviceDataset(dataset, maxbasepairdistance, mask)
-    return cache[key]
+    """Device-resident copy of a Dataset for one distance bound and pair mask,
+    cached on the dataset by the bound and the mask's content."""
+    maxbp = int(min(maxbasepairdistance, 2**31 - 1))
+    if mask is None:
+        return _cached(dataset, "_device", maxbp, lambda: lib.DeviceDataset(dataset, maxbp), 1 << 30)
+    return _cached(dataset, "_device_masked", (maxbp, mask_key(mask)),
+                   lambda: lib.DeviceDataset(dataset, maxbp, mask), MASKED_CACHE)
+
+
+def gapmask(gapfrequency, cutoff: float) -> np.ndarray:
+    """Pair mask of maskgapped! (src/RNATools.jl:441-451): pair (i, j) is allowed
+    unless gapfrequency[i] >= cutoff or gapfrequency[j] >= cutoff (uint8 L x L)."""
+    g = np.asarray(gapfrequency, dtype=np.float64)
+    ok = g < cutoff
+    return np.asfortranarray(np.logical_and.outer(ok, ok).astype(np.uint8))
+
+
+def maskgapped(probs: np.ndarray, gapfrequency, cutoff: float, v: float = 0.0) -> np.ndarray:
+    """maskgapped! (src/RNATools.jl:441-451), in place: probs[i, j] = v when
+    gapfrequency[i] >= cutoff or gapfrequency[j] >= cutoff (the reference's
+    callers pass v = -Inf for a pair log-likelihood matrix).  Returns probs."""
+    g = np.asarray(gapfrequency, dtype=np.float64)
+    if probs.shape != (len(g), len(g)):
+        raise ValueError("probs must be L x L with L = len(gapfrequency)")
+    bad = g >= cutoff
+    probs[bad, :] = v
+    probs[:, bad] = v
+    return probs
+
+
+def maskgapped_structure(paired, gapfrequency, cutoff: float) -> np.ndarray:
+    """maskgapped for a structure (src/RNATools.jl:453-466): a copy of the 1-based
+    partners with every pair (i, j) removed whose i or j has gap frequency >= cutoff."""
+    s = np.array(paired, dtype=np.int64)
+    g = np.asarray(gapfrequency, dtype=np.float64)
+    i = np.flatnonzero(s > np.arange(1, len(s) + 1))
+    j = s[i] - 1
+    drop = (g[i] >= cutoff) | (g[j] >= cutoff)
+    s[i[drop]] = 0
+    s[j[drop]] = 0
+    return s
+
+
+def felsensteincuda_gapmasked(dataset, params: ModelParameters, gapcutoff: float,
+                              maxbasepairdistance: int = 1000000, prec=lib.FP64) -> np.ndarray:
+    """felsensteincuda followed by maskgapped! at ``gapcutoff`` (the ML objective's
+    pair matrix, src/MESSI.jl:309-310), with the masked pairs never pruned: the
+    device dataset carries the gap mask, so those entries come back -inf."""
+    mask = gapmask(dataset.gapfrequency, gapcutoff)
+    M, _ = felsensteincuda(dataset, params, None, maxbasepairdistance, prec=prec, mask=mask)
+    return M
 
 
 def _stack(components):
@@ -235,18 +308,16 @@ def structure_dataset(dataset, paired) -> "lib.DeviceDataset":
     """Device dataset whose pair list is exactly the pairs of one structure (the
     reference's getindexdict(dataset, paired), src/FelsensteinCUDA.jl:77-142);
     cached on the dataset by the structure's content."""
-    p = np.asarray(paired, dtype=np.int64)
-    key = hash(p.tobytes())
-    cache = dataset.__dict__.setdefault("_device_structure", {})
-    if key not in cache:
-        L = len(p)
+    p = np.ascontiguousarray(paired, dtype=np.int64)
+    L = len(p)
+
+    def make():
         mask = np.zeros((L, L), dtype=np.uint8, order="F")
         x = np.flatnonzero(p > np.arange(1, L + 1))
         mask[x, p[x] - 1] = 1
-        if len(cache) > 8:
-            cache.clear()
-        cache[key] = lib.DeviceDataset(dataset, L, mask)
-    return cache[key]
+        return lib.DeviceDataset(dataset, L, mask)
+
+    return _cached(dataset, "_device_structure", p.tobytes(), make, 8)
 
 
 def felsensteincudapaired(dataset, paired, params: ModelParameters, museparams=None, prec=lib.FP64) -> float:

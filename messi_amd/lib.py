@@ -77,6 +77,7 @@ SIGNATURES = {
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip]),
     "messi_set_tile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_set_bulk_window": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "messi_set_band_skip": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "messi_batch_destroy": (None, [_vp]),
     "messi_batch_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
@@ -206,6 +207,11 @@ class Context:
         """Launches per bulk accumulator visit of the inside schedule (1, 2 or 4; default 4)."""
         self.check(self.lib.messi_set_bulk_window(self._h, int(window)))
 
+    def set_band_skip(self, on: bool) -> None:
+        """Whether inside/outside passes skip the terms a pairing band makes -inf
+        (default on; results do not depend on it)."""
+        self.check(self.lib.messi_set_band_skip(self._h, int(bool(on))))
+
     def close(self) -> None:
         if self._h:
             self.lib.messi_ctx_destroy(self._h)
@@ -235,6 +241,39 @@ def colmajor_P(P: np.ndarray) -> np.ndarray:
     """(..., nnodes, dim, dim) [a, d] -> contiguous column-major-per-matrix doubles."""
     P = np.asarray(P, dtype=np.float64)
     return np.ascontiguousarray(np.swapaxes(P, -1, -2))
+
+
+def _components(P, pi, logw, nnodes: int, dim: int, what: str):
+    """Column-major transition matrices (K, nnodes, dim, dim) and the matching
+    (K, dim) frequencies and (K,) log weights, shape-checked (the library reads
+    exactly K * nnodes * dim^2, K * dim and K values from these pointers)."""
+    Pc = colmajor_P(P)
+    if Pc.ndim == 3:
+        Pc = Pc[None]
+    if Pc.ndim != 4 or Pc.shape[1:] != (nnodes, dim, dim):
+        raise ValueError(f"{what} transition matrices must have shape (K, {nnodes}, {dim}, {dim}), got "
+                         f"{np.shape(P)}")
+    K = Pc.shape[0]
+    pi_ = np.ascontiguousarray(pi, dtype=np.float64)
+    lw = np.ascontiguousarray(logw, dtype=np.float64)
+    if pi_.size != K * dim or lw.size != K:
+        raise ValueError(f"{what}: {K} components need {K * dim} frequencies and {K} log weights, got "
+                         f"{pi_.size} and {lw.size}")
+    return K, Pc, pi_.reshape(K, dim), lw.reshape(K)
+
+
+def _square(a, L: int, what: str) -> np.ndarray:
+    m = np.asfortranarray(a, dtype=np.float64)
+    if m.shape != (L, L):
+        raise ValueError(f"{what} must be {L} x {L}, got {m.shape}")
+    return m
+
+
+def _table3(a, L: int, what: str) -> np.ndarray:
+    t = np.asfortranarray(a, dtype=np.float64)
+    if t.shape != (3, L, L):
+        raise ValueError(f"{what} must have shape (3, {L}, {L}), got {t.shape}")
+    return t
 
 
 class DeviceDataset:
@@ -283,21 +322,18 @@ class DeviceDataset:
         self.ctx.check(self.ctx.lib.messi_last_timings(self._h, ptr(t)))
         return t
 
+    def _mixture(self, site, pair):
+        return (_components(*site, self.nnodes, 4, "site") + _components(*pair, self.nnodes, 16, "pair"))
+
     def site_loglik(self, P4, pi4, logw, prec=FP64) -> np.ndarray:
-        P = colmajor_P(P4)
-        K = P.shape[0]
-        pi = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K, 4)
-        lw = np.ascontiguousarray(logw, dtype=np.float64).reshape(K)
+        K, P, pi, lw = _components(P4, pi4, logw, self.nnodes, 4, "site")
         out = np.empty(self.L)
         self.ctx.check(self.ctx.lib.messi_site_loglik(self._h, K, ptr(P), ptr(pi), ptr(lw), prec_code(prec),
                                                       ptr(out)))
         return out
 
     def pair_loglik(self, P16, pi16, logw, prec=FP64, percomp: bool = False):
-        P = colmajor_P(P16)
-        K = P.shape[0]
-        pi = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K, 16)
-        lw = np.ascontiguousarray(logw, dtype=np.float64).reshape(K)
+        K, P, pi, lw = _components(P16, pi16, logw, self.nnodes, 16, "pair")
         out = np.empty((self.L, self.L), order="F")
         pc = np.empty((K, self.L, self.L)) if percomp else None
         self.ctx.check(self.ctx.lib.messi_pair_loglik(self._h, K, ptr(P), ptr(pi), ptr(lw), prec_code(prec),
@@ -307,14 +343,7 @@ class DeviceDataset:
         return out
 
     def eval(self, site, pair, B: float = 1.0, prec=FP64) -> float:
-        P4, pi4, lw4 = site
-        P16, pi16, lw16 = pair
-        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
-        K4, K16 = P4c.shape[0], P16c.shape[0]
-        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
-        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
-        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
-        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        K4, P4c, pi4, lw4, K16, P16c, pi16, lw16 = self._mixture(site, pair)
         z = ctypes.c_double()
         self.ctx.check(self.ctx.lib.messi_eval(self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c),
                                                ptr(pi16), ptr(lw16), float(B), prec_code(prec),
@@ -325,14 +354,7 @@ class DeviceDataset:
         """log Z, unpaired posteriors (L) and the L x L pair posteriors (Julia
         layout).  out: an L x L float64 Fortran-ordered array to receive the pair
         posteriors (e.g. from pinned_empty, reused across calls)."""
-        P4, pi4, lw4 = site
-        P16, pi16, lw16 = pair
-        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
-        K4, K16 = P4c.shape[0], P16c.shape[0]
-        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
-        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
-        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
-        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        K4, P4c, pi4, lw4, K16, P16c, pi16, lw16 = self._mixture(site, pair)
         z = ctypes.c_double()
         unp = np.empty(self.L)
         if out is not None:
@@ -349,14 +371,7 @@ class DeviceDataset:
     def posterior_consensus(self, site, pair, prec=FP64, mea_prec=FP64, alpha: float = 2.0):
         """log Z and the MEA consensus pairing (1-based partners, 0 unpaired), with
         the posteriors kept on the device."""
-        P4, pi4, lw4 = site
-        P16, pi16, lw16 = pair
-        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
-        K4, K16 = P4c.shape[0], P16c.shape[0]
-        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
-        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
-        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
-        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        K4, P4c, pi4, lw4, K16, P16c, pi16, lw16 = self._mixture(site, pair)
         z = ctypes.c_double()
         paired = np.empty(self.L, dtype=np.intc)
         self.ctx.check(self.ctx.lib.messi_posterior_consensus(
@@ -367,14 +382,7 @@ class DeviceDataset:
     def structure_loglik(self, site, pair, paired, prec=FP64):
         """(log-likelihood of the fixed structure, (unpaired sum, pair sum)); paired
         holds 1-based partners, 0 unpaired."""
-        P4, pi4, lw4 = site
-        P16, pi16, lw16 = pair
-        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
-        K4, K16 = P4c.shape[0], P16c.shape[0]
-        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
-        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
-        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
-        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        K4, P4c, pi4, lw4, K16, P16c, pi16, lw16 = self._mixture(site, pair)
         pa = np.ascontiguousarray(paired, dtype=np.intc)
         if pa.shape != (self.L,):
             raise ValueError(f"paired must have length {self.L}")
@@ -391,14 +399,7 @@ class DeviceDataset:
         (states 1-based, unpaired log-likelihoods, paired log-likelihoods at the
         openers), plus (site conditionals K4 x L, pair components K16 x L) when
         conditionals is set (messi_sample_states)."""
-        P4, pi4, lw4 = site
-        P16, pi16, lw16 = pair
-        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
-        K4, K16 = P4c.shape[0], P16c.shape[0]
-        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
-        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
-        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
-        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        K4, P4c, pi4, lw4, K16, P16c, pi16, lw16 = self._mixture(site, pair)
         pa = np.ascontiguousarray(paired, dtype=np.intc)
         if pa.shape != (self.L,):
             raise ValueError(f"paired must have length {self.L}")
@@ -417,14 +418,7 @@ class DeviceDataset:
     def sample_posterior(self, site, pair, nsamples: int, seed: int = 0, B: float = 1.0, prec=FP64):
         """log Z, then nsamples structures (nsamples x L, 1-based partners) drawn from
         the evaluation's inside table on the device, and their KH99 log priors."""
-        P4, pi4, lw4 = site
-        P16, pi16, lw16 = pair
-        P4c, P16c = colmajor_P(P4), colmajor_P(P16)
-        K4, K16 = P4c.shape[0], P16c.shape[0]
-        pi4 = np.ascontiguousarray(pi4, dtype=np.float64).reshape(K4, 4)
-        pi16 = np.ascontiguousarray(pi16, dtype=np.float64).reshape(K16, 16)
-        lw4 = np.ascontiguousarray(lw4, dtype=np.float64).reshape(K4)
-        lw16 = np.ascontiguousarray(lw16, dtype=np.float64).reshape(K16)
+        K4, P4c, pi4, lw4, K16, P16c, pi16, lw16 = self._mixture(site, pair)
         z = ctypes.c_double()
         out = np.empty((nsamples, self.L), dtype=np.intc)
         pri = np.empty(nsamples)
@@ -452,7 +446,7 @@ def inside(u, paired, B: float = 1.0, zonly: bool = True, prec=FP64, ctx: Option
     ctx = ctx or context()
     u = np.ascontiguousarray(u, dtype=np.float64)
     L = len(u)
-    pm = np.asfortranarray(paired, dtype=np.float64)
+    pm = _square(paired, L, "paired")
     z = ctypes.c_double()
     t = None if zonly else np.empty((3, L, L), order="F")
     ctx.check(ctx.lib.messi_inside(ctx.handle, L, ptr(u), ptr(pm), float(B), int(bool(zonly)), prec_code(prec),
@@ -464,8 +458,8 @@ def outside(inside_t, u, paired, prec=FP64, ctx: Optional[Context] = None):
     ctx = ctx or context()
     u = np.ascontiguousarray(u, dtype=np.float64)
     L = len(u)
-    it = np.asfortranarray(inside_t, dtype=np.float64)
-    pm = np.asfortranarray(paired, dtype=np.float64)
+    it = _table3(inside_t, L, "inside table")
+    pm = _square(paired, L, "paired")
     out = np.empty((3, L, L), order="F")
     ctx.check(ctx.lib.messi_outside(ctx.handle, L, ptr(it), ptr(u), ptr(pm), prec_code(prec), ptr(out)))
     return out
@@ -475,9 +469,9 @@ def bpp(inside_t, outside_t, u, paired, prec=FP64, ctx: Optional[Context] = None
     ctx = ctx or context()
     u = np.ascontiguousarray(u, dtype=np.float64)
     L = len(u)
-    it = np.asfortranarray(inside_t, dtype=np.float64)
-    ot = np.asfortranarray(outside_t, dtype=np.float64)
-    pm = np.asfortranarray(paired, dtype=np.float64)
+    it = _table3(inside_t, L, "inside table")
+    ot = _table3(outside_t, L, "outside table")
+    pm = _square(paired, L, "paired")
     unp = np.empty(L)
     pp = np.empty((L, L), order="F")
     ctx.check(ctx.lib.messi_bpp(ctx.handle, L, ptr(it), ptr(ot), ptr(u), ptr(pm), prec_code(prec), ptr(unp),
@@ -629,14 +623,18 @@ class Batch:
         _live.add(self)
 
     def _inputs(self, sites, pairs):
+        """Concatenated member-major inputs; every member's arrays are checked
+        against its own node count, and all members must have the same K."""
         n = len(self.members)
-        P4 = np.concatenate([colmajor_P(s[0]).reshape(-1) for s in sites])
-        pi4 = np.ascontiguousarray(np.concatenate([np.asarray(s[1], dtype=np.float64).reshape(-1) for s in sites]))
-        lw4 = np.ascontiguousarray(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1) for s in sites]))
-        P16 = np.concatenate([colmajor_P(p[0]).reshape(-1) for p in pairs])
-        pi16 = np.ascontiguousarray(np.concatenate([np.asarray(p[1], dtype=np.float64).reshape(-1) for p in pairs]))
-        lw16 = np.ascontiguousarray(np.concatenate([np.asarray(p[2], dtype=np.float64).reshape(-1) for p in pairs]))
-        return P4, pi4, lw4, len(lw4) // n, P16, pi16, lw16, len(lw16) // n
+        if len(sites) != n or len(pairs) != n:
+            raise ValueError(f"expected inputs for {n} members, got {len(sites)} and {len(pairs)}")
+        s4 = [_components(*x, m.nnodes, 4, "site") for x, m in zip(sites, self.members)]
+        s16 = [_components(*x, m.nnodes, 16, "pair") for x, m in zip(pairs, self.members)]
+        K4, K16 = s4[0][0], s16[0][0]
+        if any(c[0] != K4 for c in s4) or any(c[0] != K16 for c in s16):
+            raise ValueError("every member must have the same number of site and pair components")
+        cat = lambda cs, i: np.ascontiguousarray(np.concatenate([c[i].reshape(-1) for c in cs]))
+        return cat(s4, 1), cat(s4, 2), cat(s4, 3), K4, cat(s16, 1), cat(s16, 2), cat(s16, 3), K16
 
     def load(self, sites, pairs, prec=FP64) -> None:
         """Copy the members' mixture components to the device (messi_batch_load)."""
@@ -666,16 +664,8 @@ class Batch:
 
     def eval(self, sites, pairs, B: float = 1.0, prec=FP64) -> np.ndarray:
         """sites[m] = (P4 (K,nn,4,4), pi4 (K,4), logw (K,)); pairs[m] likewise with 16 states."""
-        n = len(self.members)
-        P4 = np.concatenate([colmajor_P(s[0]).reshape(-1) for s in sites])
-        pi4 = np.ascontiguousarray(np.concatenate([np.asarray(s[1], dtype=np.float64).reshape(-1) for s in sites]))
-        lw4 = np.ascontiguousarray(np.concatenate([np.asarray(s[2], dtype=np.float64).reshape(-1) for s in sites]))
-        P16 = np.concatenate([colmajor_P(p[0]).reshape(-1) for p in pairs])
-        pi16 = np.ascontiguousarray(np.concatenate([np.asarray(p[1], dtype=np.float64).reshape(-1) for p in pairs]))
-        lw16 = np.ascontiguousarray(np.concatenate([np.asarray(p[2], dtype=np.float64).reshape(-1) for p in pairs]))
-        K4 = len(lw4) // n
-        K16 = len(lw16) // n
-        z = np.empty(n)
+        P4, pi4, lw4, K4, P16, pi16, lw16, K16 = self._inputs(sites, pairs)
+        z = np.empty(len(self.members))
         self.ctx.check(self.ctx.lib.messi_batch_eval(self._h, K4, ptr(P4), ptr(pi4), ptr(lw4), K16, ptr(P16),
                                                      ptr(pi16), ptr(lw16), float(B), prec_code(prec), ptr(z)))
         return z
@@ -686,13 +676,21 @@ class Batch:
         n = len(self.members)
         P4 = colmajor_P(P4)
         P16 = colmajor_P(P16)
-        if P4.shape[0] != n or P16.shape[0] != n:
-            raise ValueError(f"expected {n} members, got {P4.shape[0]} and {P16.shape[0]}")
+        nn = {m.nnodes for m in self.members}
+        if len(nn) != 1:
+            raise ValueError("stacked inputs need members with one node count")
+        nn = nn.pop()
+        if P4.ndim != 5 or P4.shape[0] != n or P4.shape[2:] != (nn, 4, 4):
+            raise ValueError(f"P4 must have shape ({n}, K4, {nn}, 4, 4), got {P4.shape}")
+        if P16.ndim != 5 or P16.shape[0] != n or P16.shape[2:] != (nn, 16, 16):
+            raise ValueError(f"P16 must have shape ({n}, K16, {nn}, 16, 16), got {P16.shape}")
+        K4, K16 = P4.shape[1], P16.shape[1]
         pi4 = np.ascontiguousarray(pi4, dtype=np.float64)
         lw4 = np.ascontiguousarray(lw4, dtype=np.float64)
         pi16 = np.ascontiguousarray(pi16, dtype=np.float64)
         lw16 = np.ascontiguousarray(lw16, dtype=np.float64)
-        K4, K16 = lw4.shape[1], lw16.shape[1]
+        if pi4.shape != (n, K4, 4) or lw4.shape != (n, K4) or pi16.shape != (n, K16, 16) or lw16.shape != (n, K16):
+            raise ValueError("frequencies and log weights must match the stacked matrices")
         z = np.empty(n)
         self.ctx.check(self.ctx.lib.messi_batch_eval(self._h, K4, ptr(P4), ptr(pi4), ptr(lw4), K16, ptr(P16),
                                                      ptr(pi16), ptr(lw16), float(B), prec_code(prec), ptr(z)))

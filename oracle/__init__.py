@@ -6,7 +6,9 @@ reference file:line of each function).  Only the tests, the smoke check and
 the CPU-baseline leg of the benchmark may import this package; the product
 library never does.
 
-``build()`` compiles the C file with gcc into ``oracle/_build/``.
+``build()`` compiles the C file with gcc (OpenMP) into ``oracle/_build/``;
+``set_threads`` sets the number of threads of its loops, which changes no
+result (each cell is computed by one thread with the serial arithmetic).
 """
 from __future__ import annotations
 
@@ -29,7 +31,8 @@ _u8 = ctypes.POINTER(ctypes.c_uint8)
 
 def build() -> str:
     os.makedirs(os.path.dirname(_LIB), exist_ok=True)
-    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fno-fast-math", "-ffp-contract=off", "-o", _LIB, _SRC, "-lm"]
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-fopenmp", "-fno-fast-math", "-ffp-contract=off", "-o", _LIB, _SRC,
+           "-lm"]
     subprocess.run(cmd, check=True)
     return _LIB
 
@@ -40,6 +43,10 @@ def lib():
         if not os.path.exists(_LIB):
             build()
         L = ctypes.CDLL(_LIB)
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_threads.restype = None
+        L.oracle_get_threads.argtypes = []
+        L.oracle_get_threads.restype = ctypes.c_int
         L.oracle_site_loglik.argtypes = [ctypes.c_int, _ip, _ip, ctypes.c_int, ctypes.c_int, _u8, _dp, _dp, _dp]
         L.oracle_pair_loglik.argtypes = [ctypes.c_int, _ip, _ip, ctypes.c_int, ctypes.c_int, _u8, _dp, _dp,
                                          ctypes.c_int, _u8, _dp]
@@ -60,6 +67,15 @@ def lib():
                                            ctypes.c_double, ctypes.c_uint64, _ip, _dp, _dp]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's OpenMP loops (results do not depend on it)."""
+    lib().oracle_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().oracle_get_threads())
 
 
 def _p(a, t=_dp):

@@ -38,6 +38,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* KH99 rule probabilities, src/InsideOutsideAlgorithm.jl:40-46. */
 #define P_S_LS 0.868534
@@ -47,6 +50,24 @@
 #define P_F_LS 0.21236
 #define P_L_s 0.894603
 #define P_L_dFd 0.105397
+
+/* Threads of the OpenMP loops above (the per-cell arithmetic, and so every
+ * result, does not depend on it); n <= 0 keeps the runtime's default. */
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+int oracle_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 static double lse(double a, double b) {
   if (b == -INFINITY) return a;
@@ -105,12 +126,16 @@ static double prune_one(int nnodes, const int *children, const int *seqindex, in
 void oracle_site_loglik(int nnodes, const int *children, const int *seqindex, int nseq, int ncol,
                         const uint8_t *codes, const double *P4, const double *pi4, double *out) {
   (void)nseq;
-  double *part = (double *)malloc(sizeof(double) * 4 * nnodes);
-  double *logm = (double *)malloc(sizeof(double) * nnodes);
-  for (int c = 0; c < ncol; ++c)
-    out[c] = prune_one(nnodes, children, seqindex, ncol, codes, c, c, 4, P4, pi4, part, logm);
-  free(part);
-  free(logm);
+#pragma omp parallel
+  {
+    double *part = (double *)malloc(sizeof(double) * 4 * nnodes);
+    double *logm = (double *)malloc(sizeof(double) * nnodes);
+#pragma omp for schedule(static)
+    for (int c = 0; c < ncol; ++c)
+      out[c] = prune_one(nnodes, children, seqindex, ncol, codes, c, c, 4, P4, pi4, part, logm);
+    free(part);
+    free(logm);
+  }
 }
 
 /* One mixture component of the pair log-likelihood matrix.  out is L x L
@@ -120,19 +145,23 @@ void oracle_pair_loglik(int nnodes, const int *children, const int *seqindex, in
                         const uint8_t *codes, const double *P16, const double *pi16, int maxbp,
                         const uint8_t *mask, double *out) {
   (void)nseq;
-  double *part = (double *)malloc(sizeof(double) * 16 * nnodes);
-  double *logm = (double *)malloc(sizeof(double) * nnodes);
-  for (int x = 0; x < ncol; ++x) {
-    int yend = x + maxbp < ncol - 1 ? x + maxbp : ncol - 1;
-    for (int y = x + 1; y <= yend; ++y) {
-      if (mask && !mask[x + (size_t)ncol * y]) continue;
-      double v = prune_one(nnodes, children, seqindex, ncol, codes, x, y, 16, P16, pi16, part, logm);
-      out[x + (size_t)ncol * y] = v;
-      out[y + (size_t)ncol * x] = v;
+#pragma omp parallel
+  {
+    double *part = (double *)malloc(sizeof(double) * 16 * nnodes);
+    double *logm = (double *)malloc(sizeof(double) * nnodes);
+#pragma omp for schedule(dynamic, 4)
+    for (int x = 0; x < ncol; ++x) {
+      int yend = x + maxbp < ncol - 1 ? x + maxbp : ncol - 1;
+      for (int y = x + 1; y <= yend; ++y) {
+        if (mask && !mask[x + (size_t)ncol * y]) continue;
+        double v = prune_one(nnodes, children, seqindex, ncol, codes, x, y, 16, P16, pi16, part, logm);
+        out[x + (size_t)ncol * y] = v;
+        out[y + (size_t)ncol * x] = v;
+      }
     }
+    free(part);
+    free(logm);
   }
-  free(part);
-  free(logm);
 }
 
 /* Accumulate a component into a running mixture: acc = lse(acc, logw + comp),
@@ -163,7 +192,9 @@ void oracle_inside(int L, const double *u, const double *paired, double B, doubl
   const int t2left[3] = {0, 1, 2};
   const double t2lp[3] = {lp[2], lp[3], lp[6]};
   for (int d = 1; d < L; ++d) {
-    for (int j = 0; j + d < L; ++j) {
+    /* the cells of one span depend only on shorter spans */
+#pragma omp parallel for schedule(static) if (L - d > 64)
+    for (int j = 0; j < L - d; ++j) {
       int y = j + d;
       for (int r = 0; r < 2; ++r) {
         double tmp = -INFINITY;
@@ -195,6 +226,8 @@ void oracle_outside(int L, const double *in, const double *u, const double *pair
   const double t2lp[3] = {lp[2], lp[3], lp[6]};
   for (int c = 2; c <= L; ++c) {
     int b = L - c;
+    /* the cells of one span depend only on longer spans */
+#pragma omp parallel for schedule(static) if (L - b > 64)
     for (int j = 0; j <= L - b - 1; ++j) {
       for (int r = 0; r < 2; ++r) {
         int A = t3left[r];
@@ -236,6 +269,7 @@ void oracle_bpp(int L, const double *in, const double *out, const double *u, con
   for (size_t k = 0; k < (size_t)L * L; ++k) pair[k] = -INFINITY;
   const int t2left[3] = {0, 1, 2};
   const double t2lp[3] = {lp[2], lp[3], lp[6]};
+#pragma omp parallel for schedule(dynamic, 8)
   for (int j = 0; j < L; ++j) {
     for (int k = j + 1; k < L; ++k) {
       double tmp = -INFINITY;
@@ -254,6 +288,7 @@ void oracle_bpp(int L, const double *in, const double *out, const double *u, con
   for (size_t k = 0; k < (size_t)L * L; ++k) if (pair[k] > m) m = pair[k];
   for (int j = 0; j < L; ++j) res[j] = exp(res[j] - m);
   for (size_t k = 0; k < (size_t)L * L; ++k) pair[k] = exp(pair[k] - m);
+#pragma omp parallel for schedule(static)
   for (int x = 0; x < L; ++x) {
     double norm = res[x];
     for (int y = 0; y < L; ++y) norm += pair[x + (size_t)L * y];

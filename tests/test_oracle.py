@@ -187,3 +187,64 @@ def test_bpp_rows_sum_to_one_on_config1():
     # Before row normalisation the matrix is symmetric and rows already sum to one,
     # so the normalised matrix is symmetric to rounding.
     assert np.allclose(pair, pair.T, atol=1e-12)
+
+
+@pytest.mark.parametrize("K", [2, 3, 7])
+def test_mixture_of_identical_components_is_the_component(K):
+    """Closed form of the mixture (src/FARCE.jl:1207-1218, 1780-1792): K copies of
+    one component with weights 1/K give that component's site and pair
+    log-likelihoods; weights w and 1-w on two copies as well."""
+    ds = _tiny_dataset(TREES[1], 9, 17)
+    p = model.getparams(_params(5), ds.tree.branchlengths)
+    (P4, pi4, _), (P16, pi16, _) = model.site_components(p)[0], model.pair_components(p)[0]
+    u1 = oracle.site_loglik(ds, [(P4, pi4, 0.0)])
+    M1 = oracle.pair_loglik(ds, [(P16, pi16, 0.0)])
+    lw = math.log(1.0 / K)
+    uK = oracle.site_loglik(ds, [(P4, pi4, lw)] * K)
+    MK = oracle.pair_loglik(ds, [(P16, pi16, lw)] * K)
+    assert np.allclose(uK, u1, rtol=0, atol=1e-12)
+    fin = np.isfinite(M1)
+    assert np.array_equal(np.isfinite(MK), fin)
+    assert np.allclose(MK[fin], M1[fin], rtol=0, atol=1e-12)
+    w = 0.3
+    u2 = oracle.site_loglik(ds, [(P4, pi4, math.log(w)), (P4, pi4, math.log(1 - w))])
+    assert np.allclose(u2, u1, rtol=0, atol=1e-12)
+
+
+def test_mixture_of_two_components_brute_force():
+    """A two-component mixture against the mixture of the enumerated per-component
+    likelihoods: log(w1 L1 + w2 L2), with L_k by brute force over internal states."""
+    ds = _tiny_dataset(TREES[0], 5, 23)
+    p1 = model.getparams(_params(6), ds.tree.branchlengths)
+    p2 = model.getparams(_params(7), ds.tree.branchlengths)
+    c1, c2 = model.site_components(p1)[0], model.site_components(p2)[0]
+    w1 = 0.35
+    got = oracle.site_loglik(ds, [(c1[0], c1[1], math.log(w1)), (c2[0], c2[1], math.log(1 - w1))])
+    for c in range(ds.numcols):
+        lv = _leafvecs(ds, [c])
+        L1 = math.exp(prune_enumerate(ds.tree.children, lv, c1[0], c1[1]))
+        L2 = math.exp(prune_enumerate(ds.tree.children, lv, c2[0], c2[1]))
+        assert got[c] == pytest.approx(math.log(w1 * L1 + (1 - w1) * L2), rel=1e-12, abs=1e-12)
+
+
+def test_oracle_results_do_not_depend_on_threads():
+    """Each cell is computed by one thread with the serial arithmetic, so one
+    thread and several give the same bits."""
+    from messi_amd import synth
+    a = synth.generate(12, 160, 5)
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths)
+    n0 = oracle.get_threads()
+    res = []
+    try:
+        for n in (1, 4):
+            oracle.set_threads(n)
+            u = oracle.site_loglik(ds, model.site_components(p))
+            M = oracle.pair_loglik(ds, model.pair_components(p))
+            ins = oracle.inside(u, M)
+            out = oracle.outside(ins, u, M)
+            res.append((u, M, ins, out) + oracle.bpp(ins, out, u, M))
+    finally:
+        oracle.set_threads(n0)
+    for a1, a2 in zip(*res):
+        assert np.array_equal(a1, a2)
