@@ -102,6 +102,18 @@ int messi_pair_loglik(messi_dataset *ds, int K, const double *P16, const double 
 int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, double B, int zonly,
                  int prec, double *Z, double *in3);
 
+/* messi_inside without any device allocation.  No counterpart in the reference:
+ * this variant is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as messi_inside, for 1 <= L <= MESSI_NOALLOC_MAX_L
+ * (larger L is rejected with an error).  Every buffer of the call is a fixed
+ * region of a static device array of about 25 MiB that each GPU the library
+ * runs on holds until the process ends; all calls on one GPU share it, so calls
+ * that overlap on one GPU (from any context or thread) race. */
+#define MESSI_NOALLOC_MAX_L 512
+int messi_inside_noalloc(messi_ctx *ctx, int L, const double *u, const double *paired, double B, int zonly,
+                         int prec, double *Z, double *in3);
+
 /* KH99 outside from an inside table.  Replaces computeoutsideKH99
  * (src/InsideCUDA.jl:55-61) and computeoutside (src/InsideOutsideAlgorithm.jl:447-483). */
 int messi_outside(messi_ctx *ctx, int L, const double *in3, const double *u, const double *paired,

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 
 #include "common.h"
 #include "prune.h"
@@ -897,6 +898,92 @@ void mea_collect(messi_ctx *ctx, bool upload) {
 
 }  // namespace
 
+namespace {
+
+// ---------------------------------------------------------------------------
+// Static scratch of messi_inside_noalloc.  Every buffer of that call is a fixed
+// region of this array, sized for the tables of L = MESSI_NOALLOC_MAX_L (both
+// precisions use 8-byte cells), so the call allocates no device memory.  Each
+// device holds its own copy; its address is taken once per device, with the
+// device current, when the first context on it is created (hipGetSymbolAddress),
+// and only that address is used on the host side.  All calls on one device share
+// the array: calls that overlap on a device race.  Nothing in it has to start at
+// zero (every cell the inside pass reads was written earlier in the same call),
+// and the launch tables are uploaded on every call, since another context may
+// have overwritten them.
+constexpr size_t na_round(size_t n) { return (n + 255) & ~(size_t)255; }
+constexpr size_t NA_L = MESSI_NOALLOC_MAX_L;
+constexpr size_t NA_TABLE = na_round(NA_L * NA_L * 8);                  // one L x L table of 8-byte cells
+constexpr size_t NA_N16 = (NA_L + 15) / 16, NA_N32 = (NA_L + 31) / 32;
+constexpr size_t NA_CELLS = std::max(NA_N16 * NA_N16 * 256, NA_N32 * NA_N32 * 1024);  // tile-major cells
+constexpr size_t NA_ROWS = std::max(NA_N16 * NA_N16 * 16, NA_N32 * NA_N32 * 32);      // tile rows
+constexpr size_t NA_VEC = na_round((NA_L + 1) * 8);
+constexpr size_t NA_RED = na_round((3 * NA_L + 16) * 8);
+constexpr size_t NA_DESC = 4096;                                          // descriptors, launch tables
+constexpr size_t NOALLOC_BYTES = 7 * NA_TABLE + na_round(NA_CELLS * 8) + 2 * na_round(NA_CELLS * 4) +
+                                 2 * na_round(NA_ROWS * 4) + 4 * NA_VEC + NA_RED + 3 * NA_TABLE + 3 * NA_DESC;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char *> g_scratch;  // per device: the address of scratch on it (null until taken)
+std::vector<std::string> g_scratch_err;  // per device: why it could not be taken
+
+// Called with the device current.  A failure is kept for messi_inside_noalloc to
+// report; it does not stop the context from being created.
+void resolve_scratch(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if ((int)g_scratch.size() <= device) {
+    g_scratch.resize(device + 1, nullptr);
+    g_scratch_err.resize(device + 1);
+  }
+  if (g_scratch[device]) return;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return;
+  }
+  g_scratch[device] = static_cast<unsigned char *>(p);
+}
+
+unsigned char *scratch_of(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  MESSI_REQUIRE(device < (int)g_scratch.size(), "static scratch not resolved on this device");
+  MESSI_REQUIRE(g_scratch[device] != nullptr, "static scratch unavailable: " + g_scratch_err[device]);
+  return g_scratch[device];
+}
+
+// Every buffer of an inside call, borrowed from the scratch at fixed offsets.
+void bind_noalloc(InsideWork &w, unsigned char *base) {
+  size_t off = 0;
+  auto take = [&](DevBuf &b, size_t n) {
+    b.borrow(base + off, n);
+    off += n;
+  };
+  for (DevBuf *b : {&w.Lr, &w.Lc, &w.Sr, &w.Sc, &w.F, &w.ptv, &w.paired}) take(*b, NA_TABLE);
+  take(w.acc, na_round(NA_CELLS * 8));
+  take(w.Lp, na_round(NA_CELLS * 4));
+  take(w.Sp, na_round(NA_CELLS * 4));
+  take(w.LpE, na_round(NA_ROWS * 4));
+  take(w.SpE, na_round(NA_ROWS * 4));
+  for (DevBuf *b : {&w.lam, &w.C, &w.uhat, &w.u}) take(*b, NA_VEC);
+  take(w.red, NA_RED);
+  take(w.out3, 3 * NA_TABLE);
+  take(w.inst1, NA_DESC);
+  take(w.ib1.inst, NA_DESC);
+  take(w.ib1.prefix, NA_DESC);
+  if (off > NOALLOC_BYTES) throw std::logic_error("static scratch layout exceeds its size");
+  w.ib1.h_inst.clear();
+  w.ib1.h_prefix.clear();
+}
+
+}  // namespace
+
 extern "C" {
 
 int messi_ctx_create(int device, messi_ctx **out) {
@@ -909,6 +996,7 @@ int messi_ctx_create(int device, messi_ctx **out) {
     MESSI_REQUIRE(device >= 0 && device < n, "no such HIP device");
     MESSI_HIP(hipSetDevice(device));
     ctx->device = device;
+    resolve_scratch(device);
     MESSI_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     for (auto &p : ctx->pipe) MESSI_HIP(hipStreamCreateWithFlags(&p, hipStreamNonBlocking));
   });
@@ -1094,36 +1182,53 @@ static int pairing_band(int L, const double *paired) {
   return band >= L - 1 ? -1 : band;
 }
 
+// The inside call of messi_inside and messi_inside_noalloc on the work w.
+static void inside_call(messi_ctx *ctx, InsideWork &w, int L, const double *u, const double *paired, double B,
+                        int zonly, int prec, double *Z, double *in3) {
+  MESSI_REQUIRE(L >= 1 && u && paired && Z, "bad argument");
+  MESSI_REQUIRE(B > 0.0, "B must be positive");
+  MESSI_REQUIRE(zonly || in3, "in3 required when zonly == 0");
+  MESSI_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  w.tile = ctx->tile;
+  w.window = ctx->window;
+  w.band = ctx->band_skip ? pairing_band(L, paired) : -1;
+  const size_t LL = (size_t)L * L;
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    inside_alloc<T>(w, L, false);
+    MESSI_HIP(hipMemcpyAsync(w.u.p, u, sizeof(double) * L, hipMemcpyHostToDevice, s));
+    MESSI_HIP(hipMemcpyAsync(w.paired.p, paired, sizeof(double) * LL, hipMemcpyHostToDevice, s));
+    inside_run<T>(s, w, L, B);
+    *Z = inside_logZ<T>(s, w, L);
+    if (!zonly) {
+      w.out3.ensure(sizeof(double) * 3 * LL);
+      inside_export<T>(s, w, L, w.out3.as<double>());
+      MESSI_HIP(hipMemcpyAsync(in3, w.out3.p, sizeof(double) * 3 * LL, hipMemcpyDeviceToHost, s));
+    }
+    MESSI_HIP(hipStreamSynchronize(s));
+  };
+  if (prec == MESSI_FP32) go(float{});
+  else go(double{});
+}
+
 int messi_inside(messi_ctx *ctx, int L, const double *u, const double *paired, double B, int zonly, int prec,
                  double *Z, double *in3) {
   if (!ctx) return 2;
+  return guard(ctx, [&] { inside_call(ctx, work(ctx, prec), L, u, paired, B, zonly, prec, Z, in3); });
+}
+
+int messi_inside_noalloc(messi_ctx *ctx, int L, const double *u, const double *paired, double B, int zonly,
+                         int prec, double *Z, double *in3) {
+  if (!ctx) return 2;
   return guard(ctx, [&] {
-    MESSI_REQUIRE(L >= 1 && u && paired && Z, "bad argument");
-    MESSI_REQUIRE(B > 0.0, "B must be positive");
-    MESSI_REQUIRE(zonly || in3, "in3 required when zonly == 0");
+    MESSI_REQUIRE(L >= 1 && L <= MESSI_NOALLOC_MAX_L,
+                  "messi_inside_noalloc supports 1 <= L <= " + std::to_string(MESSI_NOALLOC_MAX_L) +
+                      " (its static scratch holds the tables of that length)");
+    check_prec(prec);
     MESSI_HIP(hipSetDevice(ctx->device));
-    hipStream_t s = ctx->stream;
-    InsideWork &w = work(ctx, prec);
-    w.tile = ctx->tile;
-    w.window = ctx->window;
-    w.band = ctx->band_skip ? pairing_band(L, paired) : -1;
-    const size_t LL = (size_t)L * L;
-    auto go = [&](auto tag) {
-      using T = decltype(tag);
-      inside_alloc<T>(w, L, false);
-      MESSI_HIP(hipMemcpyAsync(w.u.p, u, sizeof(double) * L, hipMemcpyHostToDevice, s));
-      MESSI_HIP(hipMemcpyAsync(w.paired.p, paired, sizeof(double) * LL, hipMemcpyHostToDevice, s));
-      inside_run<T>(s, w, L, B);
-      *Z = inside_logZ<T>(s, w, L);
-      if (!zonly) {
-        w.out3.ensure(sizeof(double) * 3 * LL);
-        inside_export<T>(s, w, L, w.out3.as<double>());
-        MESSI_HIP(hipMemcpyAsync(in3, w.out3.p, sizeof(double) * 3 * LL, hipMemcpyDeviceToHost, s));
-      }
-      MESSI_HIP(hipStreamSynchronize(s));
-    };
-    if (prec == MESSI_FP32) go(float{});
-    else go(double{});
+    bind_noalloc(ctx->wna, scratch_of(ctx->device));
+    inside_call(ctx, ctx->wna, L, u, paired, B, zonly, prec, Z, in3);
   });
 }
 

@@ -48,21 +48,32 @@ __device__ __forceinline__ int xcd_chunk_id(int x, int n) {
   return k * q + (k < r ? k : r) + x / NX;
 }
 
-// Device buffer that only grows.
+// Device buffer that only grows.  A borrowed buffer is a fixed region of memory
+// the buffer does not own (a slice of a preallocated scratch array): it never
+// allocates or frees, and a request beyond its capacity is an error.
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  bool borrowed = false;
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    borrowed = false;
+  }
+  void borrow(void *q, size_t cap) {
+    release();
+    p = q;
+    bytes = cap;
+    borrowed = true;
   }
   void ensure(size_t n) {
     if (n <= bytes) return;
+    if (borrowed) throw std::length_error("scratch region too small for this call");
     release();
     MESSI_HIP(hipMalloc(&p, n < 256 ? 256 : n));
     bytes = n < 256 ? 256 : n;
@@ -140,6 +151,7 @@ struct messi_ctx {
   int window = 4;  // bulk window of the inside schedule
   bool band_skip = true;  // skip the split-point terms a pairing band makes -inf
   messi::InsideWork w64, w32;
+  messi::InsideWork wna;  // messi_inside_noalloc: every buffer borrowed from the device's static scratch
   messi::MeaWork mea;
   messi::SampleWork samp;
 };

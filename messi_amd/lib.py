@@ -44,6 +44,8 @@ SIGNATURES = {
     "messi_pair_loglik": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "messi_inside": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                     _dp, _dp]),
+    "messi_inside_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_int, _dp, _dp]),
     "messi_outside": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp]),
     "messi_bpp": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp]),
     "messi_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
@@ -451,6 +453,22 @@ def inside(u, paired, B: float = 1.0, zonly: bool = True, prec=FP64, ctx: Option
     t = None if zonly else np.empty((3, L, L), order="F")
     ctx.check(ctx.lib.messi_inside(ctx.handle, L, ptr(u), ptr(pm), float(B), int(bool(zonly)), prec_code(prec),
                                    ctypes.byref(z), ptr(t)))
+    return z.value if zonly else (z.value, t)
+
+
+NOALLOC_MAX_L = 512
+
+
+def inside_noalloc(u, paired, B: float = 1.0, zonly: bool = True, prec=FP64, ctx: Optional[Context] = None):
+    """inside() through messi_inside_noalloc: no device allocation, L <= 512."""
+    ctx = ctx or context()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    L = len(u)
+    pm = _square(paired, L, "paired")
+    z = ctypes.c_double()
+    t = None if zonly else np.empty((3, L, L), order="F")
+    ctx.check(ctx.lib.messi_inside_noalloc(ctx.handle, L, ptr(u), ptr(pm), float(B), int(bool(zonly)),
+                                           prec_code(prec), ctypes.byref(z), ptr(t)))
     return z.value if zonly else (z.value, t)
 
 
