@@ -8,7 +8,9 @@ tree-level launch and each tile-diagonal launch of the inside schedule.  The
 alignment, its dedup lists and the transition matrices of each parameter vector
 are resident in HBM when the timed region starts (messi_batch_load, untimed);
 the rate with the matrices copied from the host on every call is reported
-beside it as evals_per_s_host_inputs.  ``value`` is evaluations per second
+beside it as evals_per_s_host_inputs, and the rate from parameter vectors
+(host eigendecompositions, transition matrices built on the device, then the
+evaluation, all timed: what an MCMC step pays) as evals_per_s_from_params.  ``value`` is evaluations per second
 of the scaled FP32 path at batch B; the plain FP64 path, and both paths at
 batch 1 (one chain), are reported beside it.
 
@@ -55,29 +57,53 @@ def _perturbed(a, ds, k, seed=0):
     return out
 
 
-def cpu_baseline(ds, params, L):
-    """Oracle (single-threaded FP64 C restatement of the reference CPU path) on a
-    bounded sample: all site columns, the column pairs within distance 100
-    (scaled to all pairs by count), and the full inside recursion."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_eval(ds, params):
+    """One full evaluation by the oracle (site log-likelihoods, all L(L-1)/2 pair
+    log-likelihoods, the inside recursion): (log Z, seconds per part)."""
     import oracle
     from messi_amd import model
     t0 = time.perf_counter()
     u = oracle.site_loglik(ds, model.site_components(params))
     t1 = time.perf_counter()
-    D = 100
-    M = oracle.pair_loglik(ds, model.pair_components(params), maxbp=D)
+    M = oracle.pair_loglik(ds, model.pair_components(params))
     t2 = time.perf_counter()
-    npairs_sample = sum(min(D, L - 1 - x) for x in range(L))
-    npairs = L * (L - 1) // 2
-    pair_full = (t2 - t1) * npairs / npairs_sample
+    z = oracle.inside(u, M)[0, 0, -1]
     t3 = time.perf_counter()
-    oracle.inside(u, M)
-    t4 = time.perf_counter()
-    per_eval = (t1 - t0) + pair_full + (t4 - t3)
-    return {"value": 1.0 / per_eval, "unit": "evals/s", "cores": 1, "kind": "port",
-            "sample": f"all {L} site columns, {npairs_sample} of {npairs} column pairs (|x-y|<=100, "
-                      f"time scaled by pair count), full inside at L={L}; measured "
-                      f"{(t4 - t0):.1f} s of CPU time"}
+    return z, (t1 - t0, t2 - t1, t3 - t2)
+
+
+def cpu_baseline(ds, params, L):
+    """The oracle (FP64 C restatement of the reference CPU path, whose own code is
+    serial) timed on one full evaluation: single-threaded, as the reference runs,
+    and with its OpenMP loops on every core the process may use."""
+    import oracle
+    n_all = oracle.get_threads()
+    oracle.set_threads(1)
+    try:
+        z1, t1 = _oracle_eval(ds, params)
+    finally:
+        oracle.set_threads(n_all)
+    zN, tN = _oracle_eval(ds, params)
+    npairs = L * (L - 1) // 2
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"value": 1.0 / sum(t1), "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": f"one full evaluation: all {L} site columns, all {npairs} column pairs, the full "
+                      f"inside recursion at L={L} (site {t1[0]:.2f} s, pairs {t1[1]:.1f} s, inside {t1[2]:.1f} s)",
+            "all_cores": {"value": 1.0 / sum(tN), "cores": n_all,
+                          "seconds": {"site": tN[0], "pairs": tN[1], "inside": tN[2]},
+                          "same_logZ": bool(zN == z1)},
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": aff}
 
 
 def time_batch(b, sites, pairs, prec, steps, warmup):
@@ -113,6 +139,20 @@ def time_batch(b, sites, pairs, prec, steps, warmup):
     # HIP events around the inside step kernels alone: their mean launch duration
     b.step_launch_us = float(steps_ms.sum() / steps_n.sum() * 1e3) if steps_n.sum() else None
     return ms, z, ph, ms_host
+
+
+def time_from_params(b, pm, thetas, prec, steps, warmup):
+    """Evaluations from parameter vectors: the host builds each vector's rate
+    matrices and their eigendecompositions, the device every branch's transition
+    matrix (messi_batch_load_eigen) and then the evaluation; all of it timed."""
+    for _ in range(warmup):
+        b.load_eigen(*pm.eigen_components(thetas), prec=prec)
+        z = b.eval_loaded(1.0, prec)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.load_eigen(*pm.eigen_components(thetas), prec=prec)
+        z = b.eval_loaded(1.0, prec)
+    return (time.perf_counter() - t0) * 1e3 / steps, z
 
 
 def main_instances(args):
@@ -576,15 +616,23 @@ def main():
     bB = lib.Batch([dd] * args.batch)
     bB.set_groups(args.groups)
     b1 = lib.Batch([dd])
+    from messi_amd import mcmc
+    pm = mcmc.ParamMap(a.params, ds.tree.branchlengths)
+    thetas = pm.theta_of(a.params) + np.random.default_rng(1).normal(0, 0.02, (args.batch, mcmc.DIM))
     res = {}
     for prec in args.precs:
         ms1, z1, ph1, ms1h = time_batch(b1, sites[:1], pairs[:1], prec, args.steps, args.warmup)
         msB, zB, phB, msBh = time_batch(bB, sites, pairs, prec, args.steps, args.warmup)
+        ms1p, _ = time_from_params(b1, pm, thetas[:1], prec, args.steps, args.warmup)
+        msBp, _ = time_from_params(bB, pm, thetas, prec, args.steps, args.warmup)
         res[prec] = {"batch1": {"ms_per_eval": ms1, "evals_per_s": 1e3 / ms1, "logZ": float(z1[0]),
                                 "evals_per_s_host_inputs": 1e3 / ms1h,
+                                "evals_per_s_from_params": 1e3 / ms1p,
                                 "phase_ms": {"site": ph1[0], "pair": ph1[1], "inside": ph1[2]}},
                      f"batch{args.batch}": {"ms_per_step": msB, "evals_per_s": args.batch * 1e3 / msB,
                                             "evals_per_s_host_inputs": args.batch * 1e3 / msBh,
+                                            "evals_per_s_from_params": args.batch * 1e3 / msBp,
+                                            "ms_per_step_from_params": msBp,
                                             "phase_ms": {"site": phB[0], "pair": phB[1], "inside": phB[2]},
                                             "inside_step_launch_us": bB.step_launch_us},
                      "logZ_batch": zB.tolist()}

@@ -278,6 +278,20 @@ int messi_batch_step_timings(messi_batch *b, double *ms, int *launches);
 int messi_batch_load(messi_batch *b, int K4, const double *P4, const double *pi4, const double *logw4,
                      int K16, const double *P16, const double *pi16, const double *logw16, int prec);
 int messi_batch_eval_loaded(messi_batch *b, double B, int prec, double *logZ);
+/* messi_batch_load with the transition matrices built on the device from
+ * eigendecompositions of the rate matrices, as gettransitionmatriceslist does on the
+ * host for every evaluation (src/FARCE.jl:410-433, absmat 389-408): for node v >= 1
+ * of member m with branch length t, P = V diag(exp(w t)) Vi, entries <= 0 set to
+ * 1e-50, the identity when t == 0.  Member-major inputs: member m's V and Vi are
+ * K x dim x dim (column-major per matrix: V[a + dim*c] is entry (a, c), the columns
+ * of V the eigenvectors, Vi their inverse), w is K x dim; branchlengths holds each
+ * member's nnodes_m values (pre-order, root first), members concatenated.  pi and
+ * logw as for messi_batch_eval.  The result is what messi_batch_load of those
+ * matrices would hold, for messi_batch_eval_loaded. */
+int messi_batch_load_eigen(messi_batch *b, int K4, const double *V4, const double *w4, const double *Vi4,
+                           const double *pi4, const double *logw4, int K16, const double *V16, const double *w16,
+                           const double *Vi16, const double *pi16, const double *logw16,
+                           const double *branchlengths, int prec);
 /* Inside tile edge per member (n values, each 16, 32, or 0 for the context's
  * setting; NULL resets all to 0).  Members of one edge share one launch
  * sequence.  Set by the autotuner (messi_amd/autotune.py).  No counterpart in

@@ -166,6 +166,8 @@ struct messi_batch {
   PipeEvents pev;
   std::vector<int> tiles;   // per-member inside tile edge (0: the context's)
   DevBuf zall;              // gathered log Z of all members (2 per member)
+  DevBuf eig, gnode;        // messi_batch_load_eigen: eigendecompositions + branch lengths; (member, node) per global node
+  HostBuf eig_stage;        // their pinned staging
   Events ev;
   Events sev;                       // profiling: begin / end of the tile-16 and tile-32 step kernels
   int steps_launches[2] = {0, 0};   // step kernel launches of the last profiled evaluation, per tile edge
@@ -618,6 +620,101 @@ void load_components(messi_batch *b, Components &c, int K, int dim2, const doubl
   MESSI_HIP(hipMemcpyAsync(c.logw.p, lwh, bw, hipMemcpyHostToDevice, s));
   c.K = K;
   c.prec = prec;
+}
+
+// Transition matrices of every member, component and node from eigendecompositions,
+// gettransitionmatriceslist with absmat (src/FARCE.jl:389-433): for a node v >= 1 of
+// member m with branch length t, P(a -> d) = sum_c V(a, c) exp(w_c t) Vi(c, d), each
+// entry <= 0 replaced by 1e-50, and the identity when t == 0; the root's slot (v = 0,
+// no branch) is zero, as the reference's first list entry.  One thread per entry of
+// the component-major arena (component k, global node g, column-major entry a + dim d);
+// V and Vi are column-major per (member, component), w per (member, component).
+template <typename T>
+__global__ void eigen_transition_kernel(int dim, int K, int nn_total, const int2 *__restrict__ gnode,
+                                        const double *__restrict__ t, const double *__restrict__ V,
+                                        const double *__restrict__ w, const double *__restrict__ Vi, T *P) {
+  const size_t d2 = (size_t)dim * dim;
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)K * nn_total * d2) return;
+  const int e = (int)(idx % d2);
+  const size_t kg = idx / d2;
+  const int g = (int)(kg % nn_total), k = (int)(kg / nn_total);
+  const int a = e % dim, d = e / dim;
+  const int2 mv = gnode[g];
+  const size_t mk = (size_t)mv.x * K + k;
+  const double tv = t[g];
+  double v;
+  if (mv.y == 0) {
+    v = 0.0;
+  } else if (tv == 0.0) {
+    v = a == d ? 1.0 : 0.0;
+  } else {
+    const double *Vm = V + mk * d2, *wm = w + mk * dim, *Vim = Vi + mk * d2;
+    v = 0.0;
+    for (int c = 0; c < dim; ++c) v += Vm[a + dim * c] * exp(wm[c] * tv) * Vim[c + dim * d];
+    if (v <= 0.0) v = 1e-50;
+  }
+  P[idx] = (T)v;
+}
+
+// Member-major eigendecompositions (member m: K x (dim^2 V, dim w, dim^2 Vi)),
+// branch lengths in global node order, pi and log weights as load_components
+// takes them -> the resident component arenas of both models.
+template <typename T>
+void load_eigen_impl(messi_batch *b, int K4, const double *V4, const double *w4, const double *Vi4,
+                     const double *pi4, const double *logw4, int K16, const double *V16, const double *w16,
+                     const double *Vi16, const double *pi16, const double *logw16, const double *t) {
+  MESSI_REQUIRE(K4 >= 1 && K16 >= 1, "need at least one site and one pair component");
+  hipStream_t s = b->ctx->stream;
+  const int n = b->n, prec = sizeof(T) == 4 ? MESSI_FP32 : MESSI_FP64;
+  const int64_t nt = b->nn_total;
+  if (b->gnode.bytes < sizeof(int2) * (size_t)nt) {
+    std::vector<int2> gn((size_t)nt);
+    for (int m = 0; m < n; ++m)
+      for (int v = 0; v < b->mem[m]->nn; ++v) gn[(size_t)b->node_base[m] + v] = make_int2(m, v);
+    upload(b->gnode, gn, s);
+  }
+  // staging: V4 w4 Vi4 V16 w16 Vi16 t, then each model's pi and log weights (component-major)
+  const size_t n4 = (size_t)n * K4, n16 = (size_t)n * K16;
+  const size_t sz[7] = {n4 * 16, n4 * 4, n4 * 16, n16 * 256, n16 * 16, n16 * 256, (size_t)nt};
+  const double *src[7] = {V4, w4, Vi4, V16, w16, Vi16, t};
+  size_t tot = 0;
+  for (size_t z : sz) tot += z;
+  MESSI_HIP(hipStreamSynchronize(s));  // the staging buffer may still feed an earlier copy
+  b->eig_stage.ensure(sizeof(double) * (tot + n4 * 5 + n16 * 17));
+  double *h = b->eig_stage.as<double>();
+  size_t off[7], o = 0;
+  for (int i = 0; i < 7; ++i) {
+    off[i] = o;
+    std::memcpy(h + o, src[i], sizeof(double) * sz[i]);
+    o += sz[i];
+  }
+  b->eig.ensure(sizeof(double) * tot);
+  MESSI_HIP(hipMemcpyAsync(b->eig.p, h, sizeof(double) * tot, hipMemcpyHostToDevice, s));
+  const double *de = b->eig.as<double>();
+  auto comps = [&](Components &c, int K, int dim, size_t oV, size_t ow, size_t oVi, const double *pi,
+                   const double *logw, double *hq) {
+    const size_t dim2 = (size_t)dim * dim;
+    double *hw = hq + (size_t)K * n * dim;
+    for (int m = 0; m < n; ++m)
+      for (int k = 0; k < K; ++k) {
+        for (int q = 0; q < dim; ++q) hq[((size_t)k * n + m) * dim + q] = pi[((size_t)m * K + k) * dim + q];
+        hw[(size_t)k * n + m] = logw[(size_t)m * K + k];
+      }
+    c.P.ensure(sizeof(T) * (size_t)K * nt * dim2);
+    c.pi.ensure(sizeof(double) * (size_t)K * n * dim);
+    c.logw.ensure(sizeof(double) * (size_t)K * n);
+    MESSI_HIP(hipMemcpyAsync(c.pi.p, hq, sizeof(double) * (size_t)K * n * dim, hipMemcpyHostToDevice, s));
+    MESSI_HIP(hipMemcpyAsync(c.logw.p, hw, sizeof(double) * (size_t)K * n, hipMemcpyHostToDevice, s));
+    const size_t total = (size_t)K * nt * dim2;
+    hipLaunchKernelGGL(eigen_transition_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, dim,
+                       K, (int)nt, b->gnode.as<int2>(), de + off[6], de + oV, de + ow, de + oVi, c.P.as<T>());
+    c.K = K;
+    c.prec = prec;
+  };
+  comps(b->c4, K4, 4, off[0], off[1], off[2], pi4, logw4, h + tot);
+  comps(b->c16, K16, 16, off[3], off[4], off[5], pi16, logw16, h + tot + n4 * 5);
+  MESSI_HIP(hipGetLastError());
 }
 
 void require_loaded(const Components &c, int prec, const char *what) {
@@ -1717,6 +1814,25 @@ int messi_batch_load(messi_batch *b, int K4, const double *P4, const double *pi4
     MESSI_HIP(hipSetDevice(ctx->device));
     if (prec == MESSI_FP32) load_impl<float>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
     else load_impl<double>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+    MESSI_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int messi_batch_load_eigen(messi_batch *b, int K4, const double *V4, const double *w4, const double *Vi4,
+                           const double *pi4, const double *logw4, int K16, const double *V16, const double *w16,
+                           const double *Vi16, const double *pi16, const double *logw16,
+                           const double *branchlengths, int prec) {
+  if (!b) return 2;
+  messi_ctx *ctx = b->ctx;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(V4 && w4 && Vi4 && pi4 && logw4 && V16 && w16 && Vi16 && pi16 && logw16 && branchlengths,
+                  "null argument");
+    check_prec(prec);
+    MESSI_HIP(hipSetDevice(ctx->device));
+    if (prec == MESSI_FP32)
+      load_eigen_impl<float>(b, K4, V4, w4, Vi4, pi4, logw4, K16, V16, w16, Vi16, pi16, logw16, branchlengths);
+    else
+      load_eigen_impl<double>(b, K4, V4, w4, Vi4, pi4, logw4, K16, V16, w16, Vi16, pi16, logw16, branchlengths);
     MESSI_HIP(hipStreamSynchronize(ctx->stream));
   });
 }

@@ -88,6 +88,8 @@ SIGNATURES = {
     "messi_batch_step_timings": (ctypes.c_int, [_vp, _dp, _ip]),
     "messi_batch_load": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]),
     "messi_batch_eval_loaded": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp]),
+    "messi_batch_load_eigen": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp,
+                                              _dp, _dp, _dp, _dp, ctypes.c_int]),
     "messi_batch_set_tiles": (ctypes.c_int, [_vp, _ip]),
     "messi_batch_set_groups": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_host_alloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
@@ -659,6 +661,35 @@ class Batch:
         P4, pi4, lw4, K4, P16, pi16, lw16, K16 = self._inputs(sites, pairs)
         self.ctx.check(self.ctx.lib.messi_batch_load(self._h, K4, ptr(P4), ptr(pi4), ptr(lw4), K16, ptr(P16),
                                                      ptr(pi16), ptr(lw16), prec_code(prec)))
+
+    def load_eigen(self, V4, w4, Vi4, pi4, lw4, V16, w16, Vi16, pi16, lw16, t, prec=FP64) -> None:
+        """messi_batch_load_eigen: the members' transition matrices built on the
+        device from eigendecompositions Q = V diag(w) Vi.  Stacked inputs, one row
+        per member: V, Vi (n, K, dim, dim) indexed [a, c], w (n, K, dim), pi
+        (n, K, dim), log weights (n, K), branch lengths t (n, nn) or a list of
+        each member's nn_m values."""
+        n = len(self.members)
+        tt = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in t]))
+        if tt.size != sum(m.nnodes for m in self.members):
+            raise ValueError("branch lengths must give every node of every member")
+
+        def model(V, w, Vi, pi, lw, dim, what):
+            V, Vi = np.asarray(V, dtype=np.float64), np.asarray(Vi, dtype=np.float64)
+            if V.ndim != 4 or V.shape[0] != n or V.shape[2:] != (dim, dim) or Vi.shape != V.shape:
+                raise ValueError(f"{what} eigenvectors must have shape ({n}, K, {dim}, {dim})")
+            K = V.shape[1]
+            w = np.ascontiguousarray(w, dtype=np.float64)
+            pi = np.ascontiguousarray(pi, dtype=np.float64)
+            lw = np.ascontiguousarray(lw, dtype=np.float64)
+            if w.shape != (n, K, dim) or pi.shape != (n, K, dim) or lw.shape != (n, K):
+                raise ValueError(f"{what}: eigenvalues, frequencies and log weights must match ({n}, {K})")
+            return (K, np.ascontiguousarray(np.swapaxes(V, -1, -2)), w,
+                    np.ascontiguousarray(np.swapaxes(Vi, -1, -2)), pi, lw)
+        K4, a4, b4, c4, d4, e4 = model(V4, w4, Vi4, pi4, lw4, 4, "site")
+        K16, a16, b16, c16, d16, e16 = model(V16, w16, Vi16, pi16, lw16, 16, "pair")
+        self.ctx.check(self.ctx.lib.messi_batch_load_eigen(
+            self._h, K4, ptr(a4), ptr(b4), ptr(c4), ptr(d4), ptr(e4), K16, ptr(a16), ptr(b16), ptr(c16),
+            ptr(d16), ptr(e16), ptr(tt), prec_code(prec)))
 
     def set_tiles(self, tiles) -> None:
         """Inside tile edge per member (16, 32, or 0 for the context's); None resets."""

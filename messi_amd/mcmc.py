@@ -166,10 +166,11 @@ class ParamMap:
         return lp + jac
 
     # --- model construction (batched over vectors) -----------------------
-    def components(self, thetas):
-        """Stacked site and pair mixture inputs for ``lib.Batch.eval_stacked``;
-        equal to ``model.site_components`` / ``model.pair_components`` of
-        ``params_of`` row by row."""
+    def rate_matrices(self, thetas):
+        """(Q4 (n,4,4), pi (n,4), Q16 (n,16,16), pi16 (n,16), branch lengths (n, nn))
+        of each coordinate vector: GTR (src/RateMatrices.jl:2-37), MuSE with one
+        site-rate category (src/RateMatrices.jl:69-148) and its stationary
+        frequencies (src/RateMatrices.jl:40-66), the tree's lengths times the scale."""
         lam, q, pi, s = self.unpack(thetas)
         n = lam.shape[0]
         t = s[:, None] * self.b0[None, :]
@@ -177,7 +178,6 @@ class ParamMap:
         Q4 = S * pi[:, None, :]
         Q4[:, np.arange(4), np.arange(4)] = 0.0
         Q4[:, np.arange(4), np.arange(4)] = -Q4.sum(axis=2)
-        P4 = transition_matrices(Q4, t)[:, None]
         lamf = np.concatenate([np.ones((n, 1)), lam], axis=1)     # index 0: unpaired class
         gq = S[:, _FRM, _TO]
         site = np.ones_like(gq)          # one site-rate category: both rates are 1
@@ -186,7 +186,6 @@ class ParamMap:
         Q16[:, _H, _V] = m1 * gq * site * pi[:, _TO]
         Q16[:, _V, _H] = (1.0 / m1) * gq * site * pi[:, _FRM]
         Q16[:, np.arange(16), np.arange(16)] = -Q16.sum(axis=2)
-        P16 = transition_matrices(Q16, t)[:, None]
         piGC = pi[:, 1] * pi[:, 2]
         piAT = pi[:, 0] * pi[:, 3]
         piGT = pi[:, 2] * pi[:, 3]
@@ -195,9 +194,46 @@ class ParamMap:
         l2 = lamf ** 2
         hh, vv = np.divmod(np.arange(16), 4)
         pi16 = kappa[:, None] * l2[:, BASEPAIRINGS] * pi[:, hh] * pi[:, vv]
+        return Q4, pi, Q16, pi16, t
+
+    def components(self, thetas):
+        """Stacked site and pair mixture inputs for ``lib.Batch.eval_stacked``;
+        equal to ``model.site_components`` / ``model.pair_components`` of
+        ``params_of`` row by row."""
+        Q4, pi, Q16, pi16, t = self.rate_matrices(thetas)
+        n = Q4.shape[0]
+        P4 = transition_matrices(Q4, t)[:, None]
+        P16 = transition_matrices(Q16, t)[:, None]
         zeros = np.zeros((n, 1))
         return P4, pi[:, None, :], zeros, P16, pi16[:, None, :], zeros.copy()
 
+    def eigen_components(self, thetas):
+        """Inputs of ``lib.Batch.load_eigen``: the eigendecompositions of Q4 and Q16
+        (the device builds every branch's P(t) from them), the frequencies, zero log
+        weights (one component) and the branch lengths (n, nn)."""
+        Q4, pi, Q16, pi16, t = self.rate_matrices(thetas)
+        n = Q4.shape[0]
+        V4, w4, Vi4 = reversible_eigen(Q4, pi)
+        V16, w16, Vi16 = reversible_eigen(Q16, pi16)
+        zeros = np.zeros((n, 1))
+        return (V4[:, None], w4[:, None], Vi4[:, None], pi[:, None, :], zeros,
+                V16[:, None], w16[:, None], Vi16[:, None], pi16[:, None, :], zeros.copy(), t)
+
+
+def reversible_eigen(Q, pi):
+    """Eigendecomposition Q = V diag(w) Vi of rate matrices that satisfy detailed
+    balance with the positive vectors pi (GTR and MuSE do): the symmetric matrix
+    A = D^1/2 Q D^-1/2 (D = diag(pi)) is diagonalised by LAPACK's symmetric solver,
+    A = U diag(w) U^T, so V = D^-1/2 U and Vi = U^T D^1/2 are real and exact
+    inverses.  The reference decomposes Q with the general solver
+    (src/FARCE.jl:412-414); both give exp(Q t) = V diag(exp(w t)) Vi."""
+    r = np.sqrt(np.asarray(pi, dtype=np.float64))
+    A = Q * r[:, :, None] / r[:, None, :]
+    A = 0.5 * (A + np.swapaxes(A, 1, 2))
+    w, U = np.linalg.eigh(A)
+    V = U / r[:, :, None]
+    Vi = np.swapaxes(U, 1, 2) * r[:, None, :]
+    return V, w, Vi
 
 
 class Posterior(ParamMap):
@@ -220,7 +256,9 @@ class Posterior(ParamMap):
         if th.shape[0] != self.nchains:
             raise ValueError(f"expected {self.nchains} vectors, got {th.shape[0]}")
         t0 = time.perf_counter()
-        z = self.batch.eval_stacked(*self.components(th), 1.0, prec or self.prec)
+        pr = prec or self.prec
+        self.batch.load_eigen(*self.eigen_components(th), prec=pr)
+        z = self.batch.eval_loaded(1.0, pr)
         self.eval_seconds += time.perf_counter() - t0
         self.evals += th.shape[0]
         return z

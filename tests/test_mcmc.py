@@ -351,3 +351,33 @@ def test_independence_moves_fix_the_mode_weights():
         if it >= 500:
             fr.append(np.mean(th[:, 0] < 0))
     assert np.mean(fr) > 0.5, np.mean(fr)
+
+
+def test_eigen_components_rebuild_the_transition_matrices():
+    """V diag(exp(w t)) Vi with the 1e-50 clamp and t = 0 -> I equals the general
+    eigendecomposition route of transition_matrices (gettransitionmatriceslist,
+    src/FARCE.jl:410-433), for spread-out coordinates and a zero branch length."""
+    from messi_amd import synth
+    a = synth.generate(9, 40, 3)
+    ds = synth.dataset_of(a)
+    b0 = ds.tree.branchlengths.copy()
+    b0[3] = 0.0
+    pm = mcmc.ParamMap(a.params, b0)
+    th = pm.theta_of(a.params) + np.random.default_rng(2).normal(0, 0.6, (8, mcmc.DIM))
+    P4, _, _, P16, _, _ = pm.components(th)
+    V4, w4, Vi4, pi4, _, V16, w16, Vi16, pi16, _, t = pm.eigen_components(th)
+
+    def build(V, w, Vi, t):
+        E = np.exp(w[:, 0, None, :] * t[:, :, None])
+        P = np.einsum("nac,nvc,ncd->nvad", V[:, 0], E, Vi[:, 0])
+        P = np.where(P <= 0.0, 1e-50, P)
+        P[t == 0.0] = np.eye(P.shape[-1])
+        P[:, 0] = 0.0
+        return P
+    assert np.max(np.abs(build(V4, w4, Vi4, t) - P4[:, 0])) < 1e-13
+    assert np.max(np.abs(build(V16, w16, Vi16, t) - P16[:, 0])) < 1e-13
+    assert np.array_equal(P16[:, 0, 3], np.broadcast_to(np.eye(16), (8, 16, 16)))
+    # exact inverses and the stationary frequencies are left eigenvectors for w = 0
+    assert np.allclose(np.einsum("nkac,nkcd->nkad", V16, Vi16), np.eye(16), atol=1e-12)
+    k = np.argmax(w16[:, 0], axis=1)
+    assert np.max(np.abs(w16[np.arange(8), 0, k])) < 1e-12
