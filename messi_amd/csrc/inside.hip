@@ -30,6 +30,7 @@
 #include "common.h"
 #include "prune.h"
 #include "sr.h"
+#include "wave_layout.h"
 
 namespace messi {
 
@@ -228,19 +229,29 @@ template <class SR, int T> struct PackArea<SR, T, true> {
   PackLDS<T> q[4];  // the four products of one bulk call
 };
 
-template <class SR, int T> struct StepLDS {
-  typedef typename SR::V V;
-  V Ld[T][T + 1];   // L of the diagonal tile (I, I)
-  V Sd[T][T + 1];   // S of the diagonal tile (J, J)
-  V cP[T][T + 1];   // pair term of the target tile
+// The finalising workgroup's tables in one flat array laid out by wave::Layout
+// (diagonal tiles, pair term, then the tile's S, F and L cells with their zero
+// regions); the part after the staged tables shares its storage with the
+// split-point products, which are done before the wavefront starts.
+template <class SR, int T> struct StepProducts {
   union {
-    struct {
-      V cS[T][T + 1], cF[T][T + 1], cL[T][T + 1];
-    } w;                  // the tile being finalised
     ProdLDS<SR, T> p[2];  // the (at most two) split-point products of a launch
     PackArea<SR, T> pk;   // scaled path: up to four products of a bulk window
-  } u;
+  };
+};
+template <class SR, int T> struct StepStaged {
+  typename SR::V staged[wave::Layout<T>::STAGED];
+  StepProducts<SR, T> q;
+};
+template <class SR, int T> struct StepLDS {
+  typedef typename SR::V V;
+  typedef wave::Layout<T> X;
+  union {
+    V all[X::END];
+    StepStaged<SR, T> s;
+  } m;
   V Sb[T], Fbot[T], Fleft[T];
+  __device__ V &at(int i) { return m.all[i]; }
 };
 
 // Where the two operand tiles of a product live: element (r, c) of A is
@@ -623,7 +634,10 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
                                                             const int *__restrict__ prefix, int n, int t,
                                                             Rules<SR> r) {
   typedef typename SR::V V;
+  typedef wave::Layout<T> WL;
   __shared__ StepLDS<SR, T> sm;
+  static_assert(sizeof(StepStaged<SR, T>) == sizeof(V) * WL::STAGED + sizeof(StepProducts<SR, T>),
+                "the products follow the staged tables");
   const int tid = threadIdx.x;
   constexpr bool by_step = !SR::scaled;  // in-tile wavefront cells in step order (see below)
   constexpr int BP = bulk_per(W);
@@ -718,7 +732,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
           const int s = t - W + 1 + i;
           const int Ks[4] = {Ib[u] + s - 1, Jb[u] - (s - 1), Ib[u] + s, Jb[u] - s};
           const PackedOps ops[4] = {qb1[u][i], qb2[u][i], qb1[u][i + 1], qb2[u][i + 1]};
-          tile_products_packed4<T>(ins, Ib[u], Jb[u], ops, Ks, dk, sm.u.pk.q, av[u]);
+          tile_products_packed4<T>(ins, Ib[u], Jb[u], ops, Ks, dk, sm.m.s.q.pk.q, av[u]);
         }
       } else {
 #pragma unroll
@@ -727,10 +741,10 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
           const int s = t - W + 1 + i, k1 = Ib[u] + s - 1, k2 = Jb[u] - (s - 1);
           V acc2 = SR::zero();
           if constexpr (SR::scaled)
-            tile_products_packed<T>(ins, Ib[u], Jb[u], qb1[u][i], k1, d1[u][i], qb2[u][i], k2, d2[u][i], sm.u.p,
-                                    av[u], acc2);
+            tile_products_packed<T>(ins, Ib[u], Jb[u], qb1[u][i], k1, d1[u][i], qb2[u][i], k2, d2[u][i],
+                                    sm.m.s.q.p, av[u], acc2);
           else
-            tile_products<SR, T>(ob1[u][i], d1[u][i], ob2[u][i], d2[u][i], sm.u.p, av[u], acc2);
+            tile_products<SR, T>(ob1[u][i], d1[u][i], ob2[u][i], d2[u][i], sm.m.s.q.p, av[u], acc2);
           av[u] = SR::add(av[u], acc2);
         }
       }
@@ -770,14 +784,15 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
       int a = idx / T, b = idx % T;
       int j = IT + a, y = JT + b;
       int jd = IT + b;  // column index inside the diagonal tile (I, I)
-      sm.Ld[a][b] = (j < L && jd < L && jd >= j) ? Lr[(size_t)j * L + jd] : SR::zero();
-      sm.cP[a][b] = (j < L && y < L) ? ptv[(size_t)j * L + y] : SR::zero();
+      sm.at(WL::ld(a, b)) = (j < L && jd < L && jd >= j) ? Lr[(size_t)j * L + jd] : SR::zero();
+      sm.at(WL::cp(a, b)) = (j < L && y < L) ? ptv[(size_t)j * L + y] : SR::zero();
     }
     for (int idx = tid; idx < T * T; idx += T * T) {
       int b = idx / T, c = idx % T;  // S[JT+c, JT+b], c fastest along the column
       int y = JT + b, rr = JT + c;
-      sm.Sd[c][b] = (y < L && rr <= y) ? Sc[(size_t)y * L + rr] : SR::zero();
+      sm.at(WL::sd(c, b)) = (y < L && rr <= y) ? Sc[(size_t)y * L + rr] : SR::zero();
     }
+    for (int k = tid; k < WL::N_ZERO_STAGED; k += T * T) sm.at(WL::zero_staged(k)) = SR::zero();
     if (tid < T) {
       int b = tid, y = JT + b, rb = (I + 1) * T;
       sm.Sb[b] = (t >= 1 && y < L && rb <= y) ? Sc[(size_t)y * L + rb] : SR::zero();
@@ -789,8 +804,8 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   }
   if (do1 || do2) {
     V acc2 = SR::zero();
-    if constexpr (SR::scaled) tile_products_packed<T>(ins, I, J, q1, K1, do1, q2, K2, do2, sm.u.p, accv, acc2);
-    else tile_products<SR, T>(o1, do1, o2, do2, sm.u.p, accv, acc2);
+    if constexpr (SR::scaled) tile_products_packed<T>(ins, I, J, q1, K1, do1, q2, K2, do2, sm.m.s.q.p, accv, acc2);
+    else tile_products<SR, T>(o1, do1, o2, do2, sm.m.s.q.p, accv, acc2);
     accv = SR::add(accv, acc2);
   }
   if (!fin) {
@@ -803,12 +818,15 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
       int a = idx / T, b = idx % T;
       int j = IT + a;
       bool d = (a == b) && j < L;
-      sm.u.w.cS[a][b] = d ? Sr[(size_t)j * L + j] : SR::zero();
-      sm.u.w.cL[a][b] = d ? Lr[(size_t)j * L + j] : SR::zero();
-      sm.u.w.cF[a][b] = SR::zero();
+      sm.at(WL::cs(a, b)) = d ? Sr[(size_t)j * L + j] : SR::zero();
+      sm.at(WL::cl(a, b)) = d ? Lr[(size_t)j * L + j] : SR::zero();
+      sm.at(WL::cf(a, b)) = SR::zero();
     }
-  } else if (by_step) {
-    sm.u.w.cF[tid / T][tid % T] = accv;  // to the thread that takes the cell below
+  } else {
+    if (by_step) sm.at(WL::cf(tid / T, tid % T)) = accv;  // to the thread that takes the cell below
+    // the zero regions of the wavefront's reads and the bottom border row as row T of cS
+    for (int k = tid; k < WL::N_ZERO_WAVE; k += T * T) sm.at(WL::zero_wave(k)) = SR::zero();
+    if (tid < T) sm.at(WL::cs(T, tid)) = sm.Sb[tid];
   }
   __syncthreads();
   // One thread per cell.  A cell's split-point sum is accumulated online: at
@@ -825,7 +843,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   const int j = IT + a, y = JT + b;
   const bool mine = j < L && y < L && (t > 0 || a < b);
   const typename SR::FinRules rk = SR::fin_rules(r.p[6], r.lin[0], r.lin[2], r.lin[3], r.lin[4], r.lin[6]);
-  typename SR::Acc ac = SR::acc(by_step && t > 0 ? sm.u.w.cF[a][b] : accv);
+  typename SR::Acc ac = SR::acc(by_step && t > 0 ? sm.at(WL::cf(a, b)) : accv);
   if (t == 0) {
     const int own = b - a;
     for (int st = 1; st < T; ++st) {
@@ -835,32 +853,69 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
         const bool ok = own - st <= st - 1;
         const int c1 = a + st - 1, c2 = b - st;
         if (ok && c1 <= b - 1) {
-          l1 = sm.u.w.cL[a][c1];
-          s1 = sm.u.w.cS[c1 + 1][b];
+          l1 = sm.at(WL::cl(a, c1));
+          s1 = sm.at(WL::cs(c1 + 1, b));
         }
         if (ok && c2 >= a && c2 != c1) {
-          l2 = sm.u.w.cL[a][c2];
-          s2 = sm.u.w.cS[c2 + 1][b];
+          l2 = sm.at(WL::cl(a, c2));
+          s2 = sm.at(WL::cs(c2 + 1, b));
         }
       }
       if (fin) {
-        pt = sm.cP[a][b];
-        if (own >= 2) f = sm.u.w.cF[a + 1][b - 1];
+        pt = sm.at(WL::cp(a, b));
+        if (own >= 2) f = sm.at(WL::cf(a + 1, b - 1));
       }
       if (act) SR::acc_add2(ac, SR::mul_raw(l1, s1), SR::mul_raw(l2, s2));
       if (fin) {
         V vs, vf, vl;
         SR::finalize(ac, SR::mul_raw(pt, f), rk, vs, vf, vl);
-        sm.u.w.cS[a][b] = vs;
-        sm.u.w.cF[a][b] = vf;
-        sm.u.w.cL[a][b] = vl;
+        sm.at(WL::cs(a, b)) = vs;
+        sm.at(WL::cf(a, b)) = vf;
+        sm.at(WL::cl(a, b)) = vl;
+      }
+      __syncthreads();
+    }
+  } else if constexpr (SR::scaled) {
+    // Off-diagonal tile, scaled path.  Every step reads the lane's four cells at
+    // fixed offsets from per-lane bases (wave::Layout): terms that must be zero
+    // read a zero cell, terms after the lane's finalisation feed a dead
+    // accumulator, so a step has no index arithmetic and no masks.  The
+    // bottom-border split point h = IT+T-1 is the column term of step b (row T
+    // of cS holds S[(I+1)T, y]).  Lanes of columns y >= L compute values nothing
+    // reads (the tile's in-range cells read only in-range cells) and are not
+    // written back.
+    const int own = b + T - 1 - a;
+    const int bcl = WL::base_col_l(a, b), bcs = WL::base_col_s(a, b);
+    const int brl = WL::base_row_l(a, b), brs = WL::base_row_s(a, b);
+    // F[j+1, y-1] for the pair term (inner cell, finalised two steps earlier, or
+    // the bottom / left border), read one step ahead at a fixed position
+    const bool fin_in = a + 1 < T && b >= 1;
+    const V *fsrc = fin_in ? &sm.at(WL::cf(a + 1, b - 1)) : (a + 1 == T ? &sm.Fbot[b] : &sm.Fleft[a]);
+    const V pterm = j + 1 > y - 1 ? SR::zero() : sm.at(WL::cp(a, b));
+    V fin_next = *fsrc;
+    const int wave_last = __builtin_amdgcn_readfirstlane(2 * T - 2 - (tid & ~63) / T);
+#pragma unroll
+    for (int st = 0; st <= 2 * T - 2; ++st) {
+      if (st <= wave_last) {
+        const V fin_f = fin_next;
+        fin_next = *fsrc;
+        const V t1 = SR::mul_raw(sm.at(bcl + WL::off_col_l(st)), sm.at(bcs + WL::off_col_s(st)));
+        const V t2 = SR::mul_raw(sm.at(brl + WL::off_row_l(st)), sm.at(brs + WL::off_row_s(st)));
+        SR::acc_add2(ac, t1, t2);
+        if (st == own) {
+          V vs, vf, vl;
+          SR::finalize(ac, SR::mul_raw(pterm, fin_f), rk, vs, vf, vl);
+          sm.at(WL::cs(a, b)) = vs;
+          sm.at(WL::cf(a, b)) = vf;
+          sm.at(WL::cl(a, b)) = vl;
+        }
       }
       __syncthreads();
     }
   } else {
     const int own = b + T - 1 - a;
     // split point h = IT+T-1: S[(I+1)T, y] lies in the tile below.
-    if (mine) SR::acc_add2(ac, SR::mul_raw(sm.Ld[a][T - 1], sm.Sb[b]), SR::zero());
+    if (mine) SR::acc_add2(ac, SR::mul_raw(sm.at(WL::ld(a, T - 1)), sm.Sb[b]), SR::zero());
     // F[j+1, y-1] for the pair term: inside this tile, or on the tile's bottom /
     // left border.  The inner cell (a+1, b-1) is finalised two steps before this
     // cell (its step is own - 2, and own >= 2 for it).  Every step reads the
@@ -868,8 +923,8 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     // step, so the read overlaps a whole step instead of delaying the
     // finalisation; with no inner pair (j + 1 > y - 1) the pair term is zero.
     const bool fin_in = a + 1 < T && b >= 1;
-    const V *fsrc = fin_in ? &sm.u.w.cF[a + 1][b - 1] : (a + 1 == T ? &sm.Fbot[b] : &sm.Fleft[a]);
-    const V pterm = j + 1 > y - 1 ? SR::zero() : sm.cP[a][b];
+    const V *fsrc = fin_in ? &sm.at(WL::cf(a + 1, b - 1)) : (a + 1 == T ? &sm.Fbot[b] : &sm.Fleft[a]);
+    const V pterm = j + 1 > y - 1 ? SR::zero() : sm.at(WL::cp(a, b));
     // primed before step 0: a border cell may finalise at step 0 (own = 0) or 1;
     // an inner cell's first reads are stale but unused (own >= 2)
     V fin_next = *fsrc;
@@ -882,15 +937,15 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
       // the step has no divergent branch but the finalisation
       const int ap = b + T - st, bp = st - T + a;
       const int apc = min(max(ap, 1), T - 1), bpc = min(max(bp, 0), T - 2);
-      const V t1 = SR::mul_raw(sm.Ld[a][apc - 1], sm.u.w.cS[apc][b]);
-      const V t2 = SR::mul_raw(sm.u.w.cL[a][bpc], sm.Sd[bpc + 1][b]);
+      const V t1 = SR::mul_raw(sm.at(WL::ld(a, apc - 1)), sm.at(WL::cs(apc, b)));
+      const V t2 = SR::mul_raw(sm.at(WL::cl(a, bpc)), sm.at(WL::sd(bpc + 1, b)));
       SR::acc_add2(ac, SR::keep(t1, act && ap > a && ap < T), SR::keep(t2, act && bp >= 0 && bp < b));
       if (mine && st == own) {
         V vs, vf, vl;
         SR::finalize(ac, SR::mul_raw(pterm, fin_f), rk, vs, vf, vl);
-        sm.u.w.cS[a][b] = vs;
-        sm.u.w.cF[a][b] = vf;
-        sm.u.w.cL[a][b] = vl;
+        sm.at(WL::cs(a, b)) = vs;
+        sm.at(WL::cf(a, b)) = vf;
+        sm.at(WL::cl(a, b)) = vl;
       }
     };
     // the last step with work for this wavefront (a scalar, so the skip below is
@@ -911,30 +966,25 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     };
     // The FP64 step (exponentials and logarithms) stays a rolled loop: unrolled,
     // the kernel outgrows the instruction cache.
-    if constexpr (SR::scaled) {
-#pragma unroll
-      for (int st = 0; st <= 2 * T - 2; ++st) step(st);
-    } else {
 #pragma unroll 1
-      for (int st = 0; st <= 2 * T - 2; ++st) step(st);
-    }
+    for (int st = 0; st <= 2 * T - 2; ++st) step(st);
   }
   // ---- write the tile back (row-major copies with b fastest, column-major with a fastest)
   for (int idx = tid; idx < T * T; idx += T * T) {
     int a = idx / T, b = idx % T, j = IT + a, y = JT + b;
     if (j < L && y < L && j < y) {
       size_t rc = (size_t)j * L + y;
-      Sr[rc] = sm.u.w.cS[a][b];
-      F[rc] = sm.u.w.cF[a][b];
-      Lr[rc] = sm.u.w.cL[a][b];
+      Sr[rc] = sm.at(WL::cs(a, b));
+      F[rc] = sm.at(WL::cf(a, b));
+      Lr[rc] = sm.at(WL::cl(a, b));
     }
   }
   for (int idx = tid; idx < T * T; idx += T * T) {
     int b = idx / T, a = idx % T, j = IT + a, y = JT + b;
     if (j < L && y < L && j < y) {
       size_t cc = (size_t)y * L + j;
-      Sc[cc] = sm.u.w.cS[a][b];
-      Lc[cc] = sm.u.w.cL[a][b];
+      Sc[cc] = sm.at(WL::cs(a, b));
+      Lc[cc] = sm.at(WL::cl(a, b));
     }
   }
   if constexpr (SR::scaled) {
@@ -943,7 +993,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     {
       const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
       const bool ok = j < L && y < L;
-      const XFv v = sm.u.w.cL[a][b];
+      const XFv v = sm.at(WL::cl(a, b));
       int e = ok && v.m != 0.0f ? v.e : XF::EZ;
 #pragma unroll
       for (int off = T / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
@@ -957,7 +1007,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     {
       const int c = tid % T, b = tid / T, row = IT + 1 + c, y = JT + b;
       XFv v = XF::zero();
-      if (row < L && y < L && row <= y) v = c + 1 < T ? sm.u.w.cS[c + 1][b] : sm.Sb[b];
+      if (row < L && y < L && row <= y) v = c + 1 < T ? sm.at(WL::cs(c + 1, b)) : sm.Sb[b];
       int e = v.m != 0.0f ? v.e : XF::EZ;
 #pragma unroll
       for (int off = T / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));

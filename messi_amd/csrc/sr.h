@@ -96,7 +96,9 @@ struct LogSR {
   }
 };
 
-struct XFv {
+// 8-byte aligned, so a cell moves with one 64-bit load or store (ds_read_b64
+// takes a 16-bit immediate offset; two 32-bit halves would not).
+struct alignas(8) XFv {
   float m;
   int e;
 };
@@ -118,11 +120,11 @@ struct XF {
     d = d < -200 ? -200 : d;
     return ldexpf(a.m * b.m, d);
   }
+  // frexp maps 0 to 0, so only the exponent of a zero needs a select
   static __device__ __forceinline__ V norm(float m, int e) {
-    if (m == 0.0f) return zero();
     int x;
-    float f = frexpf(m, &x);
-    return V{f, e + x};
+    const float f = frexpf(m, &x);
+    return V{f, m == 0.0f ? EZ : e + x};
   }
   static __device__ __forceinline__ V finish(K km, S s) { return norm(s, km); }
   static __device__ __forceinline__ V add(V a, V b) {
@@ -192,7 +194,10 @@ struct XF {
     // contracts the sums at each inlined call site
     S = norm(fmaf(r.p0, xb, r.p2 * xp), ne);
     F = norm(fmaf(r.p4, xb, r.p3 * xp), ne);
-    Lv = norm(r.p6 * pf.m, pf.e);
+    // L = p6 * pf is kept unnormalised: pf's mantissa is a product of two
+    // normalised ones, in [1/4, 1), so L's lies in [p6/4, p6) and a zero pf already
+    // carries a zero's exponent (the sum of a zero's and a finite one's)
+    Lv = V{r.p6 * pf.m, pf.e};
   }
 };
 
