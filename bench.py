@@ -106,14 +106,21 @@ def cpu_baseline(ds, params, L):
             "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "affinity_cpus": aff}
 
 
-def time_batch(b, sites, pairs, prec, steps, warmup):
+def time_batch(b, sites, pairs, prec, steps, warmup, tune: bool = False):
     """Time evaluations with the mixture components resident in HBM (loaded once,
     outside the timed region), then the same with host inputs copied every call
-    (the PCIe-inclusive rate), then a profiling pass with phase events."""
-    from messi_amd import lib
+    (the PCIe-inclusive rate), then a profiling pass with phase events.  With
+    tune, the autotuner first picks the batch's schedule (untimed)."""
+    from messi_amd import autotune, lib
     ctx = lib.context()
     ctx.set_profiling(False)
     b.load(sites, pairs, prec)
+    b.schedule = None
+    if tune:
+        ch = autotune.tune_batch(b, prec, reps=3)
+        b.schedule = {"tile32_from_length": ch["threshold"], "window": ch["window"], "groups": ch["groups"],
+                      "tile_candidates_ms": ch["candidates"], "window_candidates_ms": ch["window_candidates"],
+                      "group_candidates_ms": ch["group_candidates"]}
     for _ in range(warmup):
         z = b.eval_loaded(1.0, prec)
     t0 = time.perf_counter()
@@ -136,8 +143,10 @@ def time_batch(b, sites, pairs, prec, steps, warmup):
         steps_n += sn
     ph /= nprof
     ctx.set_profiling(False)
-    # HIP events around the inside step kernels alone: their mean launch duration
+    # HIP events around the inside step kernels alone: their mean launch duration,
+    # and the launches per evaluation, per tile edge (16, 32)
     b.step_launch_us = float(steps_ms.sum() / steps_n.sum() * 1e3) if steps_n.sum() else None
+    b.step_launches = (steps_n / nprof).tolist()
     return ms, z, ph, ms_host
 
 
@@ -620,21 +629,25 @@ def main():
     pm = mcmc.ParamMap(a.params, ds.tree.branchlengths)
     thetas = pm.theta_of(a.params) + np.random.default_rng(1).normal(0, 0.02, (args.batch, mcmc.DIM))
     res = {}
+    tune = not (args.tile or args.window or args.groups)
     for prec in args.precs:
-        ms1, z1, ph1, ms1h = time_batch(b1, sites[:1], pairs[:1], prec, args.steps, args.warmup)
-        msB, zB, phB, msBh = time_batch(bB, sites, pairs, prec, args.steps, args.warmup)
+        ms1, z1, ph1, ms1h = time_batch(b1, sites[:1], pairs[:1], prec, args.steps, args.warmup, tune)
+        msB, zB, phB, msBh = time_batch(bB, sites, pairs, prec, args.steps, args.warmup, tune)
         ms1p, _ = time_from_params(b1, pm, thetas[:1], prec, args.steps, args.warmup)
         msBp, _ = time_from_params(bB, pm, thetas, prec, args.steps, args.warmup)
         res[prec] = {"batch1": {"ms_per_eval": ms1, "evals_per_s": 1e3 / ms1, "logZ": float(z1[0]),
                                 "evals_per_s_host_inputs": 1e3 / ms1h,
                                 "evals_per_s_from_params": 1e3 / ms1p,
-                                "phase_ms": {"site": ph1[0], "pair": ph1[1], "inside": ph1[2]}},
+                                "phase_ms": {"site": ph1[0], "pair": ph1[1], "inside": ph1[2]},
+                                "schedule": b1.schedule, "inside_step_launch_us": b1.step_launch_us},
                      f"batch{args.batch}": {"ms_per_step": msB, "evals_per_s": args.batch * 1e3 / msB,
                                             "evals_per_s_host_inputs": args.batch * 1e3 / msBh,
                                             "evals_per_s_from_params": args.batch * 1e3 / msBp,
                                             "ms_per_step_from_params": msBp,
                                             "phase_ms": {"site": phB[0], "pair": phB[1], "inside": phB[2]},
-                                            "inside_step_launch_us": bB.step_launch_us},
+                                            "inside_step_launch_us": bB.step_launch_us,
+                                            "inside_step_launches": bB.step_launches,
+                                            "schedule": bB.schedule},
                      "logZ_batch": zB.tolist()}
     T_in = (L ** 3 - L) / 6
     lead = "fp32" if "fp32" in res else args.precs[0]
@@ -655,22 +668,26 @@ def main():
     }
     if lead == "fp32":
         # algorithmic flops per launch (2 per bifurcation term, the batch's terms spread over
-        # the NT step launches) over the step kernel's mean launch duration (HIP events)
-        NT = -(-L // (args.tile or 16))
+        # the step launches) over the step kernel's mean launch duration (HIP events)
+        nl = r.get("inside_step_launches") or [0, 0]
+        TE = 32 if nl[1] and not nl[0] else 16
+        sch = r.get("schedule") or {}
+        W = sch.get("window") or args.window or 4
+        NT = int(round(sum(nl))) or -(-L // TE)
         flops_launch = args.batch * 2 * T_in / NT
         launch_us = r.get("inside_step_launch_us") or r["phase_ms"]["inside"] * 1e3 / NT
         achieved = flops_launch / (launch_us * 1e-6) / 1e12
         out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
                            "frac": achieved / PEAK["fp32"], "traffic": None,
-                           "kernel": f"inside_step_kernel<XF,16,4> (batch of {args.batch}: {flops_launch:.4g} flops "
-                                     f"per launch = 2 per bifurcation term, {int(T_in)} terms per evaluation, "
+                           "kernel": f"inside_step_kernel<XF,{TE},{W}> (batch of {args.batch}: {flops_launch:.4g} "
+                                     f"flops per launch = 2 per bifurcation term, {int(T_in)} terms per evaluation, "
                                      f"{NT} launches; mean launch {launch_us:.1f} us)",
                            "inside_phase_tflops": args.batch * 2 * T_in / (r["phase_ms"]["inside"] * 1e-3) / 1e12}
         # HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE passes of this same workload
         # (copy-calibrated); only quoted for the shape the counters were taken on
         tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                           "inside_xf16_config2_batch16_traffic.json")
-        if args.batch == 16 and args.tile in (0, 16) and args.window in (0, 4) and os.path.exists(tf):
+        if args.batch == 16 and TE == 16 and W == 4 and os.path.exists(tf):
             with open(tf) as f:
                 tj = json.load(f)
             out["roofline"]["traffic"] = tj["hbm_bytes_per_launch"]

@@ -305,6 +305,10 @@ int messi_batch_set_tiles(messi_batch *b, const int *tiles);
  * grouping.  With profiling on, an evaluation runs as one group.  No
  * counterpart in the reference. */
 int messi_batch_set_groups(messi_batch *b, int groups);
+/* Bulk window of this batch's inside schedule (1, 2 or 4; 0, the default, takes
+ * the context's, messi_set_bulk_window).  Results do not depend on it.  Set by
+ * the autotuner.  No counterpart in the reference. */
+int messi_batch_set_window(messi_batch *b, int window);
 
 /* Learned MCMC proposal network: din inputs -> tanh(dh) -> tanh(dh) -> 2*din
  * outputs (mean shift mu, then log-scale ell, of a diagonal Gaussian proposal

@@ -162,6 +162,7 @@ struct messi_batch {
   std::vector<std::unique_ptr<InsideWork>> w64, w32;
   InsideBatch ib, ib32;     // launch tables of the tile-16 and tile-32 groups
   int groups = 0;           // member groups of a pipelined evaluation (0: automatic)
+  int window = 0;           // bulk window of the inside schedule (0: the context's)
   InsideBatch pib[messi_ctx::NPIPE + 1], pib32[messi_ctx::NPIPE + 1];  // their launch tables
   PipeEvents pev;
   std::vector<int> tiles;   // per-member inside tile edge (0: the context's)
@@ -873,7 +874,7 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
   for (int m = 0; m < b->n; ++m) {
     ws[m] = &member_work(b, m, prec);
     ws[m]->tile = (m < (int)b->tiles.size() && b->tiles[m]) ? b->tiles[m] : ctx->tile;
-    ws[m]->window = ctx->window;
+    ws[m]->window = b->window ? b->window : ctx->window;
     ws[m]->zout = b->zall.as<double>() + 2 * m;
     // pair scores beyond the distance bound are -inf (pair_expand writes them so)
     const int D = b->mem[m]->maxbp, Lm = b->mem[m]->L;
@@ -1868,6 +1869,14 @@ int messi_batch_set_groups(messi_batch *b, int groups) {
   return guard(b->ctx, [&] {
     MESSI_REQUIRE(groups >= 0 && groups <= messi_ctx::NPIPE + 1, "groups must be 0 (automatic) .. 4");
     b->groups = groups;
+  });
+}
+
+int messi_batch_set_window(messi_batch *b, int window) {
+  if (!b) return 2;
+  return guard(b->ctx, [&] {
+    MESSI_REQUIRE(window == 0 || window == 1 || window == 2 || window == 4, "bulk window must be 0, 1, 2 or 4");
+    b->window = window;
   });
 }
 

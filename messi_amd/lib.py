@@ -92,6 +92,7 @@ SIGNATURES = {
                                               _dp, _dp, _dp, _dp, ctypes.c_int]),
     "messi_batch_set_tiles": (ctypes.c_int, [_vp, _ip]),
     "messi_batch_set_groups": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "messi_batch_set_window": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_host_alloc": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
     "messi_host_free": (ctypes.c_int, [_vp, _vp]),
     "messi_mlp_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
@@ -704,6 +705,10 @@ class Batch:
     def set_groups(self, groups: int) -> None:
         """Member groups of a pipelined evaluation (1..4; 0 = automatic)."""
         self.ctx.check(self.ctx.lib.messi_batch_set_groups(self._h, int(groups)))
+
+    def set_window(self, window: int) -> None:
+        """Bulk window of this batch's inside schedule (1, 2, 4; 0 = the context's)."""
+        self.ctx.check(self.ctx.lib.messi_batch_set_window(self._h, int(window)))
 
     def eval_loaded(self, B: float = 1.0, prec=FP64) -> np.ndarray:
         """log Z of every member from the components of the last load."""
