@@ -536,14 +536,25 @@ def main_mcmc(args):
                             long=tr["long"].samples[::10], burn=tr["burn"].samples)
     z = np.abs(np.array(res["mean_z_learned_vs_long"]))
     z_ess = np.abs(np.array(res["mean_z_learned_vs_long_ess_only"]))
+    z_rw = np.abs(np.array(res["mean_z_rw_vs_long"]))
+    mr = res.get("mixture_no_network")
+    z_mix = np.abs(np.array(mr["mean_z_vs_long"])) if mr else None
     out = {"metric": "MCMC ESS/s, learned proposal vs adaptive random walk (min ESS over 12 parameters)",
            "value": res["ess_per_s_gain_min"], "unit": "x (learned ESS/s / random-walk ESS/s)", "n_gpus": 1,
            "higher_is_better": True, "dtype": args.prec.replace("fp", "f"), "data": "synthetic",
            "config": {"workload": f"50 sequences x 1500 nt, 12 parameters, {args.chains} chains, burn-in "
-                                  f"{args.burn}, {args.mcmc_steps} steps learned and random walk, long chain "
-                                  f"{args.long_steps}, network 12-128-128-24 trained {args.train_steps} Adam steps"},
+                                  f"{args.burn}, {args.mcmc_steps} production steps for each of learned, mixture "
+                                  f"without network and random walk, long chain {args.long_steps}, network "
+                                  f"12-128-128-24 trained {args.train_steps} Adam steps", "seed": args.seed},
+           "ess_per_s_gain_min_incl_burn_and_training": res["ess_per_s_gain_min_incl_burn"],
+           "ess_per_s_gain_median": res["ess_per_s_gain_median"],
+           "network_gain_over_mixture_min": res.get("network_gain_over_mixture_min"),
+           "network_gain_over_mixture_median": res.get("network_gain_over_mixture_median"),
+           "mixture_gain_over_rw_min": res.get("mixture_gain_over_rw_min"),
            "means_within_2_mcse": int(np.sum(z <= 2.0)), "means_within_3_mcse": int(np.sum(z <= 3.0)),
            "means_within_2_mcse_ess_only": int(np.sum(z_ess <= 2.0)),
+           "means_within_2_mcse_random_walk": int(np.sum(z_rw <= 2.0)),
+           "means_within_2_mcse_mixture_no_network": int(np.sum(z_mix <= 2.0)) if mr else None,
            "max_split_rhat": {k: float(np.max(v)) for k, v in res["split_rhat"].items()},
            "evals": post.evals, "device_eval_seconds": post.eval_seconds,
            "param_names": list(mcmc.NAMES), "result": res}
@@ -576,7 +587,7 @@ def main():
     ap.add_argument("--genome-spans", type=int, nargs="+", default=[1000000, 500])
     ap.add_argument("--chains", type=int, default=16)
     ap.add_argument("--burn", type=int, default=2000)
-    ap.add_argument("--mcmc-steps", type=int, default=3000)
+    ap.add_argument("--mcmc-steps", type=int, default=20000)
     ap.add_argument("--ind-prob", type=float, default=0.25)
     ap.add_argument("--save-traces", default="", help="write the chains to this .npz (MCMC mode)")
     ap.add_argument("--mix-components", type=int, default=6)

@@ -504,12 +504,15 @@ def _gauss_logq(x, mean, ell):
 
 def learned_chain(logp: Callable, theta, lp, steps: int, net, wh: Whitening, rng, rw_chol, rw_scale: float,
                   rw_prob: float = 0.1, kappa: float = 1.0, mix: Optional[Mixture] = None,
-                  ind_prob: float = 0.0) -> Trace:
+                  ind_prob: float = 0.0, progress: Optional[Callable[[int], None]] = None) -> Trace:
     """MH with the learned proposal z' ~ N(z + mu(z), diag (kappa exp(ell(z)))^2);
     each chain takes a random-walk step instead with probability ``rw_prob``, and
     with probability ``ind_prob`` an independence step from the fitted mixture
     ``mix`` (every kernel leaves the target invariant, so the random mixture of
-    kernels does too).  Trace.kind: 0 random walk, 1 learned, 2 independence."""
+    kernels does too).  With ``net`` None there is no learned move: the chain
+    takes the independence step with probability ``ind_prob`` and the random walk
+    otherwise (the method without the network).  Trace.kind: 0 random walk,
+    1 learned, 2 independence."""
     theta = np.array(theta, dtype=np.float64)
     lp = np.array(lp, dtype=np.float64)
     n, d = theta.shape
@@ -520,15 +523,22 @@ def learned_chain(logp: Callable, theta, lp, steps: int, net, wh: Whitening, rng
     t0 = time.perf_counter()
     for it in range(steps):
         u = rng.random(n)
-        use_rw = u < rw_prob
-        use_ind = (u >= rw_prob) & (u < rw_prob + ind_prob) if mix is not None else np.zeros(n, dtype=bool)
+        use_rw = u < rw_prob if net is not None else u >= ind_prob
+        use_ind = (u >= rw_prob) & (u < rw_prob + ind_prob) if net is not None else ~use_rw
+        if mix is None:
+            use_ind = np.zeros(n, dtype=bool)
+            use_rw = use_rw if net is not None else np.ones(n, dtype=bool)
         z = wh.z(theta)
-        out = net.forward(z.astype(np.float32)).astype(np.float64)
-        mu, ell = out[:, :d], np.clip(out[:, d:], -8.0, 4.0) + np.log(kappa)
-        zp = z + mu + np.exp(ell) * rng.standard_normal((n, d))
-        outp = net.forward(zp.astype(np.float32)).astype(np.float64)
-        mup, ellp = outp[:, :d], np.clip(outp[:, d:], -8.0, 4.0) + np.log(kappa)
-        log_ratio = _gauss_logq(z, zp + mup, ellp) - _gauss_logq(zp, z + mu, ell)
+        if net is not None:
+            out = net.forward(z.astype(np.float32)).astype(np.float64)
+            mu, ell = out[:, :d], np.clip(out[:, d:], -8.0, 4.0) + np.log(kappa)
+            zp = z + mu + np.exp(ell) * rng.standard_normal((n, d))
+            outp = net.forward(zp.astype(np.float32)).astype(np.float64)
+            mup, ellp = outp[:, :d], np.clip(outp[:, d:], -8.0, 4.0) + np.log(kappa)
+            log_ratio = _gauss_logq(z, zp + mup, ellp) - _gauss_logq(zp, z + mu, ell)
+        else:
+            zp = z.copy()
+            log_ratio = np.zeros(n)
         if mix is not None and use_ind.any():
             zi = mix.sample(n, rng)
             li = mix.logq(z) - mix.logq(zi)      # q(z) / q(z'): the whitening's Jacobian cancels
@@ -544,6 +554,8 @@ def learned_chain(logp: Callable, theta, lp, steps: int, net, wh: Whitening, rng
         lp = np.where(acc, lpp, lp)
         kind = np.where(use_rw, 0, np.where(use_ind, 2, 1)).astype(np.int8)
         S[it], LP[it], A[it], K[it] = theta, lp, acc, kind
+        if progress is not None and (it + 1) % 2000 == 0:
+            progress(it + 1)
     return Trace(S, LP, A, K, time.perf_counter() - t0)
 
 
@@ -798,15 +810,45 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     trb = rw_chain(logp, tb.samples[-1], tb.logp[-1], burn - burn_rw, rw_chol, rw_scale, rng,
                    adapt_until=burn - burn_rw)
     rw_chol2, rw_scale2 = trb.extra["chol"], trb.extra["scale"]
+    rw_burn_s = tb.seconds + trb.seconds
     log(f"burn-in (random walk, rest): {burn - burn_rw} steps, acceptance {trb.acceptance:.3f}")
+    # method without the network: the same burn-in budget as the learned method,
+    # its mixture fitted and refitted as the learned method's is (on the
+    # random-walk burn-in, then on an exploration segment with the independence
+    # moves), production with the independence moves at the same probability and
+    # the random walk otherwise
+    mix_res = None
+    if ind_prob > 0.0:
+        t_m = time.perf_counter()
+        pre = burn - burn_rw - per_trial
+        tm0 = rw_chain(logp, tb.samples[-1], tb.logp[-1], pre, rw_chol, rw_scale, rng, adapt_until=pre)
+        m_chol, m_scale = tm0.extra["chol"], tm0.extra["scale"]
+        mpool = np.concatenate([segs[0], tm0.samples[pre // 2:]], axis=0)
+        whm = Whitening(mpool)
+        mixm = Mixture.fit(whm.z(mpool).reshape(-1, d), mix_components, rng, inflate=mix_inflate,
+                           nu=mix_nu).flattened(mix_flatten)
+        tme = learned_chain(logp, tm0.samples[-1], tm0.logp[-1], per_trial, None, whm, rng, m_chol, m_scale,
+                            mix=mixm, ind_prob=max(ind_prob, 0.5))
+        mixm = Mixture.fit(whm.z(np.concatenate([mpool, tme.samples], axis=0)).reshape(-1, d), mix_components,
+                           rng, inflate=mix_inflate, nu=mix_nu).flattened(mix_flatten)
+        mix_burn_s = tb.seconds + time.perf_counter() - t_m
+        tmx = learned_chain(logp, tme.samples[-1], tme.logp[-1], steps, None, whm, rng, m_chol, m_scale,
+                            mix=mixm, ind_prob=ind_prob,
+                            progress=lambda i: log(f"mixture chain (no network): {i} of {steps} steps"))
+        mind_acc = float(tmx.accepted[tmx.kind == 2].mean()) if (tmx.kind == 2).any() else 0.0
+        log(f"mixture chain (no network): {steps} steps, acceptance {tmx.acceptance:.3f}, {mind_acc:.3f} "
+            f"(independence moves), {tmx.seconds:.1f} s")
+        mix_res = (tmx, mix_burn_s, mind_acc)
 
     tl = learned_chain(logp, theta, lp, steps, net, wh, rng, rw_chol, rw_scale, rw_prob=rw_prob,
-                       kappa=best["kappa"], mix=mix, ind_prob=ind_prob if mix is not None else 0.0)
+                       kappa=best["kappa"], mix=mix, ind_prob=ind_prob if mix is not None else 0.0,
+                       progress=lambda i: log(f"learned chain: {i} of {steps} steps"))
     learned_acc = float(tl.accepted[tl.kind == 1].mean()) if (tl.kind == 1).any() else 0.0
     ind_acc = float(tl.accepted[tl.kind == 2].mean()) if (tl.kind == 2).any() else 0.0
     log(f"learned chain: {steps} steps, acceptance {learned_acc:.3f} (learned moves), {ind_acc:.3f} "
         f"(independence moves), {tl.seconds:.1f} s")
-    tr = rw_chain(logp, trb.samples[-1], trb.logp[-1], steps, rw_chol2, rw_scale2, rng)
+    tr = rw_chain(logp, trb.samples[-1], trb.logp[-1], steps, rw_chol2, rw_scale2, rng,
+                  progress=lambda i: log(f"random-walk chain: {i} of {steps} steps"))
     log(f"random-walk chain: {steps} steps, acceptance {tr.acceptance:.3f}, {tr.seconds:.1f} s")
     tlong = rw_chain(logp, trb.samples[-1], trb.logp[-1], long_steps, rw_chol2, rw_scale2, rng,
                      progress=lambda i: log(f"long random-walk chain: {i} of {long_steps} steps"))
@@ -840,5 +882,30 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     }
     res["ess_per_s_gain_min"] = res["learned"]["min_ess_per_s"] / res["random_walk"]["min_ess_per_s"]
     res["ess_per_s_gain_median"] = res["learned"]["median_ess_per_s"] / res["random_walk"]["median_ess_per_s"]
-    res["traces"] = {"learned": tl, "random_walk": tr, "long": tlong, "burn": tb}
+    # ESS per second counting each method's own burn-in (and, for the learned one,
+    # its training) as well as its production chain
+    res["learned"]["burn_seconds"] = learned_burn_s
+    res["learned"]["min_ess_per_s_incl_burn"] = float(e_l.min() / (tl.seconds + learned_burn_s))
+    res["random_walk"]["burn_seconds"] = rw_burn_s
+    res["random_walk"]["min_ess_per_s_incl_burn"] = float(e_r.min() / (tr.seconds + rw_burn_s))
+    res["ess_per_s_gain_min_incl_burn"] = (res["learned"]["min_ess_per_s_incl_burn"]
+                                           / res["random_walk"]["min_ess_per_s_incl_burn"])
+    traces = {"learned": tl, "random_walk": tr, "long": tlong, "burn": tb}
+    if mix_res is not None:
+        tmx, mix_burn_s, mind_acc = mix_res
+        e_m = ess(tmx.samples)
+        z_m = compare_means(tmx.samples, e_m, tlong.samples, e_long)
+        res["mixture_no_network"] = {
+            "seconds": tmx.seconds, "burn_seconds": mix_burn_s, "acceptance": tmx.acceptance,
+            "acceptance_independence_moves": mind_acc, "ess": e_m.tolist(), "min_ess": float(e_m.min()),
+            "min_ess_per_s": float(e_m.min() / tmx.seconds), "median_ess_per_s": float(np.median(e_m) / tmx.seconds),
+            "min_ess_per_s_incl_burn": float(e_m.min() / (tmx.seconds + mix_burn_s)),
+            "mean": tmx.samples.reshape(-1, d).mean(axis=0).tolist(), "mean_z_vs_long": z_m.tolist(),
+            "max_abs_z": float(np.max(np.abs(z_m))), "split_rhat": split_rhat(tmx.samples).tolist()}
+        mr = res["mixture_no_network"]
+        res["network_gain_over_mixture_min"] = res["learned"]["min_ess_per_s"] / mr["min_ess_per_s"]
+        res["network_gain_over_mixture_median"] = res["learned"]["median_ess_per_s"] / mr["median_ess_per_s"]
+        res["mixture_gain_over_rw_min"] = mr["min_ess_per_s"] / res["random_walk"]["min_ess_per_s"]
+        traces["mixture"] = tmx
+    res["traces"] = traces
     return res

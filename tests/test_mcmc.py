@@ -337,6 +337,14 @@ def test_independence_moves_fix_the_mode_weights():
                                rw_prob=0.1)
     assert np.mean(local.samples[:, :, 0] < 0) < 0.01
 
+    # the method without the network (net None): random walk plus the same
+    # independence moves gives the same mode weights, with no learned move
+    nm = mcmc.learned_chain(_bimodal_logp, th0, _bimodal_logp(th0), 3000, None, wh, rng, np.eye(d) * 0.3, 1.0,
+                            mix=mix, ind_prob=0.3)
+    assert not (nm.kind == 1).any() and (nm.kind == 2).any() and (nm.kind == 0).any()
+    assert abs(np.mean(nm.kind == 2) - 0.3) < 0.02
+    assert abs(np.mean(nm.samples[500:, :, 0] < 0) - 0.3) < 0.03
+
     # control: the same independence draws without the ratio q(z)/q(z') settle on
     # target x proposal (mode at -3 weighted 0.3 x 0.8 against 0.7 x 0.2)
     th = th0.copy()
