@@ -44,16 +44,18 @@ def _components(params):
     return _stack(model.site_components(params)), _stack(model.pair_components(params))
 
 
-def _perturbed(a, ds, k, seed=0):
+def _perturbed(a, ds, k, seed=0, siteCats=1, lambdaCats=1):
     from messi_amd import model
     rng = np.random.default_rng(seed)
     out = []
     for i in range(k):
         v = a.params.copy()
+        if lambdaCats > 1:
+            v[14] = 0.3     # lambda-zero weight of the lambda mixture
         if i:
             v[0:3] = np.maximum(1.0, v[0:3] * np.exp(rng.normal(0, 0.05, 3)))
             v[3:9] = v[3:9] * np.exp(rng.normal(0, 0.05, 6))
-        out.append(model.getparams(v, ds.tree.branchlengths))
+        out.append(model.getparams(v, ds.tree.branchlengths, siteCats=siteCats, lambdaCats=lambdaCats))
     return out
 
 
@@ -596,6 +598,8 @@ def main():
     ap.add_argument("--long-steps", type=int, default=30000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--prec", default="fp32")
+    ap.add_argument("--site-cats", type=int, default=1, help="site-rate categories of the model (K = 1 default)")
+    ap.add_argument("--lambda-cats", type=int, default=1, help="lambda categories of the model")
     ap.add_argument("--seed", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -630,7 +634,7 @@ def main():
     lib.context()
     dd = device_dataset(ds)
     stats = dd.stats()
-    ps = _perturbed(a, ds, args.batch)
+    ps = _perturbed(a, ds, args.batch, siteCats=args.site_cats, lambdaCats=args.lambda_cats)
     comps = [_components(p) for p in ps]
     sites, pairs = [c[0] for c in comps], [c[1] for c in comps]
     bB = lib.Batch([dd] * args.batch)
@@ -644,8 +648,10 @@ def main():
     for prec in args.precs:
         ms1, z1, ph1, ms1h = time_batch(b1, sites[:1], pairs[:1], prec, args.steps, args.warmup, tune)
         msB, zB, phB, msBh = time_batch(bB, sites, pairs, prec, args.steps, args.warmup, tune)
-        ms1p, _ = time_from_params(b1, pm, thetas[:1], prec, args.steps, args.warmup)
-        msBp, _ = time_from_params(bB, pm, thetas, prec, args.steps, args.warmup)
+        # from parameter vectors: the 12-parameter model (one site and one lambda category)
+        k1 = args.site_cats == 1 and args.lambda_cats == 1
+        ms1p = time_from_params(b1, pm, thetas[:1], prec, args.steps, args.warmup)[0] if k1 else float("nan")
+        msBp = time_from_params(bB, pm, thetas, prec, args.steps, args.warmup)[0] if k1 else float("nan")
         res[prec] = {"batch1": {"ms_per_eval": ms1, "evals_per_s": 1e3 / ms1, "logZ": float(z1[0]),
                                 "evals_per_s_host_inputs": 1e3 / ms1h,
                                 "evals_per_s_from_params": 1e3 / ms1p,
@@ -669,7 +675,9 @@ def main():
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": lead.replace("fp", "f"),
         "data": "synthetic (seeded Yule tree, GTR/MuSE simulation, 40% paired, 5% gaps)",
-        "config": {"workload": f"config2: 50 sequences x 1500 nt, K=1, full-span pairs and inside; "
+        "config": {"workload": f"config2: 50 sequences x 1500 nt, {args.site_cats} site-rate and "
+                               f"{args.lambda_cats} lambda categories (K4={args.site_cats}, "
+                               f"K16={args.site_cats ** 2 * args.lambda_cats}), full-span pairs and inside; "
                                f"{args.batch} parameter vectors per step (one per chain)",
                    "L": L, "nseq": ds.numseqs, "batch": args.batch, "sum_U": stats["sum_U"],
                    "root_entries": stats["root_entries"],

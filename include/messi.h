@@ -216,6 +216,23 @@ int messi_sample_states(messi_dataset *ds, int K4, const double *P4, const doubl
                         double B, uint64_t seed, int prec, int *states, double *unpaired, double *pairedll,
                         double *siteconds, double *paircomps);
 
+/* Posterior weights of the site-rate categories of every column:
+ * felsensteinposteriorsiterateconditionalscuda (src/FelsensteinCUDA.jl:518-717).
+ * out (cats x L, column-major: out[c + cats*x]) receives, for category c and
+ * column x, the log-sum-exp of log w_c + log unpaired[x] + log L_c(x) (the site
+ * likelihood of category c) and, for every pair (x, y) of the dataset's pair list
+ * and every pair component k = (c1, c2, lambda class), log pairprob[min + L*max]
+ * + logw16[k] + log L_k(x, y), counted for c1 at the pair's first column and for
+ * c2 at its second.  unpaired (L) and pairprob (L x L, column-major) are the
+ * posteriors messi_bpp returns; the components are ordered as for
+ * messi_sample_states (K4 == cats, K16 == cats^2 * lambda classes).  The reference
+ * computes this with FP32 pair likelihoods; both precisions are offered here.
+ * Its only caller in the reference sits in a disabled branch of processmax
+ * (src/MESSI.jl:1066-1068). */
+int messi_site_rate_posteriors(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
+                               int K16, const double *P16, const double *pi16, const double *logw16, int cats,
+                               const double *unpaired, const double *pairprob, int prec, double *out);
+
 /* ---- Input formats (host only, no device context) ----
  * Alignment handle from a FASTA file and the first line of a Newick file
  * (getalignmentfromfastaandnewick, src/AlignmentIO.jl:18-48; the Dataset

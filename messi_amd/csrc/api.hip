@@ -948,6 +948,60 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
   if (!keep_outside) collect(b, 3);
 }
 
+// Every loaded component of the batch's single member in the same launches
+// (grid row = component), each with its own partials and root entries, one
+// launch per tree level and kind: sitec (K4 x L) receives log w_c + log L_c(x)
+// per site component, b->pair_comp each pair component's root entries with a
+// stride of the returned count (root entries of the member); *dpair the pair
+// expansion descriptor.
+template <typename T>
+int64_t component_roots(messi_batch *b, int prec, double *sitec, const ExpandDesc **dpair) {
+  hipStream_t s = b->ctx->stream;
+  const int K4 = b->c4.K, K16 = b->c16.K, L = b->mem[0]->L;
+  const int64_t se = std::max<int64_t>(1, b->site_entries), sr = std::max<int64_t>(1, b->site_root_total);
+  const int64_t pe = std::max<int64_t>(1, b->pair_entries), pr = std::max<int64_t>(1, b->pair_root_total);
+  b->site_part.ensure(sizeof(T) * 4 * se * K4);
+  b->site_expo.ensure(sizeof(int) * se * K4);
+  b->site_acc.ensure(sizeof(double) * sr * K4);
+  b->pair_part.ensure(sizeof(T) * 16 * pe * K16);
+  b->pair_expo.ensure(sizeof(int) * pe * K16);
+  b->pair_acc.ensure(sizeof(double) * pr * K16);
+  b->pair_comp.ensure(sizeof(double) * pr * K16);
+  const ExpandDesc *dsite = upload_expand(b, prec, false);
+  CompBatch cs;
+  cs.K = K4;
+  cs.part = 4 * se;
+  cs.expo = se;
+  cs.root = sr;
+  cs.P = b->nn_total * 16;
+  cs.pi = (int64_t)b->n * 4;
+  cs.logw = b->n;
+  for (int h = 1; h <= b->Hmax; ++h) {
+    const int lo = b->site_lv[h - 1], hi = b->site_lv[h];
+    launch_site_level<T>(s, b->site_items.as<int2>() + lo, hi - lo, b->site_nodes.as<NodeDesc>(), b->site_idx,
+                         b->site_leaf, b->site_part.as<T>(), b->site_expo.as<int>(), b->c4.P.as<T>(),
+                         b->c4.pi.as<double>(), b->c4.logw.as<double>(), 1, b->site_acc.as<double>(), cs);
+  }
+  launch_site_gather(s, L, K4, sr, dsite, b->site_acc.as<double>(), sitec);
+  *dpair = upload_expand(b, prec, true);
+  CompBatch cp;
+  cp.K = K16;
+  cp.part = 16 * pe;
+  cp.expo = pe;
+  cp.root = pr;
+  cp.P = b->nn_total * 256;
+  cp.pi = (int64_t)b->n * 16;
+  cp.logw = b->n;
+  for (int h = 1; h <= b->Hmax; ++h) {
+    const int lo = b->pair_lv[h - 1], hi = b->pair_lv[h];
+    launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
+                                  b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), b->pair_expo.as<int>(),
+                                  b->c16.P.as<T>(), b->c16.pi.as<double>(), b->c16.logw.as<double>(), 1,
+                                  b->pair_acc.as<double>(), b->pair_comp.as<double>(), cp);
+  }
+  return pr;
+}
+
 }  // namespace
 
 namespace {
@@ -1610,52 +1664,11 @@ int messi_sample_states(messi_dataset *ds, int K4, const double *P4, const doubl
     double *sitec = ds->d_sitec.as<double>(), *pairc = ds->d_pairc.as<double>();
     double *d_unp = ds->d_sout.as<double>(), *d_pll = d_unp + L;
     int *d_states = reinterpret_cast<int *>(d_pll + L);
-    // Every component in the same launches (grid row = component), each with its
-    // own partials and root entries: one launch per tree level and kind.
     auto go = [&](auto tag) {
       using T = decltype(tag);
       load_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
-      const int64_t se = std::max<int64_t>(1, b->site_entries), sr = std::max<int64_t>(1, b->site_root_total);
-      const int64_t pe = std::max<int64_t>(1, b->pair_entries), pr = std::max<int64_t>(1, b->pair_root_total);
-      b->site_part.ensure(sizeof(T) * 4 * se * K4);
-      b->site_expo.ensure(sizeof(int) * se * K4);
-      b->site_acc.ensure(sizeof(double) * sr * K4);
-      b->pair_part.ensure(sizeof(T) * 16 * pe * K16);
-      b->pair_expo.ensure(sizeof(int) * pe * K16);
-      b->pair_acc.ensure(sizeof(double) * pr * K16);
-      b->pair_comp.ensure(sizeof(double) * pr * K16);
-      const ExpandDesc *dsite = upload_expand(b, prec, false);
-      CompBatch cs;
-      cs.K = K4;
-      cs.part = 4 * se;
-      cs.expo = se;
-      cs.root = sr;
-      cs.P = b->nn_total * 16;
-      cs.pi = (int64_t)b->n * 4;
-      cs.logw = b->n;
-      for (int h = 1; h <= b->Hmax; ++h) {
-        const int lo = b->site_lv[h - 1], hi = b->site_lv[h];
-        launch_site_level<T>(s, b->site_items.as<int2>() + lo, hi - lo, b->site_nodes.as<NodeDesc>(), b->site_idx,
-                             b->site_leaf, b->site_part.as<T>(), b->site_expo.as<int>(), b->c4.P.as<T>(),
-                             b->c4.pi.as<double>(), b->c4.logw.as<double>(), 1, b->site_acc.as<double>(), cs);
-      }
-      launch_site_gather(s, L, K4, sr, dsite, b->site_acc.as<double>(), sitec);
-      const ExpandDesc *dpair = upload_expand(b, prec, true);
-      CompBatch cp;
-      cp.K = K16;
-      cp.part = 16 * pe;
-      cp.expo = pe;
-      cp.root = pr;
-      cp.P = b->nn_total * 256;
-      cp.pi = (int64_t)b->n * 16;
-      cp.logw = b->n;
-      for (int h = 1; h <= b->Hmax; ++h) {
-        const int lo = b->pair_lv[h - 1], hi = b->pair_lv[h];
-        launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
-                                      b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), b->pair_expo.as<int>(),
-                                      b->c16.P.as<T>(), b->c16.pi.as<double>(), b->c16.logw.as<double>(), 1,
-                                      b->pair_acc.as<double>(), b->pair_comp.as<double>(), cp);
-      }
+      const ExpandDesc *dpair = nullptr;
+      const int64_t pr = component_roots<T>(b, prec, sitec, &dpair);
       launch_pair_gather(s, L, K16, pr, dpair, b->pair_comp.as<double>(), ds->d_partner.as<int>(), pairc,
                          ds->d_miss.as<int>());
       MESSI_HIP(hipGetLastError());
@@ -1675,6 +1688,44 @@ int messi_sample_states(messi_dataset *ds, int K4, const double *P4, const doubl
       MESSI_HIP(hipMemcpyAsync(paircomps, pairc, sizeof(double) * (size_t)K16 * L, hipMemcpyDeviceToHost, s));
     MESSI_HIP(hipStreamSynchronize(s));
     MESSI_REQUIRE(miss == 0, "the dataset's pair list lacks a pair of the structure (distance bound or mask)");
+  });
+}
+
+int messi_site_rate_posteriors(messi_dataset *ds, int K4, const double *P4, const double *pi4, const double *logw4,
+                               int K16, const double *P16, const double *pi16, const double *logw16, int cats,
+                               const double *unpaired, const double *pairprob, int prec, double *out) {
+  if (!ds) return 2;
+  messi_ctx *ctx = ds->ctx;
+  return guard(ctx, [&] {
+    MESSI_REQUIRE(P4 && pi4 && logw4 && P16 && pi16 && logw16 && unpaired && pairprob && out, "null argument");
+    check_prec(prec);
+    MESSI_REQUIRE(cats >= 1 && cats <= 8, "cats (site categories) must be 1 .. 8");
+    MESSI_REQUIRE(K4 == cats, "K4 must equal the number of site categories");
+    MESSI_REQUIRE(K16 >= cats * cats && K16 % (cats * cats) == 0,
+                  "K16 must be cats^2 times the number of lambda classes");
+    const int L = ds->L;
+    MESSI_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    messi_batch *b = self_batch(ds);
+    ds->d_sitec.ensure(sizeof(double) * (size_t)K4 * L);
+    ds->d_sout.ensure(sizeof(double) * ((size_t)L * L + (size_t)L + (size_t)cats * L));
+    double *sitec = ds->d_sitec.as<double>();
+    double *d_pp = ds->d_sout.as<double>(), *d_single = d_pp + (size_t)L * L, *d_out = d_single + L;
+    MESSI_HIP(hipMemcpyAsync(d_pp, pairprob, sizeof(double) * L * L, hipMemcpyHostToDevice, s));
+    MESSI_HIP(hipMemcpyAsync(d_single, unpaired, sizeof(double) * L, hipMemcpyHostToDevice, s));
+    auto go = [&](auto tag) {
+      using T = decltype(tag);
+      load_impl<T>(b, K4, P4, pi4, logw4, K16, P16, pi16, logw16);
+      const ExpandDesc *dpair = nullptr;
+      const int64_t pr = component_roots<T>(b, prec, sitec, &dpair);
+      launch_site_rate_posterior(s, L, cats, K16, pr, dpair, b->pair_comp.as<double>(), b->c16.logw.as<double>(),
+                                 sitec, d_single, d_pp, d_out);
+      MESSI_HIP(hipGetLastError());
+    };
+    if (prec == MESSI_FP32) go(float{});
+    else go(double{});
+    MESSI_HIP(hipMemcpyAsync(out, d_out, sizeof(double) * (size_t)cats * L, hipMemcpyDeviceToHost, s));
+    MESSI_HIP(hipStreamSynchronize(s));
   });
 }
 

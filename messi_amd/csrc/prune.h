@@ -66,6 +66,16 @@ void launch_site_gather(hipStream_t s, int L, int K, int64_t root_stride, const 
                         double *out);
 // out[x] = pair root value of the pair opened at x (-inf elsewhere); pairs the
 // member's pair list lacks are counted in *missing (not cleared here).
+// Posterior weights of the site-rate categories of every column
+// (felsensteinposteriorsiterateconditionalscuda, src/FelsensteinCUDA.jl:518-717):
+// out[c + cats * x] = log-sum-exp of the column's site term sitec[c L + x] +
+// log single[x] and, over every pair (x, y) of the pair list and every pair
+// component k = (c1 * cats + c2) * M + m, log pair[min + L max] + logw16[k] +
+// L_k(x, y), counted for category c1 at the pair's first column and c2 at its
+// second.  paircomp holds each component's root entries (stride root_stride).
+void launch_site_rate_posterior(hipStream_t s, int L, int cats, int K16, int64_t root_stride,
+                                const ExpandDesc *dp, const double *paircomp, const double *logw16,
+                                const double *sitec, const double *single, const double *pairprob, double *out);
 void launch_pair_gather(hipStream_t s, int L, int K, int64_t root_stride, const ExpandDesc *dp, const double *acc,
                         const int *partner, double *out, int *missing);
 // One categorical draw per unpaired column and per base pair (keyed uniforms, as

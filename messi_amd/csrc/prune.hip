@@ -386,6 +386,81 @@ __global__ void pair_gather_kernel(const ExpandDesc *__restrict__ dd, int64_t rs
   out[x] = v;
 }
 
+// One workgroup per column x; each thread takes the partners y = t, t + 256, ...
+// and keeps an online (maximum, sum) per category, then the workgroup reduces
+// them in a fixed tree.  The reference adds the terms one by one in column
+// order; the sum is the same up to rounding.
+constexpr int SRP_THREADS = 256, SRP_MAXCATS = 8;
+__global__ __launch_bounds__(SRP_THREADS) void site_rate_posterior_kernel(
+    const ExpandDesc *__restrict__ dd, int64_t rstride, const double *__restrict__ paircomp,
+    const double *__restrict__ logw16, int K16, int cats, const double *__restrict__ sitec,
+    const double *__restrict__ single, const double *__restrict__ pairprob, double *__restrict__ out) {
+  const ExpandDesc d = *dd;
+  const int L = d.L, x = blockIdx.x, t = threadIdx.x;
+  const int M = K16 / (cats * cats);
+  // per-thread accumulators (maximum, sum) of each category, in LDS (the category
+  // of a term is known only at run time)
+  __shared__ double am[SRP_MAXCATS][SRP_THREADS], as[SRP_MAXCATS][SRP_THREADS];
+  for (int c = 0; c < SRP_MAXCATS; ++c) {
+    am[c][t] = -HUGE_VAL;
+    as[c][t] = 0.0;
+  }
+  auto add = [&](int c, double v) {
+    if (v == -HUGE_VAL) return;
+    const double mc = am[c][t];
+    if (v > mc) {
+      as[c][t] = as[c][t] * exp(mc - v) + 1.0;
+      am[c][t] = v;
+    } else {
+      as[c][t] += exp(v - mc);
+    }
+  };
+  if (t == 0) {
+    const double ls = log(single[x]);
+    for (int c = 0; c < cats; ++c) {
+      const double u = sitec[(size_t)c * L + x];
+      if (u != -HUGE_VAL) add(c, u + ls);
+    }
+  }
+  for (int y = t; y < L; y += SRP_THREADS) {
+    if (y == x) continue;
+    const int a = min(x, y), b = max(x, y);
+    if (b - a > d.maxbp || (d.mask && !d.mask[(size_t)a + (size_t)L * b])) continue;
+    const int k0 = d.T0[(size_t)d.pid0[a] * d.n0 + d.pid0[b]];
+    if (k0 < 0) continue;
+    const double lpp = log(pairprob[(size_t)a + (size_t)L * b]);
+    if (lpp == -HUGE_VAL) continue;
+    const bool first = x == a;
+    for (int k = 0; k < K16; ++k) {
+      const int c1 = k / (cats * M), c2 = (k / M) % cats;
+      add(first ? c1 : c2, lpp + logw16[k] + paircomp[(size_t)k * rstride + d.rootoff + k0]);
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < cats; ++c) {
+    for (int o = SRP_THREADS / 2; o > 0; o >>= 1) {
+      if (t < o) {
+        const double m1 = am[c][t], m2 = am[c][t + o];
+        const double mm = fmax(m1, m2);
+        if (mm != -HUGE_VAL) {
+          as[c][t] = (m1 == -HUGE_VAL ? 0.0 : as[c][t] * exp(m1 - mm)) +
+                     (m2 == -HUGE_VAL ? 0.0 : as[c][t + o] * exp(m2 - mm));
+          am[c][t] = mm;
+        }
+      }
+      __syncthreads();
+    }
+    if (t == 0) out[(size_t)c + (size_t)cats * x] = as[c][0] > 0.0 ? am[c][0] + log(as[c][0]) : -HUGE_VAL;
+  }
+}
+
+void launch_site_rate_posterior(hipStream_t s, int L, int cats, int K16, int64_t root_stride,
+                                const ExpandDesc *dp, const double *paircomp, const double *logw16,
+                                const double *sitec, const double *single, const double *pairprob, double *out) {
+  hipLaunchKernelGGL(site_rate_posterior_kernel, dim3(L), dim3(SRP_THREADS), 0, s, dp, root_stride, paircomp,
+                     logw16, K16, cats, sitec, single, pairprob, out);
+}
+
 void launch_site_gather(hipStream_t s, int L, int K, int64_t root_stride, const ExpandDesc *ds, const double *acc,
                         double *out) {
   hipLaunchKernelGGL(site_gather_kernel, dim3((L + 255) / 256, K), dim3(256), 0, s, ds, root_stride, acc, out);

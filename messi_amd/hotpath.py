@@ -23,6 +23,7 @@ Function here                    Reference
 ``maskgapped``                   src/RNATools.jl:441-451 (host, in place; ``gapmask`` gives the same rule as a
                                  dataset pair mask, so the device never prunes the masked pairs)
 ``maskgapped_structure``         src/RNATools.jl:453-466
+``felsensteinposteriorsiterateconditionalscuda``  src/FelsensteinCUDA.jl:518-717
 ===============================  ==========================================================
 
 ``prec`` selects the plain FP64 path (default) or the scaled FP32 path.
@@ -341,6 +342,18 @@ def samplestates(seed: int, dataset, params: ModelParameters, paired, B: float =
                                         paired, params.siteCats, seed=seed, B=B, prec=prec)
     params.states = states
     return unp, pll, states
+
+
+def felsensteinposteriorsiterateconditionalscuda(singleprobs, pairprobs, dataset, params: ModelParameters,
+                                                 museparams: Optional[List[MuseSpecificParameters]] = None,
+                                                 maxbasepairdistance: int = 1000000,
+                                                 keepmuseconditionals: bool = False, prec=lib.FP64) -> np.ndarray:
+    """(siteCats, L) log posterior weights of the site-rate categories of every
+    column, src/FelsensteinCUDA.jl:518-717 (the lambda categories come from params;
+    museparams is accepted for signature parity)."""
+    dd = device_dataset(dataset, maxbasepairdistance)
+    return dd.site_rate_posteriors(_stack(site_components(params)), _stack(pair_components(params)),
+                                   params.siteCats, singleprobs, pairprobs, prec)
 
 
 def computemodellikelihood(dataset, params: ModelParameters, paired, samplebranchlengths: bool = False,

@@ -70,6 +70,8 @@ SIGNATURES = {
     "messi_sample_states": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                                            ctypes.c_int, _ip, ctypes.c_double, ctypes.c_uint64, ctypes.c_int,
                                            _ip, _dp, _dp, _dp, _dp]),
+    "messi_site_rate_posteriors": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
+                                                  ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]),
     "messi_sample_structures": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_uint64, ctypes.c_double, _ip, _dp]),
     "messi_sample_posterior": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
@@ -419,6 +421,20 @@ class DeviceDataset:
             ptr(pll), ptr(sc), ptr(pc)))
         out = (st.astype(np.int64), unp, pll)
         return out + (sc, pc) if conditionals else out
+
+    def site_rate_posteriors(self, site, pair, cats: int, unpaired, pairprob, prec=FP64) -> np.ndarray:
+        """(cats, L) log posterior weights of the site-rate categories of every
+        column (messi_site_rate_posteriors) from the base-pair posteriors."""
+        K4, P4c, pi4, lw4, K16, P16c, pi16, lw16 = self._mixture(site, pair)
+        u = np.ascontiguousarray(unpaired, dtype=np.float64)
+        if u.shape != (self.L,):
+            raise ValueError(f"unpaired must have length {self.L}")
+        pp = _square(pairprob, self.L, "pairprob")
+        out = np.empty((int(cats), self.L), order="F")
+        self.ctx.check(self.ctx.lib.messi_site_rate_posteriors(
+            self._h, K4, ptr(P4c), ptr(pi4), ptr(lw4), K16, ptr(P16c), ptr(pi16), ptr(lw16), int(cats), ptr(u),
+            ptr(pp), prec_code(prec), ptr(out)))
+        return out
 
     def sample_posterior(self, site, pair, nsamples: int, seed: int = 0, B: float = 1.0, prec=FP64):
         """log Z, then nsamples structures (nsamples x L, 1-based partners) drawn from

@@ -504,15 +504,18 @@ def _gauss_logq(x, mean, ell):
 
 def learned_chain(logp: Callable, theta, lp, steps: int, net, wh: Whitening, rng, rw_chol, rw_scale: float,
                   rw_prob: float = 0.1, kappa: float = 1.0, mix: Optional[Mixture] = None,
-                  ind_prob: float = 0.0, progress: Optional[Callable[[int], None]] = None) -> Trace:
+                  ind_prob: float = 0.0, progress: Optional[Callable[[int], None]] = None,
+                  mix_wh: Optional[Whitening] = None) -> Trace:
     """MH with the learned proposal z' ~ N(z + mu(z), diag (kappa exp(ell(z)))^2);
     each chain takes a random-walk step instead with probability ``rw_prob``, and
     with probability ``ind_prob`` an independence step from the fitted mixture
     ``mix`` (every kernel leaves the target invariant, so the random mixture of
     kernels does too).  With ``net`` None there is no learned move: the chain
     takes the independence step with probability ``ind_prob`` and the random walk
-    otherwise (the method without the network).  Trace.kind: 0 random walk,
-    1 learned, 2 independence."""
+    otherwise (the method without the network).  The mixture lives in the
+    coordinates of ``mix_wh`` when given (else ``wh``, the network's).
+    Trace.kind: 0 random walk, 1 learned, 2 independence."""
+    mwh = mix_wh if mix_wh is not None else wh
     theta = np.array(theta, dtype=np.float64)
     lp = np.array(lp, dtype=np.float64)
     n, d = theta.shape
@@ -539,12 +542,12 @@ def learned_chain(logp: Callable, theta, lp, steps: int, net, wh: Whitening, rng
         else:
             zp = z.copy()
             log_ratio = np.zeros(n)
+        prop = wh.theta(zp)
         if mix is not None and use_ind.any():
             zi = mix.sample(n, rng)
-            li = mix.logq(z) - mix.logq(zi)      # q(z) / q(z'): the whitening's Jacobian cancels
-            zp = np.where(use_ind[:, None], zi, zp)
+            li = mix.logq(mwh.z(theta)) - mix.logq(zi)   # q(z) / q(z'): the whitening's Jacobian cancels
+            prop = np.where(use_ind[:, None], mwh.theta(zi), prop)
             log_ratio = np.where(use_ind, li, log_ratio)
-        prop = wh.theta(zp)
         rwp = theta + rw_scale * (rng.standard_normal((n, d)) @ np.asarray(rw_chol).T)
         prop = np.where(use_rw[:, None], rwp, prop)
         log_ratio = np.where(use_rw, 0.0, log_ratio)
@@ -734,10 +737,14 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     being scored by their expected squared jump distance; a last training on
     all burn-in samples at the best noise level fixes the network, and the best
     kappa is kept for production.  A mixture of Student-t components fitted to
-    the whitened burn-in samples gives independence moves (probability
-    ``ind_prob``) that jump between the modes the burn-in visited.  Random-walk
-    method: adaptive random walk for all ``burn`` steps.  Production kernels are
-    fixed (no adaptation), so each production chain is an exact MH chain.
+    the whitened random-walk burn-in (and refitted after an exploration segment)
+    gives independence moves (probability ``ind_prob``) that jump between the
+    modes the burn-in visited; it is shared with the method without the network
+    (random walk plus the same independence moves), so the two differ only in
+    the network's moves, which take the share of the random walk that their
+    burn-in jump distance earns.  Random-walk method: adaptive random walk for
+    all ``burn`` steps.  Production kernels are fixed (no adaptation), so each
+    production chain is an exact MH chain.
     """
     theta0 = np.asarray(theta0, dtype=np.float64)
     n, d = theta0.shape
@@ -790,21 +797,6 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     train_s += time.perf_counter() - t0
     log(f"final training: noise {best['noise']}, kappa {best['kappa']}, pool {pool.shape[0]} x {n}, "
         f"held-out loss {tinfo['best_val_loss']:.3f} at step {tinfo['best_step']}")
-    mix = None
-    if ind_prob > 0.0:
-        # fit, explore with the independence moves (still burn-in), refit on
-        # everything: modes the random-walk burn-in barely visited get samples
-        mix = Mixture.fit(wh.z(pool).reshape(-1, d), mix_components, rng, inflate=mix_inflate,
-                          nu=mix_nu).flattened(mix_flatten)
-        te = learned_chain(logp, theta, lp, per_trial, net, wh, rng, rw_chol, rw_scale, rw_prob=rw_prob,
-                           kappa=best["kappa"], mix=mix, ind_prob=max(ind_prob, 0.5))
-        segs.append(te.samples)
-        theta, lp = te.samples[-1], te.logp[-1]
-        explore = np.concatenate(segs[1:], axis=0)
-        mix = Mixture.fit(wh.z(explore).reshape(-1, d), mix_components, rng, inflate=mix_inflate,
-                          nu=mix_nu).flattened(mix_flatten)
-        log(f"independence mixture: {len(mix.w)} components, weights {np.round(mix.w, 3).tolist()} "
-            f"(exploration acceptance {te.accepted[te.kind == 2].mean() if (te.kind == 2).any() else 0.0:.3f})")
     learned_burn_s = time.perf_counter() - t_start
     # random-walk method: the rest of its own burn-in
     trb = rw_chain(logp, tb.samples[-1], tb.logp[-1], burn - burn_rw, rw_chol, rw_scale, rng,
@@ -812,12 +804,14 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     rw_chol2, rw_scale2 = trb.extra["chol"], trb.extra["scale"]
     rw_burn_s = tb.seconds + trb.seconds
     log(f"burn-in (random walk, rest): {burn - burn_rw} steps, acceptance {trb.acceptance:.3f}")
-    # method without the network: the same burn-in budget as the learned method,
-    # its mixture fitted and refitted as the learned method's is (on the
-    # random-walk burn-in, then on an exploration segment with the independence
-    # moves), production with the independence moves at the same probability and
-    # the random walk otherwise
+    # The independence mixture, shared by the learned method and the method
+    # without the network (so the two differ only in the network's moves): fitted
+    # on the random-walk burn-in, used in an exploration segment with the
+    # independence moves (modes the random walk barely visited get samples), and
+    # refitted on both.  The method without the network then runs the
+    # independence moves with probability ind_prob and the random walk otherwise.
     mix_res = None
+    mixm = whm = None
     if ind_prob > 0.0:
         t_m = time.perf_counter()
         pre = burn - burn_rw - per_trial
@@ -840,8 +834,20 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
             f"(independence moves), {tmx.seconds:.1f} s")
         mix_res = (tmx, mix_burn_s, mind_acc)
 
-    tl = learned_chain(logp, theta, lp, steps, net, wh, rng, rw_chol, rw_scale, rw_prob=rw_prob,
-                       kappa=best["kappa"], mix=mix, ind_prob=ind_prob if mix is not None else 0.0,
+    # Learned method: the same independence moves plus the network's moves, which
+    # replace most of the random walk when their burn-in jump distance beat the
+    # random walk's and a small share of it otherwise.
+    rw_esjd = _esjd(np.concatenate([tb.samples[-1][None], trb.samples]), wh)
+    if mixm is not None:
+        p_learned = (1.0 - ind_prob) * (0.85 if best["esjd_min"] > rw_esjd else 0.3)
+        l_rw, l_ind = 1.0 - ind_prob - p_learned, ind_prob
+        learned_burn_s += mix_res[1] - tb.seconds      # its share of the shared mixture's burn-in
+    else:
+        p_learned, l_rw, l_ind = 1.0 - rw_prob, rw_prob, 0.0
+    log(f"learned move share {p_learned:.2f} (burn-in min ESJD: learned {best['esjd_min']:.4f}, "
+        f"random walk {rw_esjd:.4f})")
+    tl = learned_chain(logp, theta, lp, steps, net, wh, rng, rw_chol, rw_scale, rw_prob=l_rw,
+                       kappa=best["kappa"], mix=mixm, ind_prob=l_ind, mix_wh=whm,
                        progress=lambda i: log(f"learned chain: {i} of {steps} steps"))
     learned_acc = float(tl.accepted[tl.kind == 1].mean()) if (tl.kind == 1).any() else 0.0
     ind_acc = float(tl.accepted[tl.kind == 2].mean()) if (tl.kind == 2).any() else 0.0
@@ -860,9 +866,10 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     res = {
         "nchains": n, "burn": burn, "steps": steps, "long_steps": long_steps,
         "train_seconds": train_s, "learned_burn_seconds": learned_burn_s, "trials": trials,
-        "chosen": {"noise": best["noise"], "kappa": best["kappa"]},
+        "chosen": {"noise": best["noise"], "kappa": best["kappa"], "learned_share": p_learned,
+                   "burn_esjd_learned": best["esjd_min"], "burn_esjd_random_walk": rw_esjd},
         "learned": {"seconds": tl.seconds, "acceptance_learned_moves": learned_acc,
-                    "acceptance_independence_moves": ind_acc, "independence_prob": ind_prob if mix else 0.0,
+                    "acceptance_independence_moves": ind_acc, "independence_prob": l_ind,
                     "acceptance": tl.acceptance, "ess": e_l.tolist(), "min_ess": float(e_l.min()),
                     "min_ess_per_s": float(e_l.min() / tl.seconds),
                     "median_ess_per_s": float(np.median(e_l) / tl.seconds)},

@@ -10,9 +10,10 @@ definition:
 * ``muse``                -- src/RateMatrices.jl:69-148 (16-state dinucleotide MuSE model)
 * ``gettransitionmatriceslist`` -- src/FARCE.jl:389-433 (eigen, clamp <= 0 to 1e-50, t == 0 -> I,
   index 0 (the root) is an unused zero matrix)
-* ``ModelParameters`` / ``getparams`` -- src/FARCE.jl:460-657 (15-vector layout, K = 1 special case
-  at 645-652)
+* ``ModelParameters`` / ``getparams`` -- src/FARCE.jl:460-657 (15-vector layout, lambda categories
+  from discretizegamma3, the single-category special case at 645-652)
 * ``discretizegamma``     -- src/FARCE.jl:1531-1540
+* ``discretizegamma3``    -- src/FARCE.jl:1566-1583
 * ``computeprior``        -- src/FARCE.jl:2401-2436
 """
 from __future__ import annotations
@@ -140,6 +141,19 @@ def discretizegamma(shape: float, scale: float, numcategories: int) -> np.ndarra
     return np.array([gamma.ppf(v, a=shape, scale=scale) for v in vs])
 
 
+def discretizegamma3(lambdazeroweight: float, meanGC: float, meanAT: float, meanGT: float, gammavar: float,
+                     numcategories: int):
+    """Lambda categories, src/FARCE.jl:1566-1583: a rate-0 category of weight
+    lambdazeroweight, then numcategories equal-weight gamma categories
+    (discretizegamma(gammavar, 1/gammavar, n)), each lambda's rates scaled by its
+    mean minus one.  Returns (weights, ratesGC, weights, ratesAT, weights, ratesGT)."""
+    rates = np.concatenate([[0.0], discretizegamma(gammavar, 1.0 / gammavar, numcategories)
+                            if numcategories > 0 else np.zeros(0)])
+    w = np.full(numcategories + 1, (1.0 - lambdazeroweight) / numcategories if numcategories > 0 else np.inf)
+    w[0] = lambdazeroweight
+    return w, rates * (meanGC - 1.0), w.copy(), rates * (meanAT - 1.0), w.copy(), rates * (meanGT - 1.0)
+
+
 @dataclass
 class MuseSpecificParameters:
     """One lambda category, src/FARCE.jl:662-674."""
@@ -183,30 +197,41 @@ class ModelParameters:
 
 
 def getparams(params: Sequence[float], branchlengths: Sequence[float], siteCats: int = 1,
-              lambdaCats: int = 1) -> ModelParameters:
+              lambdaCats: int = 1, fixGU: bool = False, fixGCAU: bool = False) -> ModelParameters:
     """15-vector -> ModelParameters, src/FARCE.jl:609-657.
 
     Layout: [lambdaGC, lambdaAT, lambdaGT, q1..q6, lambdaGammaShape, siteGammaShape,
-    pi1, pi2, pi3, lambdazeroweight].  Only the single-lambda-category case
-    (lambdaCats == 1) is restated; it sets every lambda rate to lambda - 1 with
-    weight 1 (src/FARCE.jl:645-652), so the coevolution mixture has one
-    component per pair of site categories.
+    pi1, pi2, pi3, lambdazeroweight].  The lambda categories come from
+    discretizegamma3 with lambdaCats - 1 gamma categories and the gamma parameter
+    lambdaGammaScaleGC, which this getparams leaves at its constructor default
+    of 1 (src/FARCE.jl:629; the shape params[10] enters only the prior).  fixGCAU
+    copies the AT rates to GC, fixGU zeroes the GT rates (src/FARCE.jl:631-636).
+    With one lambda category every lambda rate is lambda - 1 with weight 1
+    (src/FARCE.jl:645-652), whatever fixGU and fixGCAU asked for.  The site rates
+    are siteCats equal-weight gamma categories (src/FARCE.jl:641-643).
     """
     p = np.asarray(params, dtype=np.float64)
-    if lambdaCats != 1:
-        raise NotImplementedError("only lambdaCats == 1 (the 12/15-parameter model) is supported")
+    if lambdaCats < 1:
+        raise ValueError("lambdaCats must be at least 1")
     freqs = np.array([p[11], p[12], p[13], 1.0 - p[11] - p[12] - p[13]])
     mp = ModelParameters(freqs=freqs, lambdaGC=p[0], lambdaAT=p[1], lambdaGT=p[2], q=p[3:9].copy(),
                          branchlengths=np.asarray(branchlengths, dtype=np.float64).copy(),
                          lambdazeroweight=p[14])
     mp.lambdaGammaShapeGC = p[9]
     mp.siteGammaShape = p[10]
+    (mp.lambdaweightsGC, mp.lambdaratesGC, _, mp.lambdaratesAT, _, mp.lambdaratesGT) = discretizegamma3(
+        mp.lambdazeroweight, mp.lambdaGC, mp.lambdaAT, mp.lambdaGT, mp.lambdaGammaScaleGC, lambdaCats - 1)
+    if fixGCAU:
+        mp.lambdaratesGC = mp.lambdaratesAT.copy()
+    if fixGU:
+        mp.lambdaratesGT = np.zeros(len(mp.lambdaratesGT))
     mp.siteWeights = np.ones(siteCats) / siteCats
     mp.siteRates = discretizegamma(mp.siteGammaShape, 1.0 / mp.siteGammaShape, siteCats)
-    mp.lambdaratesGC = np.array([mp.lambdaGC - 1.0])
-    mp.lambdaratesAT = np.array([mp.lambdaAT - 1.0])
-    mp.lambdaratesGT = np.array([mp.lambdaGT - 1.0])
-    mp.lambdaweightsGC = np.array([1.0])
+    if len(mp.lambdaweightsGC) == 1:
+        mp.lambdaratesGC = np.array([mp.lambdaGC - 1.0])
+        mp.lambdaratesAT = np.array([mp.lambdaAT - 1.0])
+        mp.lambdaratesGT = np.array([mp.lambdaGT - 1.0])
+        mp.lambdaweightsGC = np.array([1.0])
     return mp
 
 

@@ -228,3 +228,34 @@ def sample_states(sitec, pairc, logw16, partner, cats: int, B: float = 1.0, seed
     lib().oracle_sample_states(L, cats, K16 // (cats * cats), _p(sitec), _p(pairc), _p(lw), _p(part, _ip),
                                float(B), int(seed), _p(st, _ip), _p(unp), _p(pll))
     return st.astype(np.int64), unp, pll
+
+
+def site_rate_posteriors(ds, site_comps, pair_comps, cats: int, single, pairprob, maxbp: int = 1000000,
+                         mask: Optional[np.ndarray] = None) -> np.ndarray:
+    """(cats, L) log posterior weights of the site-rate categories of every column,
+    felsensteinposteriorsiterateconditionalscuda (src/FelsensteinCUDA.jl:518-717):
+    for category c and column x the log-sum-exp of log w_c + log single[x] +
+    log L_c(x) and of log pairprob[x, y] + log w_k + log L_k(x, y) over the pairs
+    (x, y), x < y, of the pair list and the pair components k = (c1, c2, lambda
+    class), counted for c1 at x and for c2 at y.  Components are ordered c1, c2,
+    lambda class (K16 = cats^2 * classes)."""
+    L = ds.numcols
+    single = np.asarray(single, dtype=np.float64)
+    pp = np.asarray(pairprob, dtype=np.float64)
+    out = np.full((cats, L), -np.inf)
+    with np.errstate(divide="ignore"):
+        ls = np.log(single)
+        lpp = np.log(pp)
+    for c, (P, pi, lw) in enumerate(site_comps):
+        u = site_loglik(ds, [(P, pi, 0.0)])
+        t = np.where(np.isfinite(u), lw + ls + u, -np.inf)
+        out[c] = np.logaddexp(out[c], t)
+    M = len(pair_comps) // (cats * cats)
+    upper = np.triu(np.ones((L, L), dtype=bool), 1)
+    for k, (P, pi, lw) in enumerate(pair_comps):
+        c1, c2 = k // (cats * M), (k // M) % cats
+        Mk = pair_loglik(ds, [(P, pi, 0.0)], maxbp=maxbp, mask=mask)
+        T = np.where(upper & np.isfinite(Mk), lpp + lw + Mk, -np.inf)
+        out[c1] = np.logaddexp(out[c1], np.logaddexp.reduce(T, axis=1))
+        out[c2] = np.logaddexp(out[c2], np.logaddexp.reduce(T, axis=0))
+    return out

@@ -364,3 +364,61 @@ def test_mixture_of_identical_components_is_the_component(ctx):
             zK = dd.eval((np.repeat(P4, K, 0), np.repeat(pi4, K, 0), w),
                          (np.repeat(P16, K, 0), np.repeat(pi16, K, 0), w), 1.0, prec)
             assert abs(zK - z1) <= tol * abs(z1)
+
+
+@pytest.mark.parametrize("siteCats,lambdaCats", [(3, 5), (2, 3)])
+def test_site_and_lambda_category_mixture_matches_oracle(ctx, siteCats, lambdaCats):
+    """The reference CLI's default mixture (3 site-rate categories, 5 lambda
+    categories: 3 site and 45 pair components) on the 20 x 120 workload: site and
+    pair log-likelihoods, log Z and posteriors against the oracle."""
+    _say(f"mixture siteCats={siteCats} lambdaCats={lambdaCats}")
+    a = synth.config1()
+    ds = synth.dataset_of(a)
+    v = a.params.copy()
+    v[14] = 0.3                      # lambda-zero weight
+    p = model.getparams(v, ds.tree.branchlengths, siteCats=siteCats, lambdaCats=lambdaCats)
+    sc, pc = model.site_components(p), model.pair_components(p)
+    assert len(sc) == siteCats and len(pc) == siteCats ** 2 * lambdaCats
+    u = oracle.site_loglik(ds, sc)
+    M = oracle.pair_loglik(ds, pc)
+    ins = oracle.inside(u, M)
+    out = oracle.outside(ins, u, M)
+    res, pair = oracle.bpp(ins, out, u, M)
+    zr = ins[0, 0, -1]
+    assert _rel(hotpath.computeunpairedlikelihoods(ds, p), u) < 1e-12
+    Mg, per = hotpath.felsensteincuda(ds, p, keepmuseconditionals=True)
+    assert _rel(Mg, M) < 1e-12
+    assert len(per) == len(pc)
+    z, r, pp = hotpath.posterior(ds, p)
+    assert abs(z - zr) <= REL64 * abs(zr)
+    assert max(np.max(np.abs(r - res)), np.max(np.abs(pp - pair))) < BPP64
+    z32, r32, p32 = hotpath.posterior(ds, p, prec="fp32")
+    assert abs(z32 - zr) <= 1e-6 * abs(zr)
+    assert max(np.max(np.abs(r32 - res)), np.max(np.abs(p32 - pair))) < BPP32
+
+
+@pytest.mark.parametrize("siteCats,lambdaCats", [(3, 1), (2, 3), (1, 3)])
+def test_site_rate_posteriors_match_oracle(ctx, siteCats, lambdaCats):
+    """felsensteinposteriorsiterateconditionalscuda on the device against the
+    oracle's restatement, from the oracle's own posteriors (20 x 120)."""
+    _say(f"site-rate posteriors siteCats={siteCats} lambdaCats={lambdaCats}")
+    a = synth.config1()
+    ds = synth.dataset_of(a)
+    v = a.params.copy()
+    v[14] = 0.3
+    p = model.getparams(v, ds.tree.branchlengths, siteCats=siteCats, lambdaCats=lambdaCats)
+    sc, pc = model.site_components(p), model.pair_components(p)
+    u = oracle.site_loglik(ds, sc)
+    M = oracle.pair_loglik(ds, pc)
+    ins = oracle.inside(u, M)
+    out = oracle.outside(ins, u, M)
+    res, pair = oracle.bpp(ins, out, u, M)
+    want = oracle.site_rate_posteriors(ds, sc, pc, siteCats, res, pair)
+    got = hotpath.felsensteinposteriorsiterateconditionalscuda(res, pair, ds, p)
+    assert got.shape == (siteCats, ds.numcols)
+    assert _rel(got, want) < 1e-12
+    got32 = hotpath.felsensteinposteriorsiterateconditionalscuda(res, pair, ds, p, prec="fp32")
+    assert _rel(got32, want) < 1e-6
+    # normalised over the categories: a distribution per column
+    w = np.exp(got - np.logaddexp.reduce(got, axis=0))
+    assert np.allclose(w.sum(axis=0), 1.0, atol=1e-12)

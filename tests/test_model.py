@@ -79,3 +79,55 @@ def test_getparams_single_category():
     assert len(comps) == 1 and comps[0][2] == 0.0
     sites = model.site_components(p)
     assert len(sites) == 1 and sites[0][2] == 0.0
+
+
+def test_discretizegamma3_closed_form():
+    """With the gamma parameter at 1 (getparams' lambdaGammaScaleGC) the rates are
+    Exp(1) quantiles at the category midpoints, -log(1 - (i - 1/2)/n), scaled by
+    lambda - 1, after a rate-0 category of weight lambdazeroweight."""
+    n = 4
+    w, gc, w2, at, w3, gt = model.discretizegamma3(0.3, 2.0, 1.5, 1.0, 1.0, n)
+    q = -np.log(1.0 - (np.arange(1, n + 1) - 0.5) / n)
+    assert np.allclose(gc, np.concatenate([[0.0], q]) * 1.0, rtol=1e-12, atol=0)
+    assert np.allclose(at, np.concatenate([[0.0], q]) * 0.5, rtol=1e-12, atol=0)
+    assert np.all(gt == 0.0)
+    assert np.allclose(w, [0.3] + [0.7 / n] * n) and abs(w.sum() - 1.0) < 1e-15
+    assert np.array_equal(w, w2) and np.array_equal(w, w3)
+
+
+def test_getparams_lambda_categories_and_fix_flags():
+    v = np.array([2.0, 1.5, 1.2, 0.3, 1.1, 0.4, 0.7, 2.0, 1.0, 1.7, 0.8, 0.2, 0.3, 0.25, 0.2])
+    bl = np.array([0.0, 0.1, 0.2])
+    p = model.getparams(v, bl, siteCats=3, lambdaCats=5)
+    q = -np.log(1.0 - (np.arange(1, 5) - 0.5) / 4)
+    assert np.allclose(p.lambdaweightsGC, [0.2] + [0.2] * 4)
+    assert np.allclose(p.lambdaratesGC[1:], q * 1.0) and np.allclose(p.lambdaratesAT[1:], q * 0.5)
+    assert np.allclose(p.lambdaratesGT[1:], q * 0.2)
+    assert len(model.site_components(p)) == 3 and len(model.pair_components(p)) == 45
+    w = np.exp([c[2] for c in model.pair_components(p)])
+    assert abs(w.sum() - 1.0) < 1e-12
+    pg = model.getparams(v, bl, siteCats=1, lambdaCats=5, fixGU=True, fixGCAU=True)
+    assert np.all(pg.lambdaratesGT == 0.0) and np.array_equal(pg.lambdaratesGC, pg.lambdaratesAT)
+    # one category: lambda - 1 with weight 1, whatever the flags asked for (src/FARCE.jl:645-652)
+    p1 = model.getparams(v, bl, lambdaCats=1, fixGU=True)
+    assert np.allclose(p1.lambdaratesGT, [0.2]) and np.allclose(p1.lambdaweightsGC, [1.0])
+
+
+def test_lambda_mixture_collapses_at_lambda_one():
+    """With every lambda equal to 1 all lambda categories are the uncoupled model
+    and their weights sum to one, so the 45-component mixture gives the pair
+    log-likelihoods of the 9-component one (closed form of the mixture)."""
+    import oracle
+    from messi_amd import synth
+    a = synth.generate(6, 14, 8)
+    ds = synth.dataset_of(a)
+    v = a.params.copy()
+    v[0:3] = 1.0
+    v[14] = 0.35
+    p5 = model.getparams(v, ds.tree.branchlengths, siteCats=3, lambdaCats=5)
+    p1 = model.getparams(v, ds.tree.branchlengths, siteCats=3, lambdaCats=1)
+    M5 = oracle.pair_loglik(ds, model.pair_components(p5))
+    M1 = oracle.pair_loglik(ds, model.pair_components(p1))
+    fin = np.isfinite(M1)
+    assert np.array_equal(np.isfinite(M5), fin)
+    assert np.max(np.abs(M5[fin] - M1[fin])) < 1e-12

@@ -248,3 +248,50 @@ def test_oracle_results_do_not_depend_on_threads():
         oracle.set_threads(n0)
     for a1, a2 in zip(*res):
         assert np.array_equal(a1, a2)
+
+
+def _posterior_inputs(ds, p):
+    u = oracle.site_loglik(ds, model.site_components(p))
+    M = oracle.pair_loglik(ds, model.pair_components(p))
+    ins = oracle.inside(u, M)
+    out = oracle.outside(ins, u, M)
+    return oracle.bpp(ins, out, u, M), M
+
+
+def test_site_rate_posteriors_identical_categories_get_equal_weight():
+    """Two site-rate categories with the same rate and weight are the same model,
+    so every column gives both the same posterior weight."""
+    from messi_amd import synth
+    a = synth.generate(7, 30, 12)
+    ds = synth.dataset_of(a)
+    p = model.getparams(a.params, ds.tree.branchlengths, siteCats=2)
+    p.siteRates = np.array([0.8, 0.8])
+    (res, pair), _ = _posterior_inputs(ds, p)
+    w = oracle.site_rate_posteriors(ds, model.site_components(p), model.pair_components(p), 2, res, pair)
+    assert np.all(np.isfinite(w))
+    assert np.max(np.abs(w[0] - w[1])) < 1e-12
+
+
+def test_site_rate_posteriors_one_category_is_the_mixture_formula():
+    """With one site category the weight of column x is
+    log(single[x] L(x) + sum_y pair[min, max] L_pair(x, y)), the pair likelihood
+    mixed over the lambda categories (the mixture the pair matrix holds)."""
+    from messi_amd import synth
+    a = synth.generate(7, 30, 13)
+    ds = synth.dataset_of(a)
+    v = a.params.copy()
+    v[14] = 0.3
+    p = model.getparams(v, ds.tree.branchlengths, siteCats=1, lambdaCats=3)
+    (res, pair), M = _posterior_inputs(ds, p)
+    w = oracle.site_rate_posteriors(ds, model.site_components(p), model.pair_components(p), 1, res, pair)
+    u = oracle.site_loglik(ds, model.site_components(p))
+    L = ds.numcols
+    lpair = np.log(np.where(pair > 0, pair, 1.0)) + np.where(pair > 0, 0.0, -np.inf)
+    for x in range(L):
+        terms = [np.log(res[x]) + u[x]]
+        for y in range(L):
+            if y != x:
+                i, j = min(x, y), max(x, y)
+                terms.append(lpair[i, j] + M[i, j])
+        want = np.logaddexp.reduce(np.array(terms))
+        assert abs(w[0, x] - want) <= 1e-12 * max(1.0, abs(want))
