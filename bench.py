@@ -433,6 +433,50 @@ def main_sample(args):
     print(json.dumps(out), flush=True)
 
 
+def main_siterate(args):
+    """Site-rate posteriors (felsensteinposteriorsiterateconditionalscuda) at 50 x 1500:
+    one call per step from base-pair posteriors already on the host (the call copies
+    them in), for the 1 x 1 and 3 x 5 category mixtures, both precisions."""
+    from messi_amd import synth, model, hotpath
+    a = synth.config2()
+    ds = synth.dataset_of(a)
+    L = ds.numcols
+    dd = hotpath.device_dataset(ds)
+    rng = np.random.default_rng(0)
+    pp = np.triu(rng.random((L, L)) / L, 1)
+    pp = pp + pp.T
+    single = 1.0 - pp.sum(axis=1)
+    res = {}
+    for sc_, lc_ in ((1, 1), (3, 5)):
+        p = model.getparams(a.params.copy(), ds.tree.branchlengths, siteCats=sc_, lambdaCats=lc_)
+        sc = hotpath._stack(model.site_components(p))
+        pc = hotpath._stack(model.pair_components(p))
+        for prec in ("fp64", "fp32"):
+            for _ in range(args.warmup):
+                dd.site_rate_posteriors(sc, pc, sc_, single, pp, prec)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                dd.site_rate_posteriors(sc, pc, sc_, single, pp, prec)
+            res[f"{sc_}x{lc_}_{prec}"] = (time.perf_counter() - t0) * 1e3 / args.steps
+    print(json.dumps({"metric": "site-rate posteriors, ms per call (host posteriors in, weights out)",
+                      "unit": "ms", "config": {"workload": "50 sequences x 1500 nt, full span",
+                                               "L": L, "steps": args.steps}, "ms": res}), flush=True)
+
+
+def main_copy_calibration(args):
+    """One device-to-device copy of 256 MiB (torch clone): the reference kernel whose
+    FETCH_SIZE / WRITE_SIZE readings scale the same counters of the likelihood
+    kernels to bytes (a copy reads and writes its size once)."""
+    import torch
+    n = 256 << 20
+    x = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    torch.cuda.synchronize()
+    y = x.clone()
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "copy calibration", "bytes_read": n, "bytes_written": n,
+                      "checksum": int(y[:4096].sum().item())}), flush=True)
+
+
 def main_structure(args):
     """Fixed-structure log-likelihood on the 50 x 1500 alignment with its true
     structure: one step = site pruning of every column, pair pruning of the
@@ -580,10 +624,15 @@ def main():
     ap.add_argument("--mea", action="store_true")
     ap.add_argument("--sample", action="store_true")
     ap.add_argument("--structure", action="store_true")
+    ap.add_argument("--siterate", action="store_true", help="time the site-rate posteriors")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one 256 MiB device copy (reference for the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--samples", type=int, default=1024)
     ap.add_argument("--mea-lengths", type=int, nargs="+", default=[1500, 9000])
     ap.add_argument("--tile", type=int, default=0, help="inside tile edge 16 or 32 (default: library's)")
     ap.add_argument("--window", type=int, default=0, help="inside bulk window 1, 2 or 4 (default: library's)")
+    ap.add_argument("--single-launch", type=int, default=-1,
+                    help="1: inside pass as one launch, 0: one launch per tile-diagonal (default: library's)")
     ap.add_argument("--precs", nargs="+", default=["fp64", "fp32"], help="precisions to time (profiling runs)")
     ap.add_argument("--no-posterior", action="store_true", help="skip the posterior error check (profiling runs)")
     ap.add_argument("--genome-spans", type=int, nargs="+", default=[1000000, 500])
@@ -612,6 +661,9 @@ def main():
     if args.window:
         from messi_amd import lib
         lib.context().set_bulk_window(args.window)
+    if args.single_launch >= 0:
+        from messi_amd import lib
+        lib.context().set_single_launch(bool(args.single_launch))
     if args.instances:
         return main_instances(args)
     if args.mcmc:
@@ -624,6 +676,10 @@ def main():
         return main_sample(args)
     if args.structure:
         return main_structure(args)
+    if args.copy_calibration:
+        return main_copy_calibration(args)
+    if args.siterate:
+        return main_siterate(args)
 
     from messi_amd import lib, synth
     from messi_amd.hotpath import device_dataset

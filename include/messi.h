@@ -55,6 +55,16 @@ int messi_set_bulk_window(messi_ctx *ctx, int window);
  * (default 1).  Results do not depend on it; 0 runs the full-span recursions,
  * as the reference always does (src/InsideOutsideAlgorithm.jl:447-483,648-749). */
 int messi_set_band_skip(messi_ctx *ctx, int on);
+/* Whether later inside passes on this context run as one launch (1) instead of
+ * one launch per tile-diagonal (0, default): resident workgroups take the same
+ * work items from a ticket counter and wait on per-tile flags, so each tile waits
+ * only for the tiles it reads and members of a batch proceed independently.
+ * Results are bit-identical either way.  A wait that does not end within a few
+ * seconds fails the call.  The allocation-free entry point always uses the
+ * launch sequence.  No counterpart in the reference, whose inside has no tiles
+ * (the schedule it replaces launches one kernel per anti-diagonal,
+ * src/InsideCUDA.jl:87-181). */
+int messi_set_single_launch(messi_ctx *ctx, int on);
 /* Checks the MFMA operand layouts the kernels rely on against a CPU product. */
 int messi_selftest(messi_ctx *ctx);
 /* Page-locked host memory on the context's device (hipHostMalloc), for callers

@@ -875,6 +875,7 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
     ws[m] = &member_work(b, m, prec);
     ws[m]->tile = (m < (int)b->tiles.size() && b->tiles[m]) ? b->tiles[m] : ctx->tile;
     ws[m]->window = b->window ? b->window : ctx->window;
+    ws[m]->persist = ctx->persist;
     ws[m]->zout = b->zall.as<double>() + 2 * m;
     // pair scores beyond the distance bound are -inf (pair_expand writes them so)
     const int D = b->mem[m]->maxbp, Lm = b->mem[m]->L;
@@ -944,7 +945,11 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
   std::vector<double> z(2 * b->n);
   MESSI_HIP(hipMemcpyAsync(z.data(), b->zall.p, sizeof(double) * 2 * b->n, hipMemcpyDeviceToHost, s));
   MESSI_HIP(hipStreamSynchronize(s));
-  for (int m = 0; m < b->n; ++m) logZ[m] = z[2 * m];
+  for (int m = 0; m < b->n; ++m) {
+    logZ[m] = z[2 * m];
+    MESSI_REQUIRE(!(ctx->persist && std::isnan(logZ[m])),
+                  "single-launch inside gave up waiting for a dependency (log Z not computed)");
+  }
   if (!keep_outside) collect(b, 3);
 }
 
@@ -1222,6 +1227,12 @@ int messi_set_band_skip(messi_ctx *ctx, int on) {
   return 0;
 }
 
+int messi_set_single_launch(messi_ctx *ctx, int on) {
+  if (!ctx) return 2;
+  ctx->persist = on != 0;
+  return 0;
+}
+
 int messi_selftest(messi_ctx *ctx) {
   if (!ctx) return 2;
   return guard(ctx, [&] {
@@ -1344,6 +1355,7 @@ static void inside_call(messi_ctx *ctx, InsideWork &w, int L, const double *u, c
   hipStream_t s = ctx->stream;
   w.tile = ctx->tile;
   w.window = ctx->window;
+  w.persist = ctx->persist && &w != &ctx->wna;  // the allocation-free call keeps the launch sequence
   w.band = ctx->band_skip ? pairing_band(L, paired) : -1;
   const size_t LL = (size_t)L * L;
   auto go = [&](auto tag) {
@@ -1353,6 +1365,8 @@ static void inside_call(messi_ctx *ctx, InsideWork &w, int L, const double *u, c
     MESSI_HIP(hipMemcpyAsync(w.paired.p, paired, sizeof(double) * LL, hipMemcpyHostToDevice, s));
     inside_run<T>(s, w, L, B);
     *Z = inside_logZ<T>(s, w, L);
+    MESSI_REQUIRE(!(w.persist && std::isnan(*Z)),
+                  "single-launch inside gave up waiting for a dependency (log Z not computed)");
     if (!zonly) {
       w.out3.ensure(sizeof(double) * 3 * LL);
       inside_export<T>(s, w, L, w.out3.as<double>());

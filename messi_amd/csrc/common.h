@@ -92,7 +92,9 @@ inline void upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
 struct InsideBatch {
   DevBuf inst;    // instance descriptors
   DevBuf prefix;  // per launch t: first workgroup of each instance (n+1 ints)
-  std::vector<unsigned char> h_inst, h_prefix;  // what inst and prefix hold (uploads only on change)
+  DevBuf ptab;    // single-launch form: ticket table
+  DevBuf pflags;  // single-launch form: ticket counter, give-up flag, then every instance's flags
+  std::vector<unsigned char> h_inst, h_prefix, h_ptab;  // what inst, prefix and ptab hold (uploads only on change)
 };
 
 // Scratch of the inside/outside/bpp phase, per precision.
@@ -100,6 +102,7 @@ struct InsideWork {
   int L = 0;
   int tile = 16;                  // tile edge of the inside schedule (16 or 32)
   int window = 4;                 // bulk window of the inside schedule (1, 2 or 4 launches)
+  bool persist = false;           // the inside pass as one launch (single-launch form)
   int band = -1;                  // paired[j,y] is -inf for y - j > band (-1: no such bound)
   const void *exp = nullptr;      // pair-root expansion descriptor of this member (device), or null
   double *zout = nullptr;         // where log Z (absolute, baseline-relative) is also written (device)
@@ -150,6 +153,7 @@ struct messi_ctx {
   int tile = 16;  // tile edge of the inside schedule
   int window = 4;  // bulk window of the inside schedule
   bool band_skip = true;  // skip the split-point terms a pairing band makes -inf
+  bool persist = false;   // inside pass as one launch (messi_set_single_launch)
   messi::InsideWork w64, w32;
   messi::InsideWork wna;  // messi_inside_noalloc: every buffer borrowed from the device's static scratch
   messi::MeaWork mea;

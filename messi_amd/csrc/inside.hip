@@ -30,6 +30,7 @@
 #include "common.h"
 #include "prune.h"
 #include "sr.h"
+#include "inside_sched.h"
 #include "wave_layout.h"
 
 namespace messi {
@@ -71,6 +72,7 @@ template <class V> struct InstDesc {
   double *lam, *C, *uhat, *z;
   const ExpandDesc *exp;  // pair term from the pair root entries when non-null
   double *zout;           // second destination of z (the batch's gathered log Z), or null
+  int *pflag;             // dependency flags of the single-launch form (pflag_ints(NT)), or null
 };
 
 // Baseline lambda, its prefix sums C and u - lambda; one workgroup per instance.
@@ -162,11 +164,12 @@ __global__ void inside_init_kernel(const InstDesc<typename SR::V> *__restrict__ 
 
 // log Z (natural, absolute) = S[0, L-1] plus the baseline of the whole span, per instance.
 template <class SR>
-__global__ void zlog_kernel(const InstDesc<typename SR::V> *__restrict__ insts, int n) {
+__global__ void zlog_kernel(const InstDesc<typename SR::V> *__restrict__ insts, int n, const int *gave_up) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   const InstDesc<typename SR::V> d = insts[m];
-  double v = SR::to_log(d.Sc[(size_t)(d.L - 1) * d.L]);
+  // a single-launch pass that gave up waiting leaves no log Z: NaN
+  double v = gave_up && *gave_up ? __builtin_nan("") : SR::to_log(d.Sc[(size_t)(d.L - 1) * d.L]);
   d.z[0] = v == -HUGE_VAL ? v : v + d.C[d.L];
   d.z[1] = v;  // baseline-relative
   if (d.zout) {
@@ -251,6 +254,7 @@ template <class SR, int T> struct StepLDS {
     StepStaged<SR, T> s;
   } m;
   V Sb[T], Fbot[T], Fleft[T];
+  int flag;  // a dependency wait's outcome (single-launch form)
   __device__ V &at(int i) { return m.all[i]; }
 };
 
@@ -506,31 +510,71 @@ __device__ __forceinline__ void tile_products_packed(const InstDesc<XFv> &d, int
   __syncthreads();
 }
 
-// Bulk items per bulk workgroup (their loads overlap): two with a window of one
-// launch, one with a wider window.
-constexpr int BULK_PER = 2;
-__host__ __device__ constexpr int bulk_per(int W) { return W == 1 ? BULK_PER : 1; }
-
 // Bulk window W: a bulk tile at distance d is visited in the launches t = d-1,
 // d-1-W, d-1-2W, ... (those with 2t - 2 >= d), each visit adding the products
 // that became ready since the previous one, so its accumulator is read and
-// written once per W launches.  Bulk items of launch t: tiles at distances
-// t+1, t+1+W, ..., NT - d of them per distance d; bulk_items(k) counts the items
-// of the first k distances.
-__host__ __device__ __forceinline__ int bulk_items(int NT, int t, int W, int k) {
-  return k * (NT - t - 1) - W * (k * (k - 1) / 2);
+// written once per W launches (inside_sched.h: item counts, decode, visits).
+using sched::bulk_items;
+using sched::bulk_per;
+using sched::had_visit;
+using sched::last_visit;
+
+// Dependency flags of the single-launch form (InstDesc::pflag, per instance):
+// tile_done (NT x NT, 1 once the tile is final), visit_done (NT x NT, t+1 after
+// the tile's bulk visit of launch t) and diag_done (NT, finalised tiles per
+// diagonal); ctl[0] the ticket counter, ctl[1] the give-up flag.  All start at
+// zero for each evaluation.
+__host__ __device__ constexpr size_t pflag_ints(int NT) { return 2 * (size_t)NT * NT + NT; }
+
+// Bound on a dependency wait (polls of about a microsecond each): a wait that
+// runs out sets the give-up flag, every workgroup then leaves, and the host
+// reports the evaluation as failed.
+constexpr int PERSIST_SPIN = 1 << 22;
+
+// Thread 0 waits until *p[i] >= v[i] for the given flags (null: none) and
+// acquires what the signalling workgroups wrote (the invalidation covers the
+// CU's L1 and the XCD's L2 that the other waves read through after the
+// barrier).  False for every thread when the wait ran out or another workgroup
+// gave up.
+__device__ __forceinline__ bool wg_wait(int *ctl, int *lds_flag, const int *p0, int v0, const int *p1, int v1,
+                                        const int *p2, int v2) {
+  if (threadIdx.x == 0) {
+    const int *ps[3] = {p0, p1, p2};
+    const int vs[3] = {v0, v1, v2};
+    int ok = 1;
+    for (int i = 0; i < 3 && ok; ++i) {
+      if (!ps[i]) continue;
+      int n = 0;
+      while (__hip_atomic_load(ps[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < vs[i]) {
+        if (++n >= PERSIST_SPIN) {
+          __hip_atomic_store(ctl + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        if ((n & 255) == 0 && __hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *lds_flag = ok;
+  }
+  __syncthreads();
+  return *lds_flag != 0;
 }
 
-// Whether a tile at distance d had a bulk visit before launch t (a bulk visit
-// in launch t, or its finalisation in launch t = d): the visits are t' = d-1,
-// d-1-W, ... >= (d+3)/2, each scheduled unless t'-1 > KB and d > t'-1+KB.
-// Every scheduled visit writes the tile's accumulator, so the accumulator is
-// read only when this holds and needs no clearing.
-__device__ __forceinline__ bool had_visit(int t, int d, int W, int KB, bool fin) {
-  const int smin = (d + 3) / 2, last = fin ? d - 1 : t - W;
-  if (last < smin) return false;
-  const int first = last - W * ((last - smin) / W);
-  return first - 1 <= KB || d <= last - 1 + KB;
+// Every thread's global writes are released (agent scope), then thread 0 sets
+// flag *p = v (and adds 1 to *q when q is non-null).
+__device__ __forceinline__ void wg_signal(int *p, int v, int *p2, int v2, int *q) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (p) __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (p2) __hip_atomic_store(p2, v2, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (q) __hip_atomic_fetch_add(q, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // Cell of a T x T off-diagonal tile that thread k takes in the in-tile wavefront:
@@ -628,37 +672,18 @@ __device__ __forceinline__ void tile_products_packed4(const InstDesc<XFv> &d, in
   __syncthreads();
 }
 
-// One launch of the inside schedule (see the file header).
-template <class SR, int T, int W>
-__global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
-                                                            const int *__restrict__ prefix, int n, int t,
-                                                            Rules<SR> r) {
+// One work item of the inside schedule (see the file header): the finalisation
+// of tile (item, item+t) of launch t, or bulk workgroup `item` of launch t.
+// With P (the single-launch form) the item first waits for what it reads and
+// afterwards signals what it wrote (pflag).
+template <class SR, int T, int W, bool P>
+__device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins, bool fin_wg, int item, int t,
+                                            const Rules<SR> &r, StepLDS<SR, T> &sm, int *ctl) {
   typedef typename SR::V V;
   typedef wave::Layout<T> WL;
-  __shared__ StepLDS<SR, T> sm;
-  static_assert(sizeof(StepStaged<SR, T>) == sizeof(V) * WL::STAGED + sizeof(StepProducts<SR, T>),
-                "the products follow the staged tables");
   const int tid = threadIdx.x;
   constexpr bool by_step = !SR::scaled;  // in-tile wavefront cells in step order (see below)
   constexpr int BP = bulk_per(W);
-  // ---- work item.  The first workgroups (in dispatch order, so they start
-  // first, round-robin over the XCDs) finalise the tiles of every instance:
-  // prefix[0..n] counts them per instance; a few padding workgroups round their
-  // number up to a multiple of 8.  The rest are bulk workgroups, prefix[n+1 ..
-  // 2n+1] per instance, renumbered so that each XCD takes a contiguous run.
-  const int *fpre = prefix, *bpre = prefix + (n + 1);
-  const int nfin_all = fpre[n], fpad = (nfin_all + 7) & ~7;
-  const bool fin_wg = (int)blockIdx.x < fpad;
-  if (fin_wg && (int)blockIdx.x >= nfin_all) return;  // padding
-  const int bx = fin_wg ? (int)blockIdx.x : xcd_chunk_id((int)blockIdx.x - fpad, (int)gridDim.x - fpad);
-  const int *pre = fin_wg ? fpre : bpre;
-  int lo = 0, hi = n;  // instance: pre[lo] <= bx < pre[lo+1]
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (pre[mid] <= bx) lo = mid;
-    else hi = mid;
-  }
-  const InstDesc<V> ins = insts[lo];
   const int L = ins.L, NT = ins.NT;
   V *__restrict__ Lr = ins.Lr;
   V *__restrict__ Lc = ins.Lc;
@@ -667,19 +692,13 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   V *__restrict__ F = ins.F;
   V *__restrict__ acc = ins.acc;
   const V *__restrict__ ptv = ins.ptv;
-  // ---- decode the work item: workgroups NT-t .. are bulk workgroups, each
-  // taking bulk_per(W) consecutive bulk items, with every item's loads issued
-  // before the first product is computed.
-  const int item = bx - pre[lo];
+  int *const tile_done = ins.pflag, *const visit_done = ins.pflag + (size_t)NT * NT;
+  int *const diag_done = ins.pflag + 2 * (size_t)NT * NT;
   if (!fin_wg) {
     // The tiles (I, I+d) at the k-th distance d = t+1+kW gather the products of
     // the launches s = max(t-W+1, smin(d)) .. t in increasing s, the two of each
     // s in the order a window of one launch adds them.
-    // beyond distance t-1+KB only a product with s-1 <= KB could be in the band,
-    // and such a product is ready only for distances <= 2KB < t-1+KB
-    int dmax = min(2 * t - 2, NT - 1);
-    if (t - 1 > ins.KB) dmax = min(dmax, t - 1 + ins.KB);
-    const int nd = dmax >= t + 1 ? (dmax - t - 1) / W + 1 : 0;
+    const int nd = sched::bulk_dists(NT, t, W, ins.KB);
     const int nbulk = bulk_items(NT, t, W, nd);
     const int first = BP * item;
     int Ib[BP], Jb[BP], smin[BP];
@@ -692,16 +711,23 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     for (int u = 0; u < BP; ++u) {
       const int idx = first + u;
       has[u] = idx < nbulk;
-      int lo = 0, hi = nd - 1;  // the distance: bulk_items(lo) <= idx < bulk_items(lo + 1)
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (bulk_items(NT, t, W, mid) <= idx) lo = mid;
-        else hi = mid - 1;
-      }
-      const int dist = t + 1 + lo * W;
-      Ib[u] = idx - bulk_items(NT, t, W, lo);
-      Jb[u] = Ib[u] + dist;
-      smin[u] = max(t - W + 1, (dist + 3) / 2);  // a product of launch s needs dist <= 2s - 2
+      sched::bulk_tile(NT, t, W, nd, idx, Ib[u], Jb[u]);
+      smin[u] = max(t - W + 1, (Jb[u] - Ib[u] + 3) / 2);  // a product of launch s needs dist <= 2s - 2
+    }
+    if constexpr (P) {
+      // every tile of the diagonals before t is final, and the tiles' previous
+      // visits have written their accumulators
+      int pv[2] = {-1, -1};
+#pragma unroll
+      for (int u = 0; u < BP; ++u) pv[u] = has[u] ? last_visit(t, Jb[u] - Ib[u], W, ins.KB, false) : -1;
+      if (!wg_wait(ctl, &sm.flag, diag_done + (t - 1), NT - (t - 1),
+                   pv[0] >= 0 ? visit_done + (size_t)Ib[0] * NT + Jb[0] : nullptr, pv[0] + 1,
+                   BP > 1 && pv[BP - 1] >= 0 ? visit_done + (size_t)Ib[BP - 1] * NT + Jb[BP - 1] : nullptr,
+                   pv[BP - 1] + 1))
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < BP; ++u) {
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
       av[u] = (has[u] && had_visit(t, Jb[u] - Ib[u], W, ins.KB, false) && j < L && y < L)
                   ? acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid]
@@ -751,11 +777,24 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
       if (j < L && y < L) acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid] = av[u];  // every visit writes
     }
+    if constexpr (P)
+      wg_signal(has[0] ? visit_done + (size_t)Ib[0] * NT + Jb[0] : nullptr, t + 1,
+                BP > 1 && has[BP - 1] ? visit_done + (size_t)Ib[BP - 1] * NT + Jb[BP - 1] : nullptr, t + 1,
+                nullptr);
     return;
   }
   const bool fin = true;
   const int I = item, J = item + t;
   const int IT = I * T, JT = J * T;
+  if constexpr (P) {
+    // the two neighbouring tiles of the previous diagonal are final (and with
+    // them every tile this one reads), and the tile's last bulk visit is done
+    const int lv = last_visit(t, t, W, ins.KB, true);
+    if (!wg_wait(ctl, &sm.flag, t >= 1 ? tile_done + (size_t)I * NT + J - 1 : nullptr, 1,
+                 t >= 1 ? tile_done + (size_t)(I + 1) * NT + J : nullptr, 1,
+                 lv >= 0 ? visit_done + (size_t)I * NT + J : nullptr, lv + 1))
+      return;
+  }
   // ---- split-point products that became ready with tile-diagonal t-1: every
   // global load (accumulator and both products' operands) is issued up front.
   // L[j, h] is -inf for h - j beyond the pairing band: such L tiles contribute nothing.
@@ -1016,6 +1055,63 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
       if (c == 0 && y < L) ins.SpE[((size_t)I * NT + J) * T + b] = e;
     }
   }
+  if constexpr (P) wg_signal(tile_done + (size_t)I * NT + J, 1, nullptr, 0, diag_done + t);
+}
+
+// One launch of the inside schedule (see the file header).
+template <class SR, int T, int W>
+__global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
+                                                            const int *__restrict__ prefix, int n, int t,
+                                                            Rules<SR> r) {
+  __shared__ StepLDS<SR, T> sm;
+  static_assert(sizeof(StepStaged<SR, T>) == sizeof(typename SR::V) * wave::Layout<T>::STAGED +
+                                                 sizeof(StepProducts<SR, T>),
+                "the products follow the staged tables");
+  // ---- work item.  The first workgroups (in dispatch order, so they start
+  // first, round-robin over the XCDs) finalise the tiles of every instance:
+  // prefix[0..n] counts them per instance; a few padding workgroups round their
+  // number up to a multiple of 8.  The rest are bulk workgroups, prefix[n+1 ..
+  // 2n+1] per instance, renumbered so that each XCD takes a contiguous run.
+  const int *fpre = prefix, *bpre = prefix + (n + 1);
+  const int nfin_all = fpre[n], fpad = (nfin_all + 7) & ~7;
+  const bool fin_wg = (int)blockIdx.x < fpad;
+  if (fin_wg && (int)blockIdx.x >= nfin_all) return;  // padding
+  const int bx = fin_wg ? (int)blockIdx.x : xcd_chunk_id((int)blockIdx.x - fpad, (int)gridDim.x - fpad);
+  const int *pre = fin_wg ? fpre : bpre;
+  int lo = 0, hi = n;  // instance: pre[lo] <= bx < pre[lo+1]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (pre[mid] <= bx) lo = mid;
+    else hi = mid;
+  }
+  inside_item<SR, T, W, false>(insts[lo], fin_wg, bx - pre[lo], t, r, sm, nullptr);
+}
+
+// The whole inside pass of a batch in one launch (single-launch form): one
+// workgroup per work item of every launch of the schedule.  A workgroup takes
+// its item from a ticket counter when it starts, in the order of the ticket
+// table (inside_sched.h), and waits on the flags of the items it reads.  An
+// item only waits for items of smaller tickets, which workgroups that already
+// started hold, so the oldest unfinished item can always proceed whatever order
+// the workgroups are dispatched in; the waits are bounded (PERSIST_SPIN).
+template <class SR, int T, int W>
+__global__ __launch_bounds__(T * T) void inside_persist_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
+                                                               const int *__restrict__ tab, int n, int NP,
+                                                               Rules<SR> r, int *ctl) {
+  typedef typename SR::V V;
+  __shared__ StepLDS<SR, T> sm;
+  if (threadIdx.x == 0)
+    sm.flag = __hip_atomic_load(ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                  ? -1
+                  : __hip_atomic_fetch_add(ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int k = __builtin_amdgcn_readfirstlane(sm.flag);  // uniform: the decode stays scalar
+  __syncthreads();  // the item's waits reuse sm.flag
+  if (k < 0 || k >= tab[NP]) return;  // another workgroup gave up
+  const sched::Ticket w = sched::decode_ticket(tab, n, NP, k);
+  const InstDesc<V> ins = insts[w.m];
+  const int item = w.kind == 2 ? w.item + sched::crit_groups(ins.NT, w.t, W, ins.KB) : w.item;
+  inside_item<SR, T, W, true>(ins, w.kind == 0, item, w.t, r, sm, ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -1480,6 +1576,7 @@ static InstDesc<typename SR::V> inst_of(InsideWork &w) {
   d.z = w.red.as<double>();
   d.exp = static_cast<const ExpandDesc *>(w.exp);
   d.zout = w.zout;
+  d.pflag = nullptr;
   return d;
 }
 
@@ -1495,6 +1592,11 @@ static void prepare_batch(hipStream_t s, const InstDesc<typename SR::V> *d_inst,
     hipLaunchKernelGGL(ptv_root_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, rootacc, B);
   else
     hipLaunchKernelGGL(ptv_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, B);
+}
+
+// Clears n ints (the single-launch form's counter and flags).
+__global__ void zero_ints_kernel(int *p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0;
 }
 
 template <typename T>
@@ -1518,6 +1620,8 @@ int inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &i
   MESSI_REQUIRE(W == 1 || W == 2 || W == 4, "bulk window must be 1, 2 or 4");
   std::vector<InstDesc<V>> d(n);
   int Lmax = 0, NTmax = 0;
+  const bool persist = ws[0]->persist;
+  size_t pf_total = 2;  // ticket counter and give-up flag, then the instances' flags
   for (int m = 0; m < n; ++m) {
     d[m] = inst_of<SR>(*ws[m]);
     d[m].NT = (d[m].L + TE - 1) / TE;
@@ -1527,6 +1631,15 @@ int inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &i
     d[m].KB = (band < 0 || band >= d[m].L) ? d[m].NT : (band == 0 ? 0 : (band - 1) / TE + 1);
     Lmax = std::max(Lmax, d[m].L);
     NTmax = std::max(NTmax, d[m].NT);
+    d[m].pflag = nullptr;
+    if (persist) {
+      d[m].pflag = (int *)(uintptr_t)pf_total;  // offset for now
+      pf_total += pflag_ints(d[m].NT);
+    }
+  }
+  if (persist) {
+    ib.pflags.ensure(sizeof(int) * pf_total);
+    for (int m = 0; m < n; ++m) d[m].pflag = ib.pflags.as<int>() + (size_t)(uintptr_t)d[m].pflag;
   }
   // Work items of launch t: the tiles (I, I+t) and the ready split-point products
   // (I, K) x (K, J) with K - I = t - 1 or J - K = t - 1; a bulk tile (I, I+dist)
@@ -1574,6 +1687,40 @@ int inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &i
   prepare_batch<SR>(s, di, n, Lmax, B, rootacc);
   hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((Lmax + 255) / 256, 1, n), dim3(256), 0, s, di, B, lp[1],
                      lp[5]);
+  if (persist) {
+    // single-launch form: the ticket table, the flags cleared, one launch
+    const int NP = NTmax + 1;
+    std::vector<int> NTs(n), KBs(n);
+    for (int m = 0; m < n; ++m) {
+      NTs[m] = d[m].NT;
+      KBs[m] = d[m].KB;
+    }
+    std::vector<int> tab(sched::table_size(n, NP));
+    sched::build_table(n, NTs.data(), KBs.data(), W, NP, tab.data());
+    refresh(ib.ptab, ib.h_ptab, tab.data(), tab.size() * sizeof(int));
+    int *ctl = ib.pflags.as<int>();
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((unsigned)std::min<size_t>((pf_total + 255) / 256, 4096)), dim3(256), 0,
+                       s, ctl, pf_total);
+    if (steps_begin) MESSI_HIP(hipEventRecord(steps_begin, s));
+    const int *dt = ib.ptab.as<int>();
+    const int tickets = tab[NP];
+    auto go = [&](auto kern, int threads) {
+      hipLaunchKernelGGL(kern, dim3(std::max(1, tickets)), dim3(threads), 0, s, di, dt, n, NP, r, ctl);
+    };
+    if (TE == 32) {
+      if (W == 4) go(inside_persist_kernel<SR, 32, 4>, 32 * 32);
+      else if (W == 2) go(inside_persist_kernel<SR, 32, 2>, 32 * 32);
+      else go(inside_persist_kernel<SR, 32, 1>, 32 * 32);
+    } else {
+      if (W == 4) go(inside_persist_kernel<SR, 16, 4>, 16 * 16);
+      else if (W == 2) go(inside_persist_kernel<SR, 16, 2>, 16 * 16);
+      else go(inside_persist_kernel<SR, 16, 1>, 16 * 16);
+    }
+    if (steps_end) MESSI_HIP(hipEventRecord(steps_end, s));
+    hipLaunchKernelGGL(zlog_kernel<SR>, dim3((n + 63) / 64), dim3(64), 0, s, di, n, (const int *)(ctl + 1));
+    MESSI_HIP(hipGetLastError());
+    return 1;
+  }
   if (steps_begin) MESSI_HIP(hipEventRecord(steps_begin, s));
   for (int t = 0; t < NTmax; ++t) {
     const int *pf = ib.prefix.as<int>() + (size_t)t * stride;
@@ -1591,7 +1738,7 @@ int inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &i
     }
   }
   if (steps_end) MESSI_HIP(hipEventRecord(steps_end, s));
-  hipLaunchKernelGGL(zlog_kernel<SR>, dim3((n + 63) / 64), dim3(64), 0, s, di, n);
+  hipLaunchKernelGGL(zlog_kernel<SR>, dim3((n + 63) / 64), dim3(64), 0, s, di, n, (const int *)nullptr);
   MESSI_HIP(hipGetLastError());
   return NTmax;
 }
@@ -1629,7 +1776,8 @@ void inside_import(hipStream_t s, InsideWork &w, int L, const double *d_in3) {
                      w.Lr.as<V>(), w.Lc.as<V>());
   std::vector<InstDesc<V>> d(1, inst_of<SR>(w));
   upload(w.inst1, d, s);
-  hipLaunchKernelGGL(zlog_kernel<SR>, dim3(1), dim3(64), 0, s, (const InstDesc<V> *)w.inst1.as<InstDesc<V>>(), 1);
+  hipLaunchKernelGGL(zlog_kernel<SR>, dim3(1), dim3(64), 0, s, (const InstDesc<V> *)w.inst1.as<InstDesc<V>>(), 1,
+                     (const int *)nullptr);
 }
 
 template <typename T>

@@ -82,6 +82,7 @@ SIGNATURES = {
     "messi_set_tile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_set_bulk_window": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_set_band_skip": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "messi_set_single_launch": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "messi_batch_destroy": (None, [_vp]),
     "messi_batch_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
@@ -218,6 +219,12 @@ class Context:
         """Whether inside/outside passes skip the terms a pairing band makes -inf
         (default on; results do not depend on it)."""
         self.check(self.lib.messi_set_band_skip(self._h, int(bool(on))))
+
+    def set_single_launch(self, on: bool) -> None:
+        """Whether inside passes run as one launch with per-tile dependency flags
+        instead of one launch per tile-diagonal (default off; results are
+        bit-identical)."""
+        self.check(self.lib.messi_set_single_launch(self._h, int(bool(on))))
 
     def close(self) -> None:
         if self._h:

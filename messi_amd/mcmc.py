@@ -726,7 +726,8 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
                    noises=(0.8, 1.1), kappas=(1.3, 1.6), log: Callable = print,
                    laplace_init: bool = True, mode_logp: Optional[Callable] = None,
                    ind_prob: float = 0.15, mix_components: int = 4, mix_inflate: float = 1.3,
-                   mix_nu: float = 7.0, mix_flatten: float = 0.5) -> dict:
+                   mix_nu: float = 7.0, mix_flatten: float = 0.5,
+                   share_candidates=(0.85, 0.3, 0.0), share_steps: int = 500) -> dict:
     """Learned proposal against the adaptive random walk, each with a burn-in of
     ``burn`` steps from the same start, then ``steps`` production steps each and
     a long random-walk chain as the yardstick for the posterior means.
@@ -741,8 +742,10 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     gives independence moves (probability ``ind_prob``) that jump between the
     modes the burn-in visited; it is shared with the method without the network
     (random walk plus the same independence moves), so the two differ only in
-    the network's moves, which take the share of the random walk that their
-    burn-in jump distance earns.  Random-walk method: adaptive random walk for
+    the network's moves, which take the share of the random walk (one of
+    ``share_candidates``, 0 included) whose burn-in segment of ``share_steps``
+    steps, independence moves included, gave the largest minimum effective
+    sample size.  Random-walk method: adaptive random walk for
     all ``burn`` steps.  Production kernels are fixed (no adaptation), so each
     production chain is an exact MH chain.
     """
@@ -838,10 +841,32 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
     # replace most of the random walk when their burn-in jump distance beat the
     # random walk's and a small share of it otherwise.
     rw_esjd = _esjd(np.concatenate([tb.samples[-1][None], trb.samples]), wh)
+    share_trials = []
     if mixm is not None:
-        p_learned = (1.0 - ind_prob) * (0.85 if best["esjd_min"] > rw_esjd else 0.3)
-        l_rw, l_ind = 1.0 - ind_prob - p_learned, ind_prob
+        # The network's share of the moves that are not independence moves is
+        # chosen in burn-in, with the independence moves present: a segment at
+        # each candidate share (0 = the method without the network), scored by
+        # its smallest effective sample size over the parameters (the jump
+        # distance favoured the network's larger local moves even where they
+        # mixed the slowest parameter worse).
         learned_burn_s += mix_res[1] - tb.seconds      # its share of the shared mixture's burn-in
+        t_s = time.perf_counter()
+        for share in share_candidates:
+            p = (1.0 - ind_prob) * share
+            ts = learned_chain(logp, theta, lp, share_steps, net if p > 0.0 else None, wh, rng, rw_chol,
+                               rw_scale, rw_prob=1.0 - ind_prob - p, kappa=best["kappa"], mix=mixm,
+                               ind_prob=ind_prob, mix_wh=whm)
+            score = float(ess(ts.samples).min())
+            share_trials.append({"share": share, "learned_prob": p, "min_ess": score,
+                                 "esjd_min": _esjd(np.concatenate([theta[None], ts.samples]), wh),
+                                 "acceptance": ts.acceptance})
+            log(f"burn-in share trial {share}: {share_steps} steps, acceptance {ts.acceptance:.3f}, "
+                f"min ESS {score:.1f}")
+            theta, lp = ts.samples[-1], ts.logp[-1]
+        learned_burn_s += time.perf_counter() - t_s
+        pick = max(share_trials, key=lambda t: t["min_ess"])
+        p_learned = pick["learned_prob"]
+        l_rw, l_ind = 1.0 - ind_prob - p_learned, ind_prob
     else:
         p_learned, l_rw, l_ind = 1.0 - rw_prob, rw_prob, 0.0
     log(f"learned move share {p_learned:.2f} (burn-in min ESJD: learned {best['esjd_min']:.4f}, "
@@ -867,7 +892,8 @@ def run_experiment(logp: Callable, theta0, net, rng, burn: int = 2000, steps: in
         "nchains": n, "burn": burn, "steps": steps, "long_steps": long_steps,
         "train_seconds": train_s, "learned_burn_seconds": learned_burn_s, "trials": trials,
         "chosen": {"noise": best["noise"], "kappa": best["kappa"], "learned_share": p_learned,
-                   "burn_esjd_learned": best["esjd_min"], "burn_esjd_random_walk": rw_esjd},
+                   "burn_esjd_learned": best["esjd_min"], "burn_esjd_random_walk": rw_esjd,
+                   "share_trials": share_trials},
         "learned": {"seconds": tl.seconds, "acceptance_learned_moves": learned_acc,
                     "acceptance_independence_moves": ind_acc, "independence_prob": l_ind,
                     "acceptance": tl.acceptance, "ess": e_l.tolist(), "min_ess": float(e_l.min()),
