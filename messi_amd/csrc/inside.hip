@@ -875,8 +875,9 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
   // (cell_by_step): a wavefront's cells then finish together and the costly
   // finalisations of one step fall in one or two wavefronts; each thread reads its
   // accumulated products back from the cell's F slot, which only the cell's own
-  // finalisation overwrites.  (In the FP32 path the extra index work costs more
-  // than it saves; its cells stay in row order.)
+  // finalisation overwrites.  (In the FP32 path step order scatters a
+  // wavefront's LDS reads over many rows; row order, whose lanes read
+  // neighbouring cells, measured faster.)
   int a = tid / T, b = tid % T;
   if (by_step && t > 0) cell_by_step<T>(tid, a, b);
   const int j = IT + a, y = JT + b;
@@ -1259,11 +1260,11 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
     if (dorow || docol) tile_products<SR, T>(orow, dorow, ocol, docol, sm.u.p, vL, vS);
   }
   // ---- finalise tile (I, J)
-  // In the FP64 path the cells are given to the threads in order of their
-  // finalisation step (as in the inside step); the products' sums reach the
-  // cell's new thread through its W and Q slots, which only the cell's own
-  // finalisation overwrites (W and Q of a cell are read only once it is final).
-  constexpr bool by_step = !SR::scaled;
+  // The cells are given to the threads in order of their finalisation step (as
+  // in the inside step); the products' sums reach the cell's new thread through
+  // its W and Q slots, which only the cell's own finalisation overwrites (W and
+  // Q of a cell are read only once it is final).
+  constexpr bool by_step = true;
   {
     const int a = a0, b = b0, j = j0, y = y0;
     const int rr = JT + a + 1, cc = JT + b;  // iSd[a][b] = iS[JT+a+1, JT+b]

@@ -127,17 +127,15 @@ struct XF {
     return V{f, m == 0.0f ? EZ : e + x};
   }
   static __device__ __forceinline__ V finish(K km, S s) { return norm(s, km); }
+  // Both operands aligned to the larger exponent and added, with no branch: the
+  // operand with that exponent is scaled by 2^0 (exact), the other by at most
+  // 2^-200 (far below a float's precision), and a zero (mantissa 0, exponent EZ)
+  // adds nothing, so this equals the case analysis "zero operand: return the
+  // other; else align the smaller to the larger" (up to the normalisation of an
+  // unnormalised operand, which changes no value) without the divergent paths.
   static __device__ __forceinline__ V add(V a, V b) {
-    if (b.m == 0.0f) return a;
-    if (a.m == 0.0f) return b;
-    if (a.e >= b.e) {
-      int d = b.e - a.e;
-      d = d < -200 ? -200 : d;
-      return norm(a.m + ldexpf(b.m, d), a.e);
-    }
-    int d = a.e - b.e;
-    d = d < -200 ? -200 : d;
-    return norm(b.m + ldexpf(a.m, d), b.e);
+    const int ne = max(a.e, b.e);
+    return norm(ldexpf(a.m, max(a.e - ne, -200)) + ldexpf(b.m, max(b.e - ne, -200)), ne);
   }
   static __device__ __forceinline__ V mul(V a, V b) {
     if (a.m == 0.0f || b.m == 0.0f) return zero();
