@@ -455,12 +455,27 @@ __device__ __forceinline__ XFv exact_cell_global(const InstDesc<XFv> &d, int I, 
   return XF::finish(km, sum);
 }
 
+// Cell (r, c) of a staged product as a raw term: the MFMA sum of aligned
+// mantissas with the row and column exponents (zero when either tile row or
+// column is all zero), or, when that sum lost its significant terms (below
+// 2^-100), the product recomputed exactly from the tables.
+template <int T, class PL>
+__device__ __forceinline__ XFv packed_term(const InstDesc<XFv> &d, const PL &p, int I, int K, int J, int r, int c) {
+  const int ra = p.rA[r], cb = p.cB[c];
+  const bool live = ra != XF::EZ && cb != XF::EZ;
+  const float cv = p.R[r][c];
+  XFv v = live ? XFv{cv, ra + cb} : XF::zero();
+  if (live && !(cv >= 0x1p-100f)) v = exact_cell_global<T>(d, I, K, J, r, c);
+  return v;
+}
+
 // tile_products for the scaled inside path from product-ready operands: the
-// same MFMA and fold, with no exponent search and half the operand bytes.
+// same MFMA, with no exponent search and half the operand bytes; sum (+)= both
+// products (each if do_k), folded in one step (XF::add3).
 template <int T>
 __device__ __forceinline__ void tile_products_packed(const InstDesc<XFv> &d, int I, int J, const PackedOps &o0,
                                                      int K0, bool do0, const PackedOps &o1, int K1, bool do1,
-                                                     ProdLDS<XF, T> *pp, XFv &acc0, XFv &acc1) {
+                                                     ProdLDS<XF, T> *pp, XFv &sum) {
   const int tid = threadIdx.x, r = tid / T, c = tid % T;
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
@@ -495,18 +510,9 @@ __device__ __forceinline__ void tile_products_packed(const InstDesc<XFv> &d, int
     for (int q = 0; q < 4; ++q) pp[k].R[bi * 16 + 4 * (lane >> 4) + q][bj * 16 + (lane & 15)] = acc[q];
   }
   __syncthreads();
-#pragma unroll
-  for (int k2 = 0; k2 < 2; ++k2) {
-    if (!(k2 ? do1 : do0)) continue;
-    const int ra = pp[k2].rA[r], cb = pp[k2].cB[c];
-    XFv contrib = XF::zero();
-    if (ra != XF::EZ && cb != XF::EZ) {
-      const float cv = pp[k2].R[r][c];
-      contrib = cv >= 0x1p-100f ? XF::norm(cv, ra + cb) : exact_cell_global<T>(d, I, k2 ? K1 : K0, J, r, c);
-    }
-    XFv &a = k2 ? acc1 : acc0;
-    a = XF::add(a, contrib);
-  }
+  const XFv t0 = do0 ? packed_term<T>(d, pp[0], I, K0, J, r, c) : XF::zero();
+  const XFv t1 = do1 ? packed_term<T>(d, pp[1], I, K1, J, r, c) : XF::zero();
+  sum = XF::add3(sum, t0, t1);
   __syncthreads();
 }
 
@@ -613,8 +619,8 @@ template <int T> __device__ __forceinline__ void out_cell_by_step(int k, int &a,
 
 // Up to four product-ready products k = 0..3 (pairs (0, 1) and (2, 3) of two
 // launches) folded into acc, one MFMA chain set per product on its own
-// wavefronts.  The fold repeats, per pair, the additions of tile_products_packed
-// followed by the caller's acc = acc + acc2, so the result has the same bits.
+// wavefronts.  Each pair is folded with the same XF::add3 as in
+// tile_products_packed, so the result has the same bits whatever the window.
 template <int T>
 __device__ __forceinline__ void tile_products_packed4(const InstDesc<XFv> &d, int I, int J, const PackedOps *o,
                                                       const int *K, const bool *dok, PackLDS<T> *pp, XFv &acc) {
@@ -652,22 +658,12 @@ __device__ __forceinline__ void tile_products_packed4(const InstDesc<XFv> &d, in
     for (int q = 0; q < 4; ++q) pp[k].R[bi * 16 + 4 * (lane >> 4) + q][bj * 16 + (lane & 15)] = m[q];
   }
   __syncthreads();
-  auto contrib = [&](int k2) {
-    const int ra = pp[k2].rA[r], cb = pp[k2].cB[c];
-    XFv v = XF::zero();
-    if (ra != XF::EZ && cb != XF::EZ) {
-      const float cv = pp[k2].R[r][c];
-      v = cv >= 0x1p-100f ? XF::norm(cv, ra + cb) : exact_cell_global<T>(d, I, K[k2], J, r, c);
-    }
-    return v;
-  };
 #pragma unroll
   for (int h = 0; h < 4; h += 2) {
     if (!(dok[h] || dok[h + 1])) continue;
-    if (dok[h]) acc = XF::add(acc, contrib(h));
-    XFv acc2 = XF::zero();
-    if (dok[h + 1]) acc2 = XF::add(acc2, contrib(h + 1));
-    acc = XF::add(acc, acc2);
+    const XFv t0 = dok[h] ? packed_term<T>(d, pp[h], I, K[h], J, r, c) : XF::zero();
+    const XFv t1 = dok[h + 1] ? packed_term<T>(d, pp[h + 1], I, K[h + 1], J, r, c) : XF::zero();
+    acc = XF::add3(acc, t0, t1);
   }
   __syncthreads();
 }
@@ -765,13 +761,14 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
         for (int i = 0; i < W; ++i) {
           if (!(d1[u][i] || d2[u][i])) continue;
           const int s = t - W + 1 + i, k1 = Ib[u] + s - 1, k2 = Jb[u] - (s - 1);
-          V acc2 = SR::zero();
-          if constexpr (SR::scaled)
+          if constexpr (SR::scaled) {
             tile_products_packed<T>(ins, Ib[u], Jb[u], qb1[u][i], k1, d1[u][i], qb2[u][i], k2, d2[u][i],
-                                    sm.m.s.q.p, av[u], acc2);
-          else
+                                    sm.m.s.q.p, av[u]);
+          } else {
+            V acc2 = SR::zero();
             tile_products<SR, T>(ob1[u][i], d1[u][i], ob2[u][i], d2[u][i], sm.m.s.q.p, av[u], acc2);
-          av[u] = SR::add(av[u], acc2);
+            av[u] = SR::add(av[u], acc2);
+          }
         }
       }
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
@@ -842,10 +839,13 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
     }
   }
   if (do1 || do2) {
-    V acc2 = SR::zero();
-    if constexpr (SR::scaled) tile_products_packed<T>(ins, I, J, q1, K1, do1, q2, K2, do2, sm.m.s.q.p, accv, acc2);
-    else tile_products<SR, T>(o1, do1, o2, do2, sm.m.s.q.p, accv, acc2);
-    accv = SR::add(accv, acc2);
+    if constexpr (SR::scaled) {
+      tile_products_packed<T>(ins, I, J, q1, K1, do1, q2, K2, do2, sm.m.s.q.p, accv);
+    } else {
+      V acc2 = SR::zero();
+      tile_products<SR, T>(o1, do1, o2, do2, sm.m.s.q.p, accv, acc2);
+      accv = SR::add(accv, acc2);
+    }
   }
   if (!fin) {
     const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;

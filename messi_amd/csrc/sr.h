@@ -137,6 +137,13 @@ struct XF {
     const int ne = max(a.e, b.e);
     return norm(ldexpf(a.m, max(a.e - ne, -200)) + ldexpf(b.m, max(b.e - ne, -200)), ne);
   }
+  // acc + t1 + t2 with one alignment and one normalisation: the split-point
+  // products that one launch makes ready are folded into a cell's sum in a
+  // single step (terms may be unnormalised; zero terms carry exponent EZ).
+  static __device__ __forceinline__ V add3(V acc, V t1, V t2) {
+    const int ne = max(acc.e, max(t1.e, t2.e));
+    return norm(ldexpf(acc.m, acc.e - ne) + ldexpf(t1.m, t1.e - ne) + ldexpf(t2.m, t2.e - ne), ne);
+  }
   static __device__ __forceinline__ V mul(V a, V b) {
     if (a.m == 0.0f || b.m == 0.0f) return zero();
     return norm(a.m * b.m, a.e + b.e);
