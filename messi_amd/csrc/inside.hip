@@ -421,15 +421,18 @@ struct PackedOps {
   int ea, eb;
 };
 
+// The product-ready copies hold whole tiles (zero mantissas and EZ exponents
+// beyond the table), so the loads need no bounds checks; tile offsets fit in 32
+// bits (NT^2 T^2 < 2^31 for every length the tables fit in memory).
 template <int T>
 __device__ __forceinline__ PackedOps fetch_packed(const InstDesc<XFv> &d, int I, int K, int J) {
-  const int tid = threadIdx.x, r = tid / T, c = tid % T, L = d.L;
-  const int ar = I * T + r, ac = K * T + c, bcol = J * T + r, brow = K * T + 1 + c;
+  const int tid = threadIdx.x, r = tid / T;
+  const unsigned ta = (unsigned)(I * d.NT + K), tb = (unsigned)(K * d.NT + J);
   PackedOps o;
-  o.a = (ar < L && ac < L) ? d.Lp[((size_t)I * d.NT + K) * (T * T) + tid] : 0.0f;
-  o.b = (bcol < L && brow < L) ? d.Sp[((size_t)K * d.NT + J) * (T * T) + tid] : 0.0f;
-  o.ea = ar < L ? d.LpE[((size_t)I * d.NT + K) * T + r] : XF::EZ;
-  o.eb = bcol < L ? d.SpE[((size_t)K * d.NT + J) * T + r] : XF::EZ;
+  o.a = d.Lp[ta * (T * T) + tid];
+  o.b = d.Sp[tb * (T * T) + tid];
+  o.ea = d.LpE[ta * T + r];
+  o.eb = d.SpE[tb * T + r];
   return o;
 }
 
@@ -726,7 +729,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
     for (int u = 0; u < BP; ++u) {
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
       av[u] = (has[u] && had_visit(t, Jb[u] - Ib[u], W, ins.KB, false) && j < L && y < L)
-                  ? acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid]
+                  ? acc[(unsigned)(Ib[u] * NT + Jb[u]) * (T * T) + tid]
                   : SR::zero();
 #pragma unroll
       for (int i = 0; i < W; ++i) {
@@ -772,7 +775,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
         }
       }
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
-      if (j < L && y < L) acc[((size_t)Ib[u] * NT + Jb[u]) * (T * T) + tid] = av[u];  // every visit writes
+      if (j < L && y < L) acc[(unsigned)(Ib[u] * NT + Jb[u]) * (T * T) + tid] = av[u];  // every visit writes
     }
     if constexpr (P)
       wg_signal(has[0] ? visit_done + (size_t)Ib[0] * NT + Jb[0] : nullptr, t + 1,
@@ -801,7 +804,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
   V accv = SR::zero();
   {
     const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
-    if (had_visit(t, t, W, ins.KB, true) && j < L && y < L) accv = acc[((size_t)I * NT + J) * (T * T) + tid];
+    if (had_visit(t, t, W, ins.KB, true) && j < L && y < L) accv = acc[(unsigned)(I * NT + J) * (T * T) + tid];
   }
   Operands<SR> o1, o2;
   PackedOps q1, q2;
@@ -1037,10 +1040,10 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       int e = ok && v.m != 0.0f ? v.e : XF::EZ;
 #pragma unroll
       for (int off = T / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
-      if (ok) {
-        ins.Lp[((size_t)I * NT + J) * (T * T) + tid] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
-        if (b == 0) ins.LpE[((size_t)I * NT + J) * T + a] = e;
-      }
+      // every lane writes (zero beyond the table), so products read whole tiles
+      // with no bounds checks
+      ins.Lp[((size_t)I * NT + J) * (T * T) + tid] = !ok || v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+      if (b == 0) ins.LpE[((size_t)I * NT + J) * T + a] = e;
     }
     // B operand: the column segment of S over rows IT+1 .. IT+T (the last one is
     // row (I+1)T of the tile below, staged in Sb), aligned to its largest exponent.
@@ -1051,9 +1054,8 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       int e = v.m != 0.0f ? v.e : XF::EZ;
 #pragma unroll
       for (int off = T / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
-      if (row < L && y < L)
-        ins.Sp[((size_t)I * NT + J) * (T * T) + tid] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
-      if (c == 0 && y < L) ins.SpE[((size_t)I * NT + J) * T + b] = e;
+      ins.Sp[((size_t)I * NT + J) * (T * T) + tid] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+      if (c == 0) ins.SpE[((size_t)I * NT + J) * T + b] = e;
     }
   }
   if constexpr (P) wg_signal(tile_done + (size_t)I * NT + J, 1, nullptr, 0, diag_done + t);
@@ -1630,6 +1632,8 @@ int inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &i
     // tile (I, K) holds an L cell with h - j <= band iff (K - I - 1) T + 1 <= band
     const int band = ws[m]->band;
     d[m].KB = (band < 0 || band >= d[m].L) ? d[m].NT : (band == 0 ? 0 : (band - 1) / TE + 1);
+    MESSI_REQUIRE((size_t)d[m].NT * d[m].NT * TE * TE < ((size_t)1 << 31),
+                  "inside: length too large for the 32-bit tile offsets");
     Lmax = std::max(Lmax, d[m].L);
     NTmax = std::max(NTmax, d[m].NT);
     d[m].pflag = nullptr;
