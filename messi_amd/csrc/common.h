@@ -37,6 +37,18 @@ inline void kh99_logprobs(double lp[7]) {
   lp[6] = std::log(0.105397);
 }
 
+// A pointer read from device memory (a descriptor field) is generic to the
+// compiler, so its loads and stores become flat instructions, which also count
+// against the LDS / scalar-memory counter and make every later wait on that
+// counter wait for them too.  gp() restates such a pointer as a global one
+// (the address space exists only in the device pass of the compiler).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MESSI_GLOBAL __attribute__((address_space(1)))
+#else
+#define MESSI_GLOBAL
+#endif
+template <class X> __device__ __forceinline__ MESSI_GLOBAL X *gp(X *p) { return (MESSI_GLOBAL X *)p; }
+
 // Workgroups are dispatched to the 8 XCDs of an MI355X round-robin by workgroup
 // id, and each XCD has its own L2.  This permutation gives XCD k a contiguous
 // chunk of the logical work items (in their natural order), so items that share

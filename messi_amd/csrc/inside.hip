@@ -135,12 +135,13 @@ __global__ void ptv_root_kernel(const InstDesc<typename SR::V> *__restrict__ ins
   int j = blockIdx.y;
   if (y >= L || j >= L) return;
   double x = -HUGE_VAL;
-  if (j < y && y - j <= e.maxbp && (!e.mask || e.mask[(size_t)j + (size_t)L * y])) {
-    const int k = e.T0[(size_t)e.pid0[j] * e.n0 + e.pid0[y]];
-    const double v = k >= 0 ? rootacc[e.rootoff + k] : -HUGE_VAL;
-    if (v != -HUGE_VAL) x = v * B - d.lam[j] - d.lam[y];
+  if (j < y && y - j <= e.maxbp && (!e.mask || gp(e.mask)[(size_t)j + (size_t)L * y])) {
+    const int *const pid0 = e.pid0;
+    const int k = gp(e.T0)[(size_t)gp(pid0)[j] * e.n0 + gp(pid0)[y]];
+    const double v = k >= 0 ? gp(rootacc)[e.rootoff + k] : -HUGE_VAL;
+    if (v != -HUGE_VAL) x = v * B - gp(d.lam)[j] - gp(d.lam)[y];
   }
-  d.ptv[(size_t)j * L + y] = SR::from_log(x);
+  gp(d.ptv)[(size_t)j * L + y] = SR::from_log(x);
 }
 
 // Set the diagonal of S and L (F has no s rule).  The bulk accumulator needs no
@@ -254,9 +255,76 @@ template <class SR, int T> struct StepLDS {
     StepStaged<SR, T> s;
   } m;
   V Sb[T], Fbot[T], Fleft[T];
-  int flag;  // a dependency wait's outcome (single-launch form)
+  double Fbd[T], Fld[T];  // scaled path, double wavefront: Fbot and Fleft at the tile's scale
+  int flag;    // a dependency wait's outcome (single-launch form)
+  int escale;  // scaled path, double wavefront: the tile's power-of-two scale
+  int bad;     // scaled path, double wavefront: a value left the safe range
   __device__ V &at(int i) { return m.all[i]; }
 };
+
+// Scaled path, off-diagonal tiles: the in-tile wavefront runs in plain double
+// precision.  FP64 FMAs issue at the unpacked FP32 rate on CDNA4, and a
+// double's range (2^+-1022) holds a tile's values at one power-of-two scale, so a
+// step is two fused multiply-adds instead of per-cell exponent alignment and
+// normalisation.  The diagonal tiles (I, I), (J, J) and the pair term enter
+// unscaled; the tile's own cells, its bulk sums and its border S / F values at
+// 2^-escale, with escale the largest exponent of the border values.  Any value
+// that would leave [2^-1000, 2^1000] (or a zero) sends the tile through the
+// per-cell-exponent wavefront instead, so the double path never loses range.
+typedef double __attribute__((may_alias)) adbl;
+constexpr double D_LO = 0x1p-1000, D_HI = 0x1p1000;
+
+__device__ __forceinline__ bool d_in_range(double x) {
+  const double ax = fabs(x);
+  return ax == 0.0 || (ax >= D_LO && ax <= D_HI);  // false for NaN and infinities
+}
+// m * 2^(e - E) as a double (0 for a zero mantissa); clears ok when a nonzero
+// value lands outside [2^-1000, 2^1000].
+__device__ __forceinline__ double xf_to_d(XFv v, int E, bool &ok) {
+  if (v.m == 0.0f) return 0.0;
+  const double x = ldexp((double)v.m, max(min(v.e - E, 1100), -1100));
+  const double ax = fabs(x);
+  ok = ok && ax >= D_LO && ax <= D_HI;
+  return x;
+}
+// A double at scale 2^E as a normalised XF cell (mantissa rounded to float).
+__device__ __forceinline__ XFv d_to_xf(double x, int E) {
+  if (x == 0.0) return XF::zero();
+  int e;
+  float f = (float)frexp(x, &e);
+  if (f == 1.0f) {  // the mantissa rounded up to 1
+    f = 0.5f;
+    ++e;
+  }
+  return XFv{f, e + E};
+}
+// Largest exponent of the nonzero values held by the first T lanes of a
+// wavefront (INT_MIN / 2 when all are zero), uniform across the wavefront: a
+// max-butterfly over each 16-lane row (DPP quad permutes, then row rotations
+// by 4 and 8), then the rows' results read from lanes 0 and 16.
+__device__ __forceinline__ int row16_max(int e) {
+  e = max(e, __builtin_amdgcn_update_dpp(e, e, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  e = max(e, __builtin_amdgcn_update_dpp(e, e, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  e = max(e, __builtin_amdgcn_update_dpp(e, e, 0x124, 0xF, 0xF, false));  // row_ror:4
+  e = max(e, __builtin_amdgcn_update_dpp(e, e, 0x128, 0xF, 0xF, false));  // row_ror:8
+  return e;
+}
+template <int T>
+__device__ __forceinline__ int border_emax(int e) {
+  e = row16_max(e);
+  int m = __builtin_amdgcn_readlane(e, 0);
+  if constexpr (T > 16) m = max(m, __builtin_amdgcn_readlane(e, 16));
+  return m;
+}
+// Maximum over each aligned group of T consecutive lanes (T = 16 or 32), in
+// every lane of the group; the whole wavefront must be active.
+template <int T>
+__device__ __forceinline__ int group_max(int e) {
+  static_assert(T == 16 || T == 32, "tile edge 16 or 32");
+  e = row16_max(e);
+  if constexpr (T == 32) e = max(e, __shfl_xor(e, 16));
+  return e;
+}
 
 // Where the two operand tiles of a product live: element (r, c) of A is
 // A[(ar0 + r) * L + ac0 + c] and element (c, r) of B is B[(br0 + r) * L + bc0 + c],
@@ -264,10 +332,10 @@ template <class SR, int T> struct StepLDS {
 // row-major table: up == true; column <= row index in a column-major one: up ==
 // false), zero elsewhere.
 template <class SR> struct OpSrc {
-  const typename SR::V *A;
+  const MESSI_GLOBAL typename SR::V *A;
   int ar0, ac0;
   bool aup;
-  const typename SR::V *B;
+  const MESSI_GLOBAL typename SR::V *B;
   int br0, bc0;
   bool bup;
 };
@@ -275,13 +343,13 @@ template <class SR> struct OpSrc {
 template <class SR>
 __device__ __forceinline__ typename SR::V op_a(const OpSrc<SR> &s, int L, int r, int c) {
   const int ar = s.ar0 + r, ac = s.ac0 + c;
-  return (ar >= 0 && ac >= 0 && ar < L && ac < L && (s.aup ? ar <= ac : ac <= ar)) ? s.A[(size_t)ar * L + ac]
+  return (ar >= 0 && ac >= 0 && ar < L && ac < L && (s.aup ? ar <= ac : ac <= ar)) ? typename SR::V(s.A[(size_t)ar * L + ac])
                                                                                  : SR::zero();
 }
 template <class SR>
 __device__ __forceinline__ typename SR::V op_b(const OpSrc<SR> &s, int L, int r, int c) {
   const int br = s.br0 + r, bc = s.bc0 + c;
-  return (br >= 0 && bc >= 0 && br < L && bc < L && (s.bup ? br <= bc : bc <= br)) ? s.B[(size_t)br * L + bc]
+  return (br >= 0 && bc >= 0 && br < L && bc < L && (s.bup ? br <= bc : bc <= br)) ? typename SR::V(s.B[(size_t)br * L + bc])
                                                                                  : SR::zero();
 }
 
@@ -302,8 +370,8 @@ __device__ __forceinline__ Operands<SR> fetch_operands(const OpSrc<SR> &s, int L
 
 // Inside split-point product (I,K) x (K,J): L[IT+a, KT+c] * S[KT+c+1, JT+b].
 template <class SR, int T>
-__device__ __forceinline__ OpSrc<SR> inside_src(int I, int K, int J, const typename SR::V *Lr,
-                                               const typename SR::V *Sc) {
+__device__ __forceinline__ OpSrc<SR> inside_src(int I, int K, int J, const MESSI_GLOBAL typename SR::V *Lr,
+                                               const MESSI_GLOBAL typename SR::V *Sc) {
   return OpSrc<SR>{Lr, I * T, K * T, true, Sc, J * T, K * T + 1, false};
 }
 
@@ -429,10 +497,10 @@ __device__ __forceinline__ PackedOps fetch_packed(const InstDesc<XFv> &d, int I,
   const int tid = threadIdx.x, r = tid / T;
   const unsigned ta = (unsigned)(I * d.NT + K), tb = (unsigned)(K * d.NT + J);
   PackedOps o;
-  o.a = d.Lp[ta * (T * T) + tid];
-  o.b = d.Sp[tb * (T * T) + tid];
-  o.ea = d.LpE[ta * T + r];
-  o.eb = d.SpE[tb * T + r];
+  o.a = gp(d.Lp)[ta * (T * T) + tid];
+  o.b = gp(d.Sp)[tb * (T * T) + tid];
+  o.ea = gp(d.LpE)[ta * T + r];
+  o.eb = gp(d.SpE)[tb * T + r];
   return o;
 }
 
@@ -443,11 +511,11 @@ __device__ __forceinline__ XFv exact_cell_global(const InstDesc<XFv> &d, int I, 
   const int L = d.L, row = I * T + r, col = J * T + c;
   auto av = [&](int q) {
     const int h = K * T + q;
-    return (row < L && h < L && row <= h) ? d.Lr[(size_t)row * L + h] : XF::zero();
+    return (row < L && h < L && row <= h) ? gp(d.Lr)[(size_t)row * L + h] : XF::zero();
   };
   auto bv = [&](int q) {
     const int s = K * T + 1 + q;
-    return (col < L && s < L && s <= col) ? d.Sc[(size_t)col * L + s] : XF::zero();
+    return (col < L && s < L && s <= col) ? gp(d.Sc)[(size_t)col * L + s] : XF::zero();
   };
   int km = XF::kmin();
 #pragma unroll 1
@@ -684,13 +752,13 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
   constexpr bool by_step = !SR::scaled;  // in-tile wavefront cells in step order (see below)
   constexpr int BP = bulk_per(W);
   const int L = ins.L, NT = ins.NT;
-  V *__restrict__ Lr = ins.Lr;
-  V *__restrict__ Lc = ins.Lc;
-  V *__restrict__ Sr = ins.Sr;
-  V *__restrict__ Sc = ins.Sc;
-  V *__restrict__ F = ins.F;
-  V *__restrict__ acc = ins.acc;
-  const V *__restrict__ ptv = ins.ptv;
+  MESSI_GLOBAL V *__restrict__ Lr = gp(ins.Lr);
+  MESSI_GLOBAL V *__restrict__ Lc = gp(ins.Lc);
+  MESSI_GLOBAL V *__restrict__ Sr = gp(ins.Sr);
+  MESSI_GLOBAL V *__restrict__ Sc = gp(ins.Sc);
+  MESSI_GLOBAL V *__restrict__ F = gp(ins.F);
+  MESSI_GLOBAL V *__restrict__ acc = gp(ins.acc);
+  const MESSI_GLOBAL V *__restrict__ ptv = gp(ins.ptv);
   int *const tile_done = ins.pflag, *const visit_done = ins.pflag + (size_t)NT * NT;
   int *const diag_done = ins.pflag + 2 * (size_t)NT * NT;
   if (!fin_wg) {
@@ -729,7 +797,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
     for (int u = 0; u < BP; ++u) {
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
       av[u] = (has[u] && had_visit(t, Jb[u] - Ib[u], W, ins.KB, false) && j < L && y < L)
-                  ? acc[(unsigned)(Ib[u] * NT + Jb[u]) * (T * T) + tid]
+                  ? typename SR::V(acc[(unsigned)(Ib[u] * NT + Jb[u]) * (T * T) + tid])
                   : SR::zero();
 #pragma unroll
       for (int i = 0; i < W; ++i) {
@@ -817,28 +885,74 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
     if (do2) o2 = fetch_operands<SR, T>(s2, L);
   }
   // ---- finalising workgroup: stage the tile's inputs now, while the product
-  // operands are in flight (these LDS arrays are not shared with the products)
-  if (fin) {
+  // operands are in flight (these LDS arrays are not shared with the products).
+  // Off-diagonal tiles of the scaled path stage the diagonal tiles and the pair
+  // term as unscaled doubles for the double wavefront (dbl); dok turns false when
+  // a value the double wavefront uses is out of its range.
+  const bool dbl = SR::scaled && t > 0;
+  adbl *const D = reinterpret_cast<adbl *>(sm.m.all);
+  bool dok = true;
+  auto stage_tables = [&](bool as_d) {
+    auto put = [&](int i, V v) {
+      if constexpr (SR::scaled) {
+        if (as_d) {
+          D[i] = xf_to_d(v, 0, dok);
+          return;
+        }
+      }
+      sm.at(i) = v;
+    };
     for (int idx = tid; idx < T * T; idx += T * T) {
       int a = idx / T, b = idx % T;
       int j = IT + a, y = JT + b;
       int jd = IT + b;  // column index inside the diagonal tile (I, I)
-      sm.at(WL::ld(a, b)) = (j < L && jd < L && jd >= j) ? Lr[(size_t)j * L + jd] : SR::zero();
-      sm.at(WL::cp(a, b)) = (j < L && y < L) ? ptv[(size_t)j * L + y] : SR::zero();
+      put(WL::ld(a, b), (j < L && jd < L && jd >= j) ? typename SR::V(Lr[(size_t)j * L + jd]) : SR::zero());
+      put(WL::cp(a, b), (j < L && y < L) ? typename SR::V(ptv[(size_t)j * L + y]) : SR::zero());
     }
     for (int idx = tid; idx < T * T; idx += T * T) {
       int b = idx / T, c = idx % T;  // S[JT+c, JT+b], c fastest along the column
       int y = JT + b, rr = JT + c;
-      sm.at(WL::sd(c, b)) = (y < L && rr <= y) ? Sc[(size_t)y * L + rr] : SR::zero();
+      put(WL::sd(c, b), (y < L && rr <= y) ? typename SR::V(Sc[(size_t)y * L + rr]) : SR::zero());
     }
-    for (int k = tid; k < WL::N_ZERO_STAGED; k += T * T) sm.at(WL::zero_staged(k)) = SR::zero();
+    for (int k = tid; k < WL::N_ZERO_STAGED; k += T * T) put(WL::zero_staged(k), SR::zero());
+  };
+  double sbd = 0.0;  // double wavefront: this lane's bottom-border S at the tile's scale (lanes < T)
+  if (fin) {
+    stage_tables(dbl);
+    V sb = SR::zero(), fb = SR::zero(), fl = SR::zero();
     if (tid < T) {
       int b = tid, y = JT + b, rb = (I + 1) * T;
-      sm.Sb[b] = (t >= 1 && y < L && rb <= y) ? Sc[(size_t)y * L + rb] : SR::zero();
+      sb = (t >= 1 && y < L && rb <= y) ? typename SR::V(Sc[(size_t)y * L + rb]) : SR::zero();
       int yc = JT + b - 1;
-      sm.Fbot[b] = (t >= 1 && rb < L && yc < L && rb <= yc) ? F[(size_t)rb * L + yc] : SR::zero();
+      fb = (t >= 1 && rb < L && yc < L && rb <= yc) ? typename SR::V(F[(size_t)rb * L + yc]) : SR::zero();
       int ja = IT + b + 1, yl = JT - 1;
-      sm.Fleft[b] = (t >= 1 && ja < L && yl >= 0 && ja <= yl) ? F[(size_t)ja * L + yl] : SR::zero();
+      fl = (t >= 1 && ja < L && yl >= 0 && ja <= yl) ? typename SR::V(F[(size_t)ja * L + yl]) : SR::zero();
+      sm.Sb[b] = sb;
+      sm.Fbot[b] = fb;
+      sm.Fleft[b] = fl;
+    }
+    if (tid == 0) sm.bad = 0;
+    if constexpr (SR::scaled) {
+      if (dbl && tid < 64) {
+        // the tile's scale: the largest exponent of its nonzero border values
+        // (S of the tile below, F below and to the left), 0 when all are zero;
+        // then the borders at that scale (the S row goes into the tile's LDS
+        // area once the products are done)
+        int e = INT_MIN / 2;
+        if (tid < T) {
+          if (sb.m != 0.0f) e = max(e, sb.e);
+          if (fb.m != 0.0f) e = max(e, fb.e);
+          if (fl.m != 0.0f) e = max(e, fl.e);
+        }
+        int E = border_emax<T>(e);
+        if (E < -(1 << 27)) E = 0;
+        if (tid < T) {
+          sbd = xf_to_d(sb, E, dok);
+          sm.Fbd[tid] = xf_to_d(fb, E, dok);
+          sm.Fld[tid] = xf_to_d(fl, E, dok);
+        }
+        if (tid == 0) sm.escale = E;
+      }
     }
   }
   if (do1 || do2) {
@@ -860,15 +974,20 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       int a = idx / T, b = idx % T;
       int j = IT + a;
       bool d = (a == b) && j < L;
-      sm.at(WL::cs(a, b)) = d ? Sr[(size_t)j * L + j] : SR::zero();
-      sm.at(WL::cl(a, b)) = d ? Lr[(size_t)j * L + j] : SR::zero();
+      sm.at(WL::cs(a, b)) = d ? typename SR::V(Sr[(size_t)j * L + j]) : SR::zero();
+      sm.at(WL::cl(a, b)) = d ? typename SR::V(Lr[(size_t)j * L + j]) : SR::zero();
       sm.at(WL::cf(a, b)) = SR::zero();
     }
   } else {
     if (by_step) sm.at(WL::cf(tid / T, tid % T)) = accv;  // to the thread that takes the cell below
     // the zero regions of the wavefront's reads and the bottom border row as row T of cS
-    for (int k = tid; k < WL::N_ZERO_WAVE; k += T * T) sm.at(WL::zero_wave(k)) = SR::zero();
-    if (tid < T) sm.at(WL::cs(T, tid)) = sm.Sb[tid];
+    if (!dbl) {
+      for (int k = tid; k < WL::N_ZERO_WAVE; k += T * T) sm.at(WL::zero_wave(k)) = SR::zero();
+      if (tid < T) sm.at(WL::cs(T, tid)) = sm.Sb[tid];
+    } else if constexpr (SR::scaled) {
+      for (int k = tid; k < WL::N_ZERO_WAVE; k += T * T) D[WL::zero_wave(k)] = 0.0;
+      if (tid < T) D[WL::cs(T, tid)] = sbd;
+    }
   }
   __syncthreads();
   // One thread per cell.  A cell's split-point sum is accumulated online: at
@@ -933,27 +1052,86 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
     // F[j+1, y-1] for the pair term (inner cell, finalised two steps earlier, or
     // the bottom / left border), read one step ahead at a fixed position
     const bool fin_in = a + 1 < T && b >= 1;
-    const V *fsrc = fin_in ? &sm.at(WL::cf(a + 1, b - 1)) : (a + 1 == T ? &sm.Fbot[b] : &sm.Fleft[a]);
-    const V pterm = j + 1 > y - 1 ? SR::zero() : sm.at(WL::cp(a, b));
-    V fin_next = *fsrc;
     const int wave_last = __builtin_amdgcn_readfirstlane(2 * T - 2 - (tid & ~63) / T);
+    bool xf_wave = !dbl;
+    if (dbl) {
+      // Double wavefront (see D_LO / D_HI): the same steps on doubles, the cells at
+      // 2^-E.  S and F of a span of two or more are positive, so a zero there, like
+      // any value out of range, or a pair term that underflowed, hands the tile to
+      // the per-cell-exponent wavefront below, which recomputes it from the tables.
+      const int E = sm.escale;
+      double acd = xf_to_d(accv, E, dok);
+      const adbl *fsrc = fin_in ? &D[WL::cf(a + 1, b - 1)]
+                                : (a + 1 == T ? reinterpret_cast<const adbl *>(&sm.Fbd[b])
+                                              : reinterpret_cast<const adbl *>(&sm.Fld[a]));
+      const double pterm = j + 1 > y - 1 ? 0.0 : D[WL::cp(a, b)];
+      const double p0 = rk.p0, p2 = rk.p2, p3 = rk.p3, p4 = rk.p4, p6 = rk.p6;
+      double fin_next = *fsrc;
+      double vs = 0.0, vf = 0.0, vl = 0.0, vpf = 0.0, vfin = 0.0;
 #pragma unroll
-    for (int st = 0; st <= 2 * T - 2; ++st) {
-      if (st <= wave_last) {
-        const V fin_f = fin_next;
-        fin_next = *fsrc;
-        const V t1 = SR::mul_raw(sm.at(bcl + WL::off_col_l(st)), sm.at(bcs + WL::off_col_s(st)));
-        const V t2 = SR::mul_raw(sm.at(brl + WL::off_row_l(st)), sm.at(brs + WL::off_row_s(st)));
-        SR::acc_add2(ac, t1, t2);
-        if (st == own) {
-          V vs, vf, vl;
-          SR::finalize(ac, SR::mul_raw(pterm, fin_f), rk, vs, vf, vl);
-          sm.at(WL::cs(a, b)) = vs;
-          sm.at(WL::cf(a, b)) = vf;
-          sm.at(WL::cl(a, b)) = vl;
+      for (int st = 0; st <= 2 * T - 2; ++st) {
+        if (st <= wave_last) {
+          const double fin_f = fin_next;
+          fin_next = *fsrc;
+          acd = fma(D[bcl + WL::off_col_l(st)], D[bcs + WL::off_col_s(st)], acd);
+          acd = fma(D[brl + WL::off_row_l(st)], D[brs + WL::off_row_s(st)], acd);
+          if (st == own) {
+            vfin = fin_f;
+            vpf = pterm * fin_f;
+            vs = fma(p0, acd, p2 * vpf);
+            vf = fma(p4, acd, p3 * vpf);
+            vl = p6 * vpf;
+            D[WL::cs(a, b)] = vs;
+            D[WL::cf(a, b)] = vf;
+            D[WL::cl(a, b)] = vl;
+          }
         }
+        __syncthreads();
       }
+      if (mine) {
+        const bool pf_ok = pterm == 0.0 || vfin == 0.0 || (vpf != 0.0 && d_in_range(vpf));
+        dok = dok && vs != 0.0 && vf != 0.0 && d_in_range(vs) && d_in_range(vf) && d_in_range(vl) && pf_ok;
+      }
+      if (!dok) atomicOr(&sm.bad, 1);
+      // the cell as an XF value in its LDS slots for the write-back (no other
+      // thread reads the doubles after the last step's barrier)
+      sm.at(WL::cs(a, b)) = d_to_xf(vs, E);
+      sm.at(WL::cf(a, b)) = d_to_xf(vf, E);
+      sm.at(WL::cl(a, b)) = d_to_xf(vl, E);
       __syncthreads();
+      xf_wave = sm.bad != 0;
+      if (xf_wave) {
+        // restage the tables, zero regions and bottom border as XF values
+        stage_tables(false);
+        for (int k = tid; k < WL::N_ZERO_WAVE; k += T * T) sm.at(WL::zero_wave(k)) = SR::zero();
+        if (tid < T) sm.at(WL::cs(T, tid)) = sm.Sb[tid];
+        __syncthreads();
+      }
+    }
+    if (xf_wave) {
+      // Per-cell-exponent wavefront: each cell keeps its own exponent, so no
+      // range limit applies.
+      const V *fsrc = fin_in ? &sm.at(WL::cf(a + 1, b - 1)) : (a + 1 == T ? &sm.Fbot[b] : &sm.Fleft[a]);
+      const V pterm = j + 1 > y - 1 ? SR::zero() : sm.at(WL::cp(a, b));
+      V fin_next = *fsrc;
+#pragma unroll
+      for (int st = 0; st <= 2 * T - 2; ++st) {
+        if (st <= wave_last) {
+          const V fin_f = fin_next;
+          fin_next = *fsrc;
+          const V t1 = SR::mul_raw(sm.at(bcl + WL::off_col_l(st)), sm.at(bcs + WL::off_col_s(st)));
+          const V t2 = SR::mul_raw(sm.at(brl + WL::off_row_l(st)), sm.at(brs + WL::off_row_s(st)));
+          SR::acc_add2(ac, t1, t2);
+          if (st == own) {
+            V vs, vf, vl;
+            SR::finalize(ac, SR::mul_raw(pterm, fin_f), rk, vs, vf, vl);
+            sm.at(WL::cs(a, b)) = vs;
+            sm.at(WL::cf(a, b)) = vf;
+            sm.at(WL::cl(a, b)) = vl;
+          }
+        }
+        __syncthreads();
+      }
     }
   } else {
     const int own = b + T - 1 - a;
@@ -1032,18 +1210,16 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
   }
   if constexpr (SR::scaled) {
     // Product-ready copies of the finished tile.  A operand: each row segment of
-    // L aligned to its largest exponent (the T lanes of a row reduce by shuffles).
+    // L aligned to its largest exponent (the T lanes of a row reduce by DPP).
     {
       const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
       const bool ok = j < L && y < L;
       const XFv v = sm.at(WL::cl(a, b));
-      int e = ok && v.m != 0.0f ? v.e : XF::EZ;
-#pragma unroll
-      for (int off = T / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
+      const int e = group_max<T>(ok && v.m != 0.0f ? v.e : XF::EZ);
       // every lane writes (zero beyond the table), so products read whole tiles
       // with no bounds checks
-      ins.Lp[((size_t)I * NT + J) * (T * T) + tid] = !ok || v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
-      if (b == 0) ins.LpE[((size_t)I * NT + J) * T + a] = e;
+      gp(ins.Lp)[((size_t)I * NT + J) * (T * T) + tid] = !ok || v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+      if (b == 0) gp(ins.LpE)[((size_t)I * NT + J) * T + a] = e;
     }
     // B operand: the column segment of S over rows IT+1 .. IT+T (the last one is
     // row (I+1)T of the tile below, staged in Sb), aligned to its largest exponent.
@@ -1051,11 +1227,9 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       const int c = tid % T, b = tid / T, row = IT + 1 + c, y = JT + b;
       XFv v = XF::zero();
       if (row < L && y < L && row <= y) v = c + 1 < T ? sm.at(WL::cs(c + 1, b)) : sm.Sb[b];
-      int e = v.m != 0.0f ? v.e : XF::EZ;
-#pragma unroll
-      for (int off = T / 2; off > 0; off >>= 1) e = max(e, __shfl_xor(e, off));
-      ins.Sp[((size_t)I * NT + J) * (T * T) + tid] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
-      if (c == 0) ins.SpE[((size_t)I * NT + J) * T + b] = e;
+      const int e = group_max<T>(v.m != 0.0f ? v.e : XF::EZ);
+      gp(ins.Sp)[((size_t)I * NT + J) * (T * T) + tid] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+      if (c == 0) gp(ins.SpE)[((size_t)I * NT + J) * T + b] = e;
     }
   }
   if constexpr (P) wg_signal(tile_done + (size_t)I * NT + J, 1, nullptr, 0, diag_done + t);
@@ -1173,17 +1347,17 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   __shared__ OutLDS<SR, T> sm;
   const int tid = threadIdx.x, L = od.L, NT = od.NT;
   const int du = NT - 1 - u;
-  const V *__restrict__ iSr = static_cast<const V *>(od.iSr);
-  const V *__restrict__ iLc = static_cast<const V *>(od.iLc);
-  const V *__restrict__ ptv = static_cast<const V *>(od.ptv);
-  V *__restrict__ oSr = static_cast<V *>(od.oSr);
-  V *__restrict__ oSc = static_cast<V *>(od.oSc);
-  V *__restrict__ oFr = static_cast<V *>(od.oFr);
-  V *__restrict__ oL = static_cast<V *>(od.oL);
-  V *__restrict__ Wr = static_cast<V *>(od.Wr);
-  V *__restrict__ Wc = static_cast<V *>(od.Wc);
-  V *__restrict__ accL = static_cast<V *>(od.accL);
-  V *__restrict__ accS = static_cast<V *>(od.accS);
+  const MESSI_GLOBAL V *__restrict__ iSr = gp(static_cast<const V *>(od.iSr));
+  const MESSI_GLOBAL V *__restrict__ iLc = gp(static_cast<const V *>(od.iLc));
+  const MESSI_GLOBAL V *__restrict__ ptv = gp(static_cast<const V *>(od.ptv));
+  MESSI_GLOBAL V *__restrict__ oSr = gp(static_cast<V *>(od.oSr));
+  MESSI_GLOBAL V *__restrict__ oSc = gp(static_cast<V *>(od.oSc));
+  MESSI_GLOBAL V *__restrict__ oFr = gp(static_cast<V *>(od.oFr));
+  MESSI_GLOBAL V *__restrict__ oL = gp(static_cast<V *>(od.oL));
+  MESSI_GLOBAL V *__restrict__ Wr = gp(static_cast<V *>(od.Wr));
+  MESSI_GLOBAL V *__restrict__ Wc = gp(static_cast<V *>(od.Wc));
+  MESSI_GLOBAL V *__restrict__ accL = gp(static_cast<V *>(od.accL));
+  MESSI_GLOBAL V *__restrict__ accS = gp(static_cast<V *>(od.accS));
   // ---- decode: u+1 finalising tiles first, then the bulk tiles by distance,
   // in plain dispatch order (round-robin over the XCDs: the bulk tiles of one
   // distance read disjoint operands, and grouping them per XCD measured slower)
@@ -1270,10 +1444,10 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   {
     const int a = a0, b = b0, j = j0, y = y0;
     const int rr = JT + a + 1, cc = JT + b;  // iSd[a][b] = iS[JT+a+1, JT+b]
-    sm.u.w.iSd[a][b] = (rr < L && cc < L && rr <= cc) ? iSr[(size_t)rr * L + cc] : SR::zero();
+    sm.u.w.iSd[a][b] = (rr < L && cc < L && rr <= cc) ? typename SR::V(iSr[(size_t)rr * L + cc]) : SR::zero();
     const int lr = IT + a, lc = IT + b - 1;  // iLd[a][b] = iL[IT+a, IT+b-1]
-    sm.u.w.iLd[a][b] = (lc >= 0 && lr < L && lc < L && lr <= lc) ? iLc[(size_t)lc * L + lr] : SR::zero();
-    sm.u.w.cP[a][b] = (j >= 1 && j <= y && y + 1 < L) ? ptv[(size_t)(j - 1) * L + (y + 1)] : SR::zero();
+    sm.u.w.iLd[a][b] = (lc >= 0 && lr < L && lc < L && lr <= lc) ? typename SR::V(iLc[(size_t)lc * L + lr]) : SR::zero();
+    sm.u.w.cP[a][b] = (j >= 1 && j <= y && y + 1 < L) ? typename SR::V(ptv[(size_t)(j - 1) * L + (y + 1)]) : SR::zero();
     sm.u.w.W[a][b] = by_step ? vL : SR::zero();
     sm.u.w.Q[a][b] = by_step ? vS : SR::zero();
   }

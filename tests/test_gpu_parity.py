@@ -174,6 +174,40 @@ def test_all_unpaired_closed_form(ctx):
         assert abs(z - closed) <= tol * abs(closed), prec
 
 
+@pytest.mark.parametrize("score", [300.0, 900.0, -900.0])
+def test_fp32_extreme_pair_scores(ctx, score):
+    """FP32 inside table against the oracle when helices carry pair scores of
+    +-300 and +-900 nats: the in-tile wavefront of the scaled path runs in double
+    precision at one scale per tile and hands a tile to its per-cell-exponent form
+    when a value would leave [2^-1000, 2^1000] (a single +-900 nat pair term is
+    2^+-1298), so both forms are exercised; both must match the oracle."""
+    _say(f"fp32 extreme pair scores {score}")
+    L = 300
+    rng = np.random.default_rng(11)
+    u = -rng.uniform(2, 6, size=L)
+    pm = np.full((L, L), -np.inf)
+    for i in range(L):
+        for j in range(i + 1, L):
+            pm[i, j] = pm[j, i] = u[i] + u[j] + rng.normal(-2, 2)
+    for (i0, j0, n) in ((20, 170, 10), (181, 260, 6)):  # two helices with extreme pair terms
+        for k in range(n):
+            pm[i0 + k, j0 - k] = pm[j0 - k, i0 + k] = u[i0 + k] + u[j0 - k] + score
+    ins = oracle.inside(u, pm)
+    for tile in (16, 32):
+        ctx.set_tile(tile)
+        try:
+            z, t = lib.inside(u, pm, zonly=False, prec="fp32")
+        finally:
+            ctx.set_tile(16)
+        zr = ins[0, 0, -1]
+        assert abs(z - zr) <= 1e-6 * abs(zr), (tile, z, zr)
+        _rel(t, ins)  # the same cells are -inf
+        fin = np.isfinite(ins)
+        err = float(np.max(np.abs(t[fin] - ins[fin])))
+        _say(f"tile {tile}: log Z rel err {abs(z - zr) / abs(zr):.2e}, max table log err {err:.2e}")
+        assert err < 1e-4, tile
+
+
 def test_config2_scale_properties(ctx):
     """Full 1,500 nt x 50 sequence workload: dedup consistency against a banded
     run, symmetry, posterior row sums and FP32 versus FP64 agreement."""
