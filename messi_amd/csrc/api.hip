@@ -155,7 +155,7 @@ struct messi_batch {
   std::vector<int64_t> node_base, site_root, pair_root;
   int64_t nn_total = 0, site_entries = 0, pair_entries = 0, site_root_total = 0, pair_root_total = 0;
   int Lmax = 0;
-  DevBuf site_part, site_expo, site_acc, pair_part, pair_expo, pair_acc, pair_comp;
+  DevBuf site_part, site_expo, site_acc, pair_part, pair_acc, pair_comp;  // pair entries carry their exponents
   Components c4, c16;  // loaded mixture components (site, pair)
   DevBuf dexp_site, dexp_pair, douts;
   std::vector<ExpandDesc> hexp_site, hexp_pair;  // what dexp_site / dexp_pair hold
@@ -390,6 +390,12 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
     }
     ds->pair_entries = part;
     ds->U_root = U[0];
+    // the pair kernel addresses a node's entries with 32-bit byte offsets (16 values
+    // of up to 8 bytes per stored entry; one 8-byte output per root entry)
+    for (int v = 0; v < n; ++v)
+      if (!isleaf(v))
+        MESSI_REQUIRE((uint64_t)U[v] * (v == 0 ? sizeof(double) : 16 * sizeof(double)) <= 0xFFFFFFFFull,
+                      "a tree node has too many distinct column-pattern pairs for 32-bit entry offsets");
     std::vector<NodeDesc> nodes(n);
     std::vector<int> idx;
     idx.reserve((size_t)2 * ds->sumU);
@@ -784,7 +790,7 @@ void pair_levels(messi_batch *b, hipStream_t s, int k, int m0, int m1) {
   for (int h = 1; h <= b->Hmax; ++h) {
     const int lo = b->pair_lvm[(size_t)(h - 1) * (n + 1) + m0], hi = b->pair_lvm[(size_t)(h - 1) * (n + 1) + m1];
     launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
-                                  b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), b->pair_expo.as<int>(),
+                                  b->pair_idx, b->pair_leaf, b->pair_part.as<T>(),
                                   b->c16.P.as<T>() + (size_t)k * b->nn_total * 256,
                                   b->c16.pi.as<double>() + (size_t)k * b->n * 16,
                                   b->c16.logw.as<double>() + (size_t)k * b->n, k == 0, b->pair_acc.as<double>(),
@@ -802,7 +808,6 @@ void run_pair(messi_batch *b, int prec, double *h_percomp, bool expand, bool pai
   require_loaded(b->c16, prec, "pair");
   const int K = b->c16.K;
   b->pair_part.ensure(std::max<int64_t>(1, b->pair_entries) * 16 * sizeof(T));
-  b->pair_expo.ensure(std::max<int64_t>(1, b->pair_entries) * sizeof(int));
   b->pair_acc.ensure(std::max<int64_t>(1, b->pair_root_total) * sizeof(double));
   b->pair_comp.ensure(std::max<int64_t>(1, b->pair_root_total) * sizeof(double));
   if (expand || h_percomp)
@@ -969,7 +974,6 @@ int64_t component_roots(messi_batch *b, int prec, double *sitec, const ExpandDes
   b->site_expo.ensure(sizeof(int) * se * K4);
   b->site_acc.ensure(sizeof(double) * sr * K4);
   b->pair_part.ensure(sizeof(T) * 16 * pe * K16);
-  b->pair_expo.ensure(sizeof(int) * pe * K16);
   b->pair_acc.ensure(sizeof(double) * pr * K16);
   b->pair_comp.ensure(sizeof(double) * pr * K16);
   const ExpandDesc *dsite = upload_expand(b, prec, false);
@@ -992,7 +996,6 @@ int64_t component_roots(messi_batch *b, int prec, double *sitec, const ExpandDes
   CompBatch cp;
   cp.K = K16;
   cp.part = 16 * pe;
-  cp.expo = pe;
   cp.root = pr;
   cp.P = b->nn_total * 256;
   cp.pi = (int64_t)b->n * 16;
@@ -1000,7 +1003,7 @@ int64_t component_roots(messi_batch *b, int prec, double *sitec, const ExpandDes
   for (int h = 1; h <= b->Hmax; ++h) {
     const int lo = b->pair_lv[h - 1], hi = b->pair_lv[h];
     launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
-                                  b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), b->pair_expo.as<int>(),
+                                  b->pair_idx, b->pair_leaf, b->pair_part.as<T>(),
                                   b->c16.P.as<T>(), b->c16.pi.as<double>(), b->c16.logw.as<double>(), 1,
                                   b->pair_acc.as<double>(), b->pair_comp.as<double>(), cp);
   }

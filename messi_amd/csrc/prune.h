@@ -33,7 +33,7 @@ struct CompBatch {
 // node g is P + 256 g (16 g for 4 states), member m uses pi + 16 m (4 m) and logw[m].
 template <typename T, int NB>
 void launch_pair_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes,
-                       const int *idx, const uint16_t *leafc, T *part, int *expo, const T *P,
+                       const int *idx, const uint16_t *leafc, T *part, const T *P,
                        const double *pi16, const double *logw, int first, double *rootacc,
                        double *rootcomp, const CompBatch &cb = CompBatch());
 template <typename T>

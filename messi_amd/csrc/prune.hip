@@ -25,6 +25,8 @@
 // its parent's MFMA with no data movement.  Partials are stored in lane order
 // (position 4g + r holds state(g, r)) so every lane loads and stores 4
 // consecutive values of one entry.
+#include <type_traits>
+
 #include "common.h"
 #include "prune.h"
 
@@ -87,12 +89,89 @@ __device__ __forceinline__ int frexp_exp(float m) {
   return e;
 }
 
-// One wavefront per item of 16*NB consecutive entries of one node.
+// A stored 16-state entry carries its int32 power-of-two exponent inside its
+// values: every value is nonnegative and below 2 (the maximum is scaled into
+// [0.5, 1)), so the sign bit and the top exponent bit (bits 31 and 30 of a
+// float, 63 and 62 of a double) are zero, and value k of the entry (position
+// 4g + r, held by lane group g) holds exponent bits 2k and 2k+1 there.  One
+// random read then brings an entry's values and exponent together; clearing the
+// two bits gives back the values exactly.
+template <typename T> struct TopBits;
+template <> struct TopBits<float> {
+  static __device__ __forceinline__ unsigned get(float x) { return __float_as_uint(x) >> 30; }
+  static __device__ __forceinline__ float clear(float x) { return __uint_as_float(__float_as_uint(x) & 0x3FFFFFFFu); }
+  static __device__ __forceinline__ float put(float x, unsigned b) { return __uint_as_float(__float_as_uint(x) | (b << 30)); }
+};
+template <> struct TopBits<double> {
+  static __device__ __forceinline__ unsigned get(double x) {
+    return (unsigned)((unsigned long long)__double_as_longlong(x) >> 62);
+  }
+  static __device__ __forceinline__ double clear(double x) {
+    return __longlong_as_double((long long)((unsigned long long)__double_as_longlong(x) & 0x3FFFFFFFFFFFFFFFull));
+  }
+  static __device__ __forceinline__ double put(double x, unsigned b) {
+    return __longlong_as_double(
+        (long long)((unsigned long long)__double_as_longlong(x) | ((unsigned long long)b << 62)));
+  }
+};
+// This lane's 4 values of a loaded entry: their exponent bits, placed at bits
+// 8g .. 8g+7, and the values with those bits cleared.
+template <typename T>
+__device__ __forceinline__ unsigned take_exp_bits(T v[4], int g) {
+  unsigned w = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    w |= TopBits<T>::get(v[r]) << (2 * r);
+    v[r] = TopBits<T>::clear(v[r]);
+  }
+  return w << (8 * g);
+}
+// Lanes col + 16g (g = 0..3) of an entry sit in the four 16-lane rows.  A
+// gfx950 row swap of a register with itself returns its even and odd rows side
+// by side (v_permlane16_swap: rows 0/1 and 2/3; v_permlane32_swap: rows 0/2 and
+// 1/3), so two swaps combine the four rows on the VALU, with no LDS round trip.
+__device__ __forceinline__ unsigned rows_or(unsigned w) {
+  const auto a = __builtin_amdgcn_permlane16_swap(w, w, false, false);
+  w = a[0] | a[1];
+  const auto b = __builtin_amdgcn_permlane32_swap(w, w, false, false);
+  return b[0] | b[1];
+}
+template <typename T> __device__ __forceinline__ T rows_max(T m);
+template <> __device__ __forceinline__ float rows_max(float m) {
+  const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(m), __float_as_uint(m), false, false);
+  m = fmaxf(__uint_as_float(a[0]), __uint_as_float(a[1]));
+  const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(m), __float_as_uint(m), false, false);
+  return fmaxf(__uint_as_float(b[0]), __uint_as_float(b[1]));
+}
+template <> __device__ __forceinline__ double rows_max(double m) {
+  m = fmax(m, __shfl_xor(m, 16));
+  return fmax(m, __shfl_xor(m, 32));
+}
+// The entry's exponent from the four lane groups' bit fields.
+__device__ __forceinline__ int join_exp_bits(unsigned w) { return (int)rows_or(w); }
+// This lane's 4 values with exponent E's bits 8g .. 8g+7 put in.
+template <typename T>
+__device__ __forceinline__ void put_exp_bits(T v[4], int E, int g) {
+  const unsigned w = ((unsigned)E >> (8 * g)) & 0xFFu;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) v[r] = TopBits<T>::put(v[r], (w >> (2 * r)) & 3u);
+}
+
+// p advanced by a 32-bit byte offset: with p uniform, the access becomes a
+// scalar-base, 32-bit-lane-offset load or store (no 64-bit address per lane).
+template <class X> __device__ __forceinline__ X *boff(X *p, unsigned bytes) {
+  return reinterpret_cast<X *>(reinterpret_cast<typename std::conditional<std::is_const<X>::value, const char, char>::type *>(p) + bytes);
+}
+
+// One wavefront per item of 16*NB consecutive entries of one node.  The kernel
+// gathers child entries at random: its speed follows the number of loads in
+// flight, so the register budget is set for 6 wavefronts per SIMD in FP32 (80
+// VGPRs, a few spilled) and 4 in FP64 (123 VGPRs, none spilled).
 template <typename T, int NB>
-__global__ __launch_bounds__(256) void pair_level_kernel(
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 4) void pair_level_kernel(
     const int2 *__restrict__ items, int nitems, const NodeDesc *__restrict__ nodes,
     const int *__restrict__ idx, const uint16_t *__restrict__ leafc, T *__restrict__ part,
-    int *__restrict__ expo, const T *__restrict__ P, const double *__restrict__ pi16,
+    const T *__restrict__ P, const double *__restrict__ pi16,
     const double *__restrict__ logws, int first, double *__restrict__ rootacc,
     double *__restrict__ rootcomp, CompBatch cb) {
   typedef typename Mfma<T>::v4 v4;
@@ -100,15 +179,17 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
   {
     const int64_t c = blockIdx.y;  // component of this grid row
     part += c * cb.part;
-    expo += c * cb.expo;
     P += c * cb.P;
     pi16 += c * cb.pi;
     logws += c * cb.logw;
     rootacc += c * cb.root;
     rootcomp += c * cb.root;
   }
-  const int wave = (int)((xcd_chunk_id(blockIdx.x, gridDim.x) * (unsigned)blockDim.x + threadIdx.x) >> 6);
-  if (wave >= nitems) return;  // uniform per wavefront
+  // the wavefront's item, uniform: the item and its node descriptor come through
+  // scalar loads, and every gather below is a scalar base plus a 32-bit lane offset
+  const int wave = __builtin_amdgcn_readfirstlane(
+      (int)((xcd_chunk_id(blockIdx.x, gridDim.x) * (unsigned)blockDim.x + threadIdx.x) >> 6));
+  if (wave >= nitems) return;
   const int2 it = items[wave];
   const NodeDesc nd = nodes[it.x];
   const int col = lane & 15, g = lane >> 4;
@@ -126,50 +207,60 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
 #pragma unroll
     for (int r = 0; r < 4; ++r) pis[r] = pi16[16 * nd.member + Mfma<T>::state(g, r)];
   }
+  const unsigned U = (unsigned)nd.U, e0 = (unsigned)it.y;  // entries of one node fit in 32 bits
+  const int *__restrict__ il = idx + nd.idx_l;
+  const int *__restrict__ ir = idx + nd.idx_r;
+  const T *pl = part + (size_t)nd.cpart_l * 16, *pr = part + (size_t)nd.cpart_r * 16;
+  const uint16_t *lcl = leafc + nd.cpart_l, *lcr = leafc + nd.cpart_r;
   // Every load of the NB blocks is issued before the first store: part is both
   // read (children) and written (this node), so loads after a store could not
   // be hoisted above it.
-  int cl[NB], cr[NB];
+  unsigned cl[NB], cr[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int64_t e = (int64_t)it.y + b * 16 + col;
-    const int64_t ec = e < nd.U ? e : nd.U - 1;
-    cl[b] = idx[nd.idx_l + ec];
-    cr[b] = idx[nd.idx_r + ec];
+    const unsigned e = e0 + b * 16 + col;
+    const unsigned ec = e < U ? e : U - 1;
+    cl[b] = (unsigned)*boff(il, 4u * ec);
+    cr[b] = (unsigned)*boff(ir, 4u * ec);
   }
   T bl[NB][4], br[NB][4];
-  int el[NB], er[NB];
+  unsigned wl[NB], wr[NB];  // this lane's exponent bit fields of the two children (0 for a leaf)
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    el[b] = 0;
-    er[b] = 0;
+    wl[b] = 0;
+    wr[b] = 0;
     if (nd.leaf_l) {
-      const unsigned cc = leafc[nd.cpart_l + cl[b]];
+      const unsigned cc = *boff(lcl, 2u * cl[b]);
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int a = Mfma<T>::state(g, s);
         bl[b][s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
       }
     } else {
-      load4(part + (size_t)(nd.cpart_l + cl[b]) * 16 + 4 * g, bl[b]);
-      el[b] = expo[nd.cpart_l + cl[b]];
+      load4(boff(pl, (cl[b] * 16u + 4u * g) * (unsigned)sizeof(T)), bl[b]);
     }
     if (nd.leaf_r) {
-      const unsigned cc = leafc[nd.cpart_r + cr[b]];
+      const unsigned cc = *boff(lcr, 2u * cr[b]);
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         const int a = Mfma<T>::state(g, s);
         br[b][s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
       }
     } else {
-      load4(part + (size_t)(nd.cpart_r + cr[b]) * 16 + 4 * g, br[b]);
-      er[b] = expo[nd.cpart_r + cr[b]];
+      load4(boff(pr, (cr[b] * 16u + 4u * g) * (unsigned)sizeof(T)), br[b]);
     }
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int64_t e = (int64_t)it.y + b * 16 + col;
-    const bool valid = e < nd.U;
+    if (!nd.leaf_l) wl[b] = take_exp_bits<T>(bl[b], g);
+    if (!nd.leaf_r) wr[b] = take_exp_bits<T>(br[b], g);
+  }
+  T *po = part + (size_t)nd.part * 16;
+  double *rc = rootcomp + nd.out, *ra = rootacc + nd.out;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const unsigned e = e0 + b * 16 + col;
+    const bool valid = e < U;
     v4 ml = {0, 0, 0, 0}, mr = {0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -179,10 +270,8 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
     T v[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) v[r] = ml[r] * mr[r];
-    T m = fmax(fmax(v[0], v[1]), fmax(v[2], v[3]));
-    m = fmax(m, __shfl_xor(m, 16));
-    m = fmax(m, __shfl_xor(m, 32));
-    int E = el[b] + er[b];
+    const T m = rows_max<T>(fmax(fmax(v[0], v[1]), fmax(v[2], v[3])));
+    int E = join_exp_bits(wl[b]) + join_exp_bits(wr[b]);
     if (m > (T)0) {
       const int ex = frexp_exp(m);
       const T sc = ldexp((T)1, -ex);
@@ -191,10 +280,8 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
       E += ex;
     }
     if (!ROOT) {
-      if (valid) {
-        store4(part + (size_t)(nd.part + e) * 16 + 4 * g, v);
-        if (g == 0) expo[nd.part + e] = E;
-      }
+      put_exp_bits<T>(v, E, g);
+      if (valid) store4(boff(po, (e * 16u + 4u * g) * (unsigned)sizeof(T)), v);
     } else {
       double s = pis[0] * (double)v[0] + pis[1] * (double)v[1] + pis[2] * (double)v[2] +
                  pis[3] * (double)v[3];
@@ -202,10 +289,10 @@ __global__ __launch_bounds__(256) void pair_level_kernel(
       s += __shfl_xor(s, 32);
       if (valid && g == 0) {
         const double ll = log(s) + (double)E * 0.69314718055994530942;
-        const int64_t o = nd.out + e;
-        rootcomp[o] = ll;
+        *boff(rc, 8u * e) = ll;
         const double t = logws[nd.member] + ll;
-        rootacc[o] = first ? t : lse_d(rootacc[o], t);
+        double *const pa = boff(ra, 8u * e);
+        *pa = first ? t : lse_d(*pa, t);
       }
     }
   }
@@ -474,12 +561,12 @@ void launch_pair_gather(hipStream_t s, int L, int K, int64_t root_stride, const 
 
 template <typename T, int NB>
 void launch_pair_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes, const int *idx,
-                       const uint16_t *leafc, T *part, int *expo, const T *P, const double *pi16,
+                       const uint16_t *leafc, T *part, const T *P, const double *pi16,
                        const double *logw, int first, double *rootacc, double *rootcomp, const CompBatch &cb) {
   if (nitems <= 0) return;
   int blocks = (nitems + 3) / 4;
   hipLaunchKernelGGL((pair_level_kernel<T, NB>), dim3(blocks, cb.K), dim3(256), 0, s, items, nitems, nodes, idx,
-                     leafc, part, expo, P, pi16, logw, first, rootacc, rootcomp, cb);
+                     leafc, part, P, pi16, logw, first, rootacc, rootcomp, cb);
 }
 
 template <typename T>
@@ -501,10 +588,10 @@ void launch_pair_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, con
 }
 
 template void launch_pair_level<double, PAIR_NB>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
-                                                 const uint16_t *, double *, int *, const double *, const double *,
+                                                 const uint16_t *, double *, const double *, const double *,
                                                  const double *, int, double *, double *, const CompBatch &);
 template void launch_pair_level<float, PAIR_NB>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
-                                                const uint16_t *, float *, int *, const float *, const double *,
+                                                const uint16_t *, float *, const float *, const double *,
                                                 const double *, int, double *, double *, const CompBatch &);
 template void launch_site_level<double>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
                                         const uint8_t *, double *, int *, const double *, const double *,
