@@ -481,6 +481,23 @@ __device__ __forceinline__ void tile_products(const Operands<SR> &o0, bool do0, 
   __syncthreads();
 }
 
+// Cell positions inside a tile of the product-ready copies and of the bulk
+// accumulator.  With T = 16 they follow V_MFMA_F32_16X16X4_F32's lane order, so
+// a wavefront computing a product loads each operand and its accumulator with
+// one contiguous vector per lane: lane l, register q takes A[l & 15][4q + l/16]
+// and B[4q + l/16][l & 15], and holds result row 4(l/16) + q of column l & 15.
+// With T = 32 the copies stay row-major (A) / column-major (B), the accumulator
+// row-major.
+template <int T> __host__ __device__ constexpr int op_a_at(int i, int k) {
+  return T == 16 ? (i + 16 * (k & 3)) * 4 + (k >> 2) : i * T + k;
+}
+template <int T> __host__ __device__ constexpr int op_b_at(int k, int j) {
+  return T == 16 ? (j + 16 * (k & 3)) * 4 + (k >> 2) : j * T + k;
+}
+template <int T, bool MF> __host__ __device__ constexpr int acc_at(int a, int b) {
+  return (MF && T == 16) ? (b + 16 * (a >> 2)) * 4 + (a & 3) : a * T + b;
+}
+
 // Scaled-path operands of one product read from the product-ready copies: this
 // thread's (r, c) = (tid / T, tid % T) element of each, as aligned float mantissas,
 // and the exponent of its row of A and of its column r of B.
@@ -497,8 +514,8 @@ __device__ __forceinline__ PackedOps fetch_packed(const InstDesc<XFv> &d, int I,
   const int tid = threadIdx.x, r = tid / T;
   const unsigned ta = (unsigned)(I * d.NT + K), tb = (unsigned)(K * d.NT + J);
   PackedOps o;
-  o.a = gp(d.Lp)[ta * (T * T) + tid];
-  o.b = gp(d.Sp)[tb * (T * T) + tid];
+  o.a = gp(d.Lp)[ta * (T * T) + op_a_at<T>(r, tid % T)];
+  o.b = gp(d.Sp)[tb * (T * T) + op_b_at<T>(tid % T, r)];
   o.ea = gp(d.LpE)[ta * T + r];
   o.eb = gp(d.SpE)[tb * T + r];
   return o;
@@ -739,6 +756,79 @@ __device__ __forceinline__ void tile_products_packed4(const InstDesc<XFv> &d, in
   __syncthreads();
 }
 
+// Bulk items handled by one workgroup of the scaled T = 16 launch sequence: one
+// per wavefront.
+constexpr int WAVE_ITEMS = 4;
+
+// One bulk visit of the scaled path with T = 16, done by one wavefront in
+// registers: bulk item idx of launch t (the same tile, products and order as
+// the workgroup form in inside_item).  Each lane loads its operand vectors of a
+// product (op_a_at / op_b_at order), runs the four MFMA steps, and folds the
+// product pairs of each launch into its four accumulator cells (acc_at order:
+// rows 4(l/16) .. 4(l/16)+3 of column l & 15) with the workgroup form's
+// packed_term / XF::add3 arithmetic, so the results are bit-identical.  No LDS,
+// no barrier.  The accumulator and product-ready tiles are whole tiles, so all
+// 256 cells are read and written with no bounds checks (cells beyond the table
+// are never used).
+template <int W>
+__device__ __forceinline__ void bulk_wave_item(const InstDesc<XFv> &ins, int idx, int t) {
+  constexpr int T = 16;
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  const int NT = ins.NT;
+  const int nd = sched::bulk_dists(NT, t, W, ins.KB);
+  if (idx >= bulk_items(NT, t, W, nd)) return;
+  int I, J;
+  sched::bulk_tile(NT, t, W, nd, idx, I, J);
+  const int smin = max(t - W + 1, (J - I + 3) / 2);  // a product of launch s needs dist <= 2s - 2
+  MESSI_GLOBAL XFv *acc = gp(ins.acc) + (unsigned)(I * NT + J) * (T * T) + lane * 4;
+  XFv av[4];
+  if (had_visit(t, J - I, W, ins.KB, false)) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) av[q] = XFv(acc[q]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) av[q] = XF::zero();
+  }
+  // the product (I, K) x (K, J) for this lane's cells, as raw XF terms
+  auto product = [&](int K, XFv out[4]) {
+    const unsigned ta = (unsigned)(I * NT + K), tb = (unsigned)(K * NT + J);
+    const float4 A = *reinterpret_cast<const MESSI_GLOBAL float4 *>(gp(ins.Lp) + ta * (T * T) + lane * 4);
+    const float4 B = *reinterpret_cast<const MESSI_GLOBAL float4 *>(gp(ins.Sp) + tb * (T * T) + lane * 4);
+    const int4 ra = *reinterpret_cast<const MESSI_GLOBAL int4 *>(gp(ins.LpE) + ta * T + 4 * g);
+    const int cb = gp(ins.SpE)[tb * T + col];
+    v4f m = {0.f, 0.f, 0.f, 0.f};
+    m = __builtin_amdgcn_mfma_f32_16x16x4f32(A.x, B.x, m, 0, 0, 0);
+    m = __builtin_amdgcn_mfma_f32_16x16x4f32(A.y, B.y, m, 0, 0, 0);
+    m = __builtin_amdgcn_mfma_f32_16x16x4f32(A.z, B.z, m, 0, 0, 0);
+    m = __builtin_amdgcn_mfma_f32_16x16x4f32(A.w, B.w, m, 0, 0, 0);
+    const int ras[4] = {ra.x, ra.y, ra.z, ra.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const bool live = ras[q] != XF::EZ && cb != XF::EZ;
+      const float cv = m[q];
+      out[q] = live ? XFv{cv, ras[q] + cb} : XF::zero();
+      if (live && !(cv >= 0x1p-100f)) out[q] = exact_cell_global<T>(ins, I, K, J, 4 * g + q, col);
+    }
+  };
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const int s = t - W + 1 + i, k1 = I + s - 1, k2 = J - (s - 1);
+    const bool d1 = s >= smin && k1 - I <= ins.KB;
+    const bool d2 = s >= smin && k2 != k1 && k2 - I <= ins.KB;
+    if (!(d1 || d2)) continue;
+    XFv t1[4], t2[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) t1[q] = t2[q] = XF::zero();
+    if (d1) product(k1, t1);
+    if (d2) product(k2, t2);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) av[q] = XF::add3(av[q], t1[q], t2[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = av[q];  // every visit writes
+}
+
 // One work item of the inside schedule (see the file header): the finalisation
 // of tile (item, item+t) of launch t, or bulk workgroup `item` of launch t.
 // With P (the single-launch form) the item first waits for what it reads and
@@ -797,7 +887,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
     for (int u = 0; u < BP; ++u) {
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
       av[u] = (has[u] && had_visit(t, Jb[u] - Ib[u], W, ins.KB, false) && j < L && y < L)
-                  ? typename SR::V(acc[(unsigned)(Ib[u] * NT + Jb[u]) * (T * T) + tid])
+                  ? typename SR::V(acc[(unsigned)(Ib[u] * NT + Jb[u]) * (T * T) + acc_at<T, SR::scaled>(a, b)])
                   : SR::zero();
 #pragma unroll
       for (int i = 0; i < W; ++i) {
@@ -843,7 +933,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
         }
       }
       const int j = Ib[u] * T + a, y = Jb[u] * T + b;
-      if (j < L && y < L) acc[(unsigned)(Ib[u] * NT + Jb[u]) * (T * T) + tid] = av[u];  // every visit writes
+      if (j < L && y < L) acc[(unsigned)(Ib[u] * NT + Jb[u]) * (T * T) + acc_at<T, SR::scaled>(a, b)] = av[u];  // every visit writes
     }
     if constexpr (P)
       wg_signal(has[0] ? visit_done + (size_t)Ib[0] * NT + Jb[0] : nullptr, t + 1,
@@ -872,7 +962,8 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
   V accv = SR::zero();
   {
     const int a = tid / T, b = tid % T, j = IT + a, y = JT + b;
-    if (had_visit(t, t, W, ins.KB, true) && j < L && y < L) accv = acc[(unsigned)(I * NT + J) * (T * T) + tid];
+    if (had_visit(t, t, W, ins.KB, true) && j < L && y < L)
+      accv = acc[(unsigned)(I * NT + J) * (T * T) + acc_at<T, SR::scaled>(a, b)];
   }
   Operands<SR> o1, o2;
   PackedOps q1, q2;
@@ -1218,7 +1309,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       const int e = group_max<T>(ok && v.m != 0.0f ? v.e : XF::EZ);
       // every lane writes (zero beyond the table), so products read whole tiles
       // with no bounds checks
-      gp(ins.Lp)[((size_t)I * NT + J) * (T * T) + tid] = !ok || v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+      gp(ins.Lp)[((size_t)I * NT + J) * (T * T) + op_a_at<T>(a, b)] = !ok || v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
       if (b == 0) gp(ins.LpE)[((size_t)I * NT + J) * T + a] = e;
     }
     // B operand: the column segment of S over rows IT+1 .. IT+T (the last one is
@@ -1228,7 +1319,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       XFv v = XF::zero();
       if (row < L && y < L && row <= y) v = c + 1 < T ? sm.at(WL::cs(c + 1, b)) : sm.Sb[b];
       const int e = group_max<T>(v.m != 0.0f ? v.e : XF::EZ);
-      gp(ins.Sp)[((size_t)I * NT + J) * (T * T) + tid] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+      gp(ins.Sp)[((size_t)I * NT + J) * (T * T) + op_b_at<T>(c, b)] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
       if (c == 0) gp(ins.SpE)[((size_t)I * NT + J) * T + b] = e;
     }
   }
@@ -1260,6 +1351,13 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
     int mid = (lo + hi) >> 1;
     if (pre[mid] <= bx) lo = mid;
     else hi = mid;
+  }
+  if constexpr (SR::scaled && T == 16) {
+    // bulk items one per wavefront (WAVE_ITEMS per workgroup), in registers
+    if (!fin_wg) {
+      bulk_wave_item<W>(insts[lo], WAVE_ITEMS * (bx - pre[lo]) + (int)(threadIdx.x >> 6), t);
+      return;
+    }
   }
   inside_item<SR, T, W, false>(insts[lo], fin_wg, bx - pre[lo], t, r, sm, nullptr);
 }
@@ -1841,7 +1939,10 @@ int inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &i
         if (t - 1 > KB) dmax = std::min(dmax, t - 1 + KB);
         for (int dist = t + 1; dist <= dmax; dist += W) nbulk += NT - dist;
         fa += NT - t;
-        ba += (nbulk + bulk_per(W) - 1) / bulk_per(W);
+        // bulk workgroups: WAVE_ITEMS items each in the scaled T = 16 launch
+        // sequence, bulk_per(W) in the other forms
+        const int per = (SR::scaled && TE == 16 && !persist) ? WAVE_ITEMS : bulk_per(W);
+        ba += (nbulk + per - 1) / per;
       }
     }
     fp[n] = fa;
