@@ -1436,6 +1436,9 @@ template <class SR, int T> struct OutLDS {
     } w;   // wavefront phase
   } u;
   V Qtop[T], Qright[T];
+  double Qtd[T], Qrd[T];  // scaled path, double wavefront: Qtop and Qright at the tile's scale
+  int escale;             // scaled path, double wavefront: the tile's power-of-two scale
+  int bad;                // scaled path, double wavefront: a value left the safe range
 };
 
 template <class SR, int T, int W>
@@ -1537,22 +1540,52 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   // The cells are given to the threads in order of their finalisation step (as
   // in the inside step); the products' sums reach the cell's new thread through
   // its W and Q slots, which only the cell's own finalisation overwrites (W and
-  // Q of a cell are read only once it is final).
+  // Q of a cell are read only once it is final).  The scaled path runs the
+  // wavefront in double precision at one power-of-two scale per tile (the
+  // largest exponent of the tile's border Q values), as the inside step does;
+  // a tile whose values would leave [2^-1000, 2^1000], or one of whose nonzero
+  // product terms would fall below 2^-1000, is recomputed by the
+  // per-cell-exponent wavefront.
   constexpr bool by_step = true;
-  {
+  adbl *const DW = reinterpret_cast<adbl *>(&sm.u.w.W[0][0]);
+  adbl *const DQ = reinterpret_cast<adbl *>(&sm.u.w.Q[0][0]);
+  adbl *const DiS = reinterpret_cast<adbl *>(&sm.u.w.iSd[0][0]);
+  adbl *const DiL = reinterpret_cast<adbl *>(&sm.u.w.iLd[0][0]);
+  adbl *const DcP = reinterpret_cast<adbl *>(&sm.u.w.cP[0][0]);
+  constexpr int RSO = T + 1;  // row stride of the wavefront arrays
+  const bool dbl = SR::scaled;
+  bool dok = true;
+  // iS / iL of the tile's own columns and rows, the pair term, and the products'
+  // sums handed to the cells' new threads; as unscaled doubles for the double
+  // wavefront (as_d), else in the representation itself
+  auto stage_tables = [&](bool as_d) {
     const int a = a0, b = b0, j = j0, y = y0;
+    auto put = [&](V &slot, adbl *darr, V v) {
+      if constexpr (SR::scaled) {
+        if (as_d) {
+          darr[a * RSO + b] = xf_to_d(v, 0, dok);
+          return;
+        }
+      }
+      slot = v;
+    };
     const int rr = JT + a + 1, cc = JT + b;  // iSd[a][b] = iS[JT+a+1, JT+b]
-    sm.u.w.iSd[a][b] = (rr < L && cc < L && rr <= cc) ? typename SR::V(iSr[(size_t)rr * L + cc]) : SR::zero();
+    put(sm.u.w.iSd[a][b], DiS, (rr < L && cc < L && rr <= cc) ? typename SR::V(iSr[(size_t)rr * L + cc]) : SR::zero());
     const int lr = IT + a, lc = IT + b - 1;  // iLd[a][b] = iL[IT+a, IT+b-1]
-    sm.u.w.iLd[a][b] = (lc >= 0 && lr < L && lc < L && lr <= lc) ? typename SR::V(iLc[(size_t)lc * L + lr]) : SR::zero();
-    sm.u.w.cP[a][b] = (j >= 1 && j <= y && y + 1 < L) ? typename SR::V(ptv[(size_t)(j - 1) * L + (y + 1)]) : SR::zero();
+    put(sm.u.w.iLd[a][b], DiL,
+        (lc >= 0 && lr < L && lc < L && lr <= lc) ? typename SR::V(iLc[(size_t)lc * L + lr]) : SR::zero());
+    put(sm.u.w.cP[a][b], DcP,
+        (j >= 1 && j <= y && y + 1 < L) ? typename SR::V(ptv[(size_t)(j - 1) * L + (y + 1)]) : SR::zero());
+    // the hand-over stays in the representation (converted by the new thread)
     sm.u.w.W[a][b] = by_step ? vL : SR::zero();
     sm.u.w.Q[a][b] = by_step ? vS : SR::zero();
-  }
+  };
+  stage_tables(dbl);
+  if (tid == 0) sm.bad = 0;
+  V qt = SR::zero(), qr = SR::zero();
   if (tid < T) {
     // Q on the row above the tile (IT-1, JT+b+1) and the column right of it (IT+b-1, JT+T)
     const int bq = tid;
-    V qt = SR::zero(), qr = SR::zero();
     const int tr = IT - 1, tc = JT + bq + 1;
     if (tr >= 0 && tc < L) {
       const size_t c = (size_t)tr * L + tc;
@@ -1566,25 +1599,36 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
     sm.Qtop[bq] = qt;
     sm.Qright[bq] = qr;
   }
+  if constexpr (SR::scaled) {
+    if (tid < 64) {
+      // the tile's scale from its nonzero border values (0 when all are zero)
+      int e = INT_MIN / 2;
+      if (tid < T) {
+        if (qt.m != 0.0f) e = max(e, qt.e);
+        if (qr.m != 0.0f) e = max(e, qr.e);
+      }
+      int E = border_emax<T>(e);
+      if (E < -(1 << 27)) E = 0;
+      if (tid < T) {
+        sm.Qtd[tid] = xf_to_d(qt, E, dok);
+        sm.Qrd[tid] = xf_to_d(qr, E, dok);
+      }
+      if (tid == 0) sm.escale = E;
+    }
+  }
   __syncthreads();
   int a = a0, b = b0;
   if constexpr (by_step) out_cell_by_step<T>(tid, a, b);
   const int j = IT + a, y = JT + b;
-  typename SR::Acc acL = SR::acc(by_step ? sm.u.w.W[a][b] : vL);
-  typename SR::Acc acS = SR::acc(by_step ? sm.u.w.Q[a][b] : vS);
   const bool mine = j < L && y < L && j <= y;
   const bool lkeep = out_row_ok(d, T, od.lband);
   const int own = (T - 1 - b) + a;
-  V fS = SR::zero(), fF = SR::zero(), fL = SR::zero();
   // Q[j-1, y+1] for the pair term: inside this tile, or on the row above / the
   // column right of it.  The inner cell (a-1, b+1) is finalised two steps before
   // this cell (own - 2; own >= 2 for it): every step reads the lane's Q cell
   // (inner or border, a fixed LDS position) for use in the next step, off the
   // finalisation's path.
   const bool q_in = a >= 1 && b + 1 < T;
-  const V *qsrc = q_in ? &sm.u.w.Q[a - 1][b + 1] : (a == 0 ? &sm.Qtop[b] : &sm.Qright[a]);
-  const V pterm = sm.u.w.cP[a][b];
-  V q_next = *qsrc;  // primed: a border cell may finalise at step 0 or 1 (see the inside step)
   // the last step with work for this wavefront (scalar: a uniform skip, one barrier a step)
   int wlast;
   if constexpr (by_step) {
@@ -1595,44 +1639,119 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
     wlast = 2 * T - 2;
   }
   const int wave_last = __builtin_amdgcn_readfirstlane(wlast);
-  auto work = [&](int st) {
-    const V q_f = q_next;
-    q_next = *qsrc;
-    const bool act = mine && st >= 1 && st <= own, fn = mine && st == own;
-    V w1 = SR::zero(), s1 = SR::zero(), l2 = SR::zero(), w2 = SR::zero();
-    if (act) {
-      const int bs = T - st + a;  // row producer (a, bs), finalised at step st-1
-      if (bs > b && bs < T) {
-        w1 = sm.u.w.W[a][bs];
-        s1 = sm.u.w.iSd[b][bs];
-      }
-      const int as = st - T + b;  // column producer (as, b), finalised at step st-1
-      if (as >= 0 && as < a) {
-        l2 = sm.u.w.iLd[as][a];
-        w2 = sm.u.w.W[as][b];
-      }
-      SR::acc_add2(acL, SR::mul_raw(w1, s1), SR::zero());
-      SR::acc_add2(acS, SR::mul_raw(l2, w2), SR::zero());
-    }
-    if (fn) {
-      fL = lkeep ? SR::acc_value(acL) : SR::zero();
-      fS = SR::acc_value(acS);
-      fF = SR::mul(pterm, q_f);
-      if (j == 0 && y == L - 1) fS = SR::from_log(0.0);  // the start symbol
-      sm.u.w.W[a][b] = SR::add(SR::mulc(fS, r.p[0]), SR::mulc(fF, r.p[4]));
-      sm.u.w.Q[a][b] = SR::add(SR::add(SR::mulc(fS, r.p[2]), SR::mulc(fF, r.p[3])), SR::mulc(fL, r.p[6]));
-    }
-  };
-  auto step = [&](int st) {
-    if (st <= wave_last) work(st);
-    __syncthreads();
-  };
-  // as in the inside step: the FP64 step stays a rolled loop
+  V fS = SR::zero(), fF = SR::zero(), fL = SR::zero(), fW = SR::zero();
+  bool xf_wave = !dbl;
   if constexpr (SR::scaled) {
-    for (int st = 0; st <= 2 * T - 2; ++st) step(st);
-  } else {
+    // ---- double wavefront
+    const int E = sm.escale;
+    double acL = xf_to_d(sm.u.w.W[a][b], E, dok), acS = xf_to_d(sm.u.w.Q[a][b], E, dok);
+    if (!mine) dok = true;  // only cells of the table are checked (the others read zeros)
+    bool ok = true;
+    const adbl *qsrc = q_in ? &DQ[(a - 1) * RSO + b + 1]
+                            : (a == 0 ? reinterpret_cast<const adbl *>(&sm.Qtd[b])
+                                      : reinterpret_cast<const adbl *>(&sm.Qrd[a]));
+    const double pterm = DcP[a * RSO + b];
+    const double p0 = r.p[0], p2 = r.p[2], p3 = r.p[3], p4 = r.p[4], p6 = r.p[6];
+    double q_next = *qsrc;
+    double dS = 0.0, dF = 0.0, dL = 0.0, dW = 0.0, dQ = 0.0;
+    // a nonzero term of nonzero factors below 2^-1000 (or flushed to zero)
+    auto tiny = [](double x, double y, double t) { return x != 0.0 && y != 0.0 && !(t >= D_LO); };
+    for (int st = 0; st <= 2 * T - 2; ++st) {
+      if (st <= wave_last) {
+        const double q_f = q_next;
+        q_next = *qsrc;
+        const bool act = mine && st >= 1 && st <= own, fn = mine && st == own;
+        if (act) {
+          const int bs = T - st + a;  // row producer (a, bs), finalised at step st-1
+          if (bs > b && bs < T) {
+            const double w1 = DW[a * RSO + bs], s1 = DiS[b * RSO + bs], t1 = w1 * s1;
+            ok = ok && !tiny(w1, s1, t1);
+            acL += t1;
+          }
+          const int as = st - T + b;  // column producer (as, b), finalised at step st-1
+          if (as >= 0 && as < a) {
+            const double l2 = DiL[as * RSO + a], w2 = DW[as * RSO + b], t2 = l2 * w2;
+            ok = ok && !tiny(l2, w2, t2);
+            acS += t2;
+          }
+        }
+        if (fn) {
+          dL = lkeep ? acL : 0.0;
+          dS = acS;
+          dF = pterm * q_f;
+          ok = ok && !tiny(pterm, q_f, dF);
+          if (j == 0 && y == L - 1) dS = ldexp(1.0, -E);  // the start symbol
+          dW = p0 * dS + p4 * dF;
+          dQ = p2 * dS + p3 * dF + p6 * dL;
+          DW[a * RSO + b] = dW;
+          DQ[a * RSO + b] = dQ;
+        }
+      }
+      __syncthreads();
+    }
+    if (mine)
+      dok = dok && ok && d_in_range(dS) && d_in_range(dF) && d_in_range(dL) && d_in_range(dW) && d_in_range(dQ);
+    if (!dok) atomicOr(&sm.bad, 1);
+    fS = d_to_xf(dS, E);
+    fF = d_to_xf(dF, E);
+    fL = d_to_xf(dL, E);
+    fW = d_to_xf(dW, E);
+    __syncthreads();
+    xf_wave = sm.bad != 0;
+    if (xf_wave) {
+      // restage the tables and the hand-over in the representation itself
+      stage_tables(false);
+      __syncthreads();
+    }
+  }
+  if (xf_wave) {
+    // ---- per-cell-exponent wavefront
+    typename SR::Acc acL = SR::acc(by_step ? sm.u.w.W[a][b] : vL);
+    typename SR::Acc acS = SR::acc(by_step ? sm.u.w.Q[a][b] : vS);
+    fS = fF = fL = SR::zero();
+    const V *qsrc = q_in ? &sm.u.w.Q[a - 1][b + 1] : (a == 0 ? &sm.Qtop[b] : &sm.Qright[a]);
+    const V pterm = sm.u.w.cP[a][b];
+    V q_next = *qsrc;  // primed: a border cell may finalise at step 0 or 1 (see the inside step)
+    auto work = [&](int st) {
+      const V q_f = q_next;
+      q_next = *qsrc;
+      const bool act = mine && st >= 1 && st <= own, fn = mine && st == own;
+      V w1 = SR::zero(), s1 = SR::zero(), l2 = SR::zero(), w2 = SR::zero();
+      if (act) {
+        const int bs = T - st + a;  // row producer (a, bs), finalised at step st-1
+        if (bs > b && bs < T) {
+          w1 = sm.u.w.W[a][bs];
+          s1 = sm.u.w.iSd[b][bs];
+        }
+        const int as = st - T + b;  // column producer (as, b), finalised at step st-1
+        if (as >= 0 && as < a) {
+          l2 = sm.u.w.iLd[as][a];
+          w2 = sm.u.w.W[as][b];
+        }
+        SR::acc_add2(acL, SR::mul_raw(w1, s1), SR::zero());
+        SR::acc_add2(acS, SR::mul_raw(l2, w2), SR::zero());
+      }
+      if (fn) {
+        fL = lkeep ? SR::acc_value(acL) : SR::zero();
+        fS = SR::acc_value(acS);
+        fF = SR::mul(pterm, q_f);
+        if (j == 0 && y == L - 1) fS = SR::from_log(0.0);  // the start symbol
+        fW = SR::add(SR::mulc(fS, r.p[0]), SR::mulc(fF, r.p[4]));
+        sm.u.w.W[a][b] = fW;
+        sm.u.w.Q[a][b] = SR::add(SR::add(SR::mulc(fS, r.p[2]), SR::mulc(fF, r.p[3])), SR::mulc(fL, r.p[6]));
+      }
+    };
+    auto step = [&](int st) {
+      if (st <= wave_last) work(st);
+      __syncthreads();
+    };
+    // as in the inside step: the FP64 step stays a rolled loop
+    if constexpr (SR::scaled) {
+      for (int st = 0; st <= 2 * T - 2; ++st) step(st);
+    } else {
 #pragma unroll 1
-    for (int st = 0; st <= 2 * T - 2; ++st) step(st);
+      for (int st = 0; st <= 2 * T - 2; ++st) step(st);
+    }
   }
   // ---- write back: row-major from registers, column-major through LDS
   if (mine) {
@@ -1640,9 +1759,10 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
     oSr[rc] = fS;
     oFr[rc] = fF;
     oL[rc] = fL;
-    Wr[rc] = sm.u.w.W[a][b];
+    Wr[rc] = fW;
   }
   sm.u.w.Q[a][b] = fS;
+  sm.u.w.W[a][b] = fW;
   __syncthreads();
   {
     const int bb = tid / T, aa = tid % T, jj = IT + aa, yy = JT + bb;

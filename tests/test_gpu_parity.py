@@ -206,6 +206,18 @@ def test_fp32_extreme_pair_scores(ctx, score):
         err = float(np.max(np.abs(t[fin] - ins[fin])))
         _say(f"tile {tile}: log Z rel err {abs(z - zr) / abs(zr):.2e}, max table log err {err:.2e}")
         assert err < 1e-4, tile
+    # the outside pass from the oracle's inside table (its in-tile wavefront has the
+    # same double / per-cell-exponent split) and the posteriors
+    out = oracle.outside(ins, u, pm)
+    res, pair = oracle.bpp(ins, out, u, pm)
+    o32 = lib.outside(ins, u, pm, prec="fp32")
+    _rel(o32, out)
+    fin = np.isfinite(out)
+    oerr = float(np.max(np.abs(o32[fin] - out[fin])))
+    r32, p32 = lib.bpp(ins, o32, u, pm, prec="fp32")
+    berr = max(float(np.max(np.abs(r32 - res))), float(np.max(np.abs(p32 - pair))))
+    _say(f"outside max log err {oerr:.2e}, posterior max abs err {berr:.2e}")
+    assert oerr < 1e-4 and berr < 1e-5
 
 
 def test_config2_scale_properties(ctx):
