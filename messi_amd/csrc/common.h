@@ -123,6 +123,9 @@ struct InsideWork {
   DevBuf Lp, LpE, Sp, SpE;        // scaled path: product-ready operand copies (see inside.hip)
   DevBuf oSr, oSc, oFr, oL;       // outside tables
   DevBuf oWr, oWc, oaL, oaS;      // outside: parent weight W (row/column-major), accumulators
+  // scaled path, tile 16: product-ready operands of the outside's bulk products
+  // (W as A and B operand, inside S and L shifted by one, see inside.hip)
+  DevBuf oWpA, oWpAE, oWpB, oWpBE, iSpB, iSpBE, iLpA, iLpAE;
   DevBuf orng;                    // outside launch tables
   int orng_key[5] = {0, 0, 0, 0, 0};  // (L, tile, band, lband, window) the launch tables were built for
   DevBuf ptv;                     // scaled pair term per cell

@@ -1410,6 +1410,14 @@ struct OutDesc {
   const void *iSr, *iLc, *ptv;              // inside S (row-major), L (column-major), pair term
   void *oSr, *oSc, *oFr, *oL, *Wr, *Wc;     // outside tables and W (row- and column-major)
   void *accL, *accS;                        // bulk accumulators (row-major)
+  // Scaled path with T = 16: product-ready operands of the bulk products, tile-major
+  // in MFMA lane order (op_a_at / op_b_at), one exponent per A row or B column.
+  //   WpA (I, K): W tile (I, K) as the A operand of a row product, rows aligned
+  //   WpB (K, J): W tile (K, J) as the B operand of a column product, columns aligned
+  //   iSpB (J, K): B[c][b] = iS[JT+b+1, KT+c] (row product), columns b aligned
+  //   iLpA (K, I): A[a][c] = iL[KT+c, IT+a-1] (column product), rows a aligned
+  float *WpA, *WpB, *iSpB, *iLpA;
+  int *WpAE, *WpBE, *iSpBE, *iLpAE;
 };
 
 // Per launch u and tile distance d < NT-1-u: the bulk tiles (I, I+d) with a
@@ -1441,6 +1449,162 @@ template <class SR, int T> struct OutLDS {
   int bad;                // scaled path, double wavefront: a value left the safe range
 };
 
+// Product-ready copies of the inside tables for the outside's bulk products
+// (scaled path, T = 16), one workgroup per tile pair p < q: iSpB (p, q) and
+// iLpA (p, q) (see OutDesc), aligned as tile_products aligns the operands it
+// stages (largest nonzero exponent of a B column / an A row), so a product from
+// them has the same bits.
+__global__ __launch_bounds__(256) void out_operands_kernel(OutDesc od) {
+  constexpr int T = 16;
+  const int p = blockIdx.y, q = blockIdx.x;
+  if (p >= q) return;
+  const int L = od.L, NT = od.NT, tid = threadIdx.x, hi = tid / T, lo = tid % T;
+  const MESSI_GLOBAL XFv *iSr = gp(static_cast<const XFv *>(od.iSr));
+  const MESSI_GLOBAL XFv *iLc = gp(static_cast<const XFv *>(od.iLc));
+  const unsigned tile = (unsigned)(p * NT + q);
+  {
+    // B[c][b] = iS[pT + b + 1, qT + c] (c = lo, b = hi), columns b aligned
+    const int c = lo, b = hi, row = p * T + b + 1, col = q * T + c;
+    const XFv v = (row < L && col < L && row <= col) ? XFv(iSr[(size_t)row * L + col]) : XF::zero();
+    const int e = group_max<T>(v.m != 0.0f ? v.e : XF::EZ);
+    gp(od.iSpB)[tile * (T * T) + op_b_at<T>(c, b)] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+    if (c == 0) gp(od.iSpBE)[tile * T + b] = e;
+  }
+  {
+    // A[a][c] = iL[pT + c, qT + a - 1] (a = hi, c = lo), rows a aligned
+    const int a = hi, c = lo, row = p * T + c, col = q * T + a - 1;
+    const XFv v = (col >= 0 && row < L && col < L && row <= col) ? XFv(iLc[(size_t)col * L + row]) : XF::zero();
+    const int e = group_max<T>(v.m != 0.0f ? v.e : XF::EZ);
+    gp(od.iLpA)[tile * (T * T) + op_a_at<T>(a, c)] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+    if (c == 0) gp(od.iLpAE)[tile * T + a] = e;
+  }
+}
+
+// Exact sums of one cell of an outside bulk product from the tables (the rare
+// cell whose aligned MFMA sum lost its significant terms), as exact_cell_xf
+// computes them from staged operands: the row product (I, K) x (J, K)^T,
+//   sum_c W[IT+a, KT+c] iS[JT+b+1, KT+c],
+// and the column product,
+//   sum_c iL[KT+c, IT+a-1] W[KT+c, JT+b].
+template <int T>
+__device__ __forceinline__ XFv exact_out_cell(const OutDesc &od, bool row_product, int I, int K, int J, int a, int b) {
+  const int L = od.L;
+  const MESSI_GLOBAL XFv *Wr = gp(static_cast<const XFv *>(od.Wr));
+  const MESSI_GLOBAL XFv *Wc = gp(static_cast<const XFv *>(od.Wc));
+  const MESSI_GLOBAL XFv *iSr = gp(static_cast<const XFv *>(od.iSr));
+  const MESSI_GLOBAL XFv *iLc = gp(static_cast<const XFv *>(od.iLc));
+  auto av = [&](int c) -> XFv {
+    if (row_product) {
+      const int r = I * T + a, k = K * T + c;
+      return (r < L && k < L && r <= k) ? XFv(Wr[(size_t)r * L + k]) : XF::zero();
+    }
+    const int r = K * T + c, k = I * T + a - 1;  // iL[r, k], column-major
+    return (k >= 0 && r < L && k < L && r <= k) ? XFv(iLc[(size_t)k * L + r]) : XF::zero();
+  };
+  auto bv = [&](int c) -> XFv {
+    if (row_product) {
+      const int r = J * T + b + 1, k = K * T + c;  // iS[r, k]
+      return (r < L && k < L && r <= k) ? XFv(iSr[(size_t)r * L + k]) : XF::zero();
+    }
+    const int r = K * T + c, k = J * T + b;  // W[r, k], column-major copy
+    return (r < L && k < L && r <= k) ? XFv(Wc[(size_t)k * L + r]) : XF::zero();
+  };
+  int km = XF::kmin();
+#pragma unroll 1
+  for (int c = 0; c < T; ++c) km = max(km, XF::key(av(c), bv(c)));
+  float sum = 0.0f;
+#pragma unroll 1
+  for (int c = 0; c < T; ++c) sum += XF::contrib(av(c), bv(c), km);
+  return XF::finish(km, sum);
+}
+
+// One bulk visit of the outside's scaled path with T = 16, by one wavefront in
+// registers (the tile, products and order of the workgroup form in
+// outside_step_kernel, with tile_products' arithmetic, so the results are
+// bit-identical): bulk item idx of launch u.  Lane l holds cells (4(l/16) + q,
+// l & 15), q = 0..3, of both accumulators.
+template <int W>
+__device__ __forceinline__ void out_bulk_wave_item(const OutDesc &od, const OutRange *rg, int u, int idx) {
+  constexpr int T = 16;
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  const int L = od.L, NT = od.NT, du = NT - 1 - u;
+  if (du < 1) return;
+  const OutRange last = rg[du - 1];
+  if (idx >= last.cum + last.rend + (last.cend - last.cbeg)) return;
+  int lo = 0, hi = du - 1;  // largest distance with cum <= idx
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (rg[mid].cum <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  const int d = lo, i0 = idx - rg[d].cum;
+  const int I = i0 < rg[d].rend ? i0 : rg[d].cbeg + (i0 - rg[d].rend);
+  const int m = du + 1 - d, J = I + d;
+  unsigned rows = 0, cols = 0;  // bit i: offset m + W - 1 - i has a row / column product
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const int mi = m + W - 1 - i;
+    if (out_row_ok(d, T, od.lband) && J + mi <= NT - 1) rows |= 1u << i;
+    if (out_col_ok(mi, T, od.band) && I - mi >= 0) cols |= 1u << i;
+  }
+  MESSI_GLOBAL XFv *accL = gp(static_cast<XFv *>(od.accL));
+  MESSI_GLOBAL XFv *accS = gp(static_cast<XFv *>(od.accS));
+  const int y = J * T + col;
+  XFv vL[4], vS[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = I * T + 4 * g + q;
+    const bool cell = j < L && y < L;
+    vL[q] = (cell && rows) ? XFv(accL[(size_t)j * L + y]) : XF::zero();
+    vS[q] = (cell && cols) ? XFv(accS[(size_t)j * L + y]) : XF::zero();
+  }
+  // the product of A (I-side) and B (J-side) tiles, folded into acc (XF::add per cell)
+  auto product = [&](const float *Ab, const int *AE, const float *Bb, const int *BE, unsigned ta, unsigned tb,
+                     bool row_product, int K, XFv acc[4]) {
+    const float4 A = *reinterpret_cast<const MESSI_GLOBAL float4 *>(gp(Ab) + ta * (T * T) + lane * 4);
+    const float4 B = *reinterpret_cast<const MESSI_GLOBAL float4 *>(gp(Bb) + tb * (T * T) + lane * 4);
+    const int4 ra = *reinterpret_cast<const MESSI_GLOBAL int4 *>(gp(AE) + ta * T + 4 * g);
+    const int cb = gp(BE)[tb * T + col];
+    v4f mm = {0.f, 0.f, 0.f, 0.f};
+    mm = __builtin_amdgcn_mfma_f32_16x16x4f32(A.x, B.x, mm, 0, 0, 0);
+    mm = __builtin_amdgcn_mfma_f32_16x16x4f32(A.y, B.y, mm, 0, 0, 0);
+    mm = __builtin_amdgcn_mfma_f32_16x16x4f32(A.z, B.z, mm, 0, 0, 0);
+    mm = __builtin_amdgcn_mfma_f32_16x16x4f32(A.w, B.w, mm, 0, 0, 0);
+    const int ras[4] = {ra.x, ra.y, ra.z, ra.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      XFv contrib = XF::zero();
+      if (ras[q] != XF::EZ && cb != XF::EZ) {
+        const float cv = mm[q];
+        contrib = cv >= 0x1p-100f ? XF::norm(cv, ras[q] + cb)
+                                  : exact_out_cell<T>(od, row_product, I, K, J, 4 * g + q, col);
+      }
+      acc[q] = XF::add(acc[q], contrib);
+    }
+  };
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const int mi = m + W - 1 - i;
+    if ((rows >> i) & 1u) {
+      const int K = J + mi;
+      product(od.WpA, od.WpAE, od.iSpB, od.iSpBE, (unsigned)(I * NT + K), (unsigned)(J * NT + K), true, K, vL);
+    }
+    if ((cols >> i) & 1u) {
+      const int K = I - mi;
+      product(od.iLpA, od.iLpAE, od.WpB, od.WpBE, (unsigned)(K * NT + I), (unsigned)(K * NT + J), false, K, vS);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = I * T + 4 * g + q;
+    if (j < L && y < L) {
+      if (rows) accL[(size_t)j * L + y] = vL[q];
+      if (cols) accS[(size_t)j * L + y] = vS[q];
+    }
+  }
+}
+
 template <class SR, int T, int W>
 __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const OutRange *__restrict__ ranges,
                                                              int u, Rules<SR> r) {
@@ -1465,6 +1629,13 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   int I, d, m;
   bool fin;
   const int item = (int)blockIdx.x;
+  if constexpr (SR::scaled && T == 16) {
+    // bulk items one per wavefront (WAVE_ITEMS per workgroup), in registers
+    if (item > u) {
+      out_bulk_wave_item<W>(od, ranges + (size_t)u * NT, u, WAVE_ITEMS * (item - (u + 1)) + (int)(threadIdx.x >> 6));
+      return;
+    }
+  }
   if (item <= u) {
     fin = true;
     I = item;
@@ -1772,6 +1943,26 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
       Wc[cc] = sm.u.w.W[aa][bb];
     }
   }
+  if constexpr (SR::scaled && T == 16) {
+    // product-ready copies of the finished W tile (see OutDesc), aligned as
+    // tile_products aligns the W operands it stages; whole tiles, zero beyond
+    // the table and below the diagonal
+    const unsigned tile = (unsigned)(I * NT + J);
+    {
+      const int ra = tid / T, rc = tid % T;  // A: element (ra, rc), rows aligned
+      const XFv v = (IT + ra < L && JT + rc < L && IT + ra <= JT + rc) ? sm.u.w.W[ra][rc] : XF::zero();
+      const int e = group_max<T>(v.m != 0.0f ? v.e : XF::EZ);
+      gp(od.WpA)[tile * (T * T) + op_a_at<T>(ra, rc)] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+      if (rc == 0) gp(od.WpAE)[tile * T + ra] = e;
+    }
+    {
+      const int c = tid % T, bb = tid / T;  // B: element (c, bb), columns aligned
+      const XFv v = (IT + c < L && JT + bb < L && IT + c <= JT + bb) ? sm.u.w.W[c][bb] : XF::zero();
+      const int e = group_max<T>(v.m != 0.0f ? v.e : XF::EZ);
+      gp(od.WpB)[tile * (T * T) + op_b_at<T>(c, bb)] = v.m == 0.0f ? 0.0f : ldexpf(v.m, max(v.e - e, -200));
+      if (c == 0) gp(od.WpBE)[tile * T + bb] = e;
+    }
+  }
 }
 
 // Zero the outside accumulators (upper triangle).
@@ -1941,6 +2132,17 @@ void inside_alloc(InsideWork &w, int L, bool outside) {
     w.oaL.ensure(n);
     w.oaS.ensure(n);
     w.lp.ensure(sizeof(double) * L * L);
+    if (SR::scaled) {
+      const size_t c16 = n16 * n16 * 256, r16 = n16 * n16 * 16;
+      w.oWpA.ensure(sizeof(float) * c16);
+      w.oWpB.ensure(sizeof(float) * c16);
+      w.iSpB.ensure(sizeof(float) * c16);
+      w.iLpA.ensure(sizeof(float) * c16);
+      w.oWpAE.ensure(sizeof(int) * r16);
+      w.oWpBE.ensure(sizeof(int) * r16);
+      w.iSpBE.ensure(sizeof(int) * r16);
+      w.iLpAE.ensure(sizeof(int) * r16);
+    }
   }
 }
 
@@ -2216,7 +2418,8 @@ void outside_run(hipStream_t s, InsideWork &w, int L, bool full_L) {
       if (!fresh) rg[(size_t)u * NT + d] = OutRange{cum, rend, cbeg, cend};
       cum += rend + (cend - cbeg);
     }
-    total[u] = (u + 1) + cum;
+    // bulk workgroups: WAVE_ITEMS items each in the scaled T = 16 form, one otherwise
+    total[u] = (u + 1) + ((SR::scaled && TE == 16) ? (cum + WAVE_ITEMS - 1) / WAVE_ITEMS : cum);
   }
   if (!fresh) {
     upload(w.orng, rg, s);
@@ -2238,8 +2441,20 @@ void outside_run(hipStream_t s, InsideWork &w, int L, bool full_L) {
   od.Wc = w.oWc.p;
   od.accL = w.oaL.p;
   od.accS = w.oaS.p;
+  od.WpA = w.oWpA.as<float>();
+  od.WpB = w.oWpB.as<float>();
+  od.iSpB = w.iSpB.as<float>();
+  od.iLpA = w.iLpA.as<float>();
+  od.WpAE = w.oWpAE.as<int>();
+  od.WpBE = w.oWpBE.as<int>();
+  od.iSpBE = w.iSpBE.as<int>();
+  od.iLpAE = w.iLpAE.as<int>();
   hipLaunchKernelGGL(outside_tiled_init_kernel<SR>, dim3((L + 255) / 256, L), dim3(256), 0, s, L, w.oaL.as<V>(),
                      w.oaS.as<V>());
+  if (SR::scaled && TE == 16) {
+    MESSI_REQUIRE(w.oWpA.p && w.iSpB.p, "outside: product-ready buffers not allocated");
+    hipLaunchKernelGGL(out_operands_kernel, dim3(NT, NT), dim3(256), 0, s, od);
+  }
   const OutRange *dr = w.orng.as<OutRange>();
   for (int u = 0; u < NT; ++u) {
     auto go = [&](auto kern, int threads) { hipLaunchKernelGGL(kern, dim3(total[u]), dim3(threads), 0, s, od, dr, u, r); };
