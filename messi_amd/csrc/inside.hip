@@ -256,6 +256,8 @@ template <class SR, int T> struct StepLDS {
   } m;
   V Sb[T], Fbot[T], Fleft[T];
   double Fbd[T], Fld[T];  // scaled path, double wavefront: Fbot and Fleft at the tile's scale
+  // scaled path, T = 16: the finalising workgroup's two products (MFMA results)
+  float Rd[2][(SR::scaled && T == 16) ? 16 : 1][17];
   int flag;    // a dependency wait's outcome (single-launch form)
   int escale;  // scaled path, double wavefront: the tile's power-of-two scale
   int bad;     // scaled path, double wavefront: a value left the safe range
@@ -967,7 +969,32 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
   }
   Operands<SR> o1, o2;
   PackedOps q1, q2;
-  if constexpr (SR::scaled) {
+  // Scaled path with T = 16: the two products run on wavefronts 0 and 1 straight
+  // from the product-ready tiles (MFMA lane order: one float4 per operand and
+  // lane), with no LDS staging of operands; every thread loads the row and column
+  // exponents of its own cell.
+  constexpr bool direct = SR::scaled && T == 16;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  float4 dA = {0.f, 0.f, 0.f, 0.f}, dB = {0.f, 0.f, 0.f, 0.f};
+  int dra[2] = {0, 0}, dcb[2] = {0, 0};
+  const bool dmine = wv == 0 ? do1 : (wv == 1 ? do2 : false);  // this wavefront runs a product
+  if constexpr (direct) {
+    const int r = tid / T, c = tid % T;
+    if (dmine) {
+      const int K = wv == 0 ? K1 : K2;
+      const unsigned ta = (unsigned)(I * NT + K), tb = (unsigned)(K * NT + J);
+      dA = *reinterpret_cast<const MESSI_GLOBAL float4 *>(gp(ins.Lp) + ta * (T * T) + (tid & 63) * 4);
+      dB = *reinterpret_cast<const MESSI_GLOBAL float4 *>(gp(ins.Sp) + tb * (T * T) + (tid & 63) * 4);
+    }
+    if (do1) {
+      dra[0] = gp(ins.LpE)[(unsigned)(I * NT + K1) * T + r];
+      dcb[0] = gp(ins.SpE)[(unsigned)(K1 * NT + J) * T + c];
+    }
+    if (do2) {
+      dra[1] = gp(ins.LpE)[(unsigned)(I * NT + K2) * T + r];
+      dcb[1] = gp(ins.SpE)[(unsigned)(K2 * NT + J) * T + c];
+    }
+  } else if constexpr (SR::scaled) {
     if (do1) q1 = fetch_packed<T>(ins, I, K1, J);
     if (do2) q2 = fetch_packed<T>(ins, I, K2, J);
   } else {
@@ -1046,7 +1073,47 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       }
     }
   }
-  if (do1 || do2) {
+  if constexpr (direct) {
+    // nothing shares the wavefront's LDS area with the products here, so its zero
+    // regions and bottom border row are written now, under the loads' latency
+    if (t > 0) {
+      if (!dbl) {
+        for (int k = tid; k < WL::N_ZERO_WAVE; k += T * T) sm.at(WL::zero_wave(k)) = SR::zero();
+        if (tid < T) sm.at(WL::cs(T, tid)) = sm.Sb[tid];
+      } else {
+        for (int k = tid; k < WL::N_ZERO_WAVE; k += T * T) D[WL::zero_wave(k)] = 0.0;
+        if (tid < T) D[WL::cs(T, tid)] = sbd;
+      }
+    }
+    if (dmine) {
+      typedef float v4f __attribute__((ext_vector_type(4)));
+      const int lane = tid & 63;
+      v4f mm = {0.f, 0.f, 0.f, 0.f};
+      mm = __builtin_amdgcn_mfma_f32_16x16x4f32(dA.x, dB.x, mm, 0, 0, 0);
+      mm = __builtin_amdgcn_mfma_f32_16x16x4f32(dA.y, dB.y, mm, 0, 0, 0);
+      mm = __builtin_amdgcn_mfma_f32_16x16x4f32(dA.z, dB.z, mm, 0, 0, 0);
+      mm = __builtin_amdgcn_mfma_f32_16x16x4f32(dA.w, dB.w, mm, 0, 0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sm.Rd[wv][4 * (lane >> 4) + q][lane & 15] = mm[q];
+    }
+    __syncthreads();
+    if (do1 || do2) {
+      // each thread folds its cell as tile_products_packed does (packed_term, XF::add3)
+      const int r = tid / T, c = tid % T;
+      XFv tk[2];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        tk[k] = XF::zero();
+        if (k == 0 ? do1 : do2) {
+          const bool live = dra[k] != XF::EZ && dcb[k] != XF::EZ;
+          const float cv = sm.Rd[k][r][c];
+          tk[k] = live ? XFv{cv, dra[k] + dcb[k]} : XF::zero();
+          if (live && !(cv >= 0x1p-100f)) tk[k] = exact_cell_global<T>(ins, I, k == 0 ? K1 : K2, J, r, c);
+        }
+      }
+      accv = XF::add3(accv, tk[0], tk[1]);
+    }
+  } else if (do1 || do2) {
     if constexpr (SR::scaled) {
       tile_products_packed<T>(ins, I, J, q1, K1, do1, q2, K2, do2, sm.m.s.q.p, accv);
     } else {
@@ -1069,7 +1136,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       sm.at(WL::cl(a, b)) = d ? typename SR::V(Lr[(size_t)j * L + j]) : SR::zero();
       sm.at(WL::cf(a, b)) = SR::zero();
     }
-  } else {
+  } else if constexpr (!direct) {
     if (by_step) sm.at(WL::cf(tid / T, tid % T)) = accv;  // to the thread that takes the cell below
     // the zero regions of the wavefront's reads and the bottom border row as row T of cS
     if (!dbl) {
@@ -1080,7 +1147,9 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       if (tid < T) D[WL::cs(T, tid)] = sbd;
     }
   }
-  __syncthreads();
+  // (the direct form's staging was ordered by its barrier above, except the
+  // diagonal tiles' cells just written)
+  if (!direct || t == 0) __syncthreads();
   // One thread per cell.  A cell's split-point sum is accumulated online: at
   // step s it adds the (at most two) terms whose newest operand was finalised at
   // step s-1, and it finalises itself at its own step.  In the FP64 path
