@@ -1010,7 +1010,23 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
   const bool dbl = SR::scaled && t > 0;
   adbl *const D = reinterpret_cast<adbl *>(sm.m.all);
   bool dok = true;
-  auto stage_tables = [&](bool as_d) {
+  // The staged tables of this thread's cell: the diagonal tiles (I, I) and
+  // (J, J) and the pair term.  The loads use clamped indices (element 0 of the
+  // table for a cell outside it) and select zero afterwards, so they issue with
+  // no branch or wait between them and their latencies overlap; the LDS writes
+  // come after every load.
+  struct StagedCell {
+    V ld, cp, sd;
+  };
+  auto load_tables = [&]() {
+    const int a = tid / T, b = tid % T, j = IT + a, y = JT + b, jd = IT + b, ys = JT + a;
+    const bool k1 = j < L && jd < L && jd >= j, k2 = j < L && y < L, k3 = ys < L && b <= a;
+    const V x1 = typename SR::V(Lr[k1 ? (size_t)j * L + jd : 0]);   // L[j, jd] of tile (I, I)
+    const V x2 = typename SR::V(ptv[k2 ? (size_t)j * L + y : 0]);   // pair term of the cell
+    const V x3 = typename SR::V(Sc[k3 ? (size_t)ys * L + y : 0]);   // S[JT+b, JT+a] of tile (J, J)
+    return StagedCell{k1 ? x1 : SR::zero(), k2 ? x2 : SR::zero(), k3 ? x3 : SR::zero()};
+  };
+  auto put_tables = [&](const StagedCell &g, bool as_d) {
     auto put = [&](int i, V v) {
       if constexpr (SR::scaled) {
         if (as_d) {
@@ -1020,34 +1036,34 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       }
       sm.at(i) = v;
     };
-    for (int idx = tid; idx < T * T; idx += T * T) {
-      int a = idx / T, b = idx % T;
-      int j = IT + a, y = JT + b;
-      int jd = IT + b;  // column index inside the diagonal tile (I, I)
-      put(WL::ld(a, b), (j < L && jd < L && jd >= j) ? typename SR::V(Lr[(size_t)j * L + jd]) : SR::zero());
-      put(WL::cp(a, b), (j < L && y < L) ? typename SR::V(ptv[(size_t)j * L + y]) : SR::zero());
-    }
-    for (int idx = tid; idx < T * T; idx += T * T) {
-      int b = idx / T, c = idx % T;  // S[JT+c, JT+b], c fastest along the column
-      int y = JT + b, rr = JT + c;
-      put(WL::sd(c, b), (y < L && rr <= y) ? typename SR::V(Sc[(size_t)y * L + rr]) : SR::zero());
-    }
+    const int a = tid / T, b = tid % T;
+    put(WL::ld(a, b), g.ld);
+    put(WL::cp(a, b), g.cp);
+    put(WL::sd(b, a), g.sd);
     for (int k = tid; k < WL::N_ZERO_STAGED; k += T * T) put(WL::zero_staged(k), SR::zero());
   };
   double sbd = 0.0;  // double wavefront: this lane's bottom-border S at the tile's scale (lanes < T)
   if (fin) {
-    stage_tables(dbl);
-    V sb = SR::zero(), fb = SR::zero(), fl = SR::zero();
+    const StagedCell g = load_tables();
+    // the border values (lanes < T): S of the tile below, F below and to the left
+    V sb, fb, fl;
+    {
+      const int b = tid, y = JT + b, rb = (I + 1) * T, yc = JT + b - 1, ja = IT + b + 1, yl = JT - 1;
+      const bool lane = tid < T && t >= 1;
+      const bool k1 = lane && y < L && rb <= y, k2 = lane && rb < L && yc < L && rb <= yc,
+                 k3 = lane && ja < L && yl >= 0 && ja <= yl;
+      const V x1 = typename SR::V(Sc[k1 ? (size_t)y * L + rb : 0]);
+      const V x2 = typename SR::V(F[k2 ? (size_t)rb * L + yc : 0]);
+      const V x3 = typename SR::V(F[k3 ? (size_t)ja * L + yl : 0]);
+      sb = k1 ? x1 : SR::zero();
+      fb = k2 ? x2 : SR::zero();
+      fl = k3 ? x3 : SR::zero();
+    }
+    put_tables(g, dbl);
     if (tid < T) {
-      int b = tid, y = JT + b, rb = (I + 1) * T;
-      sb = (t >= 1 && y < L && rb <= y) ? typename SR::V(Sc[(size_t)y * L + rb]) : SR::zero();
-      int yc = JT + b - 1;
-      fb = (t >= 1 && rb < L && yc < L && rb <= yc) ? typename SR::V(F[(size_t)rb * L + yc]) : SR::zero();
-      int ja = IT + b + 1, yl = JT - 1;
-      fl = (t >= 1 && ja < L && yl >= 0 && ja <= yl) ? typename SR::V(F[(size_t)ja * L + yl]) : SR::zero();
-      sm.Sb[b] = sb;
-      sm.Fbot[b] = fb;
-      sm.Fleft[b] = fl;
+      sm.Sb[tid] = sb;
+      sm.Fbot[tid] = fb;
+      sm.Fleft[tid] = fl;
     }
     if (tid == 0) sm.bad = 0;
     if constexpr (SR::scaled) {
@@ -1262,7 +1278,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
       xf_wave = sm.bad != 0;
       if (xf_wave) {
         // restage the tables, zero regions and bottom border as XF values
-        stage_tables(false);
+        put_tables(load_tables(), false);
         for (int k = tid; k < WL::N_ZERO_WAVE; k += T * T) sm.at(WL::zero_wave(k)) = SR::zero();
         if (tid < T) sm.at(WL::cs(T, tid)) = sm.Sb[tid];
         __syncthreads();
