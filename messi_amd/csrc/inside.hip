@@ -1811,11 +1811,28 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   constexpr int RSO = T + 1;  // row stride of the wavefront arrays
   const bool dbl = SR::scaled;
   bool dok = true;
-  // iS / iL of the tile's own columns and rows, the pair term, and the products'
-  // sums handed to the cells' new threads; as unscaled doubles for the double
-  // wavefront (as_d), else in the representation itself
-  auto stage_tables = [&](bool as_d) {
-    const int a = a0, b = b0, j = j0, y = y0;
+  // iS / iL of the tile's own columns and rows and the pair term, loaded with
+  // clamped indices (element 0 of the table for a cell outside it) and zero
+  // selected afterwards, so that these loads and the border loads below issue
+  // together, before the first LDS write; staged with the products' sums handed
+  // to the cells' new threads, as unscaled doubles for the double wavefront
+  // (as_d), else in the representation itself
+  struct StagedCell {
+    V iS, iL, cP;
+  };
+  auto load_tables = [&]() {
+    const int j = j0, y = y0;
+    const int rr = JT + a0 + 1, cc = JT + b0;  // iSd[a][b] = iS[JT+a+1, JT+b]
+    const int lr = IT + a0, lc = IT + b0 - 1;  // iLd[a][b] = iL[IT+a, IT+b-1]
+    const bool k1 = rr < L && cc < L && rr <= cc, k2 = lc >= 0 && lr < L && lc < L && lr <= lc,
+               k3 = j >= 1 && j <= y && y + 1 < L;
+    const V x1 = typename SR::V(iSr[k1 ? (size_t)rr * L + cc : 0]);
+    const V x2 = typename SR::V(iLc[k2 ? (size_t)lc * L + lr : 0]);
+    const V x3 = typename SR::V(ptv[k3 ? (size_t)(j - 1) * L + (y + 1) : 0]);
+    return StagedCell{k1 ? x1 : SR::zero(), k2 ? x2 : SR::zero(), k3 ? x3 : SR::zero()};
+  };
+  auto put_tables = [&](const StagedCell &g, bool as_d) {
+    const int a = a0, b = b0;
     auto put = [&](V &slot, adbl *darr, V v) {
       if constexpr (SR::scaled) {
         if (as_d) {
@@ -1825,35 +1842,31 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
       }
       slot = v;
     };
-    const int rr = JT + a + 1, cc = JT + b;  // iSd[a][b] = iS[JT+a+1, JT+b]
-    put(sm.u.w.iSd[a][b], DiS, (rr < L && cc < L && rr <= cc) ? typename SR::V(iSr[(size_t)rr * L + cc]) : SR::zero());
-    const int lr = IT + a, lc = IT + b - 1;  // iLd[a][b] = iL[IT+a, IT+b-1]
-    put(sm.u.w.iLd[a][b], DiL,
-        (lc >= 0 && lr < L && lc < L && lr <= lc) ? typename SR::V(iLc[(size_t)lc * L + lr]) : SR::zero());
-    put(sm.u.w.cP[a][b], DcP,
-        (j >= 1 && j <= y && y + 1 < L) ? typename SR::V(ptv[(size_t)(j - 1) * L + (y + 1)]) : SR::zero());
+    put(sm.u.w.iSd[a][b], DiS, g.iS);
+    put(sm.u.w.iLd[a][b], DiL, g.iL);
+    put(sm.u.w.cP[a][b], DcP, g.cP);
     // the hand-over stays in the representation (converted by the new thread)
     sm.u.w.W[a][b] = by_step ? vL : SR::zero();
     sm.u.w.Q[a][b] = by_step ? vS : SR::zero();
   };
-  stage_tables(dbl);
-  if (tid == 0) sm.bad = 0;
+  const StagedCell g = load_tables();
+  // Q on the row above the tile (IT-1, JT+b+1) and the column right of it
+  // (IT+b-1, JT+T), lanes b < T
   V qt = SR::zero(), qr = SR::zero();
+  {
+    const int bq = tid, tr = IT - 1, tc = JT + bq + 1, rr = IT + bq - 1, rc = JT + T;
+    const bool kt = tid < T && tr >= 0 && tc < L, kr = tid < T && bq >= 1 && rr < L && rc < L;
+    const size_t ct = kt ? (size_t)tr * L + tc : 0, cr = kr ? (size_t)rr * L + rc : 0;
+    const V s1 = typename SR::V(oSr[ct]), f1 = typename SR::V(oFr[ct]), l1 = typename SR::V(oL[ct]);
+    const V s2 = typename SR::V(oSr[cr]), f2 = typename SR::V(oFr[cr]), l2 = typename SR::V(oL[cr]);
+    if (kt) qt = SR::add(SR::add(SR::mulc(s1, r.p[2]), SR::mulc(f1, r.p[3])), SR::mulc(l1, r.p[6]));
+    if (kr) qr = SR::add(SR::add(SR::mulc(s2, r.p[2]), SR::mulc(f2, r.p[3])), SR::mulc(l2, r.p[6]));
+  }
+  put_tables(g, dbl);
+  if (tid == 0) sm.bad = 0;
   if (tid < T) {
-    // Q on the row above the tile (IT-1, JT+b+1) and the column right of it (IT+b-1, JT+T)
-    const int bq = tid;
-    const int tr = IT - 1, tc = JT + bq + 1;
-    if (tr >= 0 && tc < L) {
-      const size_t c = (size_t)tr * L + tc;
-      qt = SR::add(SR::add(SR::mulc(oSr[c], r.p[2]), SR::mulc(oFr[c], r.p[3])), SR::mulc(oL[c], r.p[6]));
-    }
-    const int rr = IT + bq - 1, rc = JT + T;
-    if (bq >= 1 && rr < L && rc < L) {
-      const size_t c = (size_t)rr * L + rc;
-      qr = SR::add(SR::add(SR::mulc(oSr[c], r.p[2]), SR::mulc(oFr[c], r.p[3])), SR::mulc(oL[c], r.p[6]));
-    }
-    sm.Qtop[bq] = qt;
-    sm.Qright[bq] = qr;
+    sm.Qtop[tid] = qt;
+    sm.Qright[tid] = qr;
   }
   if constexpr (SR::scaled) {
     if (tid < 64) {
@@ -1956,7 +1969,7 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
     xf_wave = sm.bad != 0;
     if (xf_wave) {
       // restage the tables and the hand-over in the representation itself
-      stage_tables(false);
+      put_tables(load_tables(), false);
       __syncthreads();
     }
   }
