@@ -792,38 +792,57 @@ __device__ __forceinline__ void bulk_wave_item(const InstDesc<XFv> &ins, int idx
 #pragma unroll
     for (int q = 0; q < 4; ++q) av[q] = XF::zero();
   }
-  // the product (I, K) x (K, J) for this lane's cells, as raw XF terms
-  auto product = [&](int K, XFv out[4]) {
+  // the operands of product (I, K) x (K, J) for this lane: its float4 of each
+  // tile and the exponents of its rows of A and its column of B
+  struct Ops {
+    float4 A, B;
+    int4 ra;
+    int cb;
+  };
+  auto load = [&](int K) {
     const unsigned ta = (unsigned)(I * NT + K), tb = (unsigned)(K * NT + J);
-    const float4 A = *reinterpret_cast<const MESSI_GLOBAL float4 *>(gp(ins.Lp) + ta * (T * T) + lane * 4);
-    const float4 B = *reinterpret_cast<const MESSI_GLOBAL float4 *>(gp(ins.Sp) + tb * (T * T) + lane * 4);
-    const int4 ra = *reinterpret_cast<const MESSI_GLOBAL int4 *>(gp(ins.LpE) + ta * T + 4 * g);
-    const int cb = gp(ins.SpE)[tb * T + col];
+    Ops o;
+    o.A = *reinterpret_cast<const MESSI_GLOBAL float4 *>(gp(ins.Lp) + ta * (T * T) + lane * 4);
+    o.B = *reinterpret_cast<const MESSI_GLOBAL float4 *>(gp(ins.Sp) + tb * (T * T) + lane * 4);
+    o.ra = *reinterpret_cast<const MESSI_GLOBAL int4 *>(gp(ins.LpE) + ta * T + 4 * g);
+    o.cb = gp(ins.SpE)[tb * T + col];
+    return o;
+  };
+  // the product for this lane's cells, as raw XF terms
+  auto product = [&](const Ops &o, int K, XFv out[4]) {
     v4f m = {0.f, 0.f, 0.f, 0.f};
-    m = __builtin_amdgcn_mfma_f32_16x16x4f32(A.x, B.x, m, 0, 0, 0);
-    m = __builtin_amdgcn_mfma_f32_16x16x4f32(A.y, B.y, m, 0, 0, 0);
-    m = __builtin_amdgcn_mfma_f32_16x16x4f32(A.z, B.z, m, 0, 0, 0);
-    m = __builtin_amdgcn_mfma_f32_16x16x4f32(A.w, B.w, m, 0, 0, 0);
-    const int ras[4] = {ra.x, ra.y, ra.z, ra.w};
+    m = __builtin_amdgcn_mfma_f32_16x16x4f32(o.A.x, o.B.x, m, 0, 0, 0);
+    m = __builtin_amdgcn_mfma_f32_16x16x4f32(o.A.y, o.B.y, m, 0, 0, 0);
+    m = __builtin_amdgcn_mfma_f32_16x16x4f32(o.A.z, o.B.z, m, 0, 0, 0);
+    m = __builtin_amdgcn_mfma_f32_16x16x4f32(o.A.w, o.B.w, m, 0, 0, 0);
+    const int ras[4] = {o.ra.x, o.ra.y, o.ra.z, o.ra.w};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const bool live = ras[q] != XF::EZ && cb != XF::EZ;
+      const bool live = ras[q] != XF::EZ && o.cb != XF::EZ;
       const float cv = m[q];
-      out[q] = live ? XFv{cv, ras[q] + cb} : XF::zero();
+      out[q] = live ? XFv{cv, ras[q] + o.cb} : XF::zero();
       if (live && !(cv >= 0x1p-100f)) out[q] = exact_cell_global<T>(ins, I, K, J, 4 * g + q, col);
     }
   };
+  // each launch's pair of products; both products' loads issue before either
+  // is used, so a launch's products cost one memory round trip
 #pragma unroll
   for (int i = 0; i < W; ++i) {
     const int s = t - W + 1 + i, k1 = I + s - 1, k2 = J - (s - 1);
     const bool d1 = s >= smin && k1 - I <= ins.KB;
     const bool d2 = s >= smin && k2 != k1 && k2 - I <= ins.KB;
     if (!(d1 || d2)) continue;
+    // one launch's operands in registers at a time (the next launch's loads are
+    // not hoisted above this one's products)
+    __builtin_amdgcn_sched_barrier(0);
+    Ops o1, o2;
+    if (d1) o1 = load(k1);
+    if (d2) o2 = load(k2);
     XFv t1[4], t2[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) t1[q] = t2[q] = XF::zero();
-    if (d1) product(k1, t1);
-    if (d2) product(k2, t2);
+    if (d1) product(o1, k1, t1);
+    if (d2) product(o2, k2, t2);
 #pragma unroll
     for (int q = 0; q < 4; ++q) av[q] = XF::add3(av[q], t1[q], t2[q]);
   }
@@ -1411,11 +1430,12 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
   if constexpr (P) wg_signal(tile_done + (size_t)I * NT + J, 1, nullptr, 0, diag_done + t);
 }
 
-// One launch of the inside schedule (see the file header).
+// One launch of the inside schedule (see the file header).  The scaled T = 16
+// form holds seven workgroups per CU (its LDS allows seven; the register budget
+// is set to match, 72 per lane).
 template <class SR, int T, int W>
-__global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
-                                                            const int *__restrict__ prefix, int n, int t,
-                                                            Rules<SR> r) {
+__global__ __launch_bounds__(T * T, (SR::scaled && T == 16) ? 7 : 1) void inside_step_kernel(
+    const InstDesc<typename SR::V> *__restrict__ insts, const int *__restrict__ prefix, int n, int t, Rules<SR> r) {
   __shared__ StepLDS<SR, T> sm;
   static_assert(sizeof(StepStaged<SR, T>) == sizeof(typename SR::V) * wave::Layout<T>::STAGED +
                                                  sizeof(StepProducts<SR, T>),
@@ -1440,7 +1460,7 @@ __global__ __launch_bounds__(T * T) void inside_step_kernel(const InstDesc<typen
   if constexpr (SR::scaled && T == 16) {
     // bulk items one per wavefront (WAVE_ITEMS per workgroup), in registers
     if (!fin_wg) {
-      bulk_wave_item<W>(insts[lo], WAVE_ITEMS * (bx - pre[lo]) + (int)(threadIdx.x >> 6), t);
+      bulk_wave_item<W>(insts[lo], WAVE_ITEMS * (bx - pre[lo]) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), t);
       return;
     }
   }
@@ -1717,7 +1737,8 @@ __global__ __launch_bounds__(T * T) void outside_step_kernel(OutDesc od, const O
   if constexpr (SR::scaled && T == 16) {
     // bulk items one per wavefront (WAVE_ITEMS per workgroup), in registers
     if (item > u) {
-      out_bulk_wave_item<W>(od, ranges + (size_t)u * NT, u, WAVE_ITEMS * (item - (u + 1)) + (int)(threadIdx.x >> 6));
+      out_bulk_wave_item<W>(od, ranges + (size_t)u * NT, u,
+                            WAVE_ITEMS * (item - (u + 1)) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)));
       return;
     }
   }
