@@ -257,6 +257,11 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 4) void pair_level_kernel
   }
   T *po = part + (size_t)nd.part * 16;
   double *rc = rootcomp + nd.out, *ra = rootacc + nd.out;
+  // root: lane group b keeps block b's sum and exponent, so the logarithms of the
+  // wavefront's 16 NB entries run once, on all lanes, after the blocks
+  static_assert(NB <= 4, "one lane group per block at the root");
+  double rs = 1.0;
+  int rE = 0;
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const unsigned e = e0 + b * 16 + col;
@@ -287,13 +292,20 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 4) void pair_level_kernel
                  pis[3] * (double)v[3];
       s += __shfl_xor(s, 16);
       s += __shfl_xor(s, 32);
-      if (valid && g == 0) {
-        const double ll = log(s) + (double)E * 0.69314718055994530942;
-        *boff(rc, 8u * e) = ll;
-        const double t = logws[nd.member] + ll;
-        double *const pa = boff(ra, 8u * e);
-        *pa = first ? t : lse_d(*pa, t);
+      if (g == b) {
+        rs = s;
+        rE = E;
       }
+    }
+  }
+  if (ROOT) {
+    const unsigned e = e0 + g * 16 + col;
+    if (g < NB && e < U) {
+      const double ll = log(rs) + (double)rE * 0.69314718055994530942;
+      *boff(rc, 8u * e) = ll;
+      const double t = logws[nd.member] + ll;
+      double *const pa = boff(ra, 8u * e);
+      *pa = first ? t : lse_d(*pa, t);
     }
   }
 }
