@@ -899,6 +899,7 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
       for (int m = m0; m < m1; ++m)
         if (ws[m]->tile == TE) {
           ws[m]->exp = b->dexp_pair.as<ExpandDesc>() + m;
+          ws[m]->dset = b->mem[m];
           g.push_back(ws[m]);
         }
       if (g.empty()) continue;
@@ -944,6 +945,7 @@ void eval_impl(messi_batch *b, double B, bool keep_outside, double *logZ) {
   }
   for (int m = 0; m < b->n; ++m) {
     ws[m]->exp = nullptr;
+    ws[m]->dset = nullptr;
     ws[m]->zout = nullptr;
   }
   record(b, 3);

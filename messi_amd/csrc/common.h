@@ -106,7 +106,8 @@ struct InsideBatch {
   DevBuf prefix;  // per launch t: first workgroup of each instance (n+1 ints)
   DevBuf ptab;    // single-launch form: ticket table
   DevBuf pflags;  // single-launch form: ticket counter, give-up flag, then every instance's flags
-  std::vector<unsigned char> h_inst, h_prefix, h_ptab;  // what inst, prefix and ptab hold (uploads only on change)
+  DevBuf groups;  // runs of consecutive members on one dataset (first member, count)
+  std::vector<unsigned char> h_inst, h_prefix, h_ptab, h_groups;  // what the tables hold (uploads only on change)
 };
 
 // Scratch of the inside/outside/bpp phase, per precision.
@@ -117,6 +118,7 @@ struct InsideWork {
   bool persist = false;           // the inside pass as one launch (single-launch form)
   int band = -1;                  // paired[j,y] is -inf for y - j > band (-1: no such bound)
   const void *exp = nullptr;      // pair-root expansion descriptor of this member (device), or null
+  const void *dset = nullptr;     // identity of the member's dataset while exp is set (host), or null
   double *zout = nullptr;         // where log Z (absolute, baseline-relative) is also written (device)
   DevBuf Lr, Lc, Sr, Sc, F;       // inside tables (row- and column-major copies)
   DevBuf acc;                     // bulk split-point accumulator of the tiled inside

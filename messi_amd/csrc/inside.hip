@@ -124,24 +124,41 @@ __global__ void ptv_kernel(const InstDesc<typename SR::V> *__restrict__ insts, d
 
 // The same pair term gathered straight from the pair pruning's root entries
 // (no L x L pair matrix): the pair (j, y) maps to root entry
-// T0[pid0[j], pid0[y]] when y - j <= maxbp and the mask allows it.
+// T0[pid0[j], pid0[y]] when y - j <= maxbp and the mask allows it.  One grid
+// z-slice serves a run of members on one dataset (groups[z] = first member,
+// member count): the entry of a cell is looked up once and each member's
+// value gathered from its own root entries.  Only cells y > j are written;
+// every reader of the pair term takes zero at y <= j without reading it.
 template <class SR>
-__global__ void ptv_root_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
-                                const double *__restrict__ rootacc, double B) {
-  const InstDesc<typename SR::V> d = insts[blockIdx.z];
-  const ExpandDesc e = *d.exp;
-  const int L = d.L;
-  int y = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y;
-  if (y >= L || j >= L) return;
-  double x = -HUGE_VAL;
-  if (j < y && y - j <= e.maxbp && (!e.mask || gp(e.mask)[(size_t)j + (size_t)L * y])) {
-    const int *const pid0 = e.pid0;
-    const int k = gp(e.T0)[(size_t)gp(pid0)[j] * e.n0 + gp(pid0)[y]];
-    const double v = k >= 0 ? gp(rootacc)[e.rootoff + k] : -HUGE_VAL;
-    if (v != -HUGE_VAL) x = v * B - gp(d.lam)[j] - gp(d.lam)[y];
+__global__ void ptv_group_kernel(const InstDesc<typename SR::V> *__restrict__ insts,
+                                 const int2 *__restrict__ groups, const double *__restrict__ rootacc, double B) {
+  constexpr int MU = 4;  // members whose gathers are in flight together
+  const int2 g = groups[blockIdx.z];
+  const ExpandDesc e = *insts[g.x].exp;
+  const int L = e.L;
+  const int j = blockIdx.y;
+  const int y = j + 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (y >= L) return;
+  int k = -1;
+  if (y - j <= e.maxbp && (!e.mask || gp(e.mask)[(size_t)j + (size_t)L * y]))
+    k = gp(e.T0)[(size_t)gp(e.pid0)[j] * e.n0 + gp(e.pid0)[y]];
+  const size_t c = (size_t)j * L + y;
+  for (int m0 = 0; m0 < g.y; m0 += MU) {
+    double v[MU], lj[MU], ly[MU];
+#pragma unroll
+    for (int u = 0; u < MU; ++u) {
+      const InstDesc<typename SR::V> &d = insts[g.x + min(m0 + u, g.y - 1)];
+      v[u] = k >= 0 ? gp(rootacc)[d.exp->rootoff + k] : -HUGE_VAL;
+      lj[u] = gp(d.lam)[j];
+      ly[u] = gp(d.lam)[y];
+    }
+#pragma unroll
+    for (int u = 0; u < MU; ++u)
+      if (m0 + u < g.y) {
+        const double x = v[u] == -HUGE_VAL ? -HUGE_VAL : v[u] * B - lj[u] - ly[u];
+        gp(insts[g.x + m0 + u].ptv)[c] = SR::from_log(x);
+      }
   }
-  gp(d.ptv)[(size_t)j * L + y] = SR::from_log(x);
 }
 
 // Set the diagonal of S and L (F has no s rule).  The bulk accumulator needs no
@@ -1039,7 +1056,7 @@ __device__ __forceinline__ void inside_item(const InstDesc<typename SR::V> &ins,
   };
   auto load_tables = [&]() {
     const int a = tid / T, b = tid % T, j = IT + a, y = JT + b, jd = IT + b, ys = JT + a;
-    const bool k1 = j < L && jd < L && jd >= j, k2 = j < L && y < L, k3 = ys < L && b <= a;
+    const bool k1 = j < L && jd < L && jd >= j, k2 = j < y && y < L, k3 = ys < L && b <= a;
     const V x1 = typename SR::V(Lr[k1 ? (size_t)j * L + jd : 0]);   // L[j, jd] of tile (I, I)
     const V x2 = typename SR::V(ptv[k2 ? (size_t)j * L + y : 0]);   // pair term of the cell
     const V x3 = typename SR::V(Sc[k3 ? (size_t)ys * L + y : 0]);   // S[JT+b, JT+a] of tile (J, J)
@@ -2296,18 +2313,22 @@ static InstDesc<typename SR::V> inst_of(InsideWork &w) {
   return d;
 }
 
-// Baseline and pair term of every instance (u and paired already on the device).
+// Baseline and pair term of every instance (u and paired already on the device;
+// with root entries, groups holds the ngroups runs of members on one dataset).
 template <class SR>
 static void prepare_batch(hipStream_t s, const InstDesc<typename SR::V> *d_inst, int n, int Lmax, double B,
-                          const double *rootacc = nullptr) {
+                          const double *rootacc = nullptr, const int2 *groups = nullptr, int ngroups = 0) {
   double lp[7];
   kh99_logprobs(lp);
   hipLaunchKernelGGL(baseline_kernel<typename SR::V>, dim3(n), dim3(1024), 0, s, d_inst, B, lp[0] + lp[5],
                      (int)SR::scaled);
-  if (rootacc)
-    hipLaunchKernelGGL(ptv_root_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, rootacc, B);
-  else
+  if (rootacc) {
+    if (Lmax > 1)
+      hipLaunchKernelGGL(ptv_group_kernel<SR>, dim3((Lmax - 1 + 255) / 256, Lmax, ngroups), dim3(256), 0, s, d_inst,
+                         groups, rootacc, B);
+  } else {
     hipLaunchKernelGGL(ptv_kernel<SR>, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d_inst, B);
+  }
 }
 
 // Clears n ints (the single-launch form's counter and flags).
@@ -2399,13 +2420,21 @@ int inside_batch_run(hipStream_t s, InsideWork *const *ws, int n, InsideBatch &i
   };
   refresh(ib.inst, ib.h_inst, d.data(), d.size() * sizeof(InstDesc<V>));
   refresh(ib.prefix, ib.h_prefix, prefix.data(), prefix.size() * sizeof(int));
+  // runs of consecutive members on one dataset share their pair-term lookups
+  std::vector<int2> groups;
+  for (int m = 0; m < n; ++m)
+    if (m > 0 && ws[m]->dset && ws[m]->dset == ws[m - 1]->dset && d[m].L == d[m - 1].L)
+      groups.back().y++;
+    else
+      groups.push_back(make_int2(m, 1));
+  refresh(ib.groups, ib.h_groups, groups.data(), groups.size() * sizeof(int2));
   const InstDesc<V> *di = ib.inst.as<InstDesc<V>>();
   double lp[7];
   kh99_logprobs(lp);
   Rules<SR> r = make_rules<SR>(B);
   for (int m = 0; m < n; ++m)
     MESSI_REQUIRE((rootacc != nullptr) == (ws[m]->exp != nullptr), "pair term source differs between members");
-  prepare_batch<SR>(s, di, n, Lmax, B, rootacc);
+  prepare_batch<SR>(s, di, n, Lmax, B, rootacc, ib.groups.as<int2>(), (int)groups.size());
   hipLaunchKernelGGL(inside_init_kernel<SR>, dim3((Lmax + 255) / 256, 1, n), dim3(256), 0, s, di, B, lp[1],
                      lp[5]);
   if (persist) {
