@@ -65,6 +65,13 @@ int messi_set_band_skip(messi_ctx *ctx, int on);
  * (the schedule it replaces launches one kernel per anti-diagonal,
  * src/InsideCUDA.jl:87-181). */
 int messi_set_single_launch(messi_ctx *ctx, int on);
+/* Whether datasets created later on this context compute some 16-state pair
+ * nodes inside their parent's level launch instead of storing them (1, default):
+ * a node whose entries mostly serve one entry of its parent is then never
+ * written to or read back from device memory.  Results are bit-identical either
+ * way.  No counterpart in the reference, which stores every node's partials
+ * (felsensteinstack, src/FARCE.jl:2182-2254). */
+int messi_set_pair_fusion(messi_ctx *ctx, int on);
 /* Checks the MFMA operand layouts the kernels rely on against a CPU product. */
 int messi_selftest(messi_ctx *ctx);
 /* Page-locked host memory on the context's device (hipHostMalloc), for callers

@@ -98,6 +98,7 @@ struct messi_dataset {
   // level work items (index h-1 holds height h).
   std::vector<NodeDesc> site_nodes, pair_nodes;
   std::vector<std::vector<int2>> site_items, pair_items;
+  std::vector<std::vector<int2>> pair_vitems;  // pair items of nodes whose first child is computed in registers
   int64_t site_entries = 0, pair_entries = 0, U_root = 0, sumU = 0, sumN = 0, inrange = 0;
   int64_t site_idx_n = 0, pair_idx_n = 0, site_leaf_n = 0, pair_leaf_n = 0;
   DevBuf d_site_idx, d_site_leaf, d_pair_idx, d_pair_leaf;
@@ -149,9 +150,10 @@ struct messi_batch {
   const int *site_idx = nullptr, *pair_idx = nullptr;
   const uint8_t *site_leaf = nullptr;
   const uint16_t *pair_leaf = nullptr;
-  DevBuf site_nodes, pair_nodes, site_items, pair_items;
-  std::vector<int> site_lv, pair_lv;
-  std::vector<int> pair_lvm;  // first pair item of member m at level h: [(h - 1) * (n + 1) + m]
+  DevBuf site_nodes, pair_nodes, site_items, pair_items, pair_vitems;
+  std::vector<int> site_lv, pair_lv, pair_vlv;
+  std::vector<int> pair_lvm;   // first pair item of member m at level h: [(h - 1) * (n + 1) + m]
+  std::vector<int> pair_vlvm;  // the same for the items of nodes with a child computed in registers
   std::vector<int64_t> node_base, site_root, pair_root;
   int64_t nn_total = 0, site_entries = 0, pair_entries = 0, site_root_total = 0, pair_root_total = 0;
   int Lmax = 0;
@@ -369,6 +371,49 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
         std::vector<int>().swap(ey[v]);
       }
     }
+    // Children computed in registers: an internal non-root child c of v is not
+    // stored when that moves fewer bytes.  Counting the bytes of an FP32 launch
+    // (64 per stored entry, 2 per leaf code, 4 per index), storing c costs its
+    // store, its own children's reads and indices, and one read of c per entry
+    // of v; computing c in v's launch costs the reads and indices of c's
+    // children once per entry of v.  A tree DP picks the set with the largest
+    // saving in which no computed node has a computed parent or child and no
+    // node has two computed children.
+    std::vector<char> virt(n, 0);
+    if (ds->ctx->pair_fuse) {
+      std::vector<int> parent(n, -1);
+      for (int v = 0; v < n; ++v)
+        if (!isleaf(v)) parent[ds->ch[2 * v]] = parent[ds->ch[2 * v + 1]] = v;
+      auto rd = [&](int c) { return isleaf(c) ? 2.0 : 64.0; };
+      std::vector<double> fs(n, 0.0), fv(n, -1.0);  // best saving below v with v stored / computed (-1: not allowed)
+      for (int v = n - 1; v >= 0; --v) {
+        if (isleaf(v)) continue;
+        const int l = ds->ch[2 * v], r = ds->ch[2 * v + 1];
+        double base = 0.0, best = 0.0;
+        for (int c : {l, r})
+          if (!isleaf(c)) {
+            base += fs[c];
+            if (fv[c] >= 0.0) best = std::max(best, fv[c] - fs[c]);
+          }
+        fs[v] = base + best;
+        if (v != 0) {
+          const double ca = rd(l), cb = rd(r);
+          const double gain = (double)U[v] * (64.0 + ca + cb + 8.0) + (double)U[parent[v]] * (64.0 - 4.0 - ca - cb);
+          if (gain > 0.0) fv[v] = gain + base;
+        }
+      }
+      for (int v = 0; v < n; ++v) {  // pre-order: a node's status is known before its children's
+        if (isleaf(v) || virt[v]) continue;
+        int pick = -1;
+        double best = 0.0;
+        for (int c : {ds->ch[2 * v], ds->ch[2 * v + 1]})
+          if (!isleaf(c) && fv[c] >= 0.0 && fv[c] - fs[c] > best) {
+            best = fv[c] - fs[c];
+            pick = c;
+          }
+        if (pick >= 0) virt[pick] = 1;
+      }
+    }
     std::vector<int64_t> off(n, 0);
     std::vector<uint16_t> leaf;
     int64_t part = 0;
@@ -382,7 +427,7 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
         }
       } else {
         ds->sumU += U[v];
-        if (v != 0) {
+        if (v != 0 && !virt[v]) {
           off[v] = part;
           part += U[v];
         }
@@ -399,19 +444,41 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
     std::vector<NodeDesc> nodes(n);
     std::vector<int> idx;
     idx.reserve((size_t)2 * ds->sumU);
+    // A node's children in pre-order; a computed child goes first (swapping the
+    // two sides changes no result: the sides' products are multiplied, their
+    // exponents added).  A computed child's own maps stay until its parent,
+    // which comes before it, has composed them.
     for (int v = 0; v < n; ++v) {
-      if (isleaf(v)) continue;
+      if (isleaf(v) || virt[v]) continue;
       int l = ds->ch[2 * v], r = ds->ch[2 * v + 1];
+      const bool sw = virt[r] != 0;
+      if (sw) std::swap(l, r);
+      const std::vector<int> &ml = sw ? ir[v] : il[v], &mr = sw ? il[v] : ir[v];
       NodeDesc d{};
       d.U = U[v];
-      d.idx_l = (int64_t)idx.size();
-      idx.insert(idx.end(), il[v].begin(), il[v].end());
+      if (virt[l]) {
+        const int a = ds->ch[2 * l], bb = ds->ch[2 * l + 1];
+        d.virt = 1;
+        d.idx_l = (int64_t)idx.size();
+        for (int64_t e = 0; e < U[v]; ++e) idx.push_back(il[l][ml[e]]);
+        d.idx_v2 = (int64_t)idx.size();
+        for (int64_t e = 0; e < U[v]; ++e) idx.push_back(ir[l][ml[e]]);
+        d.cpart_l = off[a];
+        d.leaf_l = isleaf(a);
+        d.cpart_v2 = off[bb];
+        d.leaf_v2 = isleaf(bb);
+        d.node_va = a;
+        d.node_vb = bb;
+      } else {
+        d.idx_l = (int64_t)idx.size();
+        idx.insert(idx.end(), ml.begin(), ml.end());
+        d.cpart_l = off[l];
+        d.leaf_l = isleaf(l);
+      }
       d.idx_r = (int64_t)idx.size();
-      idx.insert(idx.end(), ir[v].begin(), ir[v].end());
+      idx.insert(idx.end(), mr.begin(), mr.end());
       d.part = off[v];
-      d.cpart_l = off[l];
       d.cpart_r = off[r];
-      d.leaf_l = isleaf(l);
       d.leaf_r = isleaf(r);
       d.node_l = l;
       d.node_r = r;
@@ -420,11 +487,15 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
       std::vector<int>().swap(il[v]);
       std::vector<int>().swap(ir[v]);
     }
-    const int step = 16 * PAIR_NB;
+    const int step = 16 * PAIR_NB, vstep = 16 * PAIR_VNB;
     ds->pair_items.assign(H, {});
+    ds->pair_vitems.assign(H, {});
     for (int v = 0; v < n; ++v)
-      if (!isleaf(v))
-        for (int64_t e = 0; e < U[v]; e += step) ds->pair_items[ds->height[v] - 1].push_back(make_int2(v, (int)e));
+      if (!isleaf(v) && !virt[v]) {
+        std::vector<int2> &list = nodes[v].virt ? ds->pair_vitems[ds->height[v] - 1] : ds->pair_items[ds->height[v] - 1];
+        const int st = nodes[v].virt ? vstep : step;
+        for (int64_t e = 0; e < U[v]; e += st) list.push_back(make_int2(v, (int)e));
+      }
     ds->pair_nodes = nodes;
     ds->pair_idx_n = (int64_t)idx.size();
     ds->pair_leaf_n = (int64_t)leaf.size();
@@ -524,10 +595,12 @@ void build_batch(messi_batch *b) {
   b->site_root_total = sr;
   b->pair_root_total = pr;
   std::vector<NodeDesc> sn(nb), pn(nb);
-  std::vector<int2> si, pi;
+  std::vector<int2> si, pi, vi;
   b->site_lv.assign(b->Hmax + 1, 0);
   b->pair_lv.assign(b->Hmax + 1, 0);
+  b->pair_vlv.assign(b->Hmax + 1, 0);
   b->pair_lvm.assign((size_t)b->Hmax * (n + 1), 0);
+  b->pair_vlvm.assign((size_t)b->Hmax * (n + 1), 0);
   for (int m = 0; m < n; ++m) {
     messi_dataset *d = b->mem[m];
     const int di = distinct[d];
@@ -545,6 +618,12 @@ void build_batch(messi_batch *b) {
         nd.cpart_r += nd.leaf_r ? lb : pb;
         nd.node_l += (int)b->node_base[m];
         nd.node_r += (int)b->node_base[m];
+        if (nd.virt) {
+          nd.idx_v2 += ib;
+          nd.cpart_v2 += nd.leaf_v2 ? lb : pb;
+          nd.node_va += (int)b->node_base[m];
+          nd.node_vb += (int)b->node_base[m];
+        }
         nd.member = m;
         nd.out = kind ? b->pair_root[m] : b->site_root[m];
         (kind ? pn : sn)[b->node_base[m] + v] = nd;
@@ -554,21 +633,27 @@ void build_batch(messi_batch *b) {
   for (int h = 1; h <= b->Hmax; ++h) {
     b->site_lv[h - 1] = (int)si.size();
     b->pair_lv[h - 1] = (int)pi.size();
+    b->pair_vlv[h - 1] = (int)vi.size();
     for (int m = 0; m < n; ++m) {
       b->pair_lvm[(size_t)(h - 1) * (n + 1) + m] = (int)pi.size();
+      b->pair_vlvm[(size_t)(h - 1) * (n + 1) + m] = (int)vi.size();
       messi_dataset *d = b->mem[m];
       if (h > d->H) continue;
       for (int2 it : d->site_items[h - 1]) si.push_back(make_int2(it.x + (int)b->node_base[m], it.y));
       for (int2 it : d->pair_items[h - 1]) pi.push_back(make_int2(it.x + (int)b->node_base[m], it.y));
+      for (int2 it : d->pair_vitems[h - 1]) vi.push_back(make_int2(it.x + (int)b->node_base[m], it.y));
     }
     b->pair_lvm[(size_t)(h - 1) * (n + 1) + n] = (int)pi.size();
+    b->pair_vlvm[(size_t)(h - 1) * (n + 1) + n] = (int)vi.size();
   }
   b->site_lv[b->Hmax] = (int)si.size();
   b->pair_lv[b->Hmax] = (int)pi.size();
+  b->pair_vlv[b->Hmax] = (int)vi.size();
   upload(b->site_nodes, sn, s);
   upload(b->pair_nodes, pn, s);
   upload(b->site_items, si, s);
   upload(b->pair_items, pi, s);
+  upload(b->pair_vitems, vi, s);
   b->w64.clear();
   b->w32.clear();
   for (int m = 0; m < n; ++m) {
@@ -788,13 +873,19 @@ template <typename T>
 void pair_levels(messi_batch *b, hipStream_t s, int k, int m0, int m1) {
   const int n = b->n;
   for (int h = 1; h <= b->Hmax; ++h) {
-    const int lo = b->pair_lvm[(size_t)(h - 1) * (n + 1) + m0], hi = b->pair_lvm[(size_t)(h - 1) * (n + 1) + m1];
+    const size_t r = (size_t)(h - 1) * (n + 1);
+    const int lo = b->pair_lvm[r + m0], hi = b->pair_lvm[r + m1];
+    const int vlo = b->pair_vlvm[r + m0], vhi = b->pair_vlvm[r + m1];
+    const T *P = b->c16.P.as<T>() + (size_t)k * b->nn_total * 256;
+    const double *pi = b->c16.pi.as<double>() + (size_t)k * b->n * 16;
+    const double *logw = b->c16.logw.as<double>() + (size_t)k * b->n;
     launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
-                                  b->pair_idx, b->pair_leaf, b->pair_part.as<T>(),
-                                  b->c16.P.as<T>() + (size_t)k * b->nn_total * 256,
-                                  b->c16.pi.as<double>() + (size_t)k * b->n * 16,
-                                  b->c16.logw.as<double>() + (size_t)k * b->n, k == 0, b->pair_acc.as<double>(),
-                                  b->pair_comp.as<double>());
+                                  b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), P, pi, logw, k == 0,
+                                  b->pair_acc.as<double>(), b->pair_comp.as<double>());
+    launch_pair_level<T, PAIR_VNB, true>(s, b->pair_vitems.as<int2>() + vlo, vhi - vlo,
+                                         b->pair_nodes.as<NodeDesc>(), b->pair_idx, b->pair_leaf,
+                                         b->pair_part.as<T>(), P, pi, logw, k == 0, b->pair_acc.as<double>(),
+                                         b->pair_comp.as<double>());
   }
 }
 
@@ -1004,10 +1095,16 @@ int64_t component_roots(messi_batch *b, int prec, double *sitec, const ExpandDes
   cp.logw = b->n;
   for (int h = 1; h <= b->Hmax; ++h) {
     const int lo = b->pair_lv[h - 1], hi = b->pair_lv[h];
+    const int vlo = b->pair_vlv[h - 1], vhi = b->pair_vlv[h];
     launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
                                   b->pair_idx, b->pair_leaf, b->pair_part.as<T>(),
                                   b->c16.P.as<T>(), b->c16.pi.as<double>(), b->c16.logw.as<double>(), 1,
                                   b->pair_acc.as<double>(), b->pair_comp.as<double>(), cp);
+    launch_pair_level<T, PAIR_VNB, true>(s, b->pair_vitems.as<int2>() + vlo, vhi - vlo,
+                                         b->pair_nodes.as<NodeDesc>(), b->pair_idx, b->pair_leaf,
+                                         b->pair_part.as<T>(), b->c16.P.as<T>(), b->c16.pi.as<double>(),
+                                         b->c16.logw.as<double>(), 1, b->pair_acc.as<double>(),
+                                         b->pair_comp.as<double>(), cp);
   }
   return pr;
 }
@@ -1235,6 +1332,12 @@ int messi_set_band_skip(messi_ctx *ctx, int on) {
 int messi_set_single_launch(messi_ctx *ctx, int on) {
   if (!ctx) return 2;
   ctx->persist = on != 0;
+  return 0;
+}
+
+int messi_set_pair_fusion(messi_ctx *ctx, int on) {
+  if (!ctx) return 2;
+  ctx->pair_fuse = on != 0;
   return 0;
 }
 

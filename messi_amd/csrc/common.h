@@ -171,6 +171,7 @@ struct messi_ctx {
   int window = 4;  // bulk window of the inside schedule
   bool band_skip = true;  // skip the split-point terms a pairing band makes -inf
   bool persist = false;   // inside pass as one launch (messi_set_single_launch)
+  bool pair_fuse = true;  // datasets created later compute some pair nodes in their parent's launch
   messi::InsideWork w64, w32;
   messi::InsideWork wna;  // messi_inside_noalloc: every buffer borrowed from the device's static scratch
   messi::MeaWork mea;

@@ -5,7 +5,8 @@
 
 namespace messi {
 
-#define PAIR_NB 4  // 16-entry blocks per wavefront item of the pair kernel
+#define PAIR_NB 4   // 16-entry blocks per wavefront item of the pair kernel
+#define PAIR_VNB 2  // the same for nodes whose first child is computed in registers
 
 // Per internal node descriptor of a level launch (one batch member's node).
 // Offsets are in entries of the batch arenas.
@@ -19,6 +20,14 @@ struct NodeDesc {
   int node_l, node_r;       // global transition-matrix index of each child (member base + node)
   int member;               // batch member
   int root;                 // this node is its tree's root
+  // A node whose first child c is not stored but computed in registers from c's
+  // own two children (a, b) for each of the node's entries: idx_l, cpart_l and
+  // leaf_l then describe a, idx_v2, cpart_v2 and leaf_v2 describe b (the maps
+  // take the node's entries straight to a's and b's entries), node_l is c's
+  // transition matrix and node_va, node_vb are a's and b's.
+  int64_t idx_v2 = 0, cpart_v2 = 0;
+  int leaf_v2 = 0, node_va = 0, node_vb = 0;
+  int virt = 0;             // the first child is computed in registers
 };
 
 // Several mixture components in one launch (grid row c = component c): each
@@ -31,7 +40,7 @@ struct CompBatch {
 
 // P / pi / logw point at one mixture component for all members: the matrix of global
 // node g is P + 256 g (16 g for 4 states), member m uses pi + 16 m (4 m) and logw[m].
-template <typename T, int NB>
+template <typename T, int NB, bool VL = false>
 void launch_pair_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes,
                        const int *idx, const uint16_t *leafc, T *part, const T *P,
                        const double *pi16, const double *logw, int first, double *rootacc,

@@ -163,18 +163,68 @@ template <class X> __device__ __forceinline__ X *boff(X *p, unsigned bytes) {
   return reinterpret_cast<X *>(reinterpret_cast<typename std::conditional<std::is_const<X>::value, const char, char>::type *>(p) + bytes);
 }
 
+// This lane's 4 values of one child entry in the operand lane order: a leaf's
+// two codes expanded to indicator values, or a stored entry gathered (its
+// exponent bits still in place).
+template <typename T>
+__device__ __forceinline__ void child_values(bool leaf, const uint16_t *lc, const T *pp, unsigned ci, int g,
+                                             T out[4]) {
+  if (leaf) {
+    const unsigned cc = *boff(lc, 2u * ci);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int a = Mfma<T>::state(g, s);
+      out[s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
+    }
+  } else {
+    load4(boff(pp, (ci * 16u + 4u * g) * (unsigned)sizeof(T)), out);
+  }
+}
+
+// One node entry from its two children's values xl, xr (exponents El, Er): the
+// products with the children's transition matrices on MFMA, their elementwise
+// product, the maximum scaled into [0.5, 1) by an exact power of two.  The
+// values come out in the accumulator order, which is the operand order of the
+// parent's product; the entry's exponent is returned.
+template <typename T>
+__device__ __forceinline__ int node_entry(const T pl[4], const T xl[4], int El, const T pr[4], const T xr[4],
+                                          int Er, T v[4]) {
+  typedef typename Mfma<T>::v4 v4;
+  v4 ml = {0, 0, 0, 0}, mr = {0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    ml = Mfma<T>::op(pl[s], xl[s], ml);
+    mr = Mfma<T>::op(pr[s], xr[s], mr);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) v[r] = ml[r] * mr[r];
+  const T m = rows_max<T>(fmax(fmax(v[0], v[1]), fmax(v[2], v[3])));
+  int E = El + Er;
+  if (m > (T)0) {
+    const int ex = frexp_exp(m);
+    const T sc = ldexp((T)1, -ex);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] *= sc;
+    E += ex;
+  }
+  return E;
+}
+
 // One wavefront per item of 16*NB consecutive entries of one node.  The kernel
 // gathers child entries at random: its speed follows the number of loads in
 // flight, so the register budget is set for 6 wavefronts per SIMD in FP32 (80
-// VGPRs, a few spilled) and 4 in FP64 (123 VGPRs, none spilled).
-template <typename T, int NB>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 4) void pair_level_kernel(
+// VGPRs, a few spilled) and 4 in FP64 (123 VGPRs, none spilled).  With VL the
+// node's first child is not stored: its entry is computed in registers from
+// its own two children's entries (the same steps as its own launch would take,
+// so the result is the same to the bit), which saves that child's store and
+// the reread of it when most of its entries serve one entry of the node.
+template <typename T, int NB, bool VL>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? (VL ? 4 : 6) : 4) void pair_level_kernel(
     const int2 *__restrict__ items, int nitems, const NodeDesc *__restrict__ nodes,
     const int *__restrict__ idx, const uint16_t *__restrict__ leafc, T *__restrict__ part,
     const T *__restrict__ P, const double *__restrict__ pi16,
     const double *__restrict__ logws, int first, double *__restrict__ rootacc,
     double *__restrict__ rootcomp, CompBatch cb) {
-  typedef typename Mfma<T>::v4 v4;
   const int lane = threadIdx.x & 63;
   {
     const int64_t c = blockIdx.y;  // component of this grid row
@@ -193,13 +243,22 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 4) void pair_level_kernel
   const int2 it = items[wave];
   const NodeDesc nd = nodes[it.x];
   const int col = lane & 15, g = lane >> 4;
-  T al[4], ar[4];
+  T al[4], ar[4], aa[4], ab[4];
   const T *Pl = P + (size_t)nd.node_l * 256;
   const T *Pr = P + (size_t)nd.node_r * 256;
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     al[s] = Pl[col + 16 * Mfma<T>::state(g, s)];
     ar[s] = Pr[col + 16 * Mfma<T>::state(g, s)];
+  }
+  if constexpr (VL) {
+    const T *Pa = P + (size_t)nd.node_va * 256;
+    const T *Pb = P + (size_t)nd.node_vb * 256;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      aa[s] = Pa[col + 16 * Mfma<T>::state(g, s)];
+      ab[s] = Pb[col + 16 * Mfma<T>::state(g, s)];
+    }
   }
   const bool ROOT = nd.root != 0;
   double pis[4] = {0, 0, 0, 0};
@@ -210,50 +269,36 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 4) void pair_level_kernel
   const unsigned U = (unsigned)nd.U, e0 = (unsigned)it.y;  // entries of one node fit in 32 bits
   const int *__restrict__ il = idx + nd.idx_l;
   const int *__restrict__ ir = idx + nd.idx_r;
+  const int *__restrict__ ix = idx + nd.idx_v2;
   const T *pl = part + (size_t)nd.cpart_l * 16, *pr = part + (size_t)nd.cpart_r * 16;
-  const uint16_t *lcl = leafc + nd.cpart_l, *lcr = leafc + nd.cpart_r;
+  const T *px = part + (size_t)nd.cpart_v2 * 16;
+  const uint16_t *lcl = leafc + nd.cpart_l, *lcr = leafc + nd.cpart_r, *lcx = leafc + nd.cpart_v2;
   // Every load of the NB blocks is issued before the first store: part is both
   // read (children) and written (this node), so loads after a store could not
   // be hoisted above it.
-  unsigned cl[NB], cr[NB];
+  unsigned cl[NB], cr[NB], cx[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const unsigned e = e0 + b * 16 + col;
     const unsigned ec = e < U ? e : U - 1;
     cl[b] = (unsigned)*boff(il, 4u * ec);
     cr[b] = (unsigned)*boff(ir, 4u * ec);
+    if constexpr (VL) cx[b] = (unsigned)*boff(ix, 4u * ec);
   }
-  T bl[NB][4], br[NB][4];
-  unsigned wl[NB], wr[NB];  // this lane's exponent bit fields of the two children (0 for a leaf)
+  // this lane's exponent bit fields of the children (0 for a leaf)
+  T bl[NB][4], br[NB][4], bx[NB][4];
+  unsigned wl[NB], wr[NB], wx[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    wl[b] = 0;
-    wr[b] = 0;
-    if (nd.leaf_l) {
-      const unsigned cc = *boff(lcl, 2u * cl[b]);
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int a = Mfma<T>::state(g, s);
-        bl[b][s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
-      }
-    } else {
-      load4(boff(pl, (cl[b] * 16u + 4u * g) * (unsigned)sizeof(T)), bl[b]);
-    }
-    if (nd.leaf_r) {
-      const unsigned cc = *boff(lcr, 2u * cr[b]);
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int a = Mfma<T>::state(g, s);
-        br[b][s] = (T)((cc >> (a >> 2)) & 1u) * (T)((cc >> (8 + (a & 3))) & 1u);
-      }
-    } else {
-      load4(boff(pr, (cr[b] * 16u + 4u * g) * (unsigned)sizeof(T)), br[b]);
-    }
+    child_values<T>(nd.leaf_l, lcl, pl, cl[b], g, bl[b]);
+    child_values<T>(nd.leaf_r, lcr, pr, cr[b], g, br[b]);
+    if constexpr (VL) child_values<T>(nd.leaf_v2, lcx, px, cx[b], g, bx[b]);
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    if (!nd.leaf_l) wl[b] = take_exp_bits<T>(bl[b], g);
-    if (!nd.leaf_r) wr[b] = take_exp_bits<T>(br[b], g);
+    wl[b] = nd.leaf_l ? 0u : take_exp_bits<T>(bl[b], g);
+    wr[b] = nd.leaf_r ? 0u : take_exp_bits<T>(br[b], g);
+    if constexpr (VL) wx[b] = nd.leaf_v2 ? 0u : take_exp_bits<T>(bx[b], g);
   }
   T *po = part + (size_t)nd.part * 16;
   double *rc = rootcomp + nd.out, *ra = rootacc + nd.out;
@@ -266,23 +311,14 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 4) void pair_level_kernel
   for (int b = 0; b < NB; ++b) {
     const unsigned e = e0 + b * 16 + col;
     const bool valid = e < U;
-    v4 ml = {0, 0, 0, 0}, mr = {0, 0, 0, 0};
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      ml = Mfma<T>::op(al[s], bl[b][s], ml);
-      mr = Mfma<T>::op(ar[s], br[b][s], mr);
-    }
     T v[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) v[r] = ml[r] * mr[r];
-    const T m = rows_max<T>(fmax(fmax(v[0], v[1]), fmax(v[2], v[3])));
-    int E = join_exp_bits(wl[b]) + join_exp_bits(wr[b]);
-    if (m > (T)0) {
-      const int ex = frexp_exp(m);
-      const T sc = ldexp((T)1, -ex);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) v[r] *= sc;
-      E += ex;
+    int E;
+    if constexpr (VL) {
+      T vc[4];
+      const int Ec = node_entry<T>(aa, bl[b], join_exp_bits(wl[b]), ab, bx[b], join_exp_bits(wx[b]), vc);
+      E = node_entry<T>(al, vc, Ec, ar, br[b], join_exp_bits(wr[b]), v);
+    } else {
+      E = node_entry<T>(al, bl[b], join_exp_bits(wl[b]), ar, br[b], join_exp_bits(wr[b]), v);
     }
     if (!ROOT) {
       put_exp_bits<T>(v, E, g);
@@ -571,13 +607,13 @@ void launch_pair_gather(hipStream_t s, int L, int K, int64_t root_stride, const 
                      out, missing);
 }
 
-template <typename T, int NB>
+template <typename T, int NB, bool VL>
 void launch_pair_level(hipStream_t s, const int2 *items, int nitems, const NodeDesc *nodes, const int *idx,
                        const uint16_t *leafc, T *part, const T *P, const double *pi16,
                        const double *logw, int first, double *rootacc, double *rootcomp, const CompBatch &cb) {
   if (nitems <= 0) return;
   int blocks = (nitems + 3) / 4;
-  hipLaunchKernelGGL((pair_level_kernel<T, NB>), dim3(blocks, cb.K), dim3(256), 0, s, items, nitems, nodes, idx,
+  hipLaunchKernelGGL((pair_level_kernel<T, NB, VL>), dim3(blocks, cb.K), dim3(256), 0, s, items, nitems, nodes, idx,
                      leafc, part, P, pi16, logw, first, rootacc, rootcomp, cb);
 }
 
@@ -599,12 +635,15 @@ void launch_pair_expand(hipStream_t s, int n, int Lmax, const ExpandDesc *d, con
   hipLaunchKernelGGL(pair_expand_kernel, dim3((Lmax + 255) / 256, Lmax, n), dim3(256), 0, s, d, acc, outs);
 }
 
-template void launch_pair_level<double, PAIR_NB>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
-                                                 const uint16_t *, double *, const double *, const double *,
-                                                 const double *, int, double *, double *, const CompBatch &);
-template void launch_pair_level<float, PAIR_NB>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
-                                                const uint16_t *, float *, const float *, const double *,
-                                                const double *, int, double *, double *, const CompBatch &);
+#define MESSI_PAIR_LEVEL(T, NB, VL)                                                                           \
+  template void launch_pair_level<T, NB, VL>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,        \
+                                             const uint16_t *, T *, const T *, const double *, const double *, int, \
+                                             double *, double *, const CompBatch &);
+MESSI_PAIR_LEVEL(double, PAIR_NB, false)
+MESSI_PAIR_LEVEL(float, PAIR_NB, false)
+MESSI_PAIR_LEVEL(double, PAIR_VNB, true)
+MESSI_PAIR_LEVEL(float, PAIR_VNB, true)
+#undef MESSI_PAIR_LEVEL
 template void launch_site_level<double>(hipStream_t, const int2 *, int, const NodeDesc *, const int *,
                                         const uint8_t *, double *, int *, const double *, const double *,
                                         const double *, int, double *, const CompBatch &);

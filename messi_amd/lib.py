@@ -83,6 +83,7 @@ SIGNATURES = {
     "messi_set_bulk_window": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_set_band_skip": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_set_single_launch": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "messi_set_pair_fusion": (ctypes.c_int, [_vp, ctypes.c_int]),
     "messi_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "messi_batch_destroy": (None, [_vp]),
     "messi_batch_eval": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
@@ -225,6 +226,12 @@ class Context:
         instead of one launch per tile-diagonal (default off; results are
         bit-identical)."""
         self.check(self.lib.messi_set_single_launch(self._h, int(bool(on))))
+
+    def set_pair_fusion(self, on: bool) -> None:
+        """Whether datasets created later compute the pair nodes whose entries
+        mostly serve one parent entry inside the parent's launch instead of
+        storing them (default on; results are bit-identical)."""
+        self.check(self.lib.messi_set_pair_fusion(self._h, int(bool(on))))
 
     def close(self) -> None:
         if self._h:
