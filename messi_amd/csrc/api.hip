@@ -376,9 +376,12 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
     // (64 per stored entry, 2 per leaf code, 4 per index), storing c costs its
     // store, its own children's reads and indices, and one read of c per entry
     // of v; computing c in v's launch costs the reads and indices of c's
-    // children once per entry of v.  A tree DP picks the set with the largest
-    // saving in which no computed node has a computed parent or child and no
-    // node has two computed children.
+    // children once per entry of v.  The computed form also repeats c's products
+    // and scaling for every entry of v, which the byte count leaves out: c is a
+    // candidate only when v has fewer than 1.4 entries per entry of c (at 50 x
+    // 1500 a bound of 1.4 beat no bound by 1.5 % per FP32 batch-16 step).  A tree
+    // DP picks the set with the largest saving in which no computed node has a
+    // computed parent or child and no node has two computed children.
     std::vector<char> virt(n, 0);
     if (ds->ctx->pair_fuse) {
       std::vector<int> parent(n, -1);
@@ -399,7 +402,7 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
         if (v != 0) {
           const double ca = rd(l), cb = rd(r);
           const double gain = (double)U[v] * (64.0 + ca + cb + 8.0) + (double)U[parent[v]] * (64.0 - 4.0 - ca - cb);
-          if (gain > 0.0) fv[v] = gain + base;
+          if (gain > 0.0 && (double)U[parent[v]] < 1.4 * (double)U[v]) fv[v] = gain + base;
         }
       }
       for (int v = 0; v < n; ++v) {  // pre-order: a node's status is known before its children's
