@@ -871,9 +871,10 @@ void run_site(messi_batch *b, int prec) {
 }
 
 // The level launches of pair component k for members m0 .. m1-1 on stream s
-// (the items of a level are ordered by member).
+// (the items of a level are ordered by member); comp keeps the component's own
+// root values in b->pair_comp besides their sum over components in b->pair_acc.
 template <typename T>
-void pair_levels(messi_batch *b, hipStream_t s, int k, int m0, int m1) {
+void pair_levels(messi_batch *b, hipStream_t s, int k, int m0, int m1, bool comp = false) {
   const int n = b->n;
   for (int h = 1; h <= b->Hmax; ++h) {
     const size_t r = (size_t)(h - 1) * (n + 1);
@@ -882,13 +883,13 @@ void pair_levels(messi_batch *b, hipStream_t s, int k, int m0, int m1) {
     const T *P = b->c16.P.as<T>() + (size_t)k * b->nn_total * 256;
     const double *pi = b->c16.pi.as<double>() + (size_t)k * b->n * 16;
     const double *logw = b->c16.logw.as<double>() + (size_t)k * b->n;
+    double *pc = comp ? b->pair_comp.as<double>() : nullptr;
     launch_pair_level<T, PAIR_NB>(s, b->pair_items.as<int2>() + lo, hi - lo, b->pair_nodes.as<NodeDesc>(),
                                   b->pair_idx, b->pair_leaf, b->pair_part.as<T>(), P, pi, logw, k == 0,
-                                  b->pair_acc.as<double>(), b->pair_comp.as<double>());
+                                  b->pair_acc.as<double>(), pc);
     launch_pair_level<T, PAIR_VNB, true>(s, b->pair_vitems.as<int2>() + vlo, vhi - vlo,
                                          b->pair_nodes.as<NodeDesc>(), b->pair_idx, b->pair_leaf,
-                                         b->pair_part.as<T>(), P, pi, logw, k == 0, b->pair_acc.as<double>(),
-                                         b->pair_comp.as<double>());
+                                         b->pair_part.as<T>(), P, pi, logw, k == 0, b->pair_acc.as<double>(), pc);
   }
 }
 
@@ -910,7 +911,7 @@ void run_pair(messi_batch *b, int prec, double *h_percomp, bool expand, bool pai
   const ExpandDesc *dexp = upload_expand(b, prec, true);
   if (pair_only) return;
   for (int k = 0; k < K; ++k) {
-    pair_levels<T>(b, s, k, 0, b->n);
+    pair_levels<T>(b, s, k, 0, b->n, h_percomp != nullptr);
     if (h_percomp) {
       MESSI_REQUIRE(b->n == 1, "per-component output needs a single member");
       const int L = b->mem[0]->L;

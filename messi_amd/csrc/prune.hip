@@ -226,6 +226,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? (VL ? 4 : 6) : 4) void pair_l
     const double *__restrict__ logws, int first, double *__restrict__ rootacc,
     double *__restrict__ rootcomp, CompBatch cb) {
   const int lane = threadIdx.x & 63;
+  const bool comp = rootcomp != nullptr;  // keep each component's root values
   {
     const int64_t c = blockIdx.y;  // component of this grid row
     part += c * cb.part;
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? (VL ? 4 : 6) : 4) void pair_l
     pi16 += c * cb.pi;
     logws += c * cb.logw;
     rootacc += c * cb.root;
-    rootcomp += c * cb.root;
+    if (comp) rootcomp += c * cb.root;
   }
   // the wavefront's item, uniform: the item and its node descriptor come through
   // scalar loads, and every gather below is a scalar base plus a 32-bit lane offset
@@ -338,7 +339,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? (VL ? 4 : 6) : 4) void pair_l
     const unsigned e = e0 + g * 16 + col;
     if (g < NB && e < U) {
       const double ll = log(rs) + (double)rE * 0.69314718055994530942;
-      *boff(rc, 8u * e) = ll;
+      if (comp) *boff(rc, 8u * e) = ll;
       const double t = logws[nd.member] + ll;
       double *const pa = boff(ra, 8u * e);
       *pa = first ? t : lse_d(*pa, t);
