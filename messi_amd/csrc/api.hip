@@ -9,6 +9,7 @@
 
 #include "common.h"
 #include "prune.h"
+#include "pair_fuse.h"
 
 using namespace messi;
 
@@ -371,52 +372,9 @@ void build_dataset(messi_dataset *ds, const double *data, const int *children, c
         std::vector<int>().swap(ey[v]);
       }
     }
-    // Children computed in registers: an internal non-root child c of v is not
-    // stored when that moves fewer bytes.  Counting the bytes of an FP32 launch
-    // (64 per stored entry, 2 per leaf code, 4 per index), storing c costs its
-    // store, its own children's reads and indices, and one read of c per entry
-    // of v; computing c in v's launch costs the reads and indices of c's
-    // children once per entry of v.  The computed form also repeats c's products
-    // and scaling for every entry of v, which the byte count leaves out: c is a
-    // candidate only when v has fewer than 1.4 entries per entry of c (at 50 x
-    // 1500 a bound of 1.4 beat no bound by 1.5 % per FP32 batch-16 step).  A tree
-    // DP picks the set with the largest saving in which no computed node has a
-    // computed parent or child and no node has two computed children.
-    std::vector<char> virt(n, 0);
-    if (ds->ctx->pair_fuse) {
-      std::vector<int> parent(n, -1);
-      for (int v = 0; v < n; ++v)
-        if (!isleaf(v)) parent[ds->ch[2 * v]] = parent[ds->ch[2 * v + 1]] = v;
-      auto rd = [&](int c) { return isleaf(c) ? 2.0 : 64.0; };
-      std::vector<double> fs(n, 0.0), fv(n, -1.0);  // best saving below v with v stored / computed (-1: not allowed)
-      for (int v = n - 1; v >= 0; --v) {
-        if (isleaf(v)) continue;
-        const int l = ds->ch[2 * v], r = ds->ch[2 * v + 1];
-        double base = 0.0, best = 0.0;
-        for (int c : {l, r})
-          if (!isleaf(c)) {
-            base += fs[c];
-            if (fv[c] >= 0.0) best = std::max(best, fv[c] - fs[c]);
-          }
-        fs[v] = base + best;
-        if (v != 0) {
-          const double ca = rd(l), cb = rd(r);
-          const double gain = (double)U[v] * (64.0 + ca + cb + 8.0) + (double)U[parent[v]] * (64.0 - 4.0 - ca - cb);
-          if (gain > 0.0 && (double)U[parent[v]] < 1.4 * (double)U[v]) fv[v] = gain + base;
-        }
-      }
-      for (int v = 0; v < n; ++v) {  // pre-order: a node's status is known before its children's
-        if (isleaf(v) || virt[v]) continue;
-        int pick = -1;
-        double best = 0.0;
-        for (int c : {ds->ch[2 * v], ds->ch[2 * v + 1]})
-          if (!isleaf(c) && fv[c] >= 0.0 && fv[c] - fs[c] > best) {
-            best = fv[c] - fs[c];
-            pick = c;
-          }
-        if (pick >= 0) virt[pick] = 1;
-      }
-    }
+    // nodes computed in their parent's launch instead of stored (pair_fuse.h)
+    const std::vector<char> virt =
+        ds->ctx->pair_fuse ? fuse::pick(n, ds->ch.data(), U.data()) : std::vector<char>(n, 0);
     std::vector<int64_t> off(n, 0);
     std::vector<uint16_t> leaf;
     int64_t part = 0;
