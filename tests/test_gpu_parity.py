@@ -280,6 +280,28 @@ def test_batch_same_dataset_matches_single(ctx, prec):
             assert abs(zz - zr) <= REL64 * abs(zr)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_runs_of_shared_datasets(ctx, prec):
+    """Members on one dataset that follow each other share their pair-term lookups
+    (one pass per run); runs broken by other datasets or another distance bound
+    must give each member exactly its single evaluation's log Z."""
+    _say(f"batch with runs of shared datasets, {prec}")
+    a1, a2 = synth.config1(), synth.generate(16, 90, 23)
+    d1, d2 = synth.dataset_of(a1), synth.dataset_of(a2)
+    dd1, dd2, dd1b = hotpath.device_dataset(d1), hotpath.device_dataset(d2), hotpath.device_dataset(d1, 25)
+    p1, p2 = _perturbed(a1, d1, 4, seed=5), _perturbed(a2, d2, 2, seed=6)
+    members = [(dd1, p1[0]), (dd1, p1[1]), (dd2, p2[0]), (dd1, p1[2]), (dd1b, p1[3]), (dd1b, p1[0]),
+               (dd2, p2[1]), (dd2, p2[0])]
+    sp = [_stack_params(p) for _, p in members]
+    b = lib.Batch([d for d, _ in members])
+    try:
+        z = b.eval([x[0] for x in sp], [x[1] for x in sp], 1.0, prec)
+    finally:
+        b.close()
+    single = [d.eval(*x, 1.0, prec) for (d, _), x in zip(members, sp)]
+    assert np.array_equal(z, np.array(single)), (z, single)
+
+
 def test_batch_uneven_instances(ctx):
     _say("batch of 6 uneven instances (config4 generator)")
     als = synth.config4(count=6)
